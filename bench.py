@@ -1,0 +1,204 @@
+#!/usr/bin/env python3
+"""Certified-throughput benchmark of the FFCz projection post-processor on one
+MI355X.
+
+A step is one certified pass of the hot path over the synthetic field, inputs
+resident in HBM: reset the edit accumulators, the projection loop to
+convergence (iterating against bounds shrunk by the certification margins),
+extraction with the step quantiser (eps_s/8, delta/8), then the FP64
+certification of the unquantised and of the dequantised edits.  ``value`` is
+certified GB/s = steps * N * sizeof(T) / elapsed (0 if any certification fails).
+
+Prints one JSON line.  ``--gpus`` > 1 is rejected: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--config", default="c3", choices=["c1", "c2", "c3"])
+    p.add_argument("--seed", type=int, default=1234)
+    p.add_argument("--timing-reps", type=int, default=10)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-sample-iters", type=int, default=1)
+    return p.parse_args()
+
+
+def kernel_bytes(shape, s):
+    """Compulsory HBM bytes per launch of the five iteration kernels (the
+    accumulators are read/written only where a clamp fires)."""
+    nx, ny, nz = shape
+    N = nx * ny * nz
+    F = nx * ny * (nz // 2 + 1)
+    cs = 2 * s
+    return {
+        "z_fwd": N * s + F * cs,
+        "y_fwd": 2 * F * cs,
+        "x_fused": 2 * F * cs,
+        "y_inv": 2 * F * cs,
+        "z_inv": F * cs + N * s,
+    }
+
+
+def read_traffic(kernel: str, workload: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, scaled by the copy-kernel calibration), if a summary for this
+    kernel and workload is committed under profiles/."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"pmc_{kernel}_{workload}.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
+
+
+def cpu_baseline(wl, gpu_iters, sample_iters, cores):
+    """The oracle restatement of the reference loop on the host cores
+    (scipy pocketfft with ``workers=cores``), on a bounded sample: a few
+    projection iterations and one FP64 certification of the same field,
+    scaled to the GPU run's iteration count and two certifications."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import ffcz_oracle as O
+
+    x = wl.x.cpu().numpy()
+    e0 = wl.e0.cpu().numpy()
+    shape = x.shape
+    t0 = time.perf_counter()
+    r = O.solve(x, e0, wl.eps, mode="abs", delta=wl.delta, max_iter=sample_iters, tol=-1.0,
+                dtype=np.float32, workers=cores)
+    t_iter = (time.perf_counter() - t0) / sample_iters
+    t0 = time.perf_counter()
+    d = O.reconstruct_edit(shape, e0, r.df, r.ds, workers=cores)
+    O.certify(d, wl.eps, wl.delta, workers=cores)
+    t_cert = time.perf_counter() - t0
+    T = gpu_iters * t_iter + 2 * t_cert
+    nbytes = x.size * x.dtype.itemsize
+    return {
+        "value": nbytes / T / 1e9,
+        "unit": "GB/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{sample_iters} FP32 projection iteration(s) + 1 FP64 certification of the full "
+                  f"{'x'.join(map(str, shape))} field (numpy/scipy pocketfft, workers={cores}); "
+                  f"t_iter={t_iter:.3f}s t_cert={t_cert:.3f}s, scaled to {gpu_iters} iteration(s) + 2 certifications",
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+    import torch
+
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import ffcz_amd
+    from ffcz_amd import ProjectionSolver, MODE_ABS, QUANT_STEP
+    from ffcz_amd.workloads import make_config
+
+    ffcz_amd.load()
+    dev = torch.device("cuda:0")
+    wl = make_config(a.config, device=dev, seed=a.seed)
+    shape = tuple(wl.x.shape)
+    N = wl.x.numel()
+    s_in = wl.x.element_size()
+    torch.cuda.synchronize()
+    print(f"[bench] workload {a.config} shape={shape} eps={wl.eps:.3e} delta={wl.delta:.3e}", file=sys.stderr,
+          flush=True)
+
+    solver = ProjectionSolver(*shape, dtype=np.float32)
+    solver.initialize_all_abs(wl.x, wl.eps, wl.delta)
+    solver.set_margins(0.125, 0.125)
+    solver.set_quantizer(QUANT_STEP, wl.delta / 8, wl.eps / 8)
+
+    def step():
+        solver.reset_edit_accumulators()
+        solver.solve_loop(MODE_ABS, wl.e0, wl.max_iter, 1e-7)
+        solver.extract_compact_representation()
+        c1 = solver.certify(False)
+        c2 = solver.certify(True)
+        return c1, c2
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    certs = []
+    for _ in range(a.steps):
+        certs.append(step())
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    iters, viol, conv = solver.status()
+    ok = all(c1.ok and c2.ok for c1, c2 in certs)
+    value = (a.steps * N * s_in / elapsed / 1e9) if ok else 0.0
+    print(f"[bench] {a.steps} steps in {elapsed:.4f}s, iterations={iters} converged={conv} certified={ok}",
+          file=sys.stderr, flush=True)
+
+    # per-kernel timing of the iteration (HIP events on the solver stream)
+    ms = solver.time_iteration(a.timing_reps)
+    names = ["z_fwd", "y_fwd", "x_fused", "y_inv", "z_inv"]
+    kb = kernel_bytes(shape, s_in)
+    dom = int(np.argmax(ms))
+    dname = names[dom]
+    t_iter = sum(ms) * 1e-3
+    C = 2 * (shape[0] * shape[1] * (shape[2] // 2 + 1)) * s_in
+    B_iter = 4 * N * s_in + 4 * C
+    achieved = kb[dname] / (ms[dom] * 1e-3) / 1e9
+    traffic = read_traffic(dname, a.config)
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic, "kernel": dname,
+            "bytes_per_launch": kb[dname], "ms_per_launch": round(ms[dom], 5)}
+    iteration = {"ms": round(t_iter * 1e3, 4), "B_iter": B_iter,
+                 "achieved_GBps": round(B_iter / t_iter / 1e9, 1),
+                 "frac": round(B_iter / t_iter / 1e9 / HBM_PEAK_GBPS, 4),
+                 "kernels_ms": {n: round(m, 5) for n, m in zip(names, ms)}}
+    solver.reset_edit_accumulators()
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        cores = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(wl, max(iters, 1), a.cpu_sample_iters, cores)
+
+    out = {
+        "metric": "certified GB/s (both bounds met) on 1 GPU, MFMA split-prec vs plain FP32 FFT",
+        "value": round(value, 3),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(elapsed / a.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{a.config}: {'x'.join(map(str, shape))} f32 Gaussian random field, abs bounds "
+                               f"eps={wl.eps:.3e} delta={wl.delta:.3e}, plain FP32 vector-ALU FFT",
+                   "fft": "valu-fp32", "iterations": iters, "converged": conv, "certified": ok,
+                   "seed": a.seed, "margins": 0.125, "quantizer": "step eps/8, delta/8"},
+        "roofline": roof,
+        "iteration": iteration,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    solver.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,20 @@
+"""MI355X-native FFCz alternating-projection post-processor.
+
+The compute path is ``libffcz.so`` (hand-written HIP kernels for gfx950 behind
+the C ABI of ``include/ffcz.h``); this package is its Python host mirror.
+"""
+from ._lib import FFCzError, load  # noqa: F401
+from .solver import ProjectionSolver, CertResult, MODE_ABS, MODE_DIRECT, MODE_PTW, QUANT_REF16, QUANT_STEP  # noqa: F401
+from .entropy import (  # noqa: F401
+    CompressionResult,
+    cuda_huffman_zstd_compress,
+    cuda_huffman_zstd_decompress,
+    huffman_encode,
+)
+from .fft import fft_r2c, fft_c2r  # noqa: F401
+
+__all__ = [
+    "ProjectionSolver", "CertResult", "FFCzError", "load", "fft_r2c", "fft_c2r",
+    "cuda_huffman_zstd_compress", "cuda_huffman_zstd_decompress", "huffman_encode", "CompressionResult",
+    "MODE_ABS", "MODE_DIRECT", "MODE_PTW", "QUANT_REF16", "QUANT_STEP",
+]

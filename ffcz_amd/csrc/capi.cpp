@@ -1,0 +1,531 @@
+// C ABI over the HIP solver (include/ffcz.h).  Exceptions never cross the
+// boundary: every entry point returns a status and records a thread-local
+// message for ffcz_last_error().
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/ffcz.h"
+#include "extract.hpp"
+#include "huffman.hpp"
+#include "kernels.hpp"
+#include "solver.hpp"
+
+using namespace ffcz;
+
+static thread_local std::string g_err;
+
+struct ffcz_solver {
+    int dtype;
+    std::unique_ptr<Solver<float>> f;
+    std::unique_ptr<Solver<double>> d;
+};
+
+struct ffcz_coded {
+    HuffmanCoded h;
+    std::vector<uint8_t> zdata;
+    bool zstd = false;
+};
+
+template <typename F>
+static int guard(F&& fn) {
+    try {
+        fn();
+        return FFCZ_OK;
+    } catch (const Error& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_err = "host allocation failed";
+        return FFCZ_ERR_ALLOC;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return FFCZ_ERR_INVALID;
+    } catch (...) {
+        g_err = "unknown error";
+        return FFCZ_ERR_INVALID;
+    }
+}
+
+#define NEED(h) FFCZ_REQUIRE((h) != nullptr, ERR_INVALID, "null solver handle")
+
+template <typename T>
+static Solver<T>& sv(ffcz_solver* h);
+template <>
+Solver<float>& sv<float>(ffcz_solver* h) {
+    NEED(h);
+    FFCZ_REQUIRE(h->dtype == FFCZ_F32, ERR_INVALID, "solver is not float32");
+    return *h->f;
+}
+template <>
+Solver<double>& sv<double>(ffcz_solver* h) {
+    NEED(h);
+    FFCZ_REQUIRE(h->dtype == FFCZ_F64, ERR_INVALID, "solver is not float64");
+    return *h->d;
+}
+
+// dispatch on the solver dtype
+template <typename Fn>
+static void with(ffcz_solver* h, Fn&& fn) {
+    NEED(h);
+    if (h->dtype == FFCZ_F32) fn(*h->f);
+    else fn(*h->d);
+}
+
+static hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// copy device data out to a host-or-device buffer, two-call size pattern
+static void copy_out(void* out, const void* src, size_t bytes, size_t elems, size_t cap, size_t* len,
+                     hipStream_t s) {
+    if (len) *len = elems;
+    if (!out) return;
+    FFCZ_REQUIRE(cap >= elems, ERR_RANGE, "output capacity too small");
+    if (bytes) {
+        FFCZ_HIP(hipMemcpyAsync(out, src, bytes, hipMemcpyDefault, s));
+        FFCZ_HIP(hipStreamSynchronize(s));
+    }
+}
+
+template <typename S>
+static void need_extracted(const S& s) {
+    FFCZ_REQUIRE(s.extracted(), ERR_STATE, "extract_compact_representation has not run");
+}
+
+template <typename S>
+static void encode(const S* data, size_t n, int level, bool zstd, ffcz_coded** out) {
+    FFCZ_REQUIRE(out != nullptr, ERR_INVALID, "null output");
+    FFCZ_REQUIRE(n > 0 && data != nullptr, ERR_INVALID, "Empty data");  // HuffmanZSTDCoder.cuh:250-253
+    hipPointerAttribute_t attr{};
+    const bool on_device = hipPointerGetAttributes(&attr, data) == hipSuccess && attr.type == hipMemoryTypeDevice;
+    (void)hipGetLastError();
+    std::unique_ptr<ffcz_coded> c(new ffcz_coded());
+    if (on_device) {
+        c->h = huffman_encode_device<S>(data, n, nullptr);
+    } else {
+        S* d = nullptr;
+        FFCZ_HIP(hipMalloc(&d, n * sizeof(S)));
+        try {
+            FFCZ_HIP(hipMemcpy(d, data, n * sizeof(S), hipMemcpyHostToDevice));
+            c->h = huffman_encode_device<S>(d, n, nullptr);
+        } catch (...) {
+            (void)hipFree(d);
+            throw;
+        }
+        (void)hipFree(d);
+    }
+    if (zstd) {
+        c->zdata = zstd_compress(c->h.bitstream.data(), c->h.bitstream.size(), level);
+        c->zstd = true;
+    }
+    *out = c.release();
+}
+
+template <typename S>
+static void decompress(const uint8_t* comp, size_t clen, const uint32_t* sym, const uint32_t* code,
+                       const int32_t* len, size_t nsym, size_t ne, size_t bb, S* out, size_t* out_len, bool zstd) {
+    FFCZ_REQUIRE(out != nullptr && (nsym == 0 || (sym && code && len)), ERR_INVALID, "null argument");
+    HuffmanTable t;
+    t.sym.assign(sym, sym + nsym);
+    t.code.assign(code, code + nsym);
+    t.len.assign(len, len + nsym);
+    std::vector<uint8_t> bits = zstd ? zstd_decompress(comp, clen, bb) : std::vector<uint8_t>(comp, comp + clen);
+    std::vector<S> r = huffman_decode<S>(bits.data(), bits.size(), t, ne);
+    std::memcpy(out, r.data(), r.size() * sizeof(S));
+    if (out_len) *out_len = r.size();
+}
+
+template <typename T>
+static void fft_entry(size_t nx, size_t ny, size_t nz, const void* in, void* out, hipStream_t s, bool fwd) {
+    Fft3d<T> f;
+    f.create(nx, ny, nz);
+    cplx<T>* spec = nullptr;
+    try {
+        FFCZ_HIP(hipMalloc(&spec, f.spec_elems() * sizeof(cplx<T>)));
+        if (fwd) {
+            fft_r2c<T>(f, (const T*)in, spec, s);
+            spec_to_compact<T>(f, spec, (cplx<T>*)out, s);
+        } else {
+            compact_to_spec<T>(f, (const cplx<T>*)in, spec, s);
+            fft_c2r<T>(f, spec, (T*)out, T(1), s);
+        }
+        FFCZ_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+        (void)hipFree(spec);
+        f.destroy();
+        throw;
+    }
+    (void)hipFree(spec);
+    f.destroy();
+}
+
+static void unpack_flags(const uint8_t* dpack, size_t n, uint8_t* out, size_t cap, size_t* len, hipStream_t s) {
+    if (len) *len = n;
+    if (!out) return;
+    FFCZ_REQUIRE(cap >= n, ERR_RANGE, "output capacity too small");
+    std::vector<uint8_t> p((n + 7) / 8);
+    if (!p.empty()) {
+        FFCZ_HIP(hipMemcpyAsync(p.data(), dpack, p.size(), hipMemcpyDeviceToHost, s));
+        FFCZ_HIP(hipStreamSynchronize(s));
+    }
+    std::vector<uint8_t> tmp(n);
+    for (size_t i = 0; i < n; ++i) tmp[i] = (p[i >> 3] >> (i & 7)) & 1;
+    FFCZ_HIP(hipMemcpy(out, tmp.data(), n, hipMemcpyDefault));
+}
+
+extern "C" {
+
+const char* ffcz_last_error(void) { return g_err.c_str(); }
+int ffcz_abi_version(void) { return 1; }
+
+int ffcz_create(int dtype, size_t nx, size_t ny, size_t nz, ffcz_solver** out) {
+    return guard([&] {
+        FFCZ_REQUIRE(out != nullptr, ERR_INVALID, "null output handle");
+        FFCZ_REQUIRE(dtype == FFCZ_F32 || dtype == FFCZ_F64, ERR_INVALID, "dtype must be FFCZ_F32 or FFCZ_F64");
+        std::unique_ptr<ffcz_solver> h(new ffcz_solver());
+        h->dtype = dtype;
+        if (dtype == FFCZ_F32) h->f.reset(new Solver<float>(nx, ny, nz));
+        else h->d.reset(new Solver<double>(nx, ny, nz));
+        *out = h.release();
+    });
+}
+
+int ffcz_destroy(ffcz_solver* h) {
+    return guard([&] { delete h; });
+}
+
+#define INIT_PAIR(name, call)                                                                       \
+    int ffcz_##name##_f32 INIT_ARGS(float) { return guard([&] { sv<float>(h).call; }); }        \
+    int ffcz_##name##_f64 INIT_ARGS(double) { return guard([&] { sv<double>(h).call; }); }
+
+#define INIT_ARGS(T) (ffcz_solver * h, const T* x, T eps, T delta)
+INIT_PAIR(init_all_abs, init_all_abs(x, eps, delta))
+#undef INIT_ARGS
+#define INIT_ARGS(T) (ffcz_solver * h, const T* x, const size_t* idx, const T* delta_k, T eps, size_t K)
+INIT_PAIR(init_direct, init_direct(x, idx, delta_k, eps, K))
+#undef INIT_ARGS
+#define INIT_ARGS(T) (ffcz_solver * h, const T* x, T eps, T delta_ptw)
+INIT_PAIR(init_all_ptw, init_all_ptw(x, eps, delta_ptw))
+#undef INIT_ARGS
+#define INIT_ARGS(T) (ffcz_solver * h, const T* x, const size_t* idx, T delta_ptw, T eps, size_t K)
+INIT_PAIR(init_partial_ptw, init_partial_ptw(x, idx, delta_ptw, eps, K))
+#undef INIT_ARGS
+#define INIT_ARGS(T) (ffcz_solver * h, const T* e0, int max_iterations, T tol)
+INIT_PAIR(solve, solve(MODE_DIRECT, e0, max_iterations, tol))
+INIT_PAIR(solve_all_abs, solve(MODE_ABS, e0, max_iterations, tol))
+INIT_PAIR(solve_all_ptw, solve(MODE_PTW, e0, max_iterations, tol))
+#undef INIT_ARGS
+
+int ffcz_solve_loop(ffcz_solver* h, int mode, const void* e0, int max_iterations, double tol) {
+    return guard([&] {
+        NEED(h);
+        FFCZ_REQUIRE(mode >= 0 && mode <= 2, ERR_INVALID, "unknown solve mode");
+        if (h->dtype == FFCZ_F32) h->f->solve_loop(mode, (const float*)e0, max_iterations, (float)tol);
+        else h->d->solve_loop(mode, (const double*)e0, max_iterations, tol);
+    });
+}
+
+int ffcz_get_status(ffcz_solver* h, int* iterations, double* violation, int* converged) {
+    return guard([&] {
+        with(h, [&](auto& s) {
+            if (iterations) *iterations = s.iterations();
+            if (violation) *violation = s.violation();
+            if (converged) *converged = s.converged() ? 1 : 0;
+        });
+    });
+}
+
+int ffcz_extract(ffcz_solver* h) {
+    return guard([&] { with(h, [&](auto& s) { s.extract(); }); });
+}
+int ffcz_reset_edit_accumulators(ffcz_solver* h) {
+    return guard([&] { with(h, [&](auto& s) { s.reset_edit_accumulators(); }); });
+}
+
+int ffcz_get_results(ffcz_solver* h, void* e_out, size_t cap, size_t* len) {
+    return guard([&] {
+        with(h, [&](auto& s) {
+            using T = std::remove_reference_t<decltype(*s.d_e())>;
+            copy_out(e_out, s.d_e(), s.N() * sizeof(T), s.N(), cap, len, s.stream());
+        });
+    });
+}
+
+
+
+int ffcz_get_freq_flags(ffcz_solver* h, uint8_t* out, size_t cap, size_t* len) {
+    return guard([&] {
+        with(h, [&](auto& s) {
+            need_extracted(s);
+            unpack_flags(s.d_freq_pack(), 2 * s.F(), out, cap, len, s.stream());
+        });
+    });
+}
+int ffcz_get_spatial_flags(ffcz_solver* h, uint8_t* out, size_t cap, size_t* len) {
+    return guard([&] {
+        with(h, [&](auto& s) {
+            need_extracted(s);
+            unpack_flags(s.d_spatial_pack(), s.N(), out, cap, len, s.stream());
+        });
+    });
+}
+int ffcz_get_freq_flags_pack(ffcz_solver* h, uint8_t* out, size_t cap, size_t* len) {
+    return guard([&] {
+        with(h, [&](auto& s) {
+            need_extracted(s);
+            const size_t nb = (2 * s.F() + 7) / 8;
+            copy_out(out, s.d_freq_pack(), nb, nb, cap, len, s.stream());
+        });
+    });
+}
+int ffcz_get_spatial_flags_pack(ffcz_solver* h, uint8_t* out, size_t cap, size_t* len) {
+    return guard([&] {
+        with(h, [&](auto& s) {
+            need_extracted(s);
+            const size_t nb = (s.N() + 7) / 8;
+            copy_out(out, s.d_spatial_pack(), nb, nb, cap, len, s.stream());
+        });
+    });
+}
+int ffcz_get_active_freq_quant(ffcz_solver* h, uint16_t* out, size_t cap, size_t* len) {
+    return guard([&] {
+        with(h, [&](auto& s) {
+            need_extracted(s);
+            const size_t n = s.n_active_freq();
+            copy_out(out, s.d_freq_quant(), n * 2, n, cap, len, s.stream());
+        });
+    });
+}
+int ffcz_get_active_spatial_quant(ffcz_solver* h, uint16_t* out, size_t cap, size_t* len) {
+    return guard([&] {
+        with(h, [&](auto& s) {
+            need_extracted(s);
+            const size_t n = s.n_active_spatial();
+            copy_out(out, s.d_spatial_quant(), n * 2, n, cap, len, s.stream());
+        });
+    });
+}
+int ffcz_get_active_freq_edits(ffcz_solver* h, void* out, size_t cap, size_t* len) {
+    return guard([&] {
+        with(h, [&](auto& s) {
+            need_extracted(s);
+            using T = std::remove_reference_t<decltype(*s.d_e())>;
+            const size_t n = s.n_active_freq();
+            copy_out(out, s.d_freq_compact(), n * sizeof(T), n, cap, len, s.stream());
+        });
+    });
+}
+int ffcz_get_active_spatial_edits(ffcz_solver* h, void* out, size_t cap, size_t* len) {
+    return guard([&] {
+        with(h, [&](auto& s) {
+            need_extracted(s);
+            using T = std::remove_reference_t<decltype(*s.d_e())>;
+            const size_t n = s.n_active_spatial();
+            copy_out(out, s.d_spatial_compact(), n * sizeof(T), n, cap, len, s.stream());
+        });
+    });
+}
+int ffcz_get_freq_edits_full(ffcz_solver* h, void* out, size_t cap, size_t* len) {
+    return guard([&] {
+        with(h, [&](auto& s) {
+            using T = std::remove_reference_t<decltype(*s.d_e())>;
+            copy_out(out, s.d_df(), 2 * s.F() * sizeof(T), 2 * s.F(), cap, len, s.stream());
+        });
+    });
+}
+int ffcz_get_spatial_edits_full(ffcz_solver* h, void* out, size_t cap, size_t* len) {
+    return guard([&] {
+        with(h, [&](auto& s) {
+            using T = std::remove_reference_t<decltype(*s.d_e())>;
+            copy_out(out, s.d_ds(), s.N() * sizeof(T), s.N(), cap, len, s.stream());
+        });
+    });
+}
+int ffcz_get_edit_range(ffcz_solver* h, double* fmin, double* fmax, double* smin, double* smax) {
+    return guard([&] {
+        with(h, [&](auto& s) {
+            if (fmin) *fmin = s.freq_min();
+            if (fmax) *fmax = s.freq_max();
+            if (smin) *smin = s.spatial_min();
+            if (smax) *smax = s.spatial_max();
+        });
+    });
+}
+
+int ffcz_set_margins(ffcz_solver* h, double sm, double fm) {
+    return guard([&] {
+        FFCZ_REQUIRE(sm >= 0 && sm < 1 && fm >= 0 && fm < 1, ERR_INVALID, "margins must lie in [0, 1)");
+        with(h, [&](auto& s) { s.set_margins(sm, fm); });
+    });
+}
+int ffcz_set_quantizer(ffcz_solver* h, int kind, double fstep, double sstep) {
+    return guard([&] { with(h, [&](auto& s) { s.set_quantizer(kind, fstep, sstep); }); });
+}
+int ffcz_certify(ffcz_solver* h, int quantized, ffcz_cert* out) {
+    return guard([&] {
+        FFCZ_REQUIRE(out != nullptr, ERR_INVALID, "null result");
+        with(h, [&](auto& s) {
+            CertResult r = s.certify(quantized != 0);
+            out->spatial_violations = r.spatial_violations;
+            out->freq_violations = r.freq_violations;
+            out->spatial_max_excess = r.spatial_max_excess;
+            out->freq_max_excess = r.freq_max_excess;
+            out->ok = r.ok;
+        });
+    });
+}
+int ffcz_get_stream(ffcz_solver* h, void** stream) {
+    return guard([&] {
+        FFCZ_REQUIRE(stream != nullptr, ERR_INVALID, "null output");
+        with(h, [&](auto& s) { *stream = (void*)s.stream(); });
+    });
+}
+int ffcz_time_iteration(ffcz_solver* h, int reps, float* ms5) {
+    return guard([&] {
+        FFCZ_REQUIRE(ms5 != nullptr, ERR_INVALID, "null output");
+        with(h, [&](auto& s) { s.time_iteration(reps, ms5); });
+    });
+}
+
+// ---- entropy coding --------------------------------------------------------
+
+int ffcz_huffman_zstd_compress_u16(const uint16_t* data, size_t n, int level, ffcz_coded** out) {
+    return guard([&] { encode<uint16_t>(data, n, level, true, out); });
+}
+int ffcz_huffman_zstd_compress_u8(const uint8_t* data, size_t n, int level, ffcz_coded** out) {
+    return guard([&] { encode<uint8_t>(data, n, level, true, out); });
+}
+int ffcz_huffman_encode_u16(const uint16_t* data, size_t n, ffcz_coded** out) {
+    return guard([&] { encode<uint16_t>(data, n, 0, false, out); });
+}
+int ffcz_huffman_encode_u8(const uint8_t* data, size_t n, ffcz_coded** out) {
+    return guard([&] { encode<uint8_t>(data, n, 0, false, out); });
+}
+int ffcz_coded_info(const ffcz_coded* c, size_t* cb, size_t* bb, size_t* ne, size_t* ns, unsigned long long* nbits) {
+    return guard([&] {
+        FFCZ_REQUIRE(c != nullptr, ERR_INVALID, "null coded handle");
+        if (cb) *cb = c->zstd ? c->zdata.size() : c->h.bitstream.size();
+        if (bb) *bb = c->h.bitstream.size();
+        if (ne) *ne = c->h.nelem;
+        if (ns) *ns = c->h.table.sym.size();
+        if (nbits) *nbits = c->h.nbits;
+    });
+}
+int ffcz_coded_data(const ffcz_coded* c, uint8_t* out, size_t cap) {
+    return guard([&] {
+        FFCZ_REQUIRE(c != nullptr && out != nullptr, ERR_INVALID, "null argument");
+        const auto& v = c->zstd ? c->zdata : c->h.bitstream;
+        FFCZ_REQUIRE(cap >= v.size(), ERR_RANGE, "output capacity too small");
+        if (!v.empty()) std::memcpy(out, v.data(), v.size());
+    });
+}
+int ffcz_coded_bitstream(const ffcz_coded* c, uint8_t* out, size_t cap) {
+    return guard([&] {
+        FFCZ_REQUIRE(c != nullptr && out != nullptr, ERR_INVALID, "null argument");
+        const auto& v = c->h.bitstream;
+        FFCZ_REQUIRE(cap >= v.size(), ERR_RANGE, "output capacity too small");
+        if (!v.empty()) std::memcpy(out, v.data(), v.size());
+    });
+}
+int ffcz_coded_table(const ffcz_coded* c, uint32_t* symbols, uint32_t* codes, int32_t* lengths, size_t cap) {
+    return guard([&] {
+        FFCZ_REQUIRE(c != nullptr, ERR_INVALID, "null coded handle");
+        const auto& t = c->h.table;
+        FFCZ_REQUIRE(cap >= t.sym.size(), ERR_RANGE, "output capacity too small");
+        for (size_t i = 0; i < t.sym.size(); ++i) {
+            if (symbols) symbols[i] = t.sym[i];
+            if (codes) codes[i] = t.code[i];
+            if (lengths) lengths[i] = t.len[i];
+        }
+    });
+}
+void ffcz_coded_free(ffcz_coded* c) { delete c; }
+
+int ffcz_huffman_zstd_decompress_u16(const uint8_t* comp, size_t clen, const uint32_t* sym, const uint32_t* code,
+                                     const int32_t* len, size_t nsym, size_t ne, size_t bb, uint16_t* out,
+                                     size_t* out_len) {
+    return guard([&] { decompress<uint16_t>(comp, clen, sym, code, len, nsym, ne, bb, out, out_len, true); });
+}
+int ffcz_huffman_zstd_decompress_u8(const uint8_t* comp, size_t clen, const uint32_t* sym, const uint32_t* code,
+                                    const int32_t* len, size_t nsym, size_t ne, size_t bb, uint8_t* out,
+                                    size_t* out_len) {
+    return guard([&] { decompress<uint8_t>(comp, clen, sym, code, len, nsym, ne, bb, out, out_len, true); });
+}
+
+// ---- per-kernel entries ----------------------------------------------------
+
+int ffcz_fft_r2c(int dtype, size_t nx, size_t ny, size_t nz, const void* in, void* out, void* stream) {
+    return guard([&] {
+        if (dtype == FFCZ_F32) fft_entry<float>(nx, ny, nz, in, out, as_stream(stream), true);
+        else if (dtype == FFCZ_F64) fft_entry<double>(nx, ny, nz, in, out, as_stream(stream), true);
+        else throw Error(ERR_INVALID, "bad dtype");
+    });
+}
+int ffcz_fft_c2r(int dtype, size_t nx, size_t ny, size_t nz, const void* in, void* out, void* stream) {
+    return guard([&] {
+        if (dtype == FFCZ_F32) fft_entry<float>(nx, ny, nz, in, out, as_stream(stream), false);
+        else if (dtype == FFCZ_F64) fft_entry<double>(nx, ny, nz, in, out, as_stream(stream), false);
+        else throw Error(ERR_INVALID, "bad dtype");
+    });
+}
+int ffcz_k_project_spatial(int dtype, void* e, double eps, void* ds, size_t n, void* stream) {
+    return guard([&] {
+        if (dtype == FFCZ_F32) launch_project_spatial<float>((float*)e, (float)eps, (float*)ds, n, as_stream(stream));
+        else launch_project_spatial<double>((double*)e, eps, (double*)ds, n, as_stream(stream));
+        FFCZ_HIP(hipStreamSynchronize(as_stream(stream)));
+    });
+}
+int ffcz_k_project_freq_abs(int dtype, void* spec, double delta, void* df, size_t F, void* stream) {
+    return guard([&] {
+        if (dtype == FFCZ_F32)
+            launch_project_freq_abs<float>((cplx<float>*)spec, (float)delta, (float*)df, F, as_stream(stream));
+        else launch_project_freq_abs<double>((cplx<double>*)spec, delta, (double*)df, F, as_stream(stream));
+        FFCZ_HIP(hipStreamSynchronize(as_stream(stream)));
+    });
+}
+int ffcz_k_convergence(int dtype, const void* e, double eps, size_t n, double* out) {
+    return guard([&] {
+        FFCZ_REQUIRE(out != nullptr, ERR_INVALID, "null output");
+        if (dtype == FFCZ_F32) *out = convergence_metric<float>((const float*)e, (float)eps, n, nullptr);
+        else *out = convergence_metric<double>((const double*)e, eps, n, nullptr);
+    });
+}
+int ffcz_k_quantize_dequantize(int dtype, const void* v, void* out, double vmin, double vmax, size_t n,
+                               double threshold, void* stream) {
+    return guard([&] {
+        if (dtype == FFCZ_F32)
+            launch_quant_dequant<float>((const float*)v, (float*)out, (float)vmin, (float)vmax, n, (float)threshold,
+                                        as_stream(stream));
+        else launch_quant_dequant<double>((const double*)v, (double*)out, vmin, vmax, n, threshold, as_stream(stream));
+        FFCZ_HIP(hipStreamSynchronize(as_stream(stream)));
+    });
+}
+int ffcz_k_extract(int dtype, const void* v, size_t n, int kind, double step, uint8_t* pack, void* compact,
+                   uint16_t* quant, size_t* count, double* vmin, double* vmax, void* stream) {
+    return guard([&] {
+        FFCZ_REQUIRE(kind == Q_REF16 || kind == Q_STEP, ERR_INVALID, "unknown quantizer kind");
+        QuantDesc q{};
+        q.kind = kind;
+        q.step = step;
+        q.threshold = 1e-7;
+        void* scratch = nullptr;
+        FFCZ_HIP(hipMalloc(&scratch, extract_scratch_bytes(n)));
+        ExtractResult r;
+        try {
+            if (dtype == FFCZ_F32)
+                r = extract_edits<float>((const float*)v, n, q, pack, (float*)compact, quant, scratch, as_stream(stream));
+            else
+                r = extract_edits<double>((const double*)v, n, q, pack, (double*)compact, quant, scratch,
+                                          as_stream(stream));
+        } catch (...) {
+            (void)hipFree(scratch);
+            throw;
+        }
+        (void)hipFree(scratch);
+        if (count) *count = r.count;
+        if (vmin) *vmin = r.vmin;
+        if (vmax) *vmax = r.vmax;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,35 @@
+// Edit extraction: flags, min/max, stable compaction, quantisation and LSB-first
+// flag packing (reference: extract_compact_representation,
+// projection_algorithm_impl.cuh:371-456, and its helpers
+// projection_algorithm.cu:363-550).
+#pragma once
+
+#include "fft3d.hpp"
+
+namespace ffcz {
+
+constexpr int EXTRACT_TILE = 4096;     // elements per workgroup tile (16 per thread)
+
+struct ExtractResult {
+    size_t count = 0;
+    double vmin = 0, vmax = 0;
+};
+
+// Device scratch needed by extract_edits for n elements.
+size_t extract_scratch_bytes(size_t n);
+
+// v[n] -> pack[ceil(n/8) rounded up to 8 bytes], compact[count], quant[count].
+//   q.kind == Q_REF16: flag = |v| > threshold, min/max over the full array,
+//     q = n>0.5 ? ceil(n*65535) : floor(n*65535) with n = (v-min)/(max-min).
+//   q.kind == Q_STEP:  r = llround(v/step); flag = r != 0; quant = zigzag(r)
+//     (ERR_RANGE if a symbol does not fit 16 bits).
+// Returns count (synchronises the stream once).
+template <typename T>
+ExtractResult extract_edits(const T* v, size_t n, QuantDesc q, uint8_t* pack, T* compact, uint16_t* quant,
+                            void* scratch, hipStream_t s);
+
+// Dense dequantised copy (projection_algorithm.cu:615-644 for Q_REF16).
+template <typename T>
+void dequantize_dense(const T* v, size_t n, QuantDesc q, T* out, hipStream_t s);
+
+}  // namespace ffcz

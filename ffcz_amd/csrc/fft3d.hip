@@ -1,0 +1,841 @@
+// 3D R2C/C2R passes, the fused projection iteration and the FP64
+// certification transform.  See fft3d.hpp for the layouts.
+#include <cmath>
+#include <vector>
+
+#include "fft3d.hpp"
+#include "quant.hpp"
+
+namespace ffcz {
+
+// ===========================================================================
+// Host planning
+// ===========================================================================
+static FftPlan make_plan(int L) {
+    FFCZ_REQUIRE(L >= 1, ERR_INVALID, "FFT length must be >= 1");
+    FftPlan p{};
+    p.L = L;
+    p.nst = 0;
+    int rem = L, Ns = 1;
+    auto push = [&](int r) {
+        FFCZ_REQUIRE(p.nst < FFT_MAX_STAGES, ERR_INVALID, "FFT length has too many factors");
+        p.radix[p.nst] = r;
+        p.Ns[p.nst] = Ns;
+        p.twstride[p.nst] = L / (Ns * r);
+        p.ns_div[p.nst] = FastDiv((uint32_t)Ns);
+        p.lr_div[p.nst] = FastDiv((uint32_t)(L / r));
+        ++p.nst;
+        Ns *= r;
+    };
+    while (rem % 8 == 0) { push(8); rem /= 8; }
+    while (rem % 4 == 0) { push(4); rem /= 4; }
+    while (rem % 2 == 0) { push(2); rem /= 2; }
+    while (rem % 5 == 0) { push(5); rem /= 5; }
+    while (rem % 3 == 0) { push(3); rem /= 3; }
+    FFCZ_REQUIRE(rem == 1, ERR_INVALID,
+                 "FFT axis length " + std::to_string(L) + " is not 2/3/5-smooth");
+    return p;
+}
+
+template <typename R>
+static cplx<R>* upload_table(int L, int count) {
+    std::vector<cplx<R>> h((size_t)count);
+    const double two_pi = 6.283185307179586476925286766559;
+    for (int m = 0; m < count; ++m) {
+        double a = -two_pi * (double)m / (double)L;
+        h[m].x = (R)std::cos(a);
+        h[m].y = (R)std::sin(a);
+    }
+    cplx<R>* d = nullptr;
+    FFCZ_HIP(hipMalloc(&d, sizeof(cplx<R>) * (size_t)count));
+    FFCZ_HIP(hipMemcpy(d, h.data(), sizeof(cplx<R>) * (size_t)count, hipMemcpyHostToDevice));
+    return d;
+}
+
+constexpr size_t LDS_BUDGET = 48 * 1024;
+
+template <typename R>
+void Fft3d<R>::create(size_t nx_, size_t ny_, size_t nz_) {
+    FFCZ_REQUIRE(nx_ >= 1 && ny_ >= 1 && nz_ >= 1, ERR_INVALID, "field dimensions must be >= 1");
+    FFCZ_REQUIRE(nx_ <= 65536 && ny_ <= 65536 && nz_ <= 131072, ERR_INVALID, "field dimension too large");
+    nx = nx_; ny = ny_; nz = nz_;
+    N = nx * ny * nz;
+    rows = nx * ny;
+    Nh = nz / 2 + 1;
+    F = rows * Nh;
+    FFCZ_REQUIRE(2 * F < (1ull << 40) && N < (1ull << 40), ERR_INVALID, "field too large");
+    half = (nz % 2 == 0);
+    Lz = half ? (int)(nz / 2) : (int)nz;
+    Mp = (Nh + 15) / 16 * 16;
+    pz = make_plan(Lz);
+    py = make_plan((int)ny);
+    px = make_plan((int)nx);
+    const size_t cs = sizeof(cplx<R>);
+    // z tile: P pencils, element-major with pencil stride P+1 (odd => no LDS bank aliasing)
+    const size_t zrows = half ? (size_t)Lz + 1 : nz;
+    const size_t ztab = ((size_t)Lz + nz / 2 + 1) * cs;
+    Pz = 0;
+    for (int P = 256; P >= 1; P >>= 1) {
+        if ((size_t)Lz * P <= (size_t)FFT_MAX_TILE && zrows * (P + 1) * cs + ztab <= LDS_BUDGET) { Pz = P; break; }
+    }
+    FFCZ_REQUIRE(Pz >= 1, ERR_INVALID, "z axis too long for one LDS tile");
+    auto pickW = [&](size_t L) {
+        for (int W = 16; W >= 1; W >>= 1)
+            if (L * W <= (size_t)FFT_MAX_TILE && (L * W + L) * cs <= LDS_BUDGET) return W;
+        return 0;
+    };
+    Wy = pickW(ny);
+    Wx = pickW(nx);
+    FFCZ_REQUIRE(Wy >= 1 && Wx >= 1, ERR_INVALID, "x/y axis too long for one LDS tile");
+    d_tz = upload_table<R>(Lz, Lz);
+    d_tz2 = upload_table<R>((int)nz, (int)(nz / 2 + 1));
+    d_ty = upload_table<R>((int)ny, (int)ny);
+    d_tx = upload_table<R>((int)nx, (int)nx);
+}
+
+template <typename R>
+void Fft3d<R>::destroy() {
+    if (d_tz) (void)hipFree(d_tz);
+    if (d_tz2) (void)hipFree(d_tz2);
+    if (d_ty) (void)hipFree(d_ty);
+    if (d_tx) (void)hipFree(d_tx);
+    d_tz = d_tz2 = d_ty = d_tx = nullptr;
+}
+
+// ===========================================================================
+// Device helpers
+// ===========================================================================
+template <typename R>
+__device__ __forceinline__ void clamp_acc(R& v, R d, R* acc) {
+    // projection_algorithm.cu:161-172 / :351-360: the edit is formed first
+    // and then accumulated (a single writer per address).
+    if (v > d) {
+        *acc += d - v;
+        v = d;
+    } else if (v < -d) {
+        *acc += -d - v;
+        v = -d;
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ void block_max_atomic(R v, unsigned long long* target) {
+    __shared__ R red[FFT_THREADS / 64];
+    v = wave_max(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        R m = red[0];
+        for (int i = 1; i < FFT_THREADS / 64; ++i) m = red[i] > m ? red[i] : m;
+        if (m > R(0)) atomicMax(target, ordbits(m));
+    }
+}
+
+__device__ __forceinline__ void block_count_atomic(unsigned long long c, unsigned long long* target) {
+    c = wave_sum(c);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(target, c);
+}
+
+// ===========================================================================
+// z pass kernels (contiguous rows)
+// ===========================================================================
+enum ZMode : int { Z_PLAIN = 0, Z_ITER = 1, Z_CERT = 2 };
+
+template <typename R, typename T>
+struct ZArgs {
+    int nz, Lz, M, half, P, sn;
+    FastDiv dP, dM, dNh, dNp, dNz, dFill;  // P, M, nz/2+1, M/2+1, nz, nz-(nz/2+1)
+    size_t rows, Mp;
+    FftPlan pl;
+    const cplx<R>* tz;
+    const cplx<R>* tz2;
+    const R* in;          // Z_PLAIN forward input
+    R* out;               // Z_PLAIN inverse output
+    cplx<R>* spec;
+    R scale;              // inverse scale
+    // Z_ITER
+    R* e;
+    R* ds;
+    R eps;
+    DevState* st;
+    // Z_CERT
+    const T* e0;
+    const T* cds;
+    const double* dfi;
+    QuantDesc qs;
+    double ceps;
+    CertStats* stats;
+};
+
+template <typename R, typename T, int MODE>
+__global__ __launch_bounds__(FFT_THREADS) void k_zfwd(ZArgs<R, T> a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    if (MODE == Z_ITER && a.st->done) return;
+    cplx<R>* tw = reinterpret_cast<cplx<R>*>(smem);
+    cplx<R>* tw2 = tw + a.Lz;
+    cplx<R>* buf = tw2 + (a.nz / 2 + 1);
+    const int t = threadIdx.x;
+    for (int i = t; i < a.Lz; i += FFT_THREADS) tw[i] = a.tz[i];
+    for (int i = t; i <= a.nz / 2; i += FFT_THREADS) tw2[i] = a.tz2[i];
+    const size_t row0 = (size_t)blockIdx.x * a.P;
+    const int P = a.P, sn = a.sn;
+    const bool clamp = (MODE == Z_ITER) && a.st->iter > 0;
+    unsigned long long nviol = 0;
+    double vmax = 0.0;
+    auto load = [&](size_t gi) -> R {
+        if constexpr (MODE == Z_PLAIN) {
+            return a.in[gi];
+        } else if constexpr (MODE == Z_ITER) {
+            R v = a.e[gi];
+            if (clamp) clamp_acc(v, a.eps, &a.ds[gi]);
+            return v;
+        } else {
+            double d = (double)a.e0[gi];
+            if (a.dfi) d += a.dfi[gi];
+            d += dequant(a.cds[gi], a.qs);
+            const double ex = fabs(d) - a.ceps;
+            if (ex > 0.0) {
+                ++nviol;
+                vmax = ex > vmax ? ex : vmax;
+            }
+            return (R)d;
+        }
+    };
+    if (a.half) {
+        const int M = a.M;
+        const FastDiv& md = a.dM;
+        for (int q = t; q < P * M; q += FFT_THREADS) {
+            const int p = (int)fdiv((uint32_t)q, md);
+            const int n = q - p * M;
+            const size_t row = row0 + p;
+            R v0 = 0, v1 = 0;
+            if (row < a.rows) {
+                const size_t gi = row * (size_t)a.nz + 2 * (size_t)n;
+                v0 = load(gi);
+                v1 = load(gi + 1);
+            }
+            buf[n * sn + p] = mk(v0, v1);
+        }
+    } else {
+        const FastDiv& zd = a.dNz;
+        for (int q = t; q < P * a.nz; q += FFT_THREADS) {
+            const int p = (int)fdiv((uint32_t)q, zd);
+            const int n = q - p * a.nz;
+            const size_t row = row0 + p;
+            R v0 = 0;
+            if (row < a.rows) v0 = load(row * (size_t)a.nz + n);
+            buf[n * sn + p] = mk(v0, R(0));
+        }
+    }
+    if constexpr (MODE == Z_CERT) {
+        block_count_atomic(nviol, &a.stats->spatial_viol);
+        block_max_atomic(vmax, &a.stats->spatial_max_bits);
+    }
+    __syncthreads();
+    tile_fft(buf, tw, a.pl, P, a.dP, 1, sn, true, -1);
+    if (a.half) {
+        const int M = a.M;
+        const int nh = M + 1;
+        const FastDiv& hd = a.dNh;
+        for (int q = t; q < P * nh; q += FFT_THREADS) {
+            const int p = (int)fdiv((uint32_t)q, hd);
+            const int k = q - p * nh;
+            const size_t row = row0 + p;
+            if (row >= a.rows) continue;
+            const int ik = (k == M) ? 0 : k;
+            const int im = (k == 0) ? 0 : M - k;
+            const cplx<R> zk = buf[ik * sn + p];
+            const cplx<R> zm = buf[im * sn + p];
+            // X[k] = A[k] + W_nz^k B[k], A = (Zk + conj Zm)/2, B = (Zk - conj Zm)/(2i)
+            const cplx<R> A = mk(R(0.5) * (zk.x + zm.x), R(0.5) * (zk.y - zm.y));
+            const cplx<R> B = mk(R(0.5) * (zk.y + zm.y), R(0.5) * (zm.x - zk.x));
+            a.spec[row * a.Mp + k] = A + cmul(tw2[k], B);
+        }
+    } else {
+        const int nh = a.nz / 2 + 1;
+        const FastDiv& hd = a.dNh;
+        for (int q = t; q < P * nh; q += FFT_THREADS) {
+            const int p = (int)fdiv((uint32_t)q, hd);
+            const int k = q - p * nh;
+            const size_t row = row0 + p;
+            if (row >= a.rows) continue;
+            a.spec[row * a.Mp + k] = buf[k * sn + p];
+        }
+    }
+}
+
+template <typename R, typename T, int MODE>
+__global__ __launch_bounds__(FFT_THREADS) void k_zinv(ZArgs<R, T> a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    if (MODE == Z_ITER && a.st->done) return;
+    cplx<R>* tw = reinterpret_cast<cplx<R>*>(smem);
+    cplx<R>* tw2 = tw + a.Lz;
+    cplx<R>* buf = tw2 + (a.nz / 2 + 1);
+    const int t = threadIdx.x;
+    for (int i = t; i < a.Lz; i += FFT_THREADS) tw[i] = a.tz[i];
+    for (int i = t; i <= a.nz / 2; i += FFT_THREADS) tw2[i] = a.tz2[i];
+    const size_t row0 = (size_t)blockIdx.x * a.P;
+    const int P = a.P, sn = a.sn;
+    const int nh = a.nz / 2 + 1;
+    {
+        const FastDiv& hd = a.dNh;
+        for (int q = t; q < P * nh; q += FFT_THREADS) {
+            const int p = (int)fdiv((uint32_t)q, hd);
+            const int k = q - p * nh;
+            const size_t row = row0 + p;
+            buf[k * sn + p] = row < a.rows ? a.spec[row * a.Mp + k] : mk(R(0), R(0));
+        }
+    }
+    __syncthreads();
+    if (a.half) {
+        // Z[k] = (X[k] + conj X[M-k]) + i W_nz^{-k} (X[k] - conj X[M-k]); the
+        // inverse length-M transform of Z gives nz*(x[2n] + i x[2n+1]).
+        const int M = a.M;
+        const int np = M / 2 + 1;
+        const FastDiv& pd = a.dNp;
+        for (int q = t; q < P * np; q += FFT_THREADS) {
+            const int p = (int)fdiv((uint32_t)q, pd);
+            const int k = q - p * np;
+            const cplx<R> xk = buf[k * sn + p];
+            const cplx<R> xm = buf[(M - k) * sn + p];
+            {
+                const cplx<R> s = xk + conjg(xm);
+                const cplx<R> d = cmul(xk - conjg(xm), conjg(tw2[k]));
+                buf[k * sn + p] = mk(s.x - d.y, s.y + d.x);
+            }
+            if (k != 0 && M - k != k) {
+                const cplx<R> s = xm + conjg(xk);
+                const cplx<R> d = cmul(xm - conjg(xk), conjg(tw2[M - k]));
+                buf[(M - k) * sn + p] = mk(s.x - d.y, s.y + d.x);
+            }
+        }
+    } else {
+        const int nz = a.nz;
+        const int nfill = nz - nh;
+        if (nfill > 0) {
+            const FastDiv& fd = a.dFill;
+            for (int q = t; q < P * nfill; q += FFT_THREADS) {
+                const int p = (int)fdiv((uint32_t)q, fd);
+                const int k = nh + (q - p * nfill);
+                buf[k * sn + p] = conjg(buf[(nz - k) * sn + p]);
+            }
+        }
+    }
+    __syncthreads();
+    tile_fft(buf, tw, a.pl, P, a.dP, 1, sn, true, +1);
+    R vmax = R(0);
+    if (a.half) {
+        const int M = a.M;
+        const FastDiv& md = a.dM;
+        for (int q = t; q < P * M; q += FFT_THREADS) {
+            const int p = (int)fdiv((uint32_t)q, md);
+            const int n = q - p * M;
+            const size_t row = row0 + p;
+            if (row >= a.rows) continue;
+            const cplx<R> z = buf[n * sn + p];
+            const size_t gi = row * (size_t)a.nz + 2 * (size_t)n;
+            if constexpr (MODE == Z_ITER) {
+                const R v0 = z.x * a.scale, v1 = z.y * a.scale;
+                a.e[gi] = v0;
+                a.e[gi + 1] = v1;
+                const R a0 = fabs(v0), a1 = fabs(v1);
+                if (a0 > a.eps) vmax = fmax(vmax, a0 - a.eps);
+                if (a1 > a.eps) vmax = fmax(vmax, a1 - a.eps);
+            } else {
+                a.out[gi] = z.x * a.scale;
+                a.out[gi + 1] = z.y * a.scale;
+            }
+        }
+    } else {
+        const FastDiv& zd = a.dNz;
+        for (int q = t; q < P * a.nz; q += FFT_THREADS) {
+            const int p = (int)fdiv((uint32_t)q, zd);
+            const int n = q - p * a.nz;
+            const size_t row = row0 + p;
+            if (row >= a.rows) continue;
+            const R v0 = buf[n * sn + p].x * a.scale;
+            const size_t gi = row * (size_t)a.nz + n;
+            if constexpr (MODE == Z_ITER) {
+                a.e[gi] = v0;
+                const R a0 = fabs(v0);
+                if (a0 > a.eps) vmax = fmax(vmax, a0 - a.eps);
+            } else {
+                a.out[gi] = v0;
+            }
+        }
+    }
+    if constexpr (MODE == Z_ITER) {
+        // convergence metric (projection_algorithm.cu:586-612), reduced on the
+        // device; the last workgroup finalises the iteration
+        // (projection_algorithm_impl.cuh:841-848).
+        block_max_atomic(vmax, &a.st->viol_bits);
+        __shared__ bool last;
+        if (t == 0) {
+            __threadfence();
+            const unsigned int ticket = atomicAdd(&a.st->blocks_done, 1u);
+            last = (ticket == gridDim.x - 1);
+        }
+        __syncthreads();
+        if (last && t == 0) {
+            __threadfence();
+            DevState* st = a.st;
+            const unsigned long long b = atomicAdd(&st->viol_bits, 0ull);
+            const R v = from_ordbits<R>(b);
+            st->iter += 1;
+            st->last_viol = (double)v;
+            bool conv;
+            if constexpr (sizeof(R) == 4) conv = v < st->tol_f;
+            else conv = v < (R)st->tol;
+            if (conv) {
+                st->converged = 1;
+                st->done = 1;
+            } else if (st->iter >= st->max_iter) {
+                st->done = 1;
+            }
+            atomicExch(&st->viol_bits, 0ull);
+            atomicExch(&st->blocks_done, 0u);
+            __threadfence();
+        }
+    }
+}
+
+// ===========================================================================
+// strided (y / x) pass kernels
+// ===========================================================================
+enum SMode : int { S_PLAIN = 0, S_CLAMP = 1, S_CERT_CHECK = 2, S_LOAD_DF = 3 };
+
+template <typename R, typename T>
+struct SArgs {
+    cplx<R>* data;
+    FftPlan pl;
+    const cplx<R>* tw;
+    int L, W, wshift, ncols, nct;
+    FastDiv dW, dNct;
+    size_t ob, es;
+    int sign;
+    DevState* st;
+    // clamp / check / load: bin index = n*cs_n + o*cs_o + col (reference layout)
+    size_t F, cs_n, cs_o;
+    int fmode;
+    R delta;
+    const R* delta_dense;
+    R dscale;
+    R* df;
+    const cplx<R>* orig;
+    R delta_ptw;
+    // certification
+    double cdelta;
+    const T* cdelta_dense;
+    CertStats* stats;
+    const T* dfsrc;
+    QuantDesc qf;
+};
+
+template <typename R>
+__device__ __forceinline__ void ptw_clamp(cplx<R>& v, size_t idx, size_t F, const cplx<R>* orig, R dptw, R* df) {
+    // projection_algorithm.cu:199-258
+    if (idx == 0) {
+        const R lo = orig[0].x * (R(1) / sqrt(R(1) + dptw) - R(1));
+        const R hi = orig[0].x * (R(1) / sqrt(R(1) - dptw) - R(1));
+        if (v.x > hi || v.x < lo) {
+            df[0] += -v.x;
+            v.x = R(0);
+        }
+        return;
+    }
+    const cplx<R> o = orig[idx];
+    const R omag = o.x * o.x + o.y * o.y;
+    const R cr = o.x + v.x, ci = o.y + v.y;
+    const R cmag = cr * cr + ci * ci;
+    const R ratio = cmag / omag;
+    if (ratio <= sqrt(R(1) + dptw) && ratio >= sqrt(R(1) - dptw)) return;
+    const R d = fmin(R(1) - sqrt(R(1) - dptw), sqrt(R(1) + dptw) - R(1)) / (R)sqrtf(2.0f);
+    clamp_acc(v.x, d, &df[idx]);
+    clamp_acc(v.y, d, &df[idx + F]);
+}
+
+template <typename R, typename T, int MODE>
+__global__ __launch_bounds__(FFT_THREADS) void k_strided(SArgs<R, T> a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    if (a.st && a.st->done) return;
+    cplx<R>* tw = reinterpret_cast<cplx<R>*>(smem);
+    cplx<R>* buf = tw + a.L;
+    const int t = threadIdx.x;
+    for (int i = t; i < a.L; i += FFT_THREADS) tw[i] = a.tw[i];
+    const uint32_t ob_ = fdiv(blockIdx.x, a.dNct);
+    const int ct = (int)(blockIdx.x - ob_ * (uint32_t)a.nct);
+    const size_t o = ob_;
+    const int c0 = ct * a.W;
+    const int W = a.W, L = a.L, ws = a.wshift;
+    const size_t base = o * a.ob + c0;
+    for (int q = t; q < L * W; q += FFT_THREADS) {
+        const int n = q >> ws, c = q & (W - 1);
+        cplx<R> v = mk(R(0), R(0));
+        if (c0 + c < a.ncols) {
+            if constexpr (MODE == S_LOAD_DF) {
+                const size_t idx = (size_t)n * a.cs_n + o * a.cs_o + c0 + c;
+                v = mk((R)dequant(a.dfsrc[idx], a.qf), (R)dequant(a.dfsrc[idx + a.F], a.qf));
+            } else {
+                v = a.data[base + (size_t)n * a.es + c];
+            }
+        }
+        buf[q] = v;
+    }
+    __syncthreads();
+    const int sign = (MODE == S_CLAMP || MODE == S_CERT_CHECK) ? -1 : a.sign;
+    tile_fft(buf, tw, a.pl, W, a.dW, 1, W, true, sign);
+    if constexpr (MODE == S_CLAMP) {
+        for (int q = t; q < L * W; q += FFT_THREADS) {
+            const int n = q >> ws, c = q & (W - 1);
+            if (c0 + c >= a.ncols) continue;
+            const size_t idx = (size_t)n * a.cs_n + o * a.cs_o + c0 + c;
+            cplx<R> v = buf[q];
+            if (a.fmode == FM_ABS) {
+                clamp_acc(v.x, a.delta, &a.df[idx]);
+                clamp_acc(v.y, a.delta, &a.df[idx + a.F]);
+            } else if (a.fmode == FM_DENSE) {
+                const R d = a.delta_dense[idx] * a.dscale;
+                clamp_acc(v.x, d, &a.df[idx]);
+                clamp_acc(v.y, d, &a.df[idx + a.F]);
+            } else if (a.fmode == FM_PTW) {
+                ptw_clamp(v, idx, a.F, a.orig, a.delta_ptw, a.df);
+            }
+            buf[q] = v;
+        }
+        __syncthreads();
+        tile_fft(buf, tw, a.pl, W, a.dW, 1, W, true, +1);
+    }
+    if constexpr (MODE == S_CERT_CHECK) {
+        unsigned long long nv = 0;
+        double vmax = 0.0;
+        for (int q = t; q < L * W; q += FFT_THREADS) {
+            const int n = q >> ws, c = q & (W - 1);
+            if (c0 + c >= a.ncols) continue;
+            const size_t idx = (size_t)n * a.cs_n + o * a.cs_o + c0 + c;
+            const cplx<R> v = buf[q];
+            const double d = a.cdelta_dense ? (double)a.cdelta_dense[idx] : a.cdelta;
+            const double m = fmax(fabs((double)v.x), fabs((double)v.y));
+            const double ex = m - d;
+            if (ex > 0.0) {
+                ++nv;
+                vmax = ex > vmax ? ex : vmax;
+            }
+        }
+        block_count_atomic(nv, &a.stats->freq_viol);
+        block_max_atomic(vmax, &a.stats->freq_max_bits);
+        return;
+    }
+    for (int q = t; q < L * W; q += FFT_THREADS) {
+        const int n = q >> ws, c = q & (W - 1);
+        if (c0 + c < a.ncols) a.data[base + (size_t)n * a.es + c] = buf[q];
+    }
+}
+
+// ===========================================================================
+// small kernels
+// ===========================================================================
+__global__ void k_reset_state(DevState* st, unsigned int max_iter, double tol) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        st->done = max_iter == 0 ? 1u : 0u;
+        st->converged = 0;
+        st->iter = 0;
+        st->max_iter = max_iter;
+        st->blocks_done = 0;
+        st->viol_bits = 0;
+        st->tol = tol;
+        st->tol_f = (float)tol;
+        st->last_viol = 0.0;
+    }
+}
+
+template <typename R>
+__global__ void k_epilogue_clamp(R* e, R* ds, R eps, size_t n, const DevState* st) {
+    if (!(st->done && !st->converged && st->iter > 0)) return;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        R v = e[i];
+        if (v > eps || v < -eps) {
+            clamp_acc(v, eps, &ds[i]);
+            e[i] = v;
+        }
+    }
+}
+
+template <typename R>
+__global__ void k_spec_to_compact(const cplx<R>* spec, cplx<R>* comp, size_t rows, size_t Mp, size_t Nh) {
+    const size_t total = rows * Nh;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t r = i / Nh, k = i - r * Nh;
+        comp[i] = spec[r * Mp + k];
+    }
+}
+template <typename R>
+__global__ void k_compact_to_spec(const cplx<R>* comp, cplx<R>* spec, size_t rows, size_t Mp, size_t Nh) {
+    const size_t total = rows * Nh;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t r = i / Nh, k = i - r * Nh;
+        spec[r * Mp + k] = comp[i];
+    }
+}
+
+__global__ void k_reset_stats(CertStats* s) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        s->spatial_viol = 0;
+        s->freq_viol = 0;
+        s->spatial_max_bits = 0;
+        s->freq_max_bits = 0;
+    }
+}
+
+// ===========================================================================
+// Host launchers
+// ===========================================================================
+static inline unsigned grid_for(size_t n, unsigned block = 256) {
+    size_t g = (n + block - 1) / block;
+    if (g > (1u << 20)) g = 1u << 20;
+    if (g == 0) g = 1;
+    return (unsigned)g;
+}
+
+template <typename R, typename T>
+static ZArgs<R, T> zargs(const Fft3d<R>& f) {
+    ZArgs<R, T> a{};
+    a.nz = (int)f.nz;
+    a.Lz = f.Lz;
+    a.M = (int)(f.nz / 2);
+    a.half = f.half ? 1 : 0;
+    a.P = f.Pz;
+    a.sn = f.Pz + 1;
+    a.rows = f.rows;
+    a.Mp = f.Mp;
+    a.pl = f.pz;
+    a.tz = f.d_tz;
+    a.tz2 = f.d_tz2;
+    const uint32_t M = (uint32_t)(f.nz / 2), nh = M + 1;
+    a.dP = FastDiv((uint32_t)f.Pz);
+    a.dM = FastDiv(M ? M : 1);
+    a.dNh = FastDiv(nh);
+    a.dNp = FastDiv(M / 2 + 1);
+    a.dNz = FastDiv((uint32_t)f.nz);
+    a.dFill = FastDiv((uint32_t)f.nz > nh ? (uint32_t)f.nz - nh : 1u);
+    return a;
+}
+template <typename R>
+static size_t zlds(const Fft3d<R>& f) {
+    const size_t zrows = f.half ? (size_t)f.Lz + 1 : f.nz;
+    return ((size_t)f.Lz + f.nz / 2 + 1 + zrows * (f.Pz + 1)) * sizeof(cplx<R>);
+}
+template <typename R>
+static unsigned zgrid(const Fft3d<R>& f) { return (unsigned)ceil_div(f.rows, (size_t)f.Pz); }
+
+static int ilog2(int w) {
+    int s = 0;
+    while ((1 << s) < w) ++s;
+    return s;
+}
+
+template <typename R, typename T>
+static SArgs<R, T> sargs(const Fft3d<R>& f, cplx<R>* data, bool xaxis) {
+    SArgs<R, T> a{};
+    a.data = data;
+    if (xaxis) {
+        a.pl = f.px;
+        a.tw = f.d_tx;
+        a.L = (int)f.nx;
+        a.W = f.Wx;
+        a.ob = f.Mp;
+        a.es = f.ny * f.Mp;
+    } else {
+        a.pl = f.py;
+        a.tw = f.d_ty;
+        a.L = (int)f.ny;
+        a.W = f.Wy;
+        a.ob = f.ny * f.Mp;
+        a.es = f.Mp;
+    }
+    a.wshift = ilog2(a.W);
+    a.ncols = (int)f.Nh;
+    a.nct = (int)ceil_div(f.Nh, (size_t)a.W);
+    a.dW = FastDiv((uint32_t)a.W);
+    a.dNct = FastDiv((uint32_t)a.nct);
+    a.F = f.F;
+    a.cs_n = f.ny * f.Nh;
+    a.cs_o = f.Nh;
+    return a;
+}
+template <typename R, typename T>
+static void launch_strided(const Fft3d<R>& f, const SArgs<R, T>& a, int mode, bool xaxis, hipStream_t s) {
+    const size_t outer = xaxis ? f.ny : f.nx;
+    const unsigned grid = (unsigned)(outer * (size_t)a.nct);
+    const size_t lds = ((size_t)a.L + (size_t)a.L * a.W) * sizeof(cplx<R>);
+    switch (mode) {
+        case S_PLAIN: hipLaunchKernelGGL((k_strided<R, T, S_PLAIN>), dim3(grid), dim3(FFT_THREADS), lds, s, a); break;
+        case S_CLAMP: hipLaunchKernelGGL((k_strided<R, T, S_CLAMP>), dim3(grid), dim3(FFT_THREADS), lds, s, a); break;
+        case S_CERT_CHECK: hipLaunchKernelGGL((k_strided<R, T, S_CERT_CHECK>), dim3(grid), dim3(FFT_THREADS), lds, s, a); break;
+        case S_LOAD_DF: hipLaunchKernelGGL((k_strided<R, T, S_LOAD_DF>), dim3(grid), dim3(FFT_THREADS), lds, s, a); break;
+    }
+    FFCZ_HIP(hipGetLastError());
+}
+
+template <typename R>
+void fft_r2c(const Fft3d<R>& f, const R* in, cplx<R>* spec, hipStream_t s) {
+    auto z = zargs<R, R>(f);
+    z.in = in;
+    z.spec = spec;
+    hipLaunchKernelGGL((k_zfwd<R, R, Z_PLAIN>), dim3(zgrid(f)), dim3(FFT_THREADS), zlds(f), s, z);
+    FFCZ_HIP(hipGetLastError());
+    auto y = sargs<R, R>(f, spec, false);
+    y.sign = -1;
+    launch_strided(f, y, S_PLAIN, false, s);
+    auto x = sargs<R, R>(f, spec, true);
+    x.sign = -1;
+    launch_strided(f, x, S_PLAIN, true, s);
+}
+
+template <typename R>
+void fft_c2r(const Fft3d<R>& f, cplx<R>* spec, R* out, R scale, hipStream_t s) {
+    auto x = sargs<R, R>(f, spec, true);
+    x.sign = +1;
+    launch_strided(f, x, S_PLAIN, true, s);
+    auto y = sargs<R, R>(f, spec, false);
+    y.sign = +1;
+    launch_strided(f, y, S_PLAIN, false, s);
+    auto z = zargs<R, R>(f);
+    z.spec = spec;
+    z.out = out;
+    z.scale = scale;
+    hipLaunchKernelGGL((k_zinv<R, R, Z_PLAIN>), dim3(zgrid(f)), dim3(FFT_THREADS), zlds(f), s, z);
+    FFCZ_HIP(hipGetLastError());
+}
+
+template <typename R>
+void spec_to_compact(const Fft3d<R>& f, const cplx<R>* spec, cplx<R>* compact, hipStream_t s) {
+    hipLaunchKernelGGL((k_spec_to_compact<R>), dim3(grid_for(f.F)), dim3(256), 0, s, spec, compact, f.rows, f.Mp, f.Nh);
+    FFCZ_HIP(hipGetLastError());
+}
+template <typename R>
+void compact_to_spec(const Fft3d<R>& f, const cplx<R>* compact, cplx<R>* spec, hipStream_t s) {
+    hipLaunchKernelGGL((k_compact_to_spec<R>), dim3(grid_for(f.F)), dim3(256), 0, s, compact, spec, f.rows, f.Mp, f.Nh);
+    FFCZ_HIP(hipGetLastError());
+}
+
+template <typename R>
+void reset_state(DevState* st, int max_iter, double tol, hipStream_t s) {
+    hipLaunchKernelGGL(k_reset_state, dim3(1), dim3(64), 0, s, st, (unsigned)(max_iter < 0 ? 0 : max_iter), tol);
+    FFCZ_HIP(hipGetLastError());
+}
+
+template <typename R>
+void iteration(const Fft3d<R>& f, const IterArgs<R>& ia, hipStream_t s, hipEvent_t* ev) {
+    // ev (optional): 6 events recorded around the five launches
+    auto mark = [&](int i) {
+        if (ev) FFCZ_HIP(hipEventRecord(ev[i], s));
+    };
+    auto z = zargs<R, R>(f);
+    z.e = ia.e;
+    z.ds = ia.ds;
+    z.eps = ia.eps;
+    z.st = ia.st;
+    z.spec = ia.spec;
+    z.scale = ia.norm;
+    mark(0);
+    hipLaunchKernelGGL((k_zfwd<R, R, Z_ITER>), dim3(zgrid(f)), dim3(FFT_THREADS), zlds(f), s, z);
+    FFCZ_HIP(hipGetLastError());
+    mark(1);
+    auto y = sargs<R, R>(f, ia.spec, false);
+    y.st = ia.st;
+    y.sign = -1;
+    launch_strided(f, y, S_PLAIN, false, s);
+    mark(2);
+    auto x = sargs<R, R>(f, ia.spec, true);
+    x.st = ia.st;
+    x.fmode = ia.fmode;
+    x.delta = ia.delta;
+    x.delta_dense = ia.delta_dense;
+    x.dscale = ia.dscale;
+    x.df = ia.df;
+    x.orig = ia.orig;
+    x.delta_ptw = ia.delta_ptw;
+    launch_strided(f, x, ia.fmode == FM_NONE ? S_PLAIN : S_CLAMP, true, s);
+    mark(3);
+    auto y2 = sargs<R, R>(f, ia.spec, false);
+    y2.st = ia.st;
+    y2.sign = +1;
+    launch_strided(f, y2, S_PLAIN, false, s);
+    mark(4);
+    hipLaunchKernelGGL((k_zinv<R, R, Z_ITER>), dim3(zgrid(f)), dim3(FFT_THREADS), zlds(f), s, z);
+    FFCZ_HIP(hipGetLastError());
+    mark(5);
+}
+
+template <typename R>
+void epilogue_clamp(const Fft3d<R>& f, const IterArgs<R>& a, hipStream_t s) {
+    hipLaunchKernelGGL((k_epilogue_clamp<R>), dim3(grid_for(f.N)), dim3(256), 0, s, a.e, a.ds, a.eps, f.N,
+                       (const DevState*)a.st);
+    FFCZ_HIP(hipGetLastError());
+}
+
+template <typename T>
+void certify(const Fft3d<double>& f, const CertArgs<T>& c, hipStream_t s) {
+    hipLaunchKernelGGL(k_reset_stats, dim3(1), dim3(64), 0, s, c.stats);
+    FFCZ_HIP(hipGetLastError());
+    const double* dfi = nullptr;
+    if (c.have_df) {
+        // C2R(deq df)/N in FP64: x pass loads the split accumulator.
+        auto x = sargs<double, T>(f, c.work_spec, true);
+        x.sign = +1;
+        x.dfsrc = c.df;
+        x.qf = c.qf;
+        launch_strided(f, x, S_LOAD_DF, true, s);
+        auto y = sargs<double, T>(f, c.work_spec, false);
+        y.sign = +1;
+        launch_strided(f, y, S_PLAIN, false, s);
+        auto z = zargs<double, T>(f);
+        z.spec = c.work_spec;
+        z.out = c.work_real;
+        z.scale = 1.0 / (double)f.N;
+        hipLaunchKernelGGL((k_zinv<double, T, Z_PLAIN>), dim3(zgrid(f)), dim3(FFT_THREADS), zlds(f), s, z);
+        FFCZ_HIP(hipGetLastError());
+        dfi = c.work_real;
+    }
+    auto z = zargs<double, T>(f);
+    z.spec = c.work_spec;
+    z.e0 = c.e0;
+    z.cds = c.ds;
+    z.dfi = dfi;
+    z.qs = c.qs;
+    z.ceps = c.eps;
+    z.stats = c.stats;
+    hipLaunchKernelGGL((k_zfwd<double, T, Z_CERT>), dim3(zgrid(f)), dim3(FFT_THREADS), zlds(f), s, z);
+    FFCZ_HIP(hipGetLastError());
+    auto y = sargs<double, T>(f, c.work_spec, false);
+    y.sign = -1;
+    launch_strided(f, y, S_PLAIN, false, s);
+    auto x = sargs<double, T>(f, c.work_spec, true);
+    x.cdelta = c.delta;
+    x.cdelta_dense = c.delta_dense;
+    x.stats = c.stats;
+    launch_strided(f, x, S_CERT_CHECK, true, s);
+}
+
+// explicit instantiations
+template struct Fft3d<float>;
+template struct Fft3d<double>;
+template void fft_r2c<float>(const Fft3d<float>&, const float*, cplx<float>*, hipStream_t);
+template void fft_r2c<double>(const Fft3d<double>&, const double*, cplx<double>*, hipStream_t);
+template void fft_c2r<float>(const Fft3d<float>&, cplx<float>*, float*, float, hipStream_t);
+template void fft_c2r<double>(const Fft3d<double>&, cplx<double>*, double*, double, hipStream_t);
+template void spec_to_compact<float>(const Fft3d<float>&, const cplx<float>*, cplx<float>*, hipStream_t);
+template void spec_to_compact<double>(const Fft3d<double>&, const cplx<double>*, cplx<double>*, hipStream_t);
+template void compact_to_spec<float>(const Fft3d<float>&, const cplx<float>*, cplx<float>*, hipStream_t);
+template void compact_to_spec<double>(const Fft3d<double>&, const cplx<double>*, cplx<double>*, hipStream_t);
+template void reset_state<float>(DevState*, int, double, hipStream_t);
+template void reset_state<double>(DevState*, int, double, hipStream_t);
+template void iteration<float>(const Fft3d<float>&, const IterArgs<float>&, hipStream_t, hipEvent_t*);
+template void iteration<double>(const Fft3d<double>&, const IterArgs<double>&, hipStream_t, hipEvent_t*);
+template void epilogue_clamp<float>(const Fft3d<float>&, const IterArgs<float>&, hipStream_t);
+template void epilogue_clamp<double>(const Fft3d<double>&, const IterArgs<double>&, hipStream_t);
+template void certify<float>(const Fft3d<double>&, const CertArgs<float>&, hipStream_t);
+template void certify<double>(const Fft3d<double>&, const CertArgs<double>&, hipStream_t);
+
+}  // namespace ffcz

@@ -1,0 +1,135 @@
+// 3D real FFT (R2C / C2R) for row-major Nx x Ny x Nz fields, z fastest, built
+// from three LDS-tiled pencil passes.  The half spectrum is stored padded:
+// bin (kx, ky, kz), kz <= Nz/2, lives at ((kx*Ny)+ky)*Mp + kz with the row
+// pitch Mp a multiple of 16 elements (128 B for FP32), so every y/x tile load
+// is an aligned full-line segment.  The reference's compact layout (pitch
+// Nh = Nz/2+1, as cuFFT writes it) is used only for the edit accumulators.
+#pragma once
+
+#include "fft_core.hpp"
+
+namespace ffcz {
+
+// Loop bookkeeping shared by the iteration kernels (device memory).  The last
+// workgroup of the inverse z pass finalises an iteration: it publishes the max
+// spatial violation, bumps ``iter`` and sets ``done`` on convergence or when
+// ``iter`` reaches ``max_iter``; every iteration kernel returns immediately
+// once ``done`` is set, so extra enqueued iterations are no-ops.
+struct DevState {
+    unsigned int done;
+    unsigned int converged;
+    unsigned int iter;
+    unsigned int max_iter;
+    unsigned int blocks_done;
+    unsigned int pad0;
+    unsigned long long viol_bits;  // max violation of the running iteration (order-preserving bits)
+    double tol;
+    double last_viol;
+    float tol_f;
+    unsigned int pad1;
+};
+
+// Spatial-bound statistics of a certification pass (FP64).
+struct CertStats {
+    unsigned long long spatial_viol;
+    unsigned long long freq_viol;
+    unsigned long long spatial_max_bits;  // max excess |d|-eps (ordbits of double)
+    unsigned long long freq_max_bits;     // max excess max(|Re|,|Im|)-delta
+};
+
+// Quantiser descriptors (applied on the fly when reconstructing edits).
+enum QuantKind : int { Q_NONE = 0, Q_REF16 = 1, Q_STEP = 2 };
+struct QuantDesc {
+    int kind;
+    double vmin, vmax;  // Q_REF16 (cast to T inside the kernels)
+    double step;        // Q_STEP
+    double threshold;   // Q_REF16 flag threshold (1e-7 in the reference)
+};
+
+template <typename R>
+struct Fft3d {
+    size_t nx = 0, ny = 0, nz = 0;
+    size_t N = 0;       // nx*ny*nz
+    size_t Nh = 0;      // nz/2+1 bins along z
+    size_t F = 0;       // nx*ny*Nh
+    size_t Mp = 0;      // padded z pitch of the internal half spectrum
+    size_t rows = 0;    // nx*ny
+    bool half = true;   // nz even: half-length complex trick
+    int Lz = 1;         // complex length of the z transform (nz/2 or nz)
+    FftPlan pz{}, py{}, px{};
+    int Pz = 1, Wy = 1, Wx = 1;       // pencils per z tile, columns per y/x tile
+    cplx<R>* d_tz = nullptr;   // W_Lz^m
+    cplx<R>* d_tz2 = nullptr;  // W_nz^k, k <= nz/2 (R2C post / C2R pre)
+    cplx<R>* d_ty = nullptr;
+    cplx<R>* d_tx = nullptr;
+    size_t spec_elems() const { return rows * Mp; }
+
+    void create(size_t nx_, size_t ny_, size_t nz_);
+    void destroy();
+};
+
+// ---- plain transforms (initialisation, tests) ------------------------------
+// forward R2C: in[N] (real R) -> spec (padded half spectrum)
+template <typename R>
+void fft_r2c(const Fft3d<R>& f, const R* in, cplx<R>* spec, hipStream_t s);
+// inverse C2R, unnormalised times ``scale``: spec is overwritten (y/x passes
+// run in place) and out[N] receives the real field.
+template <typename R>
+void fft_c2r(const Fft3d<R>& f, cplx<R>* spec, R* out, R scale, hipStream_t s);
+// padded <-> compact (pitch Nh) layout conversion
+template <typename R>
+void spec_to_compact(const Fft3d<R>& f, const cplx<R>* spec, cplx<R>* compact, hipStream_t s);
+template <typename R>
+void compact_to_spec(const Fft3d<R>& f, const cplx<R>* compact, cplx<R>* spec, hipStream_t s);
+
+// ---- iteration passes ------------------------------------------------------
+enum FreqMode : int { FM_ABS = 0, FM_DENSE = 1, FM_PTW = 2, FM_NONE = 3 };
+
+template <typename R>
+struct IterArgs {
+    R* e;                 // spatial iterate [N]
+    R* ds;                // spatial edit accumulator [N]
+    R* df;                // frequency edit accumulator [2F] split Re|Im, reference layout
+    cplx<R>* spec;        // padded half spectrum work buffer
+    R eps;                // spatial bound used by the iteration
+    int fmode;            // FreqMode
+    R delta;              // FM_ABS bound
+    const R* delta_dense; // FM_DENSE per-bin bounds [F] (inf = unconstrained)
+    R dscale;             // FM_DENSE bound multiplier (1 = the given bounds)
+    const cplx<R>* orig;  // FM_PTW: F(x) compact layout [F]
+    R delta_ptw;
+    R norm;               // T(1)/T(N)
+    DevState* st;
+};
+
+// forward z (with the deferred spatial clamp of the previous iteration fused
+// into the load), forward y, fused x forward+clamp+inverse, inverse y,
+// inverse z (+normalise, violation max, iteration bookkeeping).
+template <typename R>
+void iteration(const Fft3d<R>& f, const IterArgs<R>& a, hipStream_t s, hipEvent_t* ev = nullptr);
+// Final spatial clamp when the loop stopped at max_iter without convergence.
+template <typename R>
+void epilogue_clamp(const Fft3d<R>& f, const IterArgs<R>& a, hipStream_t s);
+template <typename R>
+void reset_state(DevState* st, int max_iter, double tol, hipStream_t s);
+
+// ---- FP64 certification ----------------------------------------------------
+template <typename T>
+struct CertArgs {
+    size_t nx, ny, nz;
+    const T* e0;          // initial error x' - x [N]
+    const T* ds;          // spatial edits [N]
+    const T* df;          // frequency edits [2F] (reference layout)
+    QuantDesc qs, qf;     // dequantisation applied to ds / df
+    bool have_df;         // any frequency edit present
+    double eps;
+    double delta;         // uniform bound, or
+    const T* delta_dense; // per-bin bounds [F] (nullptr: uniform)
+    double* work_real;    // [N] doubles (C2R of df)
+    cplx<double>* work_spec;  // padded FP64 spectrum
+    CertStats* stats;
+};
+template <typename T>
+void certify(const Fft3d<double>& f64, const CertArgs<T>& a, hipStream_t s);
+
+}  // namespace ffcz

@@ -1,0 +1,46 @@
+// Edit quantisers shared by extraction and certification.
+#pragma once
+
+#include "fft3d.hpp"
+
+namespace ffcz {
+
+// Reference 16-bit quantiser, projection_algorithm.cu:447-455, in type T:
+// n = (v-min)/(max-min); q = n > 0.5 ? ceil(n*65535) : floor(n*65535).
+template <typename T>
+__device__ __forceinline__ T quant_ref16(T v, T mn, T mx) {
+#pragma clang fp contract(off)
+    const T n = (v - mn) / (mx - mn);
+    const T s = n * (T)65535;
+    return n > (T)0.5 ? ceil(s) : floor(s);
+}
+
+// Step quantiser: llround(v/step) in FP64, ties away from zero.
+__device__ __forceinline__ long long quant_step(double v, double step) {
+    double r = v / step;
+    const double lim = 4.0e15;
+    r = r > lim ? lim : (r < -lim ? -lim : r);
+    return r < 0.0 ? -(long long)floor(-r + 0.5) : (long long)floor(r + 0.5);
+}
+
+__device__ __forceinline__ unsigned long long zigzag64(long long q) {
+    return ((unsigned long long)q << 1) ^ (unsigned long long)(q >> 63);
+}
+
+// Value a decoder reconstructs for edit v (0 for inactive edits).
+template <typename T>
+__device__ __forceinline__ double dequant(T v, const QuantDesc& q) {
+#pragma clang fp contract(off)
+    if (q.kind == Q_NONE) return (double)v;
+    if (q.kind == Q_REF16) {
+        // projection_algorithm.cu:629-643 in type T
+        if (!(fabs(v) > (T)q.threshold)) return 0.0;
+        const T mn = (T)q.vmin, mx = (T)q.vmax;
+        const T qv = quant_ref16(v, mn, mx);
+        const T r = qv / (T)65535 * (mx - mn) + mn;
+        return (double)r;
+    }
+    return (double)quant_step((double)v, q.step) * q.step;
+}
+
+}  // namespace ffcz

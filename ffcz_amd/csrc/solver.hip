@@ -1,0 +1,415 @@
+// ProjectionSolver orchestration on one GPU: device buffers, the projection
+// loop with device-side termination, extraction and FP64 certification.
+#include <algorithm>
+#include <cmath>
+#include <limits>
+
+#include "solver.hpp"
+
+namespace ffcz {
+
+template <typename T>
+__global__ void k_zero(T* p, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = T(0);
+}
+template <typename T>
+__global__ void k_fill(T* p, size_t n, T v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+// dense per-bin bound: min over duplicate indices; flags an out-of-range index
+template <typename T>
+__global__ void k_scatter_delta(T* dense, size_t F, const size_t* idx, const T* dk, size_t K, unsigned int* bad) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < K; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t j = idx[i];
+        const T d = dk[i];
+        if (j >= F || !(d >= T(0))) {
+            atomicOr(bad, 1u);
+            continue;
+        }
+        // non-negative values order like their bit patterns
+        if constexpr (sizeof(T) == 4) atomicMin((unsigned int*)&dense[j], __float_as_uint((float)d));
+        else atomicMin((unsigned long long*)&dense[j], (unsigned long long)__double_as_longlong((double)d));
+    }
+}
+template <typename T>
+__global__ void k_any_nonzero(const T* p, size_t n, unsigned int* flag) {
+    bool nz = false;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        nz |= (p[i] != T(0));
+    if (__ballot(nz) && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
+}
+
+static unsigned gridn(size_t n) {
+    size_t g = ceil_div(n ? n : 1, (size_t)256);
+    return (unsigned)std::min<size_t>(g, 1u << 16);
+}
+
+template <typename T>
+static void dalloc(T** p, size_t n) {
+    FFCZ_REQUIRE(n > 0, ERR_INVALID, "zero-size allocation");
+    hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+    if (e != hipSuccess)
+        throw Error(ERR_ALLOC, "device allocation of " + std::to_string(n * sizeof(T)) + " bytes failed: " +
+                                   hipGetErrorString(e));
+}
+
+template <typename T>
+Solver<T>::Solver(size_t nx, size_t ny, size_t nz) {
+    f_.create(nx, ny, nz);
+    try {
+        FFCZ_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+        dalloc(&x_, f_.N);
+        dalloc(&e_, f_.N);
+        dalloc(&e0_, f_.N);
+        dalloc(&ds_, f_.N);
+        dalloc(&df_, 2 * f_.F);
+        dalloc(&spec_, f_.spec_elems());
+        dalloc(&st_, 1);
+        reset_edit_accumulators();  // projection_algorithm_impl.cuh:60
+        reset_state<T>(st_, 0, 0.0, s_);
+        FFCZ_HIP(hipStreamSynchronize(s_));
+    } catch (...) {
+        release();
+        throw;
+    }
+}
+
+template <typename T>
+Solver<T>::~Solver() {
+    release();
+}
+
+template <typename T>
+void Solver<T>::release() {
+    if (s_) (void)hipStreamSynchronize(s_);
+    void* ptrs[] = {x_, e_, e0_, ds_, df_, spec_, st_, delta_dense_, orig_, partial_idx_, fpack_, spack_,
+                    fcomp_, scomp_, fquant_, squant_, xscratch_, cert_real_, cert_spec_, cstats_};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    x_ = e_ = e0_ = ds_ = df_ = nullptr;
+    spec_ = nullptr;
+    st_ = nullptr;
+    delta_dense_ = nullptr;
+    orig_ = nullptr;
+    partial_idx_ = nullptr;
+    fpack_ = spack_ = nullptr;
+    fcomp_ = scomp_ = nullptr;
+    fquant_ = squant_ = nullptr;
+    xscratch_ = nullptr;
+    cert_real_ = nullptr;
+    cert_spec_ = nullptr;
+    cstats_ = nullptr;
+    if (f64_) f64_->destroy();
+    f64_.reset();
+    f_.destroy();
+    if (s_) (void)hipStreamDestroy(s_);
+    s_ = nullptr;
+}
+
+template <typename T>
+void Solver<T>::upload_x(const T* x) {
+    FFCZ_REQUIRE(x != nullptr, ERR_INVALID, "null original data");
+    FFCZ_HIP(hipMemcpyAsync(x_, x, f_.N * sizeof(T), hipMemcpyDefault, s_));
+}
+
+template <typename T>
+void Solver<T>::reset_edit_accumulators() {
+    hipLaunchKernelGGL((k_zero<T>), dim3(gridn(2 * f_.F)), dim3(256), 0, s_, df_, 2 * f_.F);
+    hipLaunchKernelGGL((k_zero<T>), dim3(gridn(f_.N)), dim3(256), 0, s_, ds_, f_.N);
+    FFCZ_HIP(hipGetLastError());
+    FFCZ_HIP(hipStreamSynchronize(s_));
+}
+
+template <typename T>
+void Solver<T>::init_all_abs(const T* x, T eps, T delta) {
+    // projection_algorithm_impl.cuh:639-668 (its F(x) is not needed here)
+    upload_x(x);
+    eps_ = eps;
+    delta_ = delta;
+    init_mode_ = MODE_ABS;
+    initialized_ = true;
+    FFCZ_HIP(hipStreamSynchronize(s_));
+}
+
+template <typename T>
+void Solver<T>::init_direct(const T* x, const size_t* idx, const T* dk, T eps, size_t K) {
+    // projection_algorithm_impl.cuh:598-637; the constraint list becomes a
+    // dense per-bin bound (+inf where unconstrained) read by the fused pass.
+    FFCZ_REQUIRE(K == 0 || (idx && dk), ERR_INVALID, "null constraint arrays");
+    upload_x(x);
+    eps_ = eps;
+    if (!delta_dense_) dalloc(&delta_dense_, f_.F);
+    hipLaunchKernelGGL((k_fill<T>), dim3(gridn(f_.F)), dim3(256), 0, s_, delta_dense_, f_.F,
+                       std::numeric_limits<T>::infinity());
+    if (K > 0) {
+        size_t* didx = nullptr;
+        T* ddk = nullptr;
+        unsigned int* bad = nullptr;
+        dalloc(&didx, K);
+        dalloc(&ddk, K);
+        dalloc(&bad, 1);
+        unsigned int hbad = 0;
+        try {
+            FFCZ_HIP(hipMemcpyAsync(didx, idx, K * sizeof(size_t), hipMemcpyDefault, s_));
+            FFCZ_HIP(hipMemcpyAsync(ddk, dk, K * sizeof(T), hipMemcpyDefault, s_));
+            FFCZ_HIP(hipMemsetAsync(bad, 0, sizeof(unsigned int), s_));
+            hipLaunchKernelGGL((k_scatter_delta<T>), dim3(gridn(K)), dim3(256), 0, s_, delta_dense_, f_.F, didx, ddk, K, bad);
+            FFCZ_HIP(hipGetLastError());
+            FFCZ_HIP(hipMemcpyAsync(&hbad, bad, sizeof(hbad), hipMemcpyDeviceToHost, s_));
+            FFCZ_HIP(hipStreamSynchronize(s_));
+        } catch (...) {
+            (void)hipFree(didx);
+            (void)hipFree(ddk);
+            (void)hipFree(bad);
+            throw;
+        }
+        (void)hipFree(didx);
+        (void)hipFree(ddk);
+        (void)hipFree(bad);
+        FFCZ_REQUIRE(hbad == 0, ERR_INVALID, "frequency constraint index >= F or negative/NaN bound");
+    }
+    init_mode_ = MODE_DIRECT;
+    initialized_ = true;
+    FFCZ_HIP(hipStreamSynchronize(s_));
+}
+
+template <typename T>
+void Solver<T>::init_all_ptw(const T* x, T eps, T delta_ptw) {
+    // projection_algorithm_impl.cuh:670-699: F(x) is needed by the ptw bound.
+    upload_x(x);
+    eps_ = eps;
+    delta_ptw_ = delta_ptw;
+    if (!orig_) dalloc(&orig_, f_.F);
+    fft_r2c(f_, x_, spec_, s_);
+    spec_to_compact(f_, spec_, orig_, s_);
+    init_mode_ = MODE_PTW;
+    initialized_ = true;
+    FFCZ_HIP(hipStreamSynchronize(s_));
+}
+
+template <typename T>
+void Solver<T>::init_partial_ptw(const T* x, const size_t* idx, T delta_ptw, T eps, size_t K) {
+    // projection_algorithm_impl.cuh:701-738: stores F(x) and the index list;
+    // the reference has no solve that consumes them.
+    init_all_ptw(x, eps, delta_ptw);
+    if (partial_idx_) (void)hipFree(partial_idx_);
+    partial_idx_ = nullptr;
+    partial_K_ = K;
+    if (K) {
+        FFCZ_REQUIRE(idx != nullptr, ERR_INVALID, "null index array");
+        dalloc(&partial_idx_, K);
+        FFCZ_HIP(hipMemcpyAsync(partial_idx_, idx, K * sizeof(size_t), hipMemcpyDefault, s_));
+        FFCZ_HIP(hipStreamSynchronize(s_));
+    }
+}
+
+template <typename T>
+IterArgs<T> Solver<T>::iter_args() const {
+    IterArgs<T> a{};
+    a.e = e_;
+    a.ds = ds_;
+    a.df = df_;
+    a.spec = spec_;
+    a.eps = (T)((double)eps_ * (1.0 - margin_s_));
+    if (margin_s_ == 0.0) a.eps = eps_;
+    a.fmode = init_mode_ == MODE_ABS ? FM_ABS : (init_mode_ == MODE_DIRECT ? FM_DENSE : FM_PTW);
+    a.delta = margin_f_ == 0.0 ? delta_ : (T)((double)delta_ * (1.0 - margin_f_));
+    a.dscale = margin_f_ == 0.0 ? T(1) : (T)(1.0 - margin_f_);
+    a.delta_dense = delta_dense_;
+    a.orig = orig_;
+    a.delta_ptw = delta_ptw_;
+    a.norm = T(1) / static_cast<T>(f_.N);  // projection_algorithm_impl.cuh:242
+    a.st = st_;
+    return a;
+}
+
+template <typename T>
+void Solver<T>::solve_loop(int mode, const T* e0, int max_iter, T tol) {
+    FFCZ_REQUIRE(initialized_, ERR_STATE, "Solver not initialized");  // impl:778-781
+    FFCZ_REQUIRE(mode == init_mode_, ERR_STATE,
+                 "solve mode does not match the initialize* call (direct<->initialize, abs<->initialize_all_abs, "
+                 "ptw<->initialize_all_ptw)");
+    FFCZ_REQUIRE(e0 != nullptr, ERR_INVALID, "null spatial error");
+    FFCZ_HIP(hipMemcpyAsync(e_, e0, f_.N * sizeof(T), hipMemcpyDefault, s_));
+    FFCZ_HIP(hipMemcpyAsync(e0_, e_, f_.N * sizeof(T), hipMemcpyDeviceToDevice, s_));
+    reset_state<T>(st_, max_iter, (double)tol, s_);
+    const IterArgs<T> a = iter_args();
+    int launched = 0, chunk = 1;
+    DevState h{};
+    while (launched < max_iter) {
+        const int k = std::min(chunk, max_iter - launched);
+        for (int i = 0; i < k; ++i) iteration(f_, a, s_);
+        launched += k;
+        FFCZ_HIP(hipMemcpyAsync(&h, st_, sizeof(h), hipMemcpyDeviceToHost, s_));
+        FFCZ_HIP(hipStreamSynchronize(s_));
+        if (h.done) break;
+        chunk = std::min(chunk * 2, 16);
+    }
+    epilogue_clamp(f_, a, s_);
+    FFCZ_HIP(hipMemcpyAsync(&h, st_, sizeof(h), hipMemcpyDeviceToHost, s_));
+    FFCZ_HIP(hipStreamSynchronize(s_));
+    iters_ = (int)h.iter;
+    viol_ = h.last_viol;
+    converged_ = h.converged != 0;
+    solved_ = true;
+    extracted_ = false;
+}
+
+template <typename T>
+void Solver<T>::set_quantizer(int kind, double fstep, double sstep) {
+    FFCZ_REQUIRE(kind == Q_REF16 || kind == Q_STEP, ERR_INVALID, "unknown quantizer kind");
+    if (kind == Q_STEP) FFCZ_REQUIRE(fstep > 0 && sstep > 0, ERR_INVALID, "quantizer steps must be > 0");
+    qkind_ = kind;
+    qstep_f_ = fstep;
+    qstep_s_ = sstep;
+    extracted_ = false;
+}
+
+template <typename T>
+QuantDesc Solver<T>::freq_quant() const {
+    QuantDesc q{};
+    q.kind = qkind_;
+    q.threshold = 1e-7;  // projection_algorithm_impl.cuh:389
+    q.step = qstep_f_;
+    q.vmin = xf_.vmin;
+    q.vmax = xf_.vmax;
+    return q;
+}
+template <typename T>
+QuantDesc Solver<T>::spatial_quant() const {
+    QuantDesc q{};
+    q.kind = qkind_;
+    q.threshold = 1e-7;  // projection_algorithm_impl.cuh:398
+    q.step = qstep_s_;
+    q.vmin = xs_.vmin;
+    q.vmax = xs_.vmax;
+    return q;
+}
+
+template <typename T>
+void Solver<T>::ensure_extract_buffers() {
+    if (fpack_) return;
+    dalloc(&fpack_, ceil_div(2 * f_.F, (size_t)64) * 8);
+    dalloc(&spack_, ceil_div(f_.N, (size_t)64) * 8);
+    dalloc(&fcomp_, 2 * f_.F);
+    dalloc(&scomp_, f_.N);
+    dalloc(&fquant_, 2 * f_.F);
+    dalloc(&squant_, f_.N);
+    const size_t sb = std::max(extract_scratch_bytes(2 * f_.F), extract_scratch_bytes(f_.N));
+    unsigned char* p = nullptr;
+    dalloc(&p, sb);
+    xscratch_ = p;
+}
+
+template <typename T>
+void Solver<T>::extract() {
+    // extract_compact_representation, projection_algorithm_impl.cuh:371-456
+    ensure_extract_buffers();
+    xf_ = extract_edits<T>(df_, 2 * f_.F, freq_quant(), fpack_, fcomp_, fquant_, xscratch_, s_);
+    xs_ = extract_edits<T>(ds_, f_.N, spatial_quant(), spack_, scomp_, squant_, xscratch_, s_);
+    extracted_ = true;
+}
+
+template <typename T>
+void Solver<T>::ensure_cert_buffers() {
+    if (f64_) return;
+    f64_.reset(new Fft3d<double>());
+    f64_->create(f_.nx, f_.ny, f_.nz);
+    dalloc(&cert_real_, f_.N);
+    dalloc(&cert_spec_, f64_->spec_elems());
+    dalloc(&cstats_, 1);
+}
+
+template <typename T>
+CertResult Solver<T>::certify(bool quantized) {
+    FFCZ_REQUIRE(solved_, ERR_STATE, "certify requires a completed solve");
+    FFCZ_REQUIRE(init_mode_ == MODE_ABS || init_mode_ == MODE_DIRECT, ERR_INVALID,
+                 "certification is defined for the absolute and per-bin bound modes");
+    if (quantized) FFCZ_REQUIRE(extracted_, ERR_STATE, "certify(quantized) requires extraction first");
+    ensure_cert_buffers();
+    CertArgs<T> c{};
+    c.nx = f_.nx;
+    c.ny = f_.ny;
+    c.nz = f_.nz;
+    c.e0 = e0_;
+    c.ds = ds_;
+    c.df = df_;
+    if (quantized) {
+        c.qs = spatial_quant();
+        c.qf = freq_quant();
+        c.have_df = xf_.count > 0;
+    } else {
+        c.qs.kind = Q_NONE;
+        c.qf.kind = Q_NONE;
+        unsigned int* flag = nullptr;
+        dalloc(&flag, 1);
+        unsigned int hf = 0;
+        FFCZ_HIP(hipMemsetAsync(flag, 0, sizeof(unsigned int), s_));
+        hipLaunchKernelGGL((k_any_nonzero<T>), dim3(gridn(2 * f_.F)), dim3(256), 0, s_, df_, 2 * f_.F, flag);
+        FFCZ_HIP(hipMemcpyAsync(&hf, flag, sizeof(hf), hipMemcpyDeviceToHost, s_));
+        FFCZ_HIP(hipStreamSynchronize(s_));
+        (void)hipFree(flag);
+        c.have_df = hf != 0;
+    }
+    c.eps = (double)eps_;
+    c.delta = (double)delta_;
+    c.delta_dense = init_mode_ == MODE_DIRECT ? delta_dense_ : nullptr;
+    c.work_real = cert_real_;
+    c.work_spec = cert_spec_;
+    c.stats = cstats_;
+    ffcz::certify<T>(*f64_, c, s_);
+    CertStats hs{};
+    FFCZ_HIP(hipMemcpyAsync(&hs, cstats_, sizeof(hs), hipMemcpyDeviceToHost, s_));
+    FFCZ_HIP(hipStreamSynchronize(s_));
+    CertResult r;
+    r.spatial_violations = hs.spatial_viol;
+    r.freq_violations = hs.freq_viol;
+    r.spatial_max_excess = hs.spatial_max_bits ? __builtin_bit_cast(double, hs.spatial_max_bits) : 0.0;
+    r.freq_max_excess = hs.freq_max_bits ? __builtin_bit_cast(double, hs.freq_max_bits) : 0.0;
+    r.ok = (hs.spatial_viol == 0 && hs.freq_viol == 0) ? 1 : 0;
+    return r;
+}
+
+template <typename T>
+void Solver<T>::get_results(T* e) const {
+    // projection_algorithm_impl.cuh:904-910
+    FFCZ_HIP(hipMemcpyAsync(e, e_, f_.N * sizeof(T), hipMemcpyDefault, s_));
+    FFCZ_HIP(hipStreamSynchronize(s_));
+}
+
+template <typename T>
+void Solver<T>::time_iteration(int reps, float* ms5) {
+    FFCZ_REQUIRE(solved_, ERR_STATE, "time_iteration requires a completed solve");
+    FFCZ_REQUIRE(reps >= 1, ERR_INVALID, "reps must be >= 1");
+    reset_state<T>(st_, 1 << 30, -1.0, s_);  // tol < 0: never converges
+    const IterArgs<T> a = iter_args();
+    hipEvent_t ev[6] = {};
+    for (auto& e : ev) FFCZ_HIP(hipEventCreate(&e));
+    float acc[5] = {0, 0, 0, 0, 0};
+    try {
+        iteration(f_, a, s_);  // warm-up
+        for (int r = 0; r < reps; ++r) {
+            iteration(f_, a, s_, ev);
+            FFCZ_HIP(hipEventSynchronize(ev[5]));
+            for (int k = 0; k < 5; ++k) {
+                float t = 0;
+                FFCZ_HIP(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+                acc[k] += t;
+            }
+        }
+    } catch (...) {
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        throw;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    for (int k = 0; k < 5; ++k) ms5[k] = acc[k] / (float)reps;
+    reset_state<T>(st_, 0, 0.0, s_);
+    FFCZ_HIP(hipStreamSynchronize(s_));
+}
+
+template class Solver<float>;
+template class Solver<double>;
+
+}  // namespace ffcz

@@ -1,0 +1,139 @@
+// ProjectionSolver state on one GPU (reference: class ProjectionSolver,
+// projection_algorithm.cuh:238-333, projection_algorithm_impl.cuh:1-910).
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "extract.hpp"
+#include "fft3d.hpp"
+
+namespace ffcz {
+
+enum SolveMode : int { MODE_DIRECT = 0, MODE_ABS = 1, MODE_PTW = 2 };
+
+struct CertResult {
+    unsigned long long spatial_violations = 0;
+    unsigned long long freq_violations = 0;
+    double spatial_max_excess = 0;
+    double freq_max_excess = 0;
+    int ok = 0;
+};
+
+template <typename T>
+class Solver {
+   public:
+    Solver(size_t nx, size_t ny, size_t nz);
+    ~Solver();
+    Solver(const Solver&) = delete;
+    Solver& operator=(const Solver&) = delete;
+
+    // initialize* (projection_algorithm_impl.cuh:598-738).  Pointers may be
+    // host or device memory.
+    void init_direct(const T* x, const size_t* idx, const T* delta_k, T eps, size_t K);
+    void init_all_abs(const T* x, T eps, T delta);
+    void init_all_ptw(const T* x, T eps, T delta_ptw);
+    void init_partial_ptw(const T* x, const size_t* idx, T delta_ptw, T eps, size_t K);
+
+    // Projection loop only (impl:791-808 / 835-852 / 879-896).
+    void solve_loop(int mode, const T* e0, int max_iter, T tol);
+    // solve / solve_all_abs / solve_all_ptw: loop + extraction (impl:772-902).
+    void solve(int mode, const T* e0, int max_iter, T tol) {
+        solve_loop(mode, e0, max_iter, tol);
+        extract();
+    }
+    void extract();
+    void reset_edit_accumulators();
+    CertResult certify(bool quantized);
+    // mean ms per launch of the five iteration kernels over ``reps`` iterations
+    void time_iteration(int reps, float* ms5);
+
+    // margins: the loop iterates against eps*(1-ms) and delta*(1-mf); the
+    // certification always checks the bounds given to initialize*.
+    void set_margins(double ms, double mf) {
+        margin_s_ = ms;
+        margin_f_ = mf;
+    }
+    void set_quantizer(int kind, double freq_step, double spatial_step);
+
+    // getters (copies into caller memory; host or device)
+    void get_results(T* e) const;
+    const T* d_e() const { return e_; }
+    const T* d_df() const { return df_; }
+    const T* d_ds() const { return ds_; }
+    size_t N() const { return f_.N; }
+    size_t F() const { return f_.F; }
+    size_t nx() const { return f_.nx; }
+    size_t ny() const { return f_.ny; }
+    size_t nz() const { return f_.nz; }
+    int iterations() const { return iters_; }
+    double violation() const { return viol_; }
+    bool converged() const { return converged_; }
+    bool extracted() const { return extracted_; }
+    size_t n_active_freq() const { return xf_.count; }
+    size_t n_active_spatial() const { return xs_.count; }
+    double freq_min() const { return xf_.vmin; }
+    double freq_max() const { return xf_.vmax; }
+    double spatial_min() const { return xs_.vmin; }
+    double spatial_max() const { return xs_.vmax; }
+    const uint8_t* d_freq_pack() const { return fpack_; }
+    const uint8_t* d_spatial_pack() const { return spack_; }
+    const T* d_freq_compact() const { return fcomp_; }
+    const T* d_spatial_compact() const { return scomp_; }
+    const uint16_t* d_freq_quant() const { return fquant_; }
+    const uint16_t* d_spatial_quant() const { return squant_; }
+    hipStream_t stream() const { return s_; }
+    // per-kernel timing of the iteration (for roofline reporting)
+    const Fft3d<T>& fft() const { return f_; }
+    IterArgs<T> iter_args() const;
+    QuantDesc freq_quant() const;
+    QuantDesc spatial_quant() const;
+
+   private:
+    void release();
+    void upload_x(const T* x);
+    void ensure_extract_buffers();
+    void ensure_cert_buffers();
+
+    Fft3d<T> f_;
+    hipStream_t s_ = nullptr;
+    T* x_ = nullptr;
+    T* e_ = nullptr;
+    T* e0_ = nullptr;
+    T* df_ = nullptr;
+    T* ds_ = nullptr;
+    cplx<T>* spec_ = nullptr;
+    DevState* st_ = nullptr;
+    T* delta_dense_ = nullptr;   // direct mode (and certification of it)
+    cplx<T>* orig_ = nullptr;    // F(x), compact layout (ptw modes)
+    size_t* partial_idx_ = nullptr;
+    size_t partial_K_ = 0;
+    // extraction outputs
+    uint8_t* fpack_ = nullptr;
+    uint8_t* spack_ = nullptr;
+    T* fcomp_ = nullptr;
+    T* scomp_ = nullptr;
+    uint16_t* fquant_ = nullptr;
+    uint16_t* squant_ = nullptr;
+    void* xscratch_ = nullptr;
+    ExtractResult xf_, xs_;
+    bool extracted_ = false;
+    // certification
+    std::unique_ptr<Fft3d<double>> f64_;
+    double* cert_real_ = nullptr;
+    cplx<double>* cert_spec_ = nullptr;
+    CertStats* cstats_ = nullptr;
+    // parameters
+    bool initialized_ = false;
+    int init_mode_ = -1;
+    T eps_ = 0, delta_ = 0, delta_ptw_ = 0;
+    double margin_s_ = 0, margin_f_ = 0;
+    int qkind_ = Q_REF16;
+    double qstep_f_ = 0, qstep_s_ = 0;
+    int iters_ = 0;
+    double viol_ = 0;
+    bool converged_ = false;
+    bool solved_ = false;
+};
+
+}  // namespace ffcz

@@ -1,0 +1,258 @@
+"""Python mirror of the reference ``ProjectionSolver<T>``
+(projection_algorithm.cuh:238-333) over the C ABI.
+
+Method names, argument meaning and error behaviour follow the reference: the
+same ``initialize*`` / ``solve*`` / getter surface, ``solve*`` runs the
+extraction at the end (projection_algorithm_impl.cuh:813,857,901), repeated
+solves keep accumulating edits until ``reset_edit_accumulators``, and
+``solve*`` before ``initialize*`` raises ``"Solver not initialized"``
+(projection_algorithm_impl.cuh:778-781).  Inputs may be numpy arrays (host) or
+torch tensors (host or GPU; GPU tensors are read in place).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import FFCZ_F32, FFCZ_F64, MODE_ABS, MODE_DIRECT, MODE_PTW, QUANT_REF16, QUANT_STEP, call
+
+
+def _np_dtype(dtype) -> np.dtype:
+    d = np.dtype(dtype)
+    if d not in (np.dtype(np.float32), np.dtype(np.float64)):
+        raise ValueError("ProjectionSolver supports float32 and float64")
+    return d
+
+
+def as_pointer(a, dtype=None):
+    """(void* address, keep-alive object) for a numpy array or torch tensor."""
+    if a is None:
+        return None, None
+    try:
+        import torch
+
+        if isinstance(a, torch.Tensor):
+            if not a.is_contiguous():
+                a = a.contiguous()
+            if dtype is not None and a.dtype != getattr(torch, np.dtype(dtype).name):
+                raise TypeError(f"tensor dtype {a.dtype} does not match {np.dtype(dtype)}")
+            if a.is_cuda:
+                # the library works on its own non-blocking stream
+                torch.cuda.current_stream(a.device).synchronize()
+            return C.c_void_p(a.data_ptr()), a
+    except ImportError:  # pragma: no cover
+        pass
+    arr = np.ascontiguousarray(a, dtype=dtype)
+    return C.c_void_p(arr.ctypes.data), arr
+
+
+@dataclass
+class CertResult:
+    ok: bool
+    spatial_violations: int
+    freq_violations: int
+    spatial_max_excess: float
+    freq_max_excess: float
+
+
+class ProjectionSolver:
+    """``ProjectionSolver<T>(Nx, Ny, Nz)`` on the current HIP device."""
+
+    def __init__(self, Nx: int, Ny: int, Nz: int, dtype=np.float32):
+        self.dtype = _np_dtype(dtype)
+        self._sfx = "f32" if self.dtype == np.float32 else "f64"
+        self.shape = (int(Nx), int(Ny), int(Nz))
+        self.N = self.shape[0] * self.shape[1] * self.shape[2]
+        self.F = self.shape[0] * self.shape[1] * (self.shape[2] // 2 + 1)
+        h = C.c_void_p()
+        call("ffcz_create", FFCZ_F32 if self.dtype == np.float32 else FFCZ_F64,
+             self.shape[0], self.shape[1], self.shape[2], C.byref(h))
+        self._h = h
+
+    # -- lifecycle ---------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            call("ffcz_destroy", self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover - best effort
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _scalar(self, v):
+        return float(self.dtype.type(v))
+
+    def _field(self, a, name):
+        p, keep = as_pointer(a, self.dtype)
+        n = int(np.prod(keep.shape)) if hasattr(keep, "shape") else None
+        if n is not None and n != self.N:
+            raise ValueError(f"{name} has {n} elements, expected {self.N}")
+        return p, keep
+
+    # -- initialize* (projection_algorithm_impl.cuh:598-738) ----------------
+    def initialize(self, orig_data, freq_indices, delta_k, spatial_epsilon, num_freq_constraints=None):
+        x, kx = self._field(orig_data, "orig_data")
+        idx, ki = as_pointer(np.asarray(freq_indices, dtype=np.uint64) if not _is_tensor(freq_indices) else freq_indices)
+        dk, kd = as_pointer(delta_k, self.dtype)
+        K = int(num_freq_constraints if num_freq_constraints is not None else len(ki))
+        call(f"ffcz_init_direct_{self._sfx}", self._h, x, idx, dk, self._scalar(spatial_epsilon), K)
+
+    def initialize_all_abs(self, orig_data, spatial_epsilon, delta):
+        x, kx = self._field(orig_data, "orig_data")
+        call(f"ffcz_init_all_abs_{self._sfx}", self._h, x, self._scalar(spatial_epsilon), self._scalar(delta))
+
+    def initialize_all_ptw(self, orig_data, spatial_epsilon, delta_ptw):
+        x, kx = self._field(orig_data, "orig_data")
+        call(f"ffcz_init_all_ptw_{self._sfx}", self._h, x, self._scalar(spatial_epsilon), self._scalar(delta_ptw))
+
+    def initialize_partial_ptw(self, orig_data, freq_indices, delta_ptw, spatial_epsilon, num_freq_constraints=None):
+        x, kx = self._field(orig_data, "orig_data")
+        idx, ki = as_pointer(np.asarray(freq_indices, dtype=np.uint64))
+        K = int(num_freq_constraints if num_freq_constraints is not None else len(ki))
+        call(f"ffcz_init_partial_ptw_{self._sfx}", self._h, x, idx, self._scalar(delta_ptw),
+             self._scalar(spatial_epsilon), K)
+
+    # -- solve* (projection_algorithm_impl.cuh:772-902) ---------------------
+    def solve(self, spatial_error, max_iterations, convergence_tolerance):
+        e, ke = self._field(spatial_error, "spatial_error")
+        call(f"ffcz_solve_{self._sfx}", self._h, e, int(max_iterations), self._scalar(convergence_tolerance))
+
+    def solve_all_abs(self, spatial_error, max_iterations, convergence_tolerance):
+        e, ke = self._field(spatial_error, "spatial_error")
+        call(f"ffcz_solve_all_abs_{self._sfx}", self._h, e, int(max_iterations), self._scalar(convergence_tolerance))
+
+    def solve_all_ptw(self, spatial_error, max_iterations, convergence_tolerance):
+        e, ke = self._field(spatial_error, "spatial_error")
+        call(f"ffcz_solve_all_ptw_{self._sfx}", self._h, e, int(max_iterations), self._scalar(convergence_tolerance))
+
+    def solve_loop(self, mode: int, spatial_error, max_iterations, convergence_tolerance):
+        """The projection loop without the trailing extraction."""
+        e, ke = self._field(spatial_error, "spatial_error")
+        call("ffcz_solve_loop", self._h, int(mode), e, int(max_iterations), float(convergence_tolerance))
+
+    def extract_compact_representation(self):
+        call("ffcz_extract", self._h)
+
+    def reset_edit_accumulators(self):
+        call("ffcz_reset_edit_accumulators", self._h)
+
+    # -- status ------------------------------------------------------------
+    @property
+    def iteration_count(self) -> int:
+        return self.status()[0]
+
+    @property
+    def convergence_metric(self) -> float:
+        return self.status()[1]
+
+    def status(self):
+        it, v, c = C.c_int(), C.c_double(), C.c_int()
+        call("ffcz_get_status", self._h, C.byref(it), C.byref(v), C.byref(c))
+        return it.value, v.value, bool(c.value)
+
+    # -- getters (projection_algorithm_impl.cuh:458-595, 904-910) ----------
+    def _get(self, name, dtype, out=None):
+        n = C.c_size_t()
+        call(name, self._h, None, 0, C.byref(n))
+        if out is None:
+            out = np.empty(n.value, dtype=dtype)
+        p, keep = as_pointer(out)
+        call(name, self._h, p, n.value, C.byref(n))
+        return out
+
+    def get_results(self, out=None):
+        return self._get("ffcz_get_results", self.dtype, out)
+
+    def get_freq_flags(self):
+        return self._get("ffcz_get_freq_flags", np.uint8).astype(bool)
+
+    def get_spatial_flags(self):
+        return self._get("ffcz_get_spatial_flags", np.uint8).astype(bool)
+
+    def get_freq_flags_pack(self):
+        return self._get("ffcz_get_freq_flags_pack", np.uint8)
+
+    def get_spatial_flags_pack(self):
+        return self._get("ffcz_get_spatial_flags_pack", np.uint8)
+
+    def get_active_freq_quant_edit_distances(self):
+        return self._get("ffcz_get_active_freq_quant", np.uint16)
+
+    def get_active_spatial_quant_edit_distances(self):
+        return self._get("ffcz_get_active_spatial_quant", np.uint16)
+
+    def get_active_freq_edit_distances(self):
+        return self._get("ffcz_get_active_freq_edits", self.dtype)
+
+    def get_active_spatial_edit_distances(self):
+        return self._get("ffcz_get_active_spatial_edits", self.dtype)
+
+    def get_freq_edits_full(self, out=None):
+        return self._get("ffcz_get_freq_edits_full", self.dtype, out)
+
+    def get_spatial_edits_full(self, out=None):
+        return self._get("ffcz_get_spatial_edits_full", self.dtype, out)
+
+    def _range(self):
+        v = [C.c_double() for _ in range(4)]
+        call("ffcz_get_edit_range", self._h, *[C.byref(x) for x in v])
+        return [self.dtype.type(x.value) for x in v]
+
+    def get_min_freq_edit(self):
+        return self._range()[0]
+
+    def get_max_freq_edit(self):
+        return self._range()[1]
+
+    def get_min_spatial_edit(self):
+        return self._range()[2]
+
+    def get_max_spatial_edit(self):
+        return self._range()[3]
+
+    # -- additions: certified output ----------------------------------------
+    def set_margins(self, spatial: float, freq: float):
+        call("ffcz_set_margins", self._h, float(spatial), float(freq))
+
+    def set_quantizer(self, kind: int, freq_step: float = 0.0, spatial_step: float = 0.0):
+        call("ffcz_set_quantizer", self._h, int(kind), float(freq_step), float(spatial_step))
+
+    def certify(self, quantized: bool) -> CertResult:
+        r = _lib.Cert()
+        call("ffcz_certify", self._h, int(bool(quantized)), C.byref(r))
+        return CertResult(bool(r.ok), r.spatial_violations, r.freq_violations, r.spatial_max_excess,
+                          r.freq_max_excess)
+
+    def stream_handle(self) -> int:
+        s = C.c_void_p()
+        call("ffcz_get_stream", self._h, C.byref(s))
+        return s.value or 0
+
+    def time_iteration(self, reps: int = 10):
+        ms = (C.c_float * 5)()
+        call("ffcz_time_iteration", self._h, int(reps), ms)
+        return list(ms)
+
+
+def _is_tensor(a) -> bool:
+    try:
+        import torch
+
+        return isinstance(a, torch.Tensor)
+    except ImportError:  # pragma: no cover
+        return False
+
+
+__all__ = ["ProjectionSolver", "CertResult", "as_pointer", "MODE_ABS", "MODE_DIRECT", "MODE_PTW",
+           "QUANT_REF16", "QUANT_STEP"]

@@ -1,0 +1,185 @@
+/*
+ * ffcz.h - C ABI of the MI355X FFCz alternating-projection post-processor.
+ *
+ * Plain C: opaque handles, pointers, sizes and int status codes; no HIP or
+ * torch types.  Data pointers may be host or device memory (the library uses
+ * unified addressing to tell them apart) unless stated otherwise.
+ *
+ * Each entry point names the reference interface it replaces
+ * (FastFourierCorrectionz/FFCz, file:line).  The C++ drop-in class
+ * ProjectionSolver<T> in projection_algorithm.hpp is written over these.
+ *
+ * Errors: every function returns FFCZ_OK (0) or a status code; the message of
+ * the last failure on the calling thread is ffcz_last_error().  Nothing calls
+ * exit() (the reference exits the process on CUDA/cuFFT errors,
+ * projection_algorithm.cuh:11-31).
+ */
+#ifndef FFCZ_H
+#define FFCZ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define FFCZ_OK 0
+#define FFCZ_ERR_INVALID 1
+#define FFCZ_ERR_STATE 2
+#define FFCZ_ERR_HIP 3
+#define FFCZ_ERR_RANGE 4
+#define FFCZ_ERR_ALLOC 5
+#define FFCZ_ERR_ZSTD 6
+#define FFCZ_ERR_CAPACITY 8
+
+/* element types (the reference instantiates float and double,
+ * projection_algorithm.cuh:39-53) */
+#define FFCZ_F32 0
+#define FFCZ_F64 1
+
+/* solve modes */
+#define FFCZ_MODE_DIRECT 0 /* solve()         projection_algorithm_impl.cuh:772-814 */
+#define FFCZ_MODE_ABS 1    /* solve_all_abs() projection_algorithm_impl.cuh:816-858 */
+#define FFCZ_MODE_PTW 2    /* solve_all_ptw() projection_algorithm_impl.cuh:860-902 */
+
+/* edit quantisers */
+#define FFCZ_QUANT_REF16 1 /* projection_algorithm.cu:447-455 (default) */
+#define FFCZ_QUANT_STEP 2  /* round(v/step), zigzag 16-bit symbols */
+
+typedef struct ffcz_solver ffcz_solver;
+typedef struct ffcz_coded ffcz_coded;
+
+typedef struct ffcz_cert {
+    unsigned long long spatial_violations; /* elements with |x_hat - x| > eps  */
+    unsigned long long freq_violations;    /* bins with |Re| or |Im| of F(x_hat)-F(x) > delta */
+    double spatial_max_excess;
+    double freq_max_excess;
+    int ok; /* 1 if both bounds hold everywhere (FP64) */
+} ffcz_cert;
+
+const char* ffcz_last_error(void);
+int ffcz_abi_version(void);
+
+/* ---- solver lifecycle ----------------------------------------------------
+ * ProjectionSolver(Nx,Ny,Nz)  projection_algorithm_impl.cuh:4-18 (+ :27-61, :109-122)
+ * ~ProjectionSolver()         projection_algorithm_impl.cuh:20-25            */
+int ffcz_create(int dtype, size_t nx, size_t ny, size_t nz, ffcz_solver** out);
+int ffcz_destroy(ffcz_solver* h);
+
+/* initialize_all_abs   projection_algorithm_impl.cuh:639-668 */
+int ffcz_init_all_abs_f32(ffcz_solver* h, const float* x, float eps, float delta);
+int ffcz_init_all_abs_f64(ffcz_solver* h, const double* x, double eps, double delta);
+/* initialize           projection_algorithm_impl.cuh:598-637 */
+int ffcz_init_direct_f32(ffcz_solver* h, const float* x, const size_t* idx, const float* delta_k, float eps, size_t K);
+int ffcz_init_direct_f64(ffcz_solver* h, const double* x, const size_t* idx, const double* delta_k, double eps,
+                         size_t K);
+/* initialize_all_ptw   projection_algorithm_impl.cuh:670-699 */
+int ffcz_init_all_ptw_f32(ffcz_solver* h, const float* x, float eps, float delta_ptw);
+int ffcz_init_all_ptw_f64(ffcz_solver* h, const double* x, double eps, double delta_ptw);
+/* initialize_partial_ptw projection_algorithm_impl.cuh:701-738 */
+int ffcz_init_partial_ptw_f32(ffcz_solver* h, const float* x, const size_t* idx, float delta_ptw, float eps, size_t K);
+int ffcz_init_partial_ptw_f64(ffcz_solver* h, const double* x, const size_t* idx, double delta_ptw, double eps,
+                              size_t K);
+
+/* solve / solve_all_abs / solve_all_ptw: projection loop + extraction
+ * projection_algorithm_impl.cuh:772-902.  e0 = x' - x (read only). */
+int ffcz_solve_f32(ffcz_solver* h, const float* e0, int max_iterations, float tol);
+int ffcz_solve_f64(ffcz_solver* h, const double* e0, int max_iterations, double tol);
+int ffcz_solve_all_abs_f32(ffcz_solver* h, const float* e0, int max_iterations, float tol);
+int ffcz_solve_all_abs_f64(ffcz_solver* h, const double* e0, int max_iterations, double tol);
+int ffcz_solve_all_ptw_f32(ffcz_solver* h, const float* e0, int max_iterations, float tol);
+int ffcz_solve_all_ptw_f64(ffcz_solver* h, const double* e0, int max_iterations, double tol);
+/* the loop alone (impl:791-808 / 835-852 / 879-896); e0 has the solver dtype */
+int ffcz_solve_loop(ffcz_solver* h, int mode, const void* e0, int max_iterations, double tol);
+/* iteration count and last violation: private results_ in the reference
+ * (projection_algorithm.cuh:102-103), printed at impl:810/854/898 */
+int ffcz_get_status(ffcz_solver* h, int* iterations, double* violation, int* converged);
+
+/* extract_compact_representation  projection_algorithm_impl.cuh:371-456 */
+int ffcz_extract(ffcz_solver* h);
+/* reset_edit_accumulators  projection_algorithm_impl.cuh:133-147 */
+int ffcz_reset_edit_accumulators(ffcz_solver* h);
+
+/* ---- getters (two-call size pattern: out=NULL returns the length) --------
+ * get_results                  impl:904-910  (N elements of the solver dtype) */
+int ffcz_get_results(ffcz_solver* h, void* e_out, size_t cap, size_t* len);
+/* get_freq_flags / get_spatial_flags            impl:459-485 (one byte per flag) */
+int ffcz_get_freq_flags(ffcz_solver* h, uint8_t* out, size_t cap, size_t* len);
+int ffcz_get_spatial_flags(ffcz_solver* h, uint8_t* out, size_t cap, size_t* len);
+/* get_freq_flags_pack / get_spatial_flags_pack  impl:487-505 (LSB-first) */
+int ffcz_get_freq_flags_pack(ffcz_solver* h, uint8_t* out, size_t cap, size_t* len);
+int ffcz_get_spatial_flags_pack(ffcz_solver* h, uint8_t* out, size_t cap, size_t* len);
+/* get_active_*_quant_edit_distances             impl:507-529 */
+int ffcz_get_active_freq_quant(ffcz_solver* h, uint16_t* out, size_t cap, size_t* len);
+int ffcz_get_active_spatial_quant(ffcz_solver* h, uint16_t* out, size_t cap, size_t* len);
+/* get_active_*_edit_distances                   impl:531-553 (solver dtype) */
+int ffcz_get_active_freq_edits(ffcz_solver* h, void* out, size_t cap, size_t* len);
+int ffcz_get_active_spatial_edits(ffcz_solver* h, void* out, size_t cap, size_t* len);
+/* get_*_edits_full                              impl:555-571 (solver dtype) */
+int ffcz_get_freq_edits_full(ffcz_solver* h, void* out, size_t cap, size_t* len);
+int ffcz_get_spatial_edits_full(ffcz_solver* h, void* out, size_t cap, size_t* len);
+/* get_{min,max}_{freq,spatial}_edit             impl:573-595 */
+int ffcz_get_edit_range(ffcz_solver* h, double* fmin, double* fmax, double* smin, double* smax);
+
+/* ---- additions (no counterpart in the reference) --------------------------
+ * loop bounds shrunk by (1-margin) for certified output; quantiser choice;
+ * FP64 certification of x_hat = x' + edits (quantized=1: dequantised edits). */
+int ffcz_set_margins(ffcz_solver* h, double spatial_margin, double freq_margin);
+int ffcz_set_quantizer(ffcz_solver* h, int kind, double freq_step, double spatial_step);
+int ffcz_certify(ffcz_solver* h, int quantized, ffcz_cert* out);
+/* the solver's HIP stream (hipStream_t) */
+int ffcz_get_stream(ffcz_solver* h, void** stream);
+/* times `reps` projection iterations kernel by kernel with HIP events on the
+ * solver stream (accumulators keep changing; call reset afterwards).  ms[5] =
+ * mean ms per launch of: z forward, y forward, x fused, y inverse, z inverse */
+int ffcz_time_iteration(ffcz_solver* h, int reps, float* ms5);
+
+/* ---- entropy coding: SimpleCudaHuffmanZstdCompressor<T>
+ * compress    HuffmanZSTDCoder.cuh:248-376 (symbols: host or device memory)
+ * decompress  HuffmanZSTDCoder.cuh:378-450                                   */
+int ffcz_huffman_zstd_compress_u16(const uint16_t* data, size_t n, int zstd_level, ffcz_coded** out);
+int ffcz_huffman_zstd_compress_u8(const uint8_t* data, size_t n, int zstd_level, ffcz_coded** out);
+/* Huffman only (no ZSTD): the pre-ZSTD bitstream of the same encoder */
+int ffcz_huffman_encode_u16(const uint16_t* data, size_t n, ffcz_coded** out);
+int ffcz_huffman_encode_u8(const uint8_t* data, size_t n, ffcz_coded** out);
+int ffcz_coded_info(const ffcz_coded* c, size_t* compressed_bytes, size_t* bitstream_bytes, size_t* num_elements,
+                    size_t* num_symbols, unsigned long long* num_bits);
+int ffcz_coded_data(const ffcz_coded* c, uint8_t* out, size_t cap);      /* ZSTD output (or bitstream) */
+int ffcz_coded_bitstream(const ffcz_coded* c, uint8_t* out, size_t cap); /* pre-ZSTD bytes */
+/* code table sorted by symbol: CompressionResult::code_table, HuffmanZSTDCoder.cuh:243 */
+int ffcz_coded_table(const ffcz_coded* c, uint32_t* symbols, uint32_t* codes, int32_t* lengths, size_t cap);
+void ffcz_coded_free(ffcz_coded* c);
+int ffcz_huffman_zstd_decompress_u16(const uint8_t* comp, size_t comp_len, const uint32_t* symbols,
+                                     const uint32_t* codes, const int32_t* lengths, size_t nsym,
+                                     size_t num_elements, size_t bitstream_bytes, uint16_t* out, size_t* out_len);
+int ffcz_huffman_zstd_decompress_u8(const uint8_t* comp, size_t comp_len, const uint32_t* symbols,
+                                    const uint32_t* codes, const int32_t* lengths, size_t nsym, size_t num_elements,
+                                    size_t bitstream_bytes, uint8_t* out, size_t* out_len);
+
+/* ---- per-kernel entries on device pointers (stream may be NULL) ---------
+ * 3D transforms with cuFFT's layout (compact half spectrum, interleaved
+ * complex, unnormalised): cufftExecR2C/D2Z and C2R/Z2D at
+ * projection_algorithm_impl.cuh:114-120, 155-161, 181-187 */
+int ffcz_fft_r2c(int dtype, size_t nx, size_t ny, size_t nz, const void* in, void* out_compact, void* stream);
+int ffcz_fft_c2r(int dtype, size_t nx, size_t ny, size_t nz, const void* in_compact, void* out, void* stream);
+/* project_spatial_constraints_direct   projection_algorithm.cu:335-361 */
+int ffcz_k_project_spatial(int dtype, void* e, double eps, void* ds, size_t n, void* stream);
+/* project_frequency_constraints_all_abs projection_algorithm.cu:147-188 (compact spectrum, F bins) */
+int ffcz_k_project_freq_abs(int dtype, void* spec, double delta, void* df, size_t F, void* stream);
+/* compute_convergence_metric + host max  projection_algorithm.cu:574-613, impl:741-770 */
+int ffcz_k_convergence(int dtype, const void* e, double eps, size_t n, double* out);
+/* quantize_and_dequantize              projection_algorithm.cu:615-644 */
+int ffcz_k_quantize_dequantize(int dtype, const void* v, void* out, double vmin, double vmax, size_t n,
+                               double threshold, void* stream);
+/* extract_active_flags + compact_and_quantize_active_elements + pack_bools
+ * projection_algorithm.cu:363-550 on an arbitrary device array (kind: FFCZ_QUANT_*) */
+int ffcz_k_extract(int dtype, const void* v, size_t n, int kind, double step, uint8_t* pack, void* compact,
+                   uint16_t* quant, size_t* count, double* vmin, double* vmax, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FFCZ_H */

@@ -1,0 +1,123 @@
+"""Extraction (flags, min/max, compaction, quantisation, packing) and FP64
+certification on the GPU, bit-exact against the oracle on the same
+accumulators (projection_algorithm_impl.cuh:371-456,
+projection_algorithm.cu:363-550, 615-644)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+import ffcz_amd
+from ffcz_amd import ProjectionSolver, QUANT_STEP
+from ffcz_amd._lib import call
+from oracle import ffcz_oracle as O
+from tests._cases import binding_case
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("shape", [(1, 16, 16), (6, 10, 12), (16, 20, 30), (1, 64, 64)])
+def test_extraction_bit_exact(gpu, shape, dtype):
+    x, e0, eps, delta = binding_case(shape, seed=7, dtype=dtype)
+    with ProjectionSolver(*shape, dtype=dtype) as s:
+        s.initialize_all_abs(x, eps, delta)
+        s.solve_all_abs(e0, 20, 1e-7)
+        df = s.get_freq_edits_full()
+        ds = s.get_spatial_edits_full()
+        for edits, flags, pack, quant, comp, (lo, hi) in (
+            (df, s.get_freq_flags(), s.get_freq_flags_pack(), s.get_active_freq_quant_edit_distances(),
+             s.get_active_freq_edit_distances(), (s.get_min_freq_edit(), s.get_max_freq_edit())),
+            (ds, s.get_spatial_flags(), s.get_spatial_flags_pack(), s.get_active_spatial_quant_edit_distances(),
+             s.get_active_spatial_edit_distances(), (s.get_min_spatial_edit(), s.get_max_spatial_edit())),
+        ):
+            ref = O.extract_one(edits)
+            assert np.array_equal(flags, ref.flags)
+            assert np.array_equal(pack, ref.pack)
+            assert np.array_equal(comp, ref.compact)
+            assert lo == ref.vmin and hi == ref.vmax
+            assert np.array_equal(quant, ref.quant)
+            assert ref.flags.any()
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 4095, 4096, 4097, 100_000, 1_000_003])
+def test_kernel_extract_sizes(gpu, n):
+    rng = np.random.default_rng(n)
+    v = rng.standard_normal(n).astype(np.float32) * 1e-3
+    v[rng.random(n) < 0.7] = 0.0
+    v[: min(3, n)] = [1e-8, -2e-7, 5e-8][: min(3, n)]  # around the 1e-7 threshold
+    dv = torch.from_numpy(v).to(gpu)
+    pack = torch.zeros(((n + 63) // 64) * 8, dtype=torch.uint8, device=gpu)
+    comp = torch.zeros(n, dtype=torch.float32, device=gpu)
+    quant = torch.zeros(n, dtype=torch.int16, device=gpu)
+    cnt, lo, hi = C.c_size_t(), C.c_double(), C.c_double()
+    torch.cuda.synchronize()
+    call("ffcz_k_extract", 0, dv.data_ptr(), n, 1, 0.0, pack.data_ptr(), comp.data_ptr(), quant.data_ptr(),
+         C.byref(cnt), C.byref(lo), C.byref(hi), None)
+    ref = O.extract_one(v)
+    k = cnt.value
+    assert k == ref.compact.size
+    assert np.array_equal(pack.cpu().numpy()[: (n + 7) // 8], ref.pack)
+    assert np.array_equal(comp.cpu().numpy()[:k], ref.compact)
+    assert np.array_equal(quant.cpu().numpy().view(np.uint16)[:k], ref.quant)
+    assert lo.value == ref.vmin and hi.value == ref.vmax
+
+
+def test_step_quantizer_and_certification(gpu):
+    shape = (16, 20, 30)
+    x, e0, eps, delta = binding_case(shape, seed=8)
+    with ProjectionSolver(*shape, dtype=np.float64) as s:
+        s.initialize_all_abs(x, eps, delta)
+        s.set_margins(0.125, 0.125)
+        s.set_quantizer(QUANT_STEP, delta / 8, eps / 8)
+        s.solve_all_abs(e0, 200, 1e-12)
+        c1 = s.certify(False)
+        c2 = s.certify(True)
+        df = s.get_freq_edits_full()
+        ds = s.get_spatial_edits_full()
+        q = s.get_active_spatial_quant_edit_distances()
+        rq = O.quantize_step(ds, eps / 8)
+        rq = rq[rq != 0]
+        assert np.array_equal(q, O.zigzag(rq).astype(np.uint16))
+        # oracle certification of the same edits
+        d1 = O.reconstruct_edit(shape, e0, df, ds)
+        r1 = O.certify(d1, eps, delta)
+        assert c1.ok == r1["ok"]
+        assert c1.spatial_violations == r1["spatial_violations"]
+        assert c1.freq_violations == r1["freq_violations"]
+        dq = O.reconstruct_edit(shape, e0, O.dequantize_step(O.quantize_step(df, delta / 8), delta / 8),
+                                O.dequantize_step(O.quantize_step(ds, eps / 8), eps / 8))
+        r2 = O.certify(dq, eps, delta)
+        assert c2.ok == r2["ok"]
+        assert c2.spatial_violations == r2["spatial_violations"]
+        assert c2.freq_violations == r2["freq_violations"]
+        assert c1.ok and c2.ok
+
+
+def test_certification_detects_violation(gpu):
+    # no iterations: e0 violates the spatial bound, certification must say so
+    shape = (6, 10, 12)
+    x, e0, eps, delta = binding_case(shape, seed=1)
+    with ProjectionSolver(*shape, dtype=np.float64) as s:
+        s.initialize_all_abs(x, eps, delta)
+        s.solve_all_abs(e0, 0, 1e-7)
+        c = s.certify(False)
+        r = O.certify(O.reconstruct_edit(shape, e0, np.zeros(2 * s.F), np.zeros(s.N)), eps, delta)
+        assert not c.ok
+        assert c.spatial_violations == r["spatial_violations"] > 0
+        assert c.freq_violations == r["freq_violations"] > 0
+        assert abs(c.spatial_max_excess - r["spatial_max_excess"]) < 1e-15
+
+
+def test_quantize_dequantize_kernel(gpu):
+    rng = np.random.default_rng(0)
+    v = (rng.standard_normal(10000) * 1e-3).astype(np.float32)
+    v[::7] = 0
+    lo, hi = float(v.min()), float(v.max())
+    dv = torch.from_numpy(v).to(gpu)
+    out = torch.empty_like(dv)
+    torch.cuda.synchronize()
+    call("ffcz_k_quantize_dequantize", 0, dv.data_ptr(), out.data_ptr(), lo, hi, v.size, 1e-7, None)
+    ref = O.quantize_and_dequantize_ref(v, np.float32(lo), np.float32(hi))
+    assert np.array_equal(out.cpu().numpy(), ref)
