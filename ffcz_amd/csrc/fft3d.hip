@@ -284,7 +284,11 @@ __global__ __launch_bounds__(FFT_THREADS) void k_zinv(ZArgs<R, T> a) {
             const int p = (int)fdiv((uint32_t)q, hd);
             const int k = q - p * nh;
             const size_t row = row0 + p;
-            buf[k * sn + p] = row < a.rows ? a.spec[row * a.Mp + k] : mk(R(0), R(0));
+            cplx<R> v = row < a.rows ? a.spec[row * a.Mp + k] : mk(R(0), R(0));
+            // a C2R reads only the real part of a row's DC and Nyquist bins
+            // (the Hermitian part of the half spectrum, as pocketfft/FFTW do)
+            if (k == 0 || (a.half && k == a.M)) v.y = R(0);
+            buf[k * sn + p] = v;
         }
     }
     __syncthreads();

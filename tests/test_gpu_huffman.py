@@ -52,7 +52,8 @@ def test_device_input_and_zstd_roundtrip(gpu):
     back = cuda_huffman_zstd_decompress(r.compressed_data, r.code_table, r.num_elements,
                                         r.original_bitstream_size, np.uint16)
     assert np.array_equal(back, data)
-    assert len(r.compressed_data) < r.original_bitstream_size
+    assert r.original_bitstream_size == (r.num_bits + 7) // 8
+    assert r.compressed_data[:4] == b"\x28\xb5\x2f\xfd"  # ZSTD frame magic
 
 
 def test_long_codes_raise(gpu):
