@@ -34,6 +34,8 @@ def parse():
     p.add_argument("--timing-reps", type=int, default=10)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample-iters", type=int, default=1)
+    p.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                   help="only run a plain device copy of BYTES (counter calibration) and exit")
     return p.parse_args()
 
 
@@ -107,6 +109,16 @@ def main():
         sys.exit(2)
     import torch
 
+    if a.copy_calibration:
+        n = a.copy_calibration // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n * 4, "launches": 3}))
+        return
+
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     import ffcz_amd
     from ffcz_amd import ProjectionSolver, MODE_ABS, QUANT_STEP
@@ -151,12 +163,12 @@ def main():
           file=sys.stderr, flush=True)
 
     # per-kernel timing of the iteration (HIP events on the solver stream)
-    ms = solver.time_iteration(a.timing_reps)
+    timing = solver.time_iteration(a.timing_reps)
+    ms, t_iter = timing[:5], timing[5] * 1e-3
     names = ["z_fwd", "y_fwd", "x_fused", "y_inv", "z_inv"]
     kb = kernel_bytes(shape, s_in)
     dom = int(np.argmax(ms))
     dname = names[dom]
-    t_iter = sum(ms) * 1e-3
     C = 2 * (shape[0] * shape[1] * (shape[2] // 2 + 1)) * s_in
     B_iter = 4 * N * s_in + 4 * C
     achieved = kb[dname] / (ms[dom] * 1e-3) / 1e9

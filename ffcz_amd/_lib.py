@@ -89,6 +89,8 @@ SIGNATURES = {
     "ffcz_set_margins": [_h, _d, _d],
     "ffcz_set_quantizer": [_h, _i, _d, _d],
     "ffcz_certify": [_h, _i, C.POINTER(Cert)],
+    "ffcz_set_tuning": [_i, C.c_long],
+    "ffcz_get_tuning": [C.POINTER(_i), C.POINTER(C.c_long)],
     "ffcz_get_stream": [_h, C.POINTER(_vp)],
     "ffcz_time_iteration": [_h, _i, C.POINTER(_f)],
     "ffcz_huffman_zstd_compress_u16": [_vp, _sz, _i, C.POINTER(_h)],
@@ -139,3 +141,12 @@ def check(status: int) -> None:
 
 def call(name: str, *args) -> None:
     check(getattr(load(), name)(*args))
+
+
+def set_tuning(force_generic: int = -1, slab_mb: int | None = None):
+    """Process-wide kernel tuning (see ffcz_set_tuning); returns the previous
+    (force_generic, slab_mb)."""
+    g, m = C.c_int(), C.c_long()
+    call("ffcz_get_tuning", C.byref(g), C.byref(m))
+    call("ffcz_set_tuning", int(force_generic), int(m.value if slab_mb is None else slab_mb))
+    return g.value, m.value

@@ -8,6 +8,7 @@
 
 #include "../../include/ffcz.h"
 #include "extract.hpp"
+#include "fft_fast.hpp"
 #include "huffman.hpp"
 #include "kernels.hpp"
 #include "solver.hpp"
@@ -372,6 +373,18 @@ int ffcz_certify(ffcz_solver* h, int quantized, ffcz_cert* out) {
             out->freq_max_excess = r.freq_max_excess;
             out->ok = r.ok;
         });
+    });
+}
+int ffcz_set_tuning(int force_generic, long slab_mb) {
+    return guard([&] {
+        if (force_generic >= 0) tuning().force_generic = force_generic ? 1 : 0;
+        tuning().slab_mb = slab_mb;
+    });
+}
+int ffcz_get_tuning(int* force_generic, long* slab_mb) {
+    return guard([&] {
+        if (force_generic) *force_generic = tuning().force_generic.load();
+        if (slab_mb) *slab_mb = tuning().slab_mb.load();
     });
 }
 int ffcz_get_stream(ffcz_solver* h, void** stream) {

@@ -8,7 +8,7 @@
 
 namespace ffcz {
 
-constexpr int EXTRACT_TILE = 4096;     // elements per workgroup tile (16 per thread)
+constexpr int EXTRACT_TILE = 8192;     // elements per workgroup tile (8 per thread x 4 rounds)
 
 struct ExtractResult {
     size_t count = 0;

@@ -1,10 +1,13 @@
 // 3D R2C/C2R passes, the fused projection iteration and the FP64
 // certification transform.  See fft3d.hpp for the layouts.
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "fft3d.hpp"
-#include "quant.hpp"
+#include "fft_kernels_common.hpp"
+#include "fft_fast.hpp"
 
 namespace ffcz {
 
@@ -105,69 +108,9 @@ void Fft3d<R>::destroy() {
 // ===========================================================================
 // Device helpers
 // ===========================================================================
-template <typename R>
-__device__ __forceinline__ void clamp_acc(R& v, R d, R* acc) {
-    // projection_algorithm.cu:161-172 / :351-360: the edit is formed first
-    // and then accumulated (a single writer per address).
-    if (v > d) {
-        *acc += d - v;
-        v = d;
-    } else if (v < -d) {
-        *acc += -d - v;
-        v = -d;
-    }
-}
-
-template <typename R>
-__device__ __forceinline__ void block_max_atomic(R v, unsigned long long* target) {
-    __shared__ R red[FFT_THREADS / 64];
-    v = wave_max(v);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) red[w] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        R m = red[0];
-        for (int i = 1; i < FFT_THREADS / 64; ++i) m = red[i] > m ? red[i] : m;
-        if (m > R(0)) atomicMax(target, ordbits(m));
-    }
-}
-
-__device__ __forceinline__ void block_count_atomic(unsigned long long c, unsigned long long* target) {
-    c = wave_sum(c);
-    if ((threadIdx.x & 63) == 0 && c) atomicAdd(target, c);
-}
-
 // ===========================================================================
 // z pass kernels (contiguous rows)
 // ===========================================================================
-enum ZMode : int { Z_PLAIN = 0, Z_ITER = 1, Z_CERT = 2 };
-
-template <typename R, typename T>
-struct ZArgs {
-    int nz, Lz, M, half, P, sn;
-    FastDiv dP, dM, dNh, dNp, dNz, dFill;  // P, M, nz/2+1, M/2+1, nz, nz-(nz/2+1)
-    size_t rows, Mp;
-    FftPlan pl;
-    const cplx<R>* tz;
-    const cplx<R>* tz2;
-    const R* in;          // Z_PLAIN forward input
-    R* out;               // Z_PLAIN inverse output
-    cplx<R>* spec;
-    R scale;              // inverse scale
-    // Z_ITER
-    R* e;
-    R* ds;
-    R eps;
-    DevState* st;
-    // Z_CERT
-    const T* e0;
-    const T* cds;
-    const double* dfi;
-    QuantDesc qs;
-    double ceps;
-    CertStats* stats;
-};
-
 template <typename R, typename T, int MODE>
 __global__ __launch_bounds__(FFT_THREADS) void k_zfwd(ZArgs<R, T> a) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -370,95 +313,15 @@ __global__ __launch_bounds__(FFT_THREADS) void k_zinv(ZArgs<R, T> a) {
         }
     }
     if constexpr (MODE == Z_ITER) {
-        // convergence metric (projection_algorithm.cu:586-612), reduced on the
-        // device; the last workgroup finalises the iteration
-        // (projection_algorithm_impl.cuh:841-848).
-        block_max_atomic(vmax, &a.st->viol_bits);
-        __shared__ bool last;
-        if (t == 0) {
-            __threadfence();
-            const unsigned int ticket = atomicAdd(&a.st->blocks_done, 1u);
-            last = (ticket == gridDim.x - 1);
-        }
-        __syncthreads();
-        if (last && t == 0) {
-            __threadfence();
-            DevState* st = a.st;
-            const unsigned long long b = atomicAdd(&st->viol_bits, 0ull);
-            const R v = from_ordbits<R>(b);
-            st->iter += 1;
-            st->last_viol = (double)v;
-            bool conv;
-            if constexpr (sizeof(R) == 4) conv = v < st->tol_f;
-            else conv = v < (R)st->tol;
-            if (conv) {
-                st->converged = 1;
-                st->done = 1;
-            } else if (st->iter >= st->max_iter) {
-                st->done = 1;
-            }
-            atomicExch(&st->viol_bits, 0ull);
-            atomicExch(&st->blocks_done, 0u);
-            __threadfence();
-        }
+        // convergence metric (projection_algorithm.cu:586-612): one partial
+        // per workgroup, reduced by k_finalize_iteration
+        block_max_store<R, FFT_THREADS>(vmax, &a.vpart[a.row_off / a.P + blockIdx.x]);
     }
 }
 
 // ===========================================================================
 // strided (y / x) pass kernels
 // ===========================================================================
-enum SMode : int { S_PLAIN = 0, S_CLAMP = 1, S_CERT_CHECK = 2, S_LOAD_DF = 3 };
-
-template <typename R, typename T>
-struct SArgs {
-    cplx<R>* data;
-    FftPlan pl;
-    const cplx<R>* tw;
-    int L, W, wshift, ncols, nct;
-    FastDiv dW, dNct;
-    size_t ob, es;
-    int sign;
-    DevState* st;
-    // clamp / check / load: bin index = n*cs_n + o*cs_o + col (reference layout)
-    size_t F, cs_n, cs_o;
-    int fmode;
-    R delta;
-    const R* delta_dense;
-    R dscale;
-    R* df;
-    const cplx<R>* orig;
-    R delta_ptw;
-    // certification
-    double cdelta;
-    const T* cdelta_dense;
-    CertStats* stats;
-    const T* dfsrc;
-    QuantDesc qf;
-};
-
-template <typename R>
-__device__ __forceinline__ void ptw_clamp(cplx<R>& v, size_t idx, size_t F, const cplx<R>* orig, R dptw, R* df) {
-    // projection_algorithm.cu:199-258
-    if (idx == 0) {
-        const R lo = orig[0].x * (R(1) / sqrt(R(1) + dptw) - R(1));
-        const R hi = orig[0].x * (R(1) / sqrt(R(1) - dptw) - R(1));
-        if (v.x > hi || v.x < lo) {
-            df[0] += -v.x;
-            v.x = R(0);
-        }
-        return;
-    }
-    const cplx<R> o = orig[idx];
-    const R omag = o.x * o.x + o.y * o.y;
-    const R cr = o.x + v.x, ci = o.y + v.y;
-    const R cmag = cr * cr + ci * ci;
-    const R ratio = cmag / omag;
-    if (ratio <= sqrt(R(1) + dptw) && ratio >= sqrt(R(1) - dptw)) return;
-    const R d = fmin(R(1) - sqrt(R(1) - dptw), sqrt(R(1) + dptw) - R(1)) / (R)sqrtf(2.0f);
-    clamp_acc(v.x, d, &df[idx]);
-    clamp_acc(v.y, d, &df[idx + F]);
-}
-
 template <typename R, typename T, int MODE>
 __global__ __launch_bounds__(FFT_THREADS) void k_strided(SArgs<R, T> a) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -490,23 +353,16 @@ __global__ __launch_bounds__(FFT_THREADS) void k_strided(SArgs<R, T> a) {
     const int sign = (MODE == S_CLAMP || MODE == S_CERT_CHECK) ? -1 : a.sign;
     tile_fft(buf, tw, a.pl, W, a.dW, 1, W, true, sign);
     if constexpr (MODE == S_CLAMP) {
+        bool fired = false;
         for (int q = t; q < L * W; q += FFT_THREADS) {
             const int n = q >> ws, c = q & (W - 1);
             if (c0 + c >= a.ncols) continue;
             const size_t idx = (size_t)n * a.cs_n + o * a.cs_o + c0 + c;
             cplx<R> v = buf[q];
-            if (a.fmode == FM_ABS) {
-                clamp_acc(v.x, a.delta, &a.df[idx]);
-                clamp_acc(v.y, a.delta, &a.df[idx + a.F]);
-            } else if (a.fmode == FM_DENSE) {
-                const R d = a.delta_dense[idx] * a.dscale;
-                clamp_acc(v.x, d, &a.df[idx]);
-                clamp_acc(v.y, d, &a.df[idx + a.F]);
-            } else if (a.fmode == FM_PTW) {
-                ptw_clamp(v, idx, a.F, a.orig, a.delta_ptw, a.df);
-            }
+            fired |= freq_clamp_bin(v, idx, a);
             buf[q] = v;
         }
+        if (fired) a.st->freq_clamped = 1u;
         __syncthreads();
         tile_fft(buf, tw, a.pl, W, a.dW, 1, W, true, +1);
     }
@@ -539,10 +395,29 @@ __global__ __launch_bounds__(FFT_THREADS) void k_strided(SArgs<R, T> a) {
 // ===========================================================================
 // small kernels
 // ===========================================================================
+// Reduces the inverse z pass's per-workgroup violations and finalises the
+// iteration (projection_algorithm_impl.cuh:741-770 and 841-848, on the device).
+template <typename R>
+__global__ __launch_bounds__(1024) void k_finalize_iteration(const R* part, size_t n, DevState* st) {
+    if (st->done) return;
+    R m = R(0);
+    for (size_t i = threadIdx.x; i < n; i += 1024) m = part[i] > m ? part[i] : m;
+    __shared__ R red[16];
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < 16; ++i) m = red[i] > m ? red[i] : m;
+        st->viol_bits = ordbits(red[0] > m ? red[0] : m);
+        finalize_iteration<R>(st);
+    }
+}
+
 __global__ void k_reset_state(DevState* st, unsigned int max_iter, double tol) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         st->done = max_iter == 0 ? 1u : 0u;
         st->converged = 0;
+        st->freq_clamped = 0;
         st->iter = 0;
         st->max_iter = max_iter;
         st->blocks_done = 0;
@@ -632,6 +507,19 @@ static size_t zlds(const Fft3d<R>& f) {
 template <typename R>
 static unsigned zgrid(const Fft3d<R>& f) { return (unsigned)ceil_div(f.rows, (size_t)f.Pz); }
 
+template <typename R, typename T, int MODE>
+static void launch_zfwd(const Fft3d<R>& f, const ZArgs<R, T>& z, hipStream_t s) {
+    if (!(fast_fft_enabled() && fast_zfwd<R, T>(MODE, z, s)))
+        hipLaunchKernelGGL((k_zfwd<R, T, MODE>), dim3(zgrid(f)), dim3(FFT_THREADS), zlds(f), s, z);
+    FFCZ_HIP(hipGetLastError());
+}
+template <typename R, typename T, int MODE>
+static void launch_zinv(const Fft3d<R>& f, const ZArgs<R, T>& z, hipStream_t s) {
+    if (!(fast_fft_enabled() && fast_zinv<R, T>(MODE, z, s)))
+        hipLaunchKernelGGL((k_zinv<R, T, MODE>), dim3(zgrid(f)), dim3(FFT_THREADS), zlds(f), s, z);
+    FFCZ_HIP(hipGetLastError());
+}
+
 static int ilog2(int w) {
     int s = 0;
     while ((1 << s) < w) ++s;
@@ -668,8 +556,13 @@ static SArgs<R, T> sargs(const Fft3d<R>& f, cplx<R>* data, bool xaxis) {
     return a;
 }
 template <typename R, typename T>
-static void launch_strided(const Fft3d<R>& f, const SArgs<R, T>& a, int mode, bool xaxis, hipStream_t s) {
-    const size_t outer = xaxis ? f.ny : f.nx;
+static void launch_strided(const Fft3d<R>& f, const SArgs<R, T>& a, int mode, bool xaxis, hipStream_t s,
+                           size_t outer_override = 0) {
+    const size_t outer = outer_override ? outer_override : (xaxis ? f.ny : f.nx);
+    if (fast_fft_enabled() && fast_strided<R, T>(mode, a, outer, s)) {
+        FFCZ_HIP(hipGetLastError());
+        return;
+    }
     const unsigned grid = (unsigned)(outer * (size_t)a.nct);
     const size_t lds = ((size_t)a.L + (size_t)a.L * a.W) * sizeof(cplx<R>);
     switch (mode) {
@@ -686,8 +579,7 @@ void fft_r2c(const Fft3d<R>& f, const R* in, cplx<R>* spec, hipStream_t s) {
     auto z = zargs<R, R>(f);
     z.in = in;
     z.spec = spec;
-    hipLaunchKernelGGL((k_zfwd<R, R, Z_PLAIN>), dim3(zgrid(f)), dim3(FFT_THREADS), zlds(f), s, z);
-    FFCZ_HIP(hipGetLastError());
+    launch_zfwd<R, R, Z_PLAIN>(f, z, s);
     auto y = sargs<R, R>(f, spec, false);
     y.sign = -1;
     launch_strided(f, y, S_PLAIN, false, s);
@@ -708,8 +600,7 @@ void fft_c2r(const Fft3d<R>& f, cplx<R>* spec, R* out, R scale, hipStream_t s) {
     z.spec = spec;
     z.out = out;
     z.scale = scale;
-    hipLaunchKernelGGL((k_zinv<R, R, Z_PLAIN>), dim3(zgrid(f)), dim3(FFT_THREADS), zlds(f), s, z);
-    FFCZ_HIP(hipGetLastError());
+    launch_zinv<R, R, Z_PLAIN>(f, z, s);
 }
 
 template <typename R>
@@ -729,28 +620,54 @@ void reset_state(DevState* st, int max_iter, double tol, hipStream_t s) {
     FFCZ_HIP(hipGetLastError());
 }
 
+// x-planes per slab: the forward z and y passes (and the inverse y and z
+// passes) run slab by slab so the half spectrum a z pass just wrote is still
+// in the 256 MB memory-side cache when the y pass reads it.  The slab row
+// count is a multiple of 512 so every z-pass workgroup maps to one partial.
 template <typename R>
-void iteration(const Fft3d<R>& f, const IterArgs<R>& ia, hipStream_t s, hipEvent_t* ev) {
-    // ev (optional): 6 events recorded around the five launches
-    auto mark = [&](int i) {
-        if (ev) FFCZ_HIP(hipEventRecord(ev[i], s));
+static size_t slab_planes(const Fft3d<R>& f) {
+    const long mb = tuning().slab_mb.load();
+    if (mb <= 0) return f.nx;
+    const size_t plane = f.ny * f.Mp * sizeof(cplx<R>);
+    size_t xs = std::max<size_t>(1, ((size_t)mb << 20) / plane);
+    while (xs < f.nx && (xs * f.ny) % 512 != 0) ++xs;
+    return xs >= f.nx ? f.nx : xs;
+}
+
+template <typename R>
+void iteration(const Fft3d<R>& f, const IterArgs<R>& ia, hipStream_t s, EventLog* log) {
+    auto mark = [&](int kind) {
+        if (log) log->mark(kind, s);
     };
-    auto z = zargs<R, R>(f);
-    z.e = ia.e;
-    z.ds = ia.ds;
-    z.eps = ia.eps;
-    z.st = ia.st;
-    z.spec = ia.spec;
-    z.scale = ia.norm;
-    mark(0);
-    hipLaunchKernelGGL((k_zfwd<R, R, Z_ITER>), dim3(zgrid(f)), dim3(FFT_THREADS), zlds(f), s, z);
-    FFCZ_HIP(hipGetLastError());
-    mark(1);
-    auto y = sargs<R, R>(f, ia.spec, false);
-    y.st = ia.st;
-    y.sign = -1;
-    launch_strided(f, y, S_PLAIN, false, s);
-    mark(2);
+    const size_t xs = slab_planes(f);
+    mark(-1);
+    auto zslab = [&](size_t x0, size_t x1) {
+        auto z = zargs<R, R>(f);
+        const size_t r0 = x0 * f.ny;
+        z.e = ia.e + r0 * f.nz;
+        z.ds = ia.ds + r0 * f.nz;
+        z.eps = ia.eps;
+        z.st = ia.st;
+        z.spec = ia.spec + r0 * f.Mp;
+        z.scale = ia.norm;
+        z.vpart = ia.vpart;
+        z.row_off = r0;
+        z.rows = (x1 - x0) * f.ny;
+        return z;
+    };
+    auto yslab = [&](size_t x0, size_t x1, int sign) {
+        auto y = sargs<R, R>(f, ia.spec + x0 * f.ny * f.Mp, false);
+        y.st = ia.st;
+        y.sign = sign;
+        launch_strided(f, y, S_PLAIN, false, s, x1 - x0);
+    };
+    for (size_t x0 = 0; x0 < f.nx; x0 += xs) {
+        const size_t x1 = std::min(f.nx, x0 + xs);
+        launch_zfwd<R, R, Z_ITER>(f, zslab(x0, x1), s);
+        mark(0);
+        yslab(x0, x1, -1);
+        mark(1);
+    }
     auto x = sargs<R, R>(f, ia.spec, true);
     x.st = ia.st;
     x.fmode = ia.fmode;
@@ -761,20 +678,59 @@ void iteration(const Fft3d<R>& f, const IterArgs<R>& ia, hipStream_t s, hipEvent
     x.orig = ia.orig;
     x.delta_ptw = ia.delta_ptw;
     launch_strided(f, x, ia.fmode == FM_NONE ? S_PLAIN : S_CLAMP, true, s);
-    mark(3);
-    auto y2 = sargs<R, R>(f, ia.spec, false);
-    y2.st = ia.st;
-    y2.sign = +1;
-    launch_strided(f, y2, S_PLAIN, false, s);
-    mark(4);
-    hipLaunchKernelGGL((k_zinv<R, R, Z_ITER>), dim3(zgrid(f)), dim3(FFT_THREADS), zlds(f), s, z);
+    mark(2);
+    for (size_t x0 = 0; x0 < f.nx; x0 += xs) {
+        const size_t x1 = std::min(f.nx, x0 + xs);
+        yslab(x0, x1, +1);
+        mark(3);
+        launch_zinv<R, R, Z_ITER>(f, zslab(x0, x1), s);
+        mark(4);
+    }
+    const int zr = fast_z_rows(f.Lz, f.half);
+    const size_t nparts = ceil_div(f.rows, (size_t)(zr ? zr : f.Pz));
+    hipLaunchKernelGGL((k_finalize_iteration<R>), dim3(1), dim3(1024), 0, s, (const R*)ia.vpart, nparts, ia.st);
     FFCZ_HIP(hipGetLastError());
     mark(5);
 }
 
 template <typename R>
+void iteration_kernel(const Fft3d<R>& f, const IterArgs<R>& ia, hipStream_t s, int kind) {
+    auto z = zargs<R, R>(f);
+    z.e = ia.e;
+    z.ds = ia.ds;
+    z.eps = ia.eps;
+    z.st = ia.st;
+    z.spec = ia.spec;
+    z.scale = ia.norm;
+    z.vpart = ia.vpart;
+    if (kind == 0) {
+        launch_zfwd<R, R, Z_ITER>(f, z, s);
+    } else if (kind == 1 || kind == 3) {
+        auto y = sargs<R, R>(f, ia.spec, false);
+        y.st = ia.st;
+        y.sign = kind == 1 ? -1 : +1;
+        launch_strided(f, y, S_PLAIN, false, s);
+    } else if (kind == 2) {
+        auto x = sargs<R, R>(f, ia.spec, true);
+        x.st = ia.st;
+        x.fmode = ia.fmode;
+        x.delta = ia.delta;
+        x.delta_dense = ia.delta_dense;
+        x.dscale = ia.dscale;
+        x.df = ia.df;
+        x.orig = ia.orig;
+        x.delta_ptw = ia.delta_ptw;
+        launch_strided(f, x, ia.fmode == FM_NONE ? S_PLAIN : S_CLAMP, true, s);
+    } else {
+        launch_zinv<R, R, Z_ITER>(f, z, s);
+    }
+}
+
+template <typename R>
 void epilogue_clamp(const Fft3d<R>& f, const IterArgs<R>& a, hipStream_t s) {
-    hipLaunchKernelGGL((k_epilogue_clamp<R>), dim3(grid_for(f.N)), dim3(256), 0, s, a.e, a.ds, a.eps, f.N,
+    // a small grid: the kernel usually returns at once (converged loop)
+    const unsigned g = (unsigned)std::min<size_t>(ceil_div(f.N, (size_t)256), 2048);
+    hipLaunchKernelGGL((k_epilogue_clamp<R>), dim3(g), dim3(256), 0, s, a.e, a.ds, a.eps, f.N,
                        (const DevState*)a.st);
     FFCZ_HIP(hipGetLastError());
 }
@@ -798,8 +754,7 @@ void certify(const Fft3d<double>& f, const CertArgs<T>& c, hipStream_t s) {
         z.spec = c.work_spec;
         z.out = c.work_real;
         z.scale = 1.0 / (double)f.N;
-        hipLaunchKernelGGL((k_zinv<double, T, Z_PLAIN>), dim3(zgrid(f)), dim3(FFT_THREADS), zlds(f), s, z);
-        FFCZ_HIP(hipGetLastError());
+        launch_zinv<double, T, Z_PLAIN>(f, z, s);
         dfi = c.work_real;
     }
     auto z = zargs<double, T>(f);
@@ -810,8 +765,7 @@ void certify(const Fft3d<double>& f, const CertArgs<T>& c, hipStream_t s) {
     z.qs = c.qs;
     z.ceps = c.eps;
     z.stats = c.stats;
-    hipLaunchKernelGGL((k_zfwd<double, T, Z_CERT>), dim3(zgrid(f)), dim3(FFT_THREADS), zlds(f), s, z);
-    FFCZ_HIP(hipGetLastError());
+    launch_zfwd<double, T, Z_CERT>(f, z, s);
     auto y = sargs<double, T>(f, c.work_spec, false);
     y.sign = -1;
     launch_strided(f, y, S_PLAIN, false, s);
@@ -835,8 +789,10 @@ template void compact_to_spec<float>(const Fft3d<float>&, const cplx<float>*, cp
 template void compact_to_spec<double>(const Fft3d<double>&, const cplx<double>*, cplx<double>*, hipStream_t);
 template void reset_state<float>(DevState*, int, double, hipStream_t);
 template void reset_state<double>(DevState*, int, double, hipStream_t);
-template void iteration<float>(const Fft3d<float>&, const IterArgs<float>&, hipStream_t, hipEvent_t*);
-template void iteration<double>(const Fft3d<double>&, const IterArgs<double>&, hipStream_t, hipEvent_t*);
+template void iteration_kernel<float>(const Fft3d<float>&, const IterArgs<float>&, hipStream_t, int);
+template void iteration_kernel<double>(const Fft3d<double>&, const IterArgs<double>&, hipStream_t, int);
+template void iteration<float>(const Fft3d<float>&, const IterArgs<float>&, hipStream_t, EventLog*);
+template void iteration<double>(const Fft3d<double>&, const IterArgs<double>&, hipStream_t, EventLog*);
 template void epilogue_clamp<float>(const Fft3d<float>&, const IterArgs<float>&, hipStream_t);
 template void epilogue_clamp<double>(const Fft3d<double>&, const IterArgs<double>&, hipStream_t);
 template void certify<float>(const Fft3d<double>&, const CertArgs<float>&, hipStream_t);

@@ -6,6 +6,8 @@
 // Nh = Nz/2+1, as cuFFT writes it) is used only for the edit accumulators.
 #pragma once
 
+#include <vector>
+
 #include "fft_core.hpp"
 
 namespace ffcz {
@@ -21,7 +23,7 @@ struct DevState {
     unsigned int iter;
     unsigned int max_iter;
     unsigned int blocks_done;
-    unsigned int pad0;
+    unsigned int freq_clamped;  // some frequency clamp fired during this solve
     unsigned long long viol_bits;  // max violation of the running iteration (order-preserving bits)
     double tol;
     double last_viol;
@@ -100,13 +102,40 @@ struct IterArgs {
     R delta_ptw;
     R norm;               // T(1)/T(N)
     DevState* st;
+    R* vpart;             // [rows] per-workgroup violation partials (zero-initialised)
 };
 
 // forward z (with the deferred spatial clamp of the previous iteration fused
 // into the load), forward y, fused x forward+clamp+inverse, inverse y,
 // inverse z (+normalise, violation max, iteration bookkeeping).
+// Optional per-launch event log (kind: 0 z fwd, 1 y fwd, 2 x fused,
+// 3 y inv, 4 z inv; -1 start).  Events come from a caller-owned pool.
+struct EventLog {
+    std::vector<hipEvent_t> pool;
+    std::vector<int> kinds;
+    size_t used = 0;
+    void mark(int kind, hipStream_t s) {
+        if (used >= pool.size()) {
+            hipEvent_t e;
+            // no system-scope fence: recording must not flush the caches
+            // between the kernels it separates
+            if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return;
+            pool.push_back(e);
+        }
+        (void)hipEventRecord(pool[used], s);
+        if (kinds.size() <= used) kinds.resize(used + 1);
+        kinds[used++] = kind;
+    }
+    ~EventLog() {
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
 template <typename R>
-void iteration(const Fft3d<R>& f, const IterArgs<R>& a, hipStream_t s, hipEvent_t* ev = nullptr);
+void iteration(const Fft3d<R>& f, const IterArgs<R>& a, hipStream_t s, EventLog* log = nullptr);
+// One of the iteration's passes over the whole field (kind as in EventLog),
+// for timing the kernels one at a time.
+template <typename R>
+void iteration_kernel(const Fft3d<R>& f, const IterArgs<R>& a, hipStream_t s, int kind);
 // Final spatial clamp when the loop stopped at max_iter without convergence.
 template <typename R>
 void epilogue_clamp(const Fft3d<R>& f, const IterArgs<R>& a, hipStream_t s);

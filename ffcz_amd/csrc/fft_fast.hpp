@@ -1,0 +1,45 @@
+// Length-specialised pass kernels (fft_fast_impl.hpp).  Each returns false
+// when the axis length has no specialisation, and the caller then runs the
+// generic run-time-planned kernels of fft3d.hip.
+#pragma once
+
+#include <atomic>
+
+#include "fft_kernels_common.hpp"
+
+namespace ffcz {
+
+template <typename R, typename T>
+bool fast_zfwd(int mode, const ZArgs<R, T>& a, hipStream_t s);
+template <typename R, typename T>
+bool fast_zinv(int mode, const ZArgs<R, T>& a, hipStream_t s);
+template <typename R, typename T>
+bool fast_strided(int mode, const SArgs<R, T>& a, size_t outer, hipStream_t s);
+
+// specialisations defined by fft_fast_f32.hip / fft_fast_f64.hip / fft_fast_c32.hip
+#define FFCZ_FAST_DECL(R, T)                                                                  \
+    template <>                                                                               \
+    bool fast_zfwd<R, T>(int mode, const ZArgs<R, T>& a, hipStream_t s);                      \
+    template <>                                                                               \
+    bool fast_zinv<R, T>(int mode, const ZArgs<R, T>& a, hipStream_t s);                      \
+    template <>                                                                               \
+    bool fast_strided<R, T>(int mode, const SArgs<R, T>& a, size_t outer, hipStream_t s);
+FFCZ_FAST_DECL(float, float)
+FFCZ_FAST_DECL(double, double)
+FFCZ_FAST_DECL(double, float)
+#undef FFCZ_FAST_DECL
+
+// Process-wide tuning knobs (C ABI: ffcz_set_tuning).
+struct Tuning {
+    std::atomic<int> force_generic{0};  // 1: run the generic kernels only
+    std::atomic<long> slab_mb{0};       // slab size of the z/y interleave (<= 0: whole field)
+    Tuning() = default;
+    Tuning(const Tuning& o) : force_generic(o.force_generic.load()), slab_mb(o.slab_mb.load()) {}
+};
+Tuning& tuning();
+bool fast_fft_enabled();
+// rows per workgroup of the specialised z kernels for half length Lz (0: the
+// generic kernels run)
+int fast_z_rows(int Lz, bool half);
+
+}  // namespace ffcz

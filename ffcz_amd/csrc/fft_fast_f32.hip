@@ -1,0 +1,49 @@
+// Length-specialised passes for R=float, T=float.
+#include <cstdlib>
+
+#include "fft_fast_impl.hpp"
+
+namespace ffcz {
+
+template <>
+bool fast_zfwd<float, float>(int mode, const ZArgs<float, float>& a, hipStream_t s) {
+    return zfwd_dispatch<float, float, (1u << Z_PLAIN) | (1u << Z_ITER)>(mode, a, s);
+}
+template <>
+bool fast_zinv<float, float>(int mode, const ZArgs<float, float>& a, hipStream_t s) {
+    return zinv_dispatch<float, float, (1u << Z_PLAIN) | (1u << Z_ITER)>(mode, a, s);
+}
+template <>
+bool fast_strided<float, float>(int mode, const SArgs<float, float>& a, size_t outer, hipStream_t s) {
+    return strided_dispatch<float, float, (1u << S_PLAIN) | (1u << S_CLAMP)>(mode, a, outer, s);
+}
+
+bool fast_fft_enabled() { return tuning().force_generic.load() == 0; }
+
+int fast_z_rows(int Lz, bool half) {
+    if (!half || !fast_fft_enabled()) return 0;
+    switch (Lz) {
+#define FFCZ_ZR(Lv) \
+    case Lv:        \
+        return z_rows(Lv);
+        FFCZ_FAST_Z_LENGTHS(FFCZ_ZR)
+#undef FFCZ_ZR
+        default:
+            return 0;
+    }
+}
+
+Tuning& tuning() {
+    // defaults from the environment: FFCZ_GENERIC_FFT=1, FFCZ_SLAB_MB=<MiB>
+    static Tuning t = [] {
+        Tuning v;
+        const char* g = std::getenv("FFCZ_GENERIC_FFT");
+        v.force_generic = (g && g[0] == '1') ? 1 : 0;
+        const char* m = std::getenv("FFCZ_SLAB_MB");
+        v.slab_mb = m ? std::atol(m) : 0L;
+        return v;
+    }();
+    return t;
+}
+
+}  // namespace ffcz

@@ -1,0 +1,335 @@
+// Length-specialised z / y / x pass kernels built on the pencil engine.
+// Included by the per-precision translation units (fft_fast_*.hip), which
+// instantiate the dispatchers for their (R, T) pair and pass modes.
+#pragma once
+
+#include "fft_fast.hpp"
+#include "fft_pencil.hpp"
+
+namespace ffcz {
+
+// Supported lengths: y/x axes, and half lengths nz/2 of the z axis.  They
+// cover the 2/3/5-smooth axes of the benchmark configurations
+// {60,72,96,120,150,180,240,250,256,300,512,600} and powers of two.
+#define FFCZ_FAST_S_LENGTHS(X) \
+    X(1) X(8) X(16) X(32) X(60) X(64) X(72) X(96) X(120) X(128) X(150) X(180) X(240) X(250) X(256) X(300) X(512) X(600)
+#define FFCZ_FAST_Z_LENGTHS(X) \
+    X(4) X(8) X(16) X(30) X(32) X(36) X(48) X(60) X(64) X(75) X(90) X(120) X(125) X(128) X(150) X(256) X(300)
+
+template <typename R>
+constexpr int s_width() { return sizeof(R) == 4 ? 16 : 8; }  // 128-byte column segments
+constexpr int z_tp(int LZ) { return pen::threads_per_pencil(LZ); }
+constexpr int z_rows(int LZ) {
+    int p = 1;
+    while (p * 2 * z_tp(LZ) <= 512) p *= 2;
+    return p;
+}
+
+// ---------------------------------------------------------------------------
+// forward z (R2C, nz even): loads fuse the deferred spatial clamp (ITER) or
+// the FP64 reconstruction and spatial check (CERT); the half-length complex
+// FFT ends in LDS and the R2C split X[k] = A[k] + W_nz^k B[k] writes the
+// padded half spectrum.
+// ---------------------------------------------------------------------------
+template <typename R, typename T, int LZ, int MODE>
+__global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ)) void k2_zfwd(ZArgs<R, T> a) {
+    constexpr int TP = z_tp(LZ), P = z_rows(LZ), NT = TP * P;
+    constexpr int NZ = 2 * LZ, M = LZ, NH = M + 1, SN = P + 1;
+    __shared__ cplx<R> tw[LZ];
+    __shared__ cplx<R> tw2[NH];
+    __shared__ cplx<R> buf[LZ * SN];
+    if (MODE == Z_ITER && a.st->done) return;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < LZ; i += NT) tw[i] = a.tz[i];
+    for (int i = tid; i < NH; i += NT) tw2[i] = a.tz2[i];
+    __syncthreads();
+    const int t = tid % TP, p = tid / TP;
+    const size_t row0 = (size_t)blockIdx.x * P;
+    const size_t row = row0 + p;
+    const bool rv = row < a.rows;
+    const bool clamp = (MODE == Z_ITER) && a.st->iter > 0;
+    unsigned long long nviol = 0;
+    double vmax = 0.0;
+    auto gld = [&](int n) -> cplx<R> {
+        if (!rv) return mk(R(0), R(0));
+        const size_t gi = row * (size_t)NZ + 2 * (size_t)n;
+        if constexpr (MODE == Z_ITER) {
+            // one 2-element vector load of the iterate (gi is even)
+            cplx<R> v = *reinterpret_cast<const cplx<R>*>(a.e + gi);
+            if (clamp) {
+                clamp_acc(v.x, a.eps, &a.ds[gi]);
+                clamp_acc(v.y, a.eps, &a.ds[gi + 1]);
+            }
+            return v;
+        } else if constexpr (MODE == Z_PLAIN) {
+            return *reinterpret_cast<const cplx<R>*>(a.in + gi);
+        } else {
+            // x_hat - x = e0 + C2R(df)/N + ds, element pairs as vectors
+            const cplx<T> e2 = *reinterpret_cast<const cplx<T>*>(a.e0 + gi);
+            const cplx<T> s2 = *reinterpret_cast<const cplx<T>*>(a.cds + gi);
+            double d0 = (double)e2.x, d1 = (double)e2.y;
+            if (a.dfi) {
+                const cplx<double> f2 = *reinterpret_cast<const cplx<double>*>(a.dfi + gi);
+                d0 += f2.x;
+                d1 += f2.y;
+            }
+            d0 += dequant(s2.x, a.qs);
+            d1 += dequant(s2.y, a.qs);
+            const double x0 = fabs(d0) - a.ceps, x1 = fabs(d1) - a.ceps;
+            if (x0 > 0.0) {
+                ++nviol;
+                vmax = x0 > vmax ? x0 : vmax;
+            }
+            if (x1 > 0.0) {
+                ++nviol;
+                vmax = x1 > vmax ? x1 : vmax;
+            }
+            return mk((R)d0, (R)d1);
+        }
+    };
+    auto lld = [&](int n) { return buf[n * SN + p]; };
+    auto lst = [&](int n, cplx<R> v) { buf[n * SN + p] = v; };
+    pen::transform<R, LZ, TP, false, true>(gld, lst, lld, lst, tw, t, -1);
+    if constexpr (MODE == Z_CERT) {
+        block_count_atomic(nviol, &a.stats->spatial_viol);
+        block_max_atomic<double, NT>(vmax, &a.stats->spatial_max_bits);
+    }
+    __syncthreads();
+    for (int q = tid; q < P * NH; q += NT) {
+        const int pp = q / NH;
+        const int k = q - pp * NH;
+        const size_t r2 = row0 + pp;
+        if (r2 >= a.rows) continue;
+        const int ik = (k == M) ? 0 : k;
+        const int im = (k == 0) ? 0 : M - k;
+        const cplx<R> zk = buf[ik * SN + pp];
+        const cplx<R> zm = buf[im * SN + pp];
+        const cplx<R> A = mk(R(0.5) * (zk.x + zm.x), R(0.5) * (zk.y - zm.y));
+        const cplx<R> B = mk(R(0.5) * (zk.y + zm.y), R(0.5) * (zm.x - zk.x));
+        a.spec[r2 * a.Mp + k] = A + cmul(tw2[k], B);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// inverse z (C2R, nz even): the first stage loads X[n] and X[M-n] straight
+// from HBM and forms Z[n] = (X[n] + conj X[M-n]) + i W^-n (X[n] - conj X[M-n])
+// (imaginary parts of the DC and Nyquist bins dropped); the last stage writes
+// the real row scaled by 1/N, and ITER tracks the spatial violation and
+// finalises the iteration in the last workgroup.
+// ---------------------------------------------------------------------------
+template <typename R, typename T, int LZ, int MODE>
+__global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ)) void k2_zinv(ZArgs<R, T> a) {
+    constexpr int TP = z_tp(LZ), P = z_rows(LZ), NT = TP * P;
+    constexpr int NZ = 2 * LZ, M = LZ, NH = M + 1, SN = P + 1;
+    __shared__ cplx<R> tw[LZ];
+    __shared__ cplx<R> tw2[NH];
+    __shared__ cplx<R> buf[LZ * SN];
+    if (MODE == Z_ITER && a.st->done) return;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < LZ; i += NT) tw[i] = a.tz[i];
+    for (int i = tid; i < NH; i += NT) tw2[i] = a.tz2[i];
+    __syncthreads();
+    const int t = tid % TP, p = tid / TP;
+    const size_t row = (size_t)blockIdx.x * P + p;
+    const bool rv = row < a.rows;
+    const cplx<R>* srow = a.spec + row * a.Mp;
+    auto gld = [&](int n) -> cplx<R> {
+        if (!rv) return mk(R(0), R(0));
+        cplx<R> xk = srow[n];
+        cplx<R> xm = srow[M - n];
+        if (n == 0) {
+            xk.y = R(0);
+            xm.y = R(0);
+        }
+        const cplx<R> s = xk + conjg(xm);
+        const cplx<R> d = cmul(xk - conjg(xm), conjg(tw2[n]));
+        return mk(s.x - d.y, s.y + d.x);
+    };
+    R vm = R(0);
+    auto gst = [&](int n, cplx<R> z) {
+        if (!rv) return;
+        const size_t gi = row * (size_t)NZ + 2 * (size_t)n;
+        const R v0 = z.x * a.scale, v1 = z.y * a.scale;
+        if constexpr (MODE == Z_ITER) {
+            *reinterpret_cast<cplx<R>*>(a.e + gi) = mk(v0, v1);
+            const R a0 = fabs(v0), a1 = fabs(v1);
+            if (a0 > a.eps) vm = fmax(vm, a0 - a.eps);
+            if (a1 > a.eps) vm = fmax(vm, a1 - a.eps);
+        } else {
+            *reinterpret_cast<cplx<R>*>(a.out + gi) = mk(v0, v1);
+        }
+    };
+    auto lld = [&](int n) { return buf[n * SN + p]; };
+    auto lst = [&](int n, cplx<R> v) { buf[n * SN + p] = v; };
+    pen::transform<R, LZ, TP, false, false>(gld, gst, lld, lst, tw, t, +1);
+    if constexpr (MODE == Z_ITER) block_max_store<R, NT>(vm, &a.vpart[a.row_off / P + blockIdx.x]);
+}
+
+// ---------------------------------------------------------------------------
+// y / x passes over 128-byte column tiles.
+// ---------------------------------------------------------------------------
+template <typename R, typename T, int L, int MODE>
+__global__ __launch_bounds__(pen::threads_per_pencil(L) * s_width<R>()) void k2_strided(SArgs<R, T> a) {
+    constexpr int TP = pen::threads_per_pencil(L), W = s_width<R>(), NT = TP * W;
+    __shared__ cplx<R> tw[L];
+    __shared__ cplx<R> buf[L * W];
+    if (a.st && a.st->done) return;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < L; i += NT) tw[i] = a.tw[i];
+    __syncthreads();
+    const int p = tid % W, t = tid / W;
+    const uint32_t ob_ = fdiv(blockIdx.x, a.dNct);
+    const int ct = (int)(blockIdx.x - ob_ * (uint32_t)a.nct);
+    const size_t o = ob_;
+    const int col = ct * W + p;
+    const bool valid = col < a.ncols;
+    cplx<R>* base = a.data + o * a.ob + col;
+    auto gld = [&](int n) -> cplx<R> {
+        if (!valid) return mk(R(0), R(0));
+        if constexpr (MODE == S_LOAD_DF) {
+            const size_t idx = (size_t)n * a.cs_n + o * a.cs_o + col;
+            return mk((R)dequant(a.dfsrc[idx], a.qf), (R)dequant(a.dfsrc[idx + a.F], a.qf));
+        } else {
+            return base[(size_t)n * a.es];
+        }
+    };
+    auto gst = [&](int n, cplx<R> v) {
+        if (valid) base[(size_t)n * a.es] = v;
+    };
+    auto lld = [&](int n) { return buf[n * W + p]; };
+    auto lst = [&](int n, cplx<R> v) { buf[n * W + p] = v; };
+    if constexpr (MODE == S_PLAIN || MODE == S_LOAD_DF) {
+        pen::transform<R, L, TP, false, false>(gld, gst, lld, lst, tw, t, MODE == S_LOAD_DF ? +1 : a.sign);
+    } else if constexpr (MODE == S_CLAMP) {
+        bool fired = false;
+        auto cst = [&](int n, cplx<R> v) {
+            if (valid) fired |= freq_clamp_bin(v, (size_t)n * a.cs_n + o * a.cs_o + col, a);
+            buf[n * W + p] = v;
+        };
+        pen::transform<R, L, TP, false, true>(gld, cst, lld, lst, tw, t, -1);
+        if (fired) a.st->freq_clamped = 1u;
+        __syncthreads();
+        pen::transform<R, L, TP, true, false>(lld, gst, lld, lst, tw, t, +1);
+    } else {  // S_CERT_CHECK
+        unsigned long long nv = 0;
+        double vmax = 0.0;
+        auto chk = [&](int n, cplx<R> v) {
+            if (!valid) return;
+            const size_t idx = (size_t)n * a.cs_n + o * a.cs_o + col;
+            const double d = a.cdelta_dense ? (double)a.cdelta_dense[idx] : a.cdelta;
+            const double ex = fmax(fabs((double)v.x), fabs((double)v.y)) - d;
+            if (ex > 0.0) {
+                ++nv;
+                vmax = ex > vmax ? ex : vmax;
+            }
+        };
+        pen::transform<R, L, TP, false, false>(gld, chk, lld, lst, tw, t, -1);
+        block_count_atomic(nv, &a.stats->freq_viol);
+        block_max_atomic<double, NT>(vmax, &a.stats->freq_max_bits);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// dispatch
+// ---------------------------------------------------------------------------
+template <typename R, typename T, int MODE, int LZ>
+inline void launch_zfwd_len(const ZArgs<R, T>& a, hipStream_t s) {
+    const unsigned grid = (unsigned)ceil_div(a.rows, (size_t)z_rows(LZ));
+    hipLaunchKernelGGL((k2_zfwd<R, T, LZ, MODE>), dim3(grid), dim3(z_tp(LZ) * z_rows(LZ)), 0, s, a);
+}
+template <typename R, typename T, int MODE, int LZ>
+inline void launch_zinv_len(const ZArgs<R, T>& a, hipStream_t s) {
+    const unsigned grid = (unsigned)ceil_div(a.rows, (size_t)z_rows(LZ));
+    hipLaunchKernelGGL((k2_zinv<R, T, LZ, MODE>), dim3(grid), dim3(z_tp(LZ) * z_rows(LZ)), 0, s, a);
+}
+template <typename R, typename T, int MODE, int L>
+inline void launch_strided_len(SArgs<R, T> a, size_t outer, hipStream_t s) {
+    constexpr int W = s_width<R>();
+    a.W = W;
+    a.nct = (int)ceil_div((size_t)a.ncols, (size_t)W);
+    a.dNct = FastDiv((uint32_t)a.nct);
+    const unsigned grid = (unsigned)(outer * (size_t)a.nct);
+    hipLaunchKernelGGL((k2_strided<R, T, L, MODE>), dim3(grid), dim3(pen::threads_per_pencil(L) * W), 0, s, a);
+}
+
+template <typename R, typename T, int MODE>
+bool zfwd_by_len(const ZArgs<R, T>& a, hipStream_t s) {
+    if (!a.half) return false;
+    switch (a.Lz) {
+#define FFCZ_ZF(Lv)                                 \
+    case Lv:                                        \
+        launch_zfwd_len<R, T, MODE, Lv>(a, s);      \
+        return true;
+        FFCZ_FAST_Z_LENGTHS(FFCZ_ZF)
+#undef FFCZ_ZF
+        default:
+            return false;
+    }
+}
+template <typename R, typename T, int MODE>
+bool zinv_by_len(const ZArgs<R, T>& a, hipStream_t s) {
+    if (!a.half) return false;
+    switch (a.Lz) {
+#define FFCZ_ZI(Lv)                                 \
+    case Lv:                                        \
+        launch_zinv_len<R, T, MODE, Lv>(a, s);      \
+        return true;
+        FFCZ_FAST_Z_LENGTHS(FFCZ_ZI)
+#undef FFCZ_ZI
+        default:
+            return false;
+    }
+}
+template <typename R, typename T, int MODE>
+bool strided_by_len(const SArgs<R, T>& a, size_t outer, hipStream_t s) {
+    switch (a.L) {
+#define FFCZ_SL(Lv)                                      \
+    case Lv:                                             \
+        launch_strided_len<R, T, MODE, Lv>(a, outer, s); \
+        return true;
+        FFCZ_FAST_S_LENGTHS(FFCZ_SL)
+#undef FFCZ_SL
+        default:
+            return false;
+    }
+}
+
+// MASK bits select the pass modes a translation unit instantiates.
+template <typename R, typename T, unsigned MASK>
+bool zfwd_dispatch(int mode, const ZArgs<R, T>& a, hipStream_t s) {
+    // the specialised kernels move element pairs as one vector
+    if (mode == Z_PLAIN && ((uintptr_t)a.in % (2 * sizeof(R))) != 0) return false;
+    if (mode == Z_PLAIN) {
+        if constexpr ((MASK & (1u << Z_PLAIN)) != 0) return zfwd_by_len<R, T, Z_PLAIN>(a, s);
+    } else if (mode == Z_ITER) {
+        if constexpr ((MASK & (1u << Z_ITER)) != 0) return zfwd_by_len<R, T, Z_ITER>(a, s);
+    } else if (mode == Z_CERT) {
+        if constexpr ((MASK & (1u << Z_CERT)) != 0) return zfwd_by_len<R, T, Z_CERT>(a, s);
+    }
+    return false;
+}
+template <typename R, typename T, unsigned MASK>
+bool zinv_dispatch(int mode, const ZArgs<R, T>& a, hipStream_t s) {
+    if (mode == Z_PLAIN && ((uintptr_t)a.out % (2 * sizeof(R))) != 0) return false;
+    if (mode == Z_PLAIN) {
+        if constexpr ((MASK & (1u << Z_PLAIN)) != 0) return zinv_by_len<R, T, Z_PLAIN>(a, s);
+    } else if (mode == Z_ITER) {
+        if constexpr ((MASK & (1u << Z_ITER)) != 0) return zinv_by_len<R, T, Z_ITER>(a, s);
+    }
+    return false;
+}
+template <typename R, typename T, unsigned MASK>
+bool strided_dispatch(int mode, const SArgs<R, T>& a, size_t outer, hipStream_t s) {
+    if (mode == S_PLAIN) {
+        if constexpr ((MASK & (1u << S_PLAIN)) != 0) return strided_by_len<R, T, S_PLAIN>(a, outer, s);
+    } else if (mode == S_CLAMP) {
+        if constexpr ((MASK & (1u << S_CLAMP)) != 0) return strided_by_len<R, T, S_CLAMP>(a, outer, s);
+    } else if (mode == S_CERT_CHECK) {
+        if constexpr ((MASK & (1u << S_CERT_CHECK)) != 0) return strided_by_len<R, T, S_CERT_CHECK>(a, outer, s);
+    } else if (mode == S_LOAD_DF) {
+        if constexpr ((MASK & (1u << S_LOAD_DF)) != 0) return strided_by_len<R, T, S_LOAD_DF>(a, outer, s);
+    }
+    return false;
+}
+
+}  // namespace ffcz

@@ -1,0 +1,182 @@
+// Kernel argument blocks and device helpers shared by the generic and the
+// length-specialised FFT pass kernels.
+#pragma once
+
+#include "fft3d.hpp"
+#include "quant.hpp"
+
+namespace ffcz {
+
+template <typename R>
+__device__ __forceinline__ void clamp_acc(R& v, R d, R* acc) {
+    // projection_algorithm.cu:161-172 / :351-360: the edit is formed first
+    // and then accumulated (a single writer per address).
+    if (v > d) {
+        *acc += d - v;
+        v = d;
+    } else if (v < -d) {
+        *acc += -d - v;
+        v = -d;
+    }
+}
+
+template <typename R, int NT = FFT_THREADS>
+__device__ __forceinline__ void block_max_atomic(R v, unsigned long long* target) {
+    __shared__ R red[(NT + 63) / 64];
+    v = wave_max(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        R m = red[0];
+        for (int i = 1; i < (NT + 63) / 64; ++i) m = red[i] > m ? red[i] : m;
+        if (m > R(0)) atomicMax(target, ordbits(m));
+    }
+}
+
+// Workgroup max written to its own slot (no atomics on a shared address).
+template <typename R, int NT>
+__device__ __forceinline__ void block_max_store(R v, R* slot) {
+    __shared__ R red[(NT + 63) / 64];
+    v = wave_max(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        R m = red[0];
+        for (int i = 1; i < (NT + 63) / 64; ++i) m = red[i] > m ? red[i] : m;
+        *slot = m;
+    }
+}
+
+// Run by one thread of the last workgroup of an iteration's inverse z pass:
+// publish the max spatial violation, count the iteration and decide
+// termination (projection_algorithm_impl.cuh:841-848).
+template <typename R>
+__device__ __forceinline__ void finalize_iteration(DevState* st) {
+    __threadfence();
+    const unsigned long long b = atomicAdd(&st->viol_bits, 0ull);
+    const R v = from_ordbits<R>(b);
+    st->iter += 1;
+    st->last_viol = (double)v;
+    bool conv;
+    if constexpr (sizeof(R) == 4) conv = v < st->tol_f;
+    else conv = v < (R)st->tol;
+    if (conv) {
+        st->converged = 1;
+        st->done = 1;
+    } else if (st->iter >= st->max_iter) {
+        st->done = 1;
+    }
+    atomicExch(&st->viol_bits, 0ull);
+    atomicExch(&st->blocks_done, 0u);
+    __threadfence();
+}
+
+__device__ __forceinline__ void block_count_atomic(unsigned long long c, unsigned long long* target) {
+    c = wave_sum(c);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(target, c);
+}
+
+enum ZMode : int { Z_PLAIN = 0, Z_ITER = 1, Z_CERT = 2 };
+
+template <typename R, typename T>
+struct ZArgs {
+    int nz, Lz, M, half, P, sn;
+    FastDiv dP, dM, dNh, dNp, dNz, dFill;  // P, M, nz/2+1, M/2+1, nz, nz-(nz/2+1)
+    size_t rows, Mp;
+    FftPlan pl;
+    const cplx<R>* tz;
+    const cplx<R>* tz2;
+    const R* in;          // Z_PLAIN forward input
+    R* out;               // Z_PLAIN inverse output
+    cplx<R>* spec;
+    R scale;              // inverse scale
+    // Z_ITER
+    R* e;
+    R* ds;
+    R eps;
+    DevState* st;
+    R* vpart;  // per-workgroup max violation of the inverse z pass
+    size_t row_off;  // first row of this launch within the field (slab launches)
+    // Z_CERT
+    const T* e0;
+    const T* cds;
+    const double* dfi;
+    QuantDesc qs;
+    double ceps;
+    CertStats* stats;
+};
+
+enum SMode : int { S_PLAIN = 0, S_CLAMP = 1, S_CERT_CHECK = 2, S_LOAD_DF = 3 };
+
+template <typename R, typename T>
+struct SArgs {
+    cplx<R>* data;
+    FftPlan pl;
+    const cplx<R>* tw;
+    int L, W, wshift, ncols, nct;
+    FastDiv dW, dNct;
+    size_t ob, es;
+    int sign;
+    DevState* st;
+    // clamp / check / load: bin index = n*cs_n + o*cs_o + col (reference layout)
+    size_t F, cs_n, cs_o;
+    int fmode;
+    R delta;
+    const R* delta_dense;
+    R dscale;
+    R* df;
+    const cplx<R>* orig;
+    R delta_ptw;
+    // certification
+    double cdelta;
+    const T* cdelta_dense;
+    CertStats* stats;
+    const T* dfsrc;
+    QuantDesc qf;
+};
+
+template <typename R>
+__device__ __forceinline__ void ptw_clamp(cplx<R>& v, size_t idx, size_t F, const cplx<R>* orig, R dptw, R* df) {
+    // projection_algorithm.cu:199-258
+    if (idx == 0) {
+        const R lo = orig[0].x * (R(1) / sqrt(R(1) + dptw) - R(1));
+        const R hi = orig[0].x * (R(1) / sqrt(R(1) - dptw) - R(1));
+        if (v.x > hi || v.x < lo) {
+            df[0] += -v.x;
+            v.x = R(0);
+        }
+        return;
+    }
+    const cplx<R> o = orig[idx];
+    const R omag = o.x * o.x + o.y * o.y;
+    const R cr = o.x + v.x, ci = o.y + v.y;
+    const R cmag = cr * cr + ci * ci;
+    const R ratio = cmag / omag;
+    if (ratio <= sqrt(R(1) + dptw) && ratio >= sqrt(R(1) - dptw)) return;
+    const R d = fmin(R(1) - sqrt(R(1) - dptw), sqrt(R(1) + dptw) - R(1)) / (R)sqrtf(2.0f);
+    clamp_acc(v.x, d, &df[idx]);
+    clamp_acc(v.y, d, &df[idx + F]);
+}
+
+// The frequency projection of one half-spectrum bin for the pass's mode
+// (projection_algorithm.cu:147-188 abs, 93-145 per-bin, 190-259 ptw).
+// Returns true when a clamp changed the bin.
+template <typename R, typename T>
+__device__ __forceinline__ bool freq_clamp_bin(cplx<R>& v, size_t idx, const SArgs<R, T>& a) {
+    const cplx<R> v0 = v;
+    if (a.fmode == FM_ABS) {
+        clamp_acc(v.x, a.delta, &a.df[idx]);
+        clamp_acc(v.y, a.delta, &a.df[idx + a.F]);
+    } else if (a.fmode == FM_DENSE) {
+        const R d = a.delta_dense[idx] * a.dscale;
+        clamp_acc(v.x, d, &a.df[idx]);
+        clamp_acc(v.y, d, &a.df[idx + a.F]);
+    } else if (a.fmode == FM_PTW) {
+        ptw_clamp(v, idx, a.F, a.orig, a.delta_ptw, a.df);
+    }
+    return v.x != v0.x || v.y != v0.y;
+}
+
+}  // namespace ffcz

@@ -1,0 +1,129 @@
+// Compile-time-specialised pencil FFT engine.
+//
+// For a length L known at compile time the radix sequence (8s, then 4 or 2,
+// then 5s, then 3s — the same sequence as the run-time planner) and every
+// stage constant are constexpr, so all index arithmetic folds to shifts and
+// multiplies and every loop unrolls.  A pencil is transformed by TP threads;
+// in stage s each thread owns ceil((L/R_s)/TP) butterflies of radix R_s in
+// registers (Stockham DIT, twiddle before butterfly, as in fft_core.hpp).
+// The first stage reads its inputs through a caller-supplied loader (straight
+// from HBM, with the iteration's fused element-wise work), the last stage
+// writes through a caller-supplied storer; only the exchanges between stages
+// go through LDS.
+#pragma once
+
+#include "fft_core.hpp"
+
+namespace ffcz {
+namespace pen {
+
+constexpr int fac_next(int L) {
+    return L % 8 == 0 ? 8 : L % 4 == 0 ? 4 : L % 2 == 0 ? 2 : L % 5 == 0 ? 5 : L % 3 == 0 ? 3 : 0;
+}
+constexpr int nstages(int L) {
+    int n = 0;
+    while (L > 1) {
+        const int r = fac_next(L);
+        if (!r) return -1;
+        L /= r;
+        ++n;
+    }
+    return n;
+}
+constexpr int radix(int L, int s) {
+    int r = 1;
+    for (int i = 0; i <= s; ++i) {
+        r = fac_next(L);
+        L /= r;
+    }
+    return r;
+}
+constexpr int nsub(int L, int s) {
+    int ns = 1;
+    for (int i = 0; i < s; ++i) {
+        const int r = fac_next(L);
+        L /= r;
+        ns *= r;
+    }
+    return ns;
+}
+// threads per pencil: ~8 elements per thread up to L = 512, ~16 beyond
+constexpr int threads_per_pencil(int L) { return L <= 1 ? 1 : (L <= 512 ? (L + 7) / 8 : (L + 15) / 16); }
+
+template <typename R, int L, int TP, int S, bool SYNC_MID, class LD, class ST>
+__device__ __forceinline__ void run_stage(const LD& ld, const ST& st, const cplx<R>* __restrict__ tw, int t,
+                                          int sign) {
+    constexpr int RAD = radix(L, S);
+    constexpr int NS = nsub(L, S);
+    constexpr int LR = L / RAD;
+    constexpr int NB = (LR + TP - 1) / TP;
+    constexpr int TWS = L / (NS * RAD);
+    cplx<R> v[NB][RAD];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int j = t + b * TP;
+        if ((LR % TP == 0) || j < LR) {
+#pragma unroll
+            for (int r = 0; r < RAD; ++r) v[b][r] = ld(j + r * LR);
+            if constexpr (NS > 1) {
+                const int k = j % NS;
+#pragma unroll
+                for (int r = 1; r < RAD; ++r) {
+                    cplx<R> w = tw[k * r * TWS];
+                    w.y = sign > 0 ? -w.y : w.y;
+                    v[b][r] = cmul(v[b][r], w);
+                }
+            }
+            dftR<R, RAD>(v[b], sign);
+        }
+    }
+    if constexpr (SYNC_MID) __syncthreads();
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int j = t + b * TP;
+        if ((LR % TP == 0) || j < LR) {
+            const int k = j % NS;
+            const int base = (j - k) * RAD + k;
+#pragma unroll
+            for (int r = 0; r < RAD; ++r) st(base + r * NS, v[b][r]);
+        }
+    }
+}
+
+// Full transform of one pencil.  gld/gst: first-stage loader and last-stage
+// storer; lld/lst: LDS exchange between stages.  IN_LDS / OUT_LDS say that
+// gld / gst themselves address the exchange buffer (so the stage must barrier
+// between its reads and its writes).  Returns after the last stage's stores
+// without a trailing barrier.
+template <typename R, int L, int TP, bool IN_LDS, bool OUT_LDS, int S = 0, class GLD, class GST, class LLD, class LST>
+__device__ __forceinline__ void transform(const GLD& gld, const GST& gst, const LLD& lld, const LST& lst,
+                                          const cplx<R>* __restrict__ tw, int t, int sign) {
+    constexpr int NST = nstages(L);
+    static_assert(NST >= 0, "length must be 2/3/5-smooth");
+    if constexpr (NST == 0) {
+        if (t == 0) {
+            const cplx<R> v = gld(0);
+            if constexpr (IN_LDS && OUT_LDS) __syncthreads();
+            gst(0, v);
+        } else if constexpr (IN_LDS && OUT_LDS) {
+            __syncthreads();
+        }
+    } else if constexpr (S < NST) {
+        constexpr bool FIRST = (S == 0), LAST = (S == NST - 1);
+        if constexpr (FIRST && LAST) {
+            run_stage<R, L, TP, S, IN_LDS && OUT_LDS>(gld, gst, tw, t, sign);
+        } else if constexpr (FIRST) {
+            run_stage<R, L, TP, S, IN_LDS>(gld, lst, tw, t, sign);
+            __syncthreads();
+        } else if constexpr (LAST) {
+            run_stage<R, L, TP, S, OUT_LDS>(lld, gst, tw, t, sign);
+        } else {
+            run_stage<R, L, TP, S, true>(lld, lst, tw, t, sign);
+            __syncthreads();
+        }
+        transform<R, L, TP, IN_LDS, OUT_LDS, S + 1>(gld, gst, lld, lst, tw, t, sign);
+    }
+}
+
+}  // namespace pen
+}  // namespace ffcz

@@ -33,14 +33,6 @@ __global__ void k_scatter_delta(T* dense, size_t F, const size_t* idx, const T* 
         else atomicMin((unsigned long long*)&dense[j], (unsigned long long)__double_as_longlong((double)d));
     }
 }
-template <typename T>
-__global__ void k_any_nonzero(const T* p, size_t n, unsigned int* flag) {
-    bool nz = false;
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        nz |= (p[i] != T(0));
-    if (__ballot(nz) && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
-}
-
 static unsigned gridn(size_t n) {
     size_t g = ceil_div(n ? n : 1, (size_t)256);
     return (unsigned)std::min<size_t>(g, 1u << 16);
@@ -67,6 +59,8 @@ Solver<T>::Solver(size_t nx, size_t ny, size_t nz) {
         dalloc(&df_, 2 * f_.F);
         dalloc(&spec_, f_.spec_elems());
         dalloc(&st_, 1);
+        dalloc(&vpart_, f_.rows);
+        hipLaunchKernelGGL((k_zero<T>), dim3(gridn(f_.rows)), dim3(256), 0, s_, vpart_, f_.rows);
         reset_edit_accumulators();  // projection_algorithm_impl.cuh:60
         reset_state<T>(st_, 0, 0.0, s_);
         FFCZ_HIP(hipStreamSynchronize(s_));
@@ -84,11 +78,11 @@ Solver<T>::~Solver() {
 template <typename T>
 void Solver<T>::release() {
     if (s_) (void)hipStreamSynchronize(s_);
-    void* ptrs[] = {x_, e_, e0_, ds_, df_, spec_, st_, delta_dense_, orig_, partial_idx_, fpack_, spack_,
+    void* ptrs[] = {x_, e_, e0_, ds_, df_, spec_, st_, vpart_, delta_dense_, orig_, partial_idx_, fpack_, spack_,
                     fcomp_, scomp_, fquant_, squant_, xscratch_, cert_real_, cert_spec_, cstats_};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
-    x_ = e_ = e0_ = ds_ = df_ = nullptr;
+    x_ = e_ = e0_ = ds_ = df_ = vpart_ = nullptr;
     spec_ = nullptr;
     st_ = nullptr;
     delta_dense_ = nullptr;
@@ -116,6 +110,7 @@ void Solver<T>::upload_x(const T* x) {
 
 template <typename T>
 void Solver<T>::reset_edit_accumulators() {
+    df_touched_ = false;
     hipLaunchKernelGGL((k_zero<T>), dim3(gridn(2 * f_.F)), dim3(256), 0, s_, df_, 2 * f_.F);
     hipLaunchKernelGGL((k_zero<T>), dim3(gridn(f_.N)), dim3(256), 0, s_, ds_, f_.N);
     FFCZ_HIP(hipGetLastError());
@@ -222,6 +217,7 @@ IterArgs<T> Solver<T>::iter_args() const {
     a.delta_ptw = delta_ptw_;
     a.norm = T(1) / static_cast<T>(f_.N);  // projection_algorithm_impl.cuh:242
     a.st = st_;
+    a.vpart = vpart_;
     return a;
 }
 
@@ -253,6 +249,7 @@ void Solver<T>::solve_loop(int mode, const T* e0, int max_iter, T tol) {
     iters_ = (int)h.iter;
     viol_ = h.last_viol;
     converged_ = h.converged != 0;
+    df_touched_ = df_touched_ || h.freq_clamped != 0;
     solved_ = true;
     extracted_ = false;
 }
@@ -343,15 +340,8 @@ CertResult Solver<T>::certify(bool quantized) {
     } else {
         c.qs.kind = Q_NONE;
         c.qf.kind = Q_NONE;
-        unsigned int* flag = nullptr;
-        dalloc(&flag, 1);
-        unsigned int hf = 0;
-        FFCZ_HIP(hipMemsetAsync(flag, 0, sizeof(unsigned int), s_));
-        hipLaunchKernelGGL((k_any_nonzero<T>), dim3(gridn(2 * f_.F)), dim3(256), 0, s_, df_, 2 * f_.F, flag);
-        FFCZ_HIP(hipMemcpyAsync(&hf, flag, sizeof(hf), hipMemcpyDeviceToHost, s_));
-        FFCZ_HIP(hipStreamSynchronize(s_));
-        (void)hipFree(flag);
-        c.have_df = hf != 0;
+        // the frequency accumulator is non-zero only if some clamp fired
+        c.have_df = df_touched_;
     }
     c.eps = (double)eps_;
     c.delta = (double)delta_;
@@ -381,30 +371,38 @@ void Solver<T>::get_results(T* e) const {
 
 template <typename T>
 void Solver<T>::time_iteration(int reps, float* ms5) {
+    // ms5[0..4]: the five passes; ms5[5]: a whole iteration
     FFCZ_REQUIRE(solved_, ERR_STATE, "time_iteration requires a completed solve");
     FFCZ_REQUIRE(reps >= 1, ERR_INVALID, "reps must be >= 1");
     reset_state<T>(st_, 1 << 30, -1.0, s_);  // tol < 0: never converges
     const IterArgs<T> a = iter_args();
-    hipEvent_t ev[6] = {};
-    for (auto& e : ev) FFCZ_HIP(hipEventCreate(&e));
-    float acc[5] = {0, 0, 0, 0, 0};
-    try {
-        iteration(f_, a, s_);  // warm-up
-        for (int r = 0; r < reps; ++r) {
-            iteration(f_, a, s_, ev);
-            FFCZ_HIP(hipEventSynchronize(ev[5]));
-            for (int k = 0; k < 5; ++k) {
-                float t = 0;
-                FFCZ_HIP(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
-                acc[k] += t;
-            }
+    // passes: events between the launches of real iterations (an interval
+    // is attributed to the launch that ends it)
+    float acc[6] = {0, 0, 0, 0, 0, 0};  // [5]: violation reduction
+    EventLog log;
+    iteration(f_, a, s_);  // warm-up
+    for (int r = 0; r < reps; ++r) {
+        log.used = 0;
+        iteration(f_, a, s_, &log);
+        FFCZ_REQUIRE(log.used >= 2, ERR_HIP, "event creation failed");
+        FFCZ_HIP(hipEventSynchronize(log.pool[log.used - 1]));
+        for (size_t i = 1; i < log.used; ++i) {
+            float t = 0;
+            FFCZ_HIP(hipEventElapsedTime(&t, log.pool[i - 1], log.pool[i]));
+            acc[log.kinds[i]] += t;
         }
-    } catch (...) {
-        for (auto& e : ev) (void)hipEventDestroy(e);
-        throw;
     }
-    for (auto& e : ev) (void)hipEventDestroy(e);
     for (int k = 0; k < 5; ++k) ms5[k] = acc[k] / (float)reps;
+    // whole iterations (five passes + the violation reduction), no events
+    // between the launches
+    hipEvent_t e0 = log.pool[0], e1 = log.pool[1];
+    FFCZ_HIP(hipEventRecord(e0, s_));
+    for (int r = 0; r < reps; ++r) iteration(f_, a, s_);
+    FFCZ_HIP(hipEventRecord(e1, s_));
+    FFCZ_HIP(hipEventSynchronize(e1));
+    float t = 0;
+    FFCZ_HIP(hipEventElapsedTime(&t, e0, e1));
+    ms5[5] = t / (float)reps;
     reset_state<T>(st_, 0, 0.0, s_);
     FFCZ_HIP(hipStreamSynchronize(s_));
 }

@@ -45,8 +45,8 @@ class Solver {
     void extract();
     void reset_edit_accumulators();
     CertResult certify(bool quantized);
-    // mean ms per launch of the five iteration kernels over ``reps`` iterations
-    void time_iteration(int reps, float* ms5);
+    // mean ms per launch of the five iteration passes, then per whole iteration
+    void time_iteration(int reps, float* ms6);
 
     // margins: the loop iterates against eps*(1-ms) and delta*(1-mf); the
     // certification always checks the bounds given to initialize*.
@@ -104,6 +104,7 @@ class Solver {
     T* ds_ = nullptr;
     cplx<T>* spec_ = nullptr;
     DevState* st_ = nullptr;
+    T* vpart_ = nullptr;  // per-workgroup violation partials of the inverse z pass
     T* delta_dense_ = nullptr;   // direct mode (and certification of it)
     cplx<T>* orig_ = nullptr;    // F(x), compact layout (ptw modes)
     size_t* partial_idx_ = nullptr;
@@ -134,6 +135,7 @@ class Solver {
     double viol_ = 0;
     bool converged_ = false;
     bool solved_ = false;
+    bool df_touched_ = false;  // a frequency clamp fired since the last accumulator reset
 };
 
 }  // namespace ffcz

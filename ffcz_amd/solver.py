@@ -240,7 +240,7 @@ class ProjectionSolver:
         return s.value or 0
 
     def time_iteration(self, reps: int = 10):
-        ms = (C.c_float * 5)()
+        ms = (C.c_float * 6)()
         call("ffcz_time_iteration", self._h, int(reps), ms)
         return list(ms)
 

@@ -129,12 +129,19 @@ int ffcz_get_edit_range(ffcz_solver* h, double* fmin, double* fmax, double* smin
 int ffcz_set_margins(ffcz_solver* h, double spatial_margin, double freq_margin);
 int ffcz_set_quantizer(ffcz_solver* h, int kind, double freq_step, double spatial_step);
 int ffcz_certify(ffcz_solver* h, int quantized, ffcz_cert* out);
+/* process-wide tuning: force_generic=1 runs the run-time-planned FFT kernels
+ * instead of the length-specialised ones; slab_mb sets the slab size of the
+ * z/y pass interleave (<= 0: whole field).  Negative force_generic leaves it. */
+int ffcz_set_tuning(int force_generic, long slab_mb);
+int ffcz_get_tuning(int* force_generic, long* slab_mb);
 /* the solver's HIP stream (hipStream_t) */
 int ffcz_get_stream(ffcz_solver* h, void** stream);
-/* times `reps` projection iterations kernel by kernel with HIP events on the
- * solver stream (accumulators keep changing; call reset afterwards).  ms[5] =
- * mean ms per launch of: z forward, y forward, x fused, y inverse, z inverse */
-int ffcz_time_iteration(ffcz_solver* h, int reps, float* ms5);
+/* times the projection iteration with HIP events on the solver stream, after
+ * a completed solve (the accumulators keep changing: reset them afterwards).
+ * ms[6] = mean ms per launch of the z forward, y forward, fused x, y inverse
+ * and z inverse passes (each launched `reps` times back to back), and mean ms
+ * per whole iteration over `reps` back-to-back iterations */
+int ffcz_time_iteration(ffcz_solver* h, int reps, float* ms6);
 
 /* ---- entropy coding: SimpleCudaHuffmanZstdCompressor<T>
  * compress    HuffmanZSTDCoder.cuh:248-376 (symbols: host or device memory)

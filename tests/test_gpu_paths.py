@@ -1,0 +1,59 @@
+"""The length-specialised FFT kernels and the slab-interleaved iteration give
+the same results as the generic run-time-planned kernels and the whole-field
+iteration (FP64: to rounding; the two paths use different radix schedules
+only where noted)."""
+import numpy as np
+import pytest
+import torch
+
+import ffcz_amd
+from ffcz_amd import ProjectionSolver
+from ffcz_amd._lib import set_tuning
+from tests._cases import binding_case
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture
+def restore_tuning():
+    prev = set_tuning()
+    yield
+    set_tuning(*prev)
+
+
+def run(shape, dtype, force_generic, slab_mb):
+    set_tuning(force_generic, slab_mb)
+    x, e0, eps, delta = binding_case(shape, seed=12, dtype=dtype)
+    with ProjectionSolver(*shape, dtype=dtype) as s:
+        s.initialize_all_abs(x, eps, delta)
+        s.solve_all_abs(e0, 25, 1e-12 if dtype == np.float64 else 1e-7)
+        return s.iteration_count, s.get_results(), s.get_freq_edits_full(), s.get_spatial_edits_full()
+
+
+@pytest.mark.parametrize("shape", [(32, 64, 128), (24, 60, 72), (64, 16, 256)])
+def test_fast_matches_generic_f64(gpu, restore_tuning, shape):
+    a = run(shape, np.float64, 0, 0)
+    b = run(shape, np.float64, 1, 0)
+    assert a[0] == b[0]
+    for u, v in zip(a[1:], b[1:]):
+        assert np.linalg.norm(u - v) <= 1e-11 * max(np.linalg.norm(v), 1e-300)
+
+
+@pytest.mark.parametrize("shape", [(32, 64, 128), (16, 32, 256)])
+def test_slabs_match_whole_field(gpu, restore_tuning, shape):
+    # 1 MiB slabs force several slabs on these sizes; results are bit-identical
+    a = run(shape, np.float32, 0, 0)
+    b = run(shape, np.float32, 0, 1)
+    assert a[0] == b[0]
+    for u, v in zip(a[1:], b[1:]):
+        assert np.array_equal(u, v)
+
+
+@pytest.mark.parametrize("shape", [(8, 60, 72), (4, 512, 16), (2, 600, 8)])
+def test_fast_fft_matches_numpy(gpu, restore_tuning, shape):
+    set_tuning(0, 64)
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(shape)
+    X = ffcz_amd.fft_r2c(torch.from_numpy(x).to(gpu)).cpu().numpy()
+    ref = np.fft.rfftn(x)
+    assert np.abs(X - ref).max() < 1e-13 * np.abs(ref).max()
