@@ -29,7 +29,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--config", default="c3", choices=["c1", "c2", "c3"])
+    p.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c5"])
     p.add_argument("--seed", type=int, default=1234)
     p.add_argument("--timing-reps", type=int, default=10)
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -81,13 +81,15 @@ def cpu_baseline(wl, gpu_iters, sample_iters, cores):
     x = wl.x.cpu().numpy()
     e0 = wl.e0.cpu().numpy()
     shape = x.shape
+    dk = None if wl.delta_k is None else wl.delta_k.cpu().numpy()
+    kw = (dict(mode="abs", delta=wl.delta) if dk is None else
+          dict(mode="direct", idx=np.arange(dk.size), delta_k=dk))
     t0 = time.perf_counter()
-    r = O.solve(x, e0, wl.eps, mode="abs", delta=wl.delta, max_iter=sample_iters, tol=-1.0,
-                dtype=np.float32, workers=cores)
+    r = O.solve(x, e0, wl.eps, max_iter=sample_iters, tol=-1.0, dtype=x.dtype, workers=cores, **kw)
     t_iter = (time.perf_counter() - t0) / sample_iters
     t0 = time.perf_counter()
     d = O.reconstruct_edit(shape, e0, r.df, r.ds, workers=cores)
-    O.certify(d, wl.eps, wl.delta, workers=cores)
+    O.certify(d, wl.eps, wl.delta, delta_dense=dk, workers=cores)
     t_cert = time.perf_counter() - t0
     T = gpu_iters * t_iter + 2 * t_cert
     nbytes = x.size * x.dtype.itemsize
@@ -96,7 +98,7 @@ def cpu_baseline(wl, gpu_iters, sample_iters, cores):
         "unit": "GB/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"{sample_iters} FP32 projection iteration(s) + 1 FP64 certification of the full "
+        "sample": f"{sample_iters} {x.dtype} projection iteration(s) + 1 FP64 certification of the full "
                   f"{'x'.join(map(str, shape))} field (numpy/scipy pocketfft, workers={cores}); "
                   f"t_iter={t_iter:.3f}s t_cert={t_cert:.3f}s, scaled to {gpu_iters} iteration(s) + 2 certifications",
     }
@@ -121,7 +123,7 @@ def main():
 
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     import ffcz_amd
-    from ffcz_amd import ProjectionSolver, MODE_ABS, QUANT_STEP
+    from ffcz_amd import ProjectionSolver, MODE_ABS, MODE_DIRECT, QUANT_STEP
     from ffcz_amd.workloads import make_config
 
     ffcz_amd.load()
@@ -134,14 +136,25 @@ def main():
     print(f"[bench] workload {a.config} shape={shape} eps={wl.eps:.3e} delta={wl.delta:.3e}", file=sys.stderr,
           flush=True)
 
-    solver = ProjectionSolver(*shape, dtype=np.float32)
-    solver.initialize_all_abs(wl.x, wl.eps, wl.delta)
+    np_dtype = np.float32 if wl.x.dtype == torch.float32 else np.float64
+    solver = ProjectionSolver(*shape, dtype=np_dtype)
+    direct = wl.delta_k is not None
+    if direct:
+        # per-bin bounds on every half-spectrum bin (compact index order)
+        idx = torch.arange(wl.delta_k.numel(), device=dev, dtype=torch.int64)
+        solver.initialize(wl.x, idx, wl.delta_k, wl.eps, wl.delta_k.numel())
+        fstep = float(wl.delta_k.min()) / 8
+        del idx
+    else:
+        solver.initialize_all_abs(wl.x, wl.eps, wl.delta)
+        fstep = wl.delta / 8
+    mode = MODE_DIRECT if direct else MODE_ABS
     solver.set_margins(0.125, 0.125)
-    solver.set_quantizer(QUANT_STEP, wl.delta / 8, wl.eps / 8)
+    solver.set_quantizer(QUANT_STEP, fstep, wl.eps / 8)
 
     def step():
         solver.reset_edit_accumulators()
-        solver.solve_loop(MODE_ABS, wl.e0, wl.max_iter, 1e-7)
+        solver.solve_loop(mode, wl.e0, wl.max_iter, 1e-7)
         solver.extract_compact_representation()
         c1 = solver.certify(False)
         c2 = solver.certify(True)
@@ -187,6 +200,14 @@ def main():
         cores = min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(wl, max(iters, 1), a.cpu_sample_iters, cores)
 
+    prec = "f32" if np_dtype == np.float32 else "f64"
+    if direct:
+        bounds = (f"eps={wl.eps:.3e}, per-bin delta_k in "
+                  f"[{float(wl.delta_k.min()):.3e}, {float(wl.delta_k.max()):.3e}]")
+    else:
+        bounds = f"eps={wl.eps:.3e} delta={wl.delta:.3e}"
+    workload_desc = (f"{a.config}: {'x'.join(map(str, shape))} {prec} Gaussian random field, "
+                     f"{'per-bin' if direct else 'abs'} bounds {bounds}, {prec.upper()} vector-ALU FFT")
     out = {
         "metric": "certified GB/s (both bounds met) on 1 GPU, MFMA split-prec vs plain FP32 FFT",
         "value": round(value, 3),
@@ -198,10 +219,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32" if np_dtype == np.float32 else "f64",
         "data": "synthetic",
-        "config": {"workload": f"{a.config}: {'x'.join(map(str, shape))} f32 Gaussian random field, abs bounds "
-                               f"eps={wl.eps:.3e} delta={wl.delta:.3e}, plain FP32 vector-ALU FFT",
+        "config": {"workload": workload_desc,
                    "fft": "valu-fp32", "iterations": iters, "converged": conv, "certified": ok,
                    "seed": a.seed, "margins": 0.125, "quantizer": "step eps/8, delta/8"},
         "roofline": roof,
