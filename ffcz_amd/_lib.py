@@ -24,6 +24,8 @@ FFCZ_F32 = 0
 FFCZ_F64 = 1
 MODE_DIRECT, MODE_ABS, MODE_PTW = 0, 1, 2
 QUANT_REF16, QUANT_STEP = 1, 2
+FFT_VALU, FFT_BF16X3, FFT_FP16X3 = 0, 1, 2
+FFT_VARIANTS = {"valu-fp32": FFT_VALU, "mfma-bf16x3": FFT_BF16X3, "mfma-fp16x3": FFT_FP16X3}
 
 
 class FFCzError(RuntimeError):
@@ -88,6 +90,8 @@ SIGNATURES = {
     "ffcz_get_edit_range": [_h, C.POINTER(_d), C.POINTER(_d), C.POINTER(_d), C.POINTER(_d)],
     "ffcz_set_margins": [_h, _d, _d],
     "ffcz_set_quantizer": [_h, _i, _d, _d],
+    "ffcz_set_fft_precision": [_h, _i],
+    "ffcz_get_fft_precision": [_h, C.POINTER(_i)],
     "ffcz_certify": [_h, _i, C.POINTER(Cert)],
     "ffcz_set_tuning": [_i, C.c_long],
     "ffcz_get_tuning": [C.POINTER(_i), C.POINTER(C.c_long)],
@@ -106,6 +110,8 @@ SIGNATURES = {
     "ffcz_huffman_zstd_decompress_u8": [_vp, _sz, _vp, _vp, _vp, _sz, _sz, _sz, _vp, _szp],
     "ffcz_fft_r2c": [_i, _sz, _sz, _sz, _vp, _vp, _vp],
     "ffcz_fft_c2r": [_i, _sz, _sz, _sz, _vp, _vp, _vp],
+    "ffcz_fft_r2c_prec": [_i, _i, _sz, _sz, _sz, _vp, _vp, _vp],
+    "ffcz_fft_c2r_prec": [_i, _i, _sz, _sz, _sz, _vp, _vp, _vp],
     "ffcz_k_project_spatial": [_i, _vp, _d, _vp, _sz, _vp],
     "ffcz_k_project_freq_abs": [_i, _vp, _d, _vp, _sz, _vp],
     "ffcz_k_convergence": [_i, _vp, _d, _sz, C.POINTER(_d)],

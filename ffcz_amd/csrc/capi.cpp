@@ -137,9 +137,13 @@ static void decompress(const uint8_t* comp, size_t clen, const uint32_t* sym, co
 }
 
 template <typename T>
-static void fft_entry(size_t nx, size_t ny, size_t nz, const void* in, void* out, hipStream_t s, bool fwd) {
+static void fft_entry(size_t nx, size_t ny, size_t nz, const void* in, void* out, hipStream_t s, bool fwd,
+                      int prec = PREC_VALU) {
+    FFCZ_REQUIRE(prec >= PREC_VALU && prec <= PREC_FP16X3, ERR_INVALID, "unknown FFT variant");
+    FFCZ_REQUIRE(prec == PREC_VALU || sizeof(T) == 4, ERR_INVALID, "the MFMA FFT variants are FP32 transforms");
     Fft3d<T> f;
     f.create(nx, ny, nz);
+    f.prec = prec;
     cplx<T>* spec = nullptr;
     try {
         FFCZ_HIP(hipMalloc(&spec, f.spec_elems() * sizeof(cplx<T>)));
@@ -359,6 +363,15 @@ int ffcz_set_margins(ffcz_solver* h, double sm, double fm) {
         with(h, [&](auto& s) { s.set_margins(sm, fm); });
     });
 }
+int ffcz_set_fft_precision(ffcz_solver* h, int variant) {
+    return guard([&] { with(h, [&](auto& s) { s.set_fft_precision(variant); }); });
+}
+int ffcz_get_fft_precision(ffcz_solver* h, int* variant) {
+    return guard([&] {
+        FFCZ_REQUIRE(variant != nullptr, ERR_INVALID, "null output");
+        with(h, [&](auto& s) { *variant = s.fft_precision(); });
+    });
+}
 int ffcz_set_quantizer(ffcz_solver* h, int kind, double fstep, double sstep) {
     return guard([&] { with(h, [&](auto& s) { s.set_quantizer(kind, fstep, sstep); }); });
 }
@@ -478,6 +491,22 @@ int ffcz_fft_c2r(int dtype, size_t nx, size_t ny, size_t nz, const void* in, voi
     return guard([&] {
         if (dtype == FFCZ_F32) fft_entry<float>(nx, ny, nz, in, out, as_stream(stream), false);
         else if (dtype == FFCZ_F64) fft_entry<double>(nx, ny, nz, in, out, as_stream(stream), false);
+        else throw Error(ERR_INVALID, "bad dtype");
+    });
+}
+int ffcz_fft_r2c_prec(int dtype, int prec, size_t nx, size_t ny, size_t nz, const void* in, void* out,
+                      void* stream) {
+    return guard([&] {
+        if (dtype == FFCZ_F32) fft_entry<float>(nx, ny, nz, in, out, as_stream(stream), true, prec);
+        else if (dtype == FFCZ_F64) fft_entry<double>(nx, ny, nz, in, out, as_stream(stream), true, prec);
+        else throw Error(ERR_INVALID, "bad dtype");
+    });
+}
+int ffcz_fft_c2r_prec(int dtype, int prec, size_t nx, size_t ny, size_t nz, const void* in, void* out,
+                      void* stream) {
+    return guard([&] {
+        if (dtype == FFCZ_F32) fft_entry<float>(nx, ny, nz, in, out, as_stream(stream), false, prec);
+        else if (dtype == FFCZ_F64) fft_entry<double>(nx, ny, nz, in, out, as_stream(stream), false, prec);
         else throw Error(ERR_INVALID, "bad dtype");
     });
 }

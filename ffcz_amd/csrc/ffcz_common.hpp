@@ -11,6 +11,11 @@
 
 namespace ffcz {
 
+// FFT pass variants of the FP32 iteration: vector-ALU FP32, or radix-16
+// stages as DFT-matrix GEMMs on MFMA in split BF16x3 / power-of-two-scaled
+// split FP16x3 with FP32 accumulation.
+enum FftPrec : int { PREC_VALU = 0, PREC_BF16X3 = 1, PREC_FP16X3 = 2 };
+
 struct Error : std::runtime_error {
     int code;
     Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}

@@ -490,6 +490,7 @@ static ZArgs<R, T> zargs(const Fft3d<R>& f) {
     a.pl = f.pz;
     a.tz = f.d_tz;
     a.tz2 = f.d_tz2;
+    a.prec = f.prec;
     const uint32_t M = (uint32_t)(f.nz / 2), nh = M + 1;
     a.dP = FastDiv((uint32_t)f.Pz);
     a.dM = FastDiv(M ? M : 1);
@@ -553,6 +554,7 @@ static SArgs<R, T> sargs(const Fft3d<R>& f, cplx<R>* data, bool xaxis) {
     a.F = f.F;
     a.cs_n = f.ny * f.Nh;
     a.cs_o = f.Nh;
+    a.prec = f.prec;
     return a;
 }
 template <typename R, typename T>

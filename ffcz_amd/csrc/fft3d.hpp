@@ -60,6 +60,7 @@ struct Fft3d {
     int Lz = 1;         // complex length of the z transform (nz/2 or nz)
     FftPlan pz{}, py{}, px{};
     int Pz = 1, Wy = 1, Wx = 1;       // pencils per z tile, columns per y/x tile
+    int prec = 0;                     // FftPrec of the FP32 pass kernels (0: vector ALU)
     cplx<R>* d_tz = nullptr;   // W_Lz^m
     cplx<R>* d_tz2 = nullptr;  // W_nz^k, k <= nz/2 (R2C post / C2R pre)
     cplx<R>* d_ty = nullptr;

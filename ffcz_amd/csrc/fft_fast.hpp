@@ -29,6 +29,12 @@ FFCZ_FAST_DECL(double, double)
 FFCZ_FAST_DECL(double, float)
 #undef FFCZ_FAST_DECL
 
+// MFMA split-precision variants of the float passes (fft_fast_mfma.hip):
+// false when a.prec is PREC_VALU or the length has no radix-16 kernel.
+bool mfma_zfwd(int mode, const ZArgs<float, float>& a, hipStream_t s);
+bool mfma_zinv(int mode, const ZArgs<float, float>& a, hipStream_t s);
+bool mfma_strided(int mode, const SArgs<float, float>& a, size_t outer, hipStream_t s);
+
 // Process-wide tuning knobs (C ABI: ffcz_set_tuning).
 struct Tuning {
     std::atomic<int> force_generic{0};  // 1: run the generic kernels only

@@ -7,14 +7,17 @@ namespace ffcz {
 
 template <>
 bool fast_zfwd<float, float>(int mode, const ZArgs<float, float>& a, hipStream_t s) {
+    if (a.prec != PREC_VALU && mfma_zfwd(mode, a, s)) return true;
     return zfwd_dispatch<float, float, (1u << Z_PLAIN) | (1u << Z_ITER)>(mode, a, s);
 }
 template <>
 bool fast_zinv<float, float>(int mode, const ZArgs<float, float>& a, hipStream_t s) {
+    if (a.prec != PREC_VALU && mfma_zinv(mode, a, s)) return true;
     return zinv_dispatch<float, float, (1u << Z_PLAIN) | (1u << Z_ITER)>(mode, a, s);
 }
 template <>
 bool fast_strided<float, float>(int mode, const SArgs<float, float>& a, size_t outer, hipStream_t s) {
+    if (a.prec != PREC_VALU && mfma_strided(mode, a, outer, s)) return true;
     return strided_dispatch<float, float, (1u << S_PLAIN) | (1u << S_CLAMP)>(mode, a, outer, s);
 }
 

@@ -4,7 +4,7 @@
 #pragma once
 
 #include "fft_fast.hpp"
-#include "fft_pencil.hpp"
+#include "fft_mfma.hpp"
 
 namespace ffcz {
 
@@ -31,7 +31,7 @@ constexpr int z_rows(int LZ) {
 // FFT ends in LDS and the R2C split X[k] = A[k] + W_nz^k B[k] writes the
 // padded half spectrum.
 // ---------------------------------------------------------------------------
-template <typename R, typename T, int LZ, int MODE>
+template <typename R, typename T, int LZ, int MODE, int PREC = PREC_VALU>
 __global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ)) void k2_zfwd(ZArgs<R, T> a) {
     constexpr int TP = z_tp(LZ), P = z_rows(LZ), NT = TP * P;
     constexpr int NZ = 2 * LZ, M = LZ, NH = M + 1, SN = P + 1;
@@ -89,7 +89,12 @@ __global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ)) void k2_zfwd(ZArgs<R, T> a) 
     };
     auto lld = [&](int n) { return buf[n * SN + p]; };
     auto lst = [&](int n, cplx<R> v) { buf[n * SN + p] = v; };
-    pen::transform<R, LZ, TP, false, true>(gld, lst, lld, lst, tw, t, -1);
+    if constexpr (PREC == PREC_VALU) {
+        pen::transform<R, LZ, TP, false, true>(gld, lst, lld, lst, tw, t, -1);
+    } else {
+        const mx::Bmat<PREC> bm = mx::dft16_operand<PREC>(tid & 63);
+        mx::transform<PREC, LZ, TP, SN, P, NT, false, true>(gld, lst, lld, lst, buf, tw, bm, tid, t, -1);
+    }
     if constexpr (MODE == Z_CERT) {
         block_count_atomic(nviol, &a.stats->spatial_viol);
         block_max_atomic<double, NT>(vmax, &a.stats->spatial_max_bits);
@@ -117,7 +122,7 @@ __global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ)) void k2_zfwd(ZArgs<R, T> a) 
 // the real row scaled by 1/N, and ITER tracks the spatial violation and
 // finalises the iteration in the last workgroup.
 // ---------------------------------------------------------------------------
-template <typename R, typename T, int LZ, int MODE>
+template <typename R, typename T, int LZ, int MODE, int PREC = PREC_VALU>
 __global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ)) void k2_zinv(ZArgs<R, T> a) {
     constexpr int TP = z_tp(LZ), P = z_rows(LZ), NT = TP * P;
     constexpr int NZ = 2 * LZ, M = LZ, NH = M + 1, SN = P + 1;
@@ -161,14 +166,19 @@ __global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ)) void k2_zinv(ZArgs<R, T> a) 
     };
     auto lld = [&](int n) { return buf[n * SN + p]; };
     auto lst = [&](int n, cplx<R> v) { buf[n * SN + p] = v; };
-    pen::transform<R, LZ, TP, false, false>(gld, gst, lld, lst, tw, t, +1);
+    if constexpr (PREC == PREC_VALU) {
+        pen::transform<R, LZ, TP, false, false>(gld, gst, lld, lst, tw, t, +1);
+    } else {
+        const mx::Bmat<PREC> bm = mx::dft16_operand<PREC>(tid & 63);
+        mx::transform<PREC, LZ, TP, SN, P, NT, false, false>(gld, gst, lld, lst, buf, tw, bm, tid, t, +1);
+    }
     if constexpr (MODE == Z_ITER) block_max_store<R, NT>(vm, &a.vpart[a.row_off / P + blockIdx.x]);
 }
 
 // ---------------------------------------------------------------------------
 // y / x passes over 128-byte column tiles.
 // ---------------------------------------------------------------------------
-template <typename R, typename T, int L, int MODE>
+template <typename R, typename T, int L, int MODE, int PREC = PREC_VALU>
 __global__ __launch_bounds__(pen::threads_per_pencil(L) * s_width<R>()) void k2_strided(SArgs<R, T> a) {
     constexpr int TP = pen::threads_per_pencil(L), W = s_width<R>(), NT = TP * W;
     __shared__ cplx<R> tw[L];
@@ -198,19 +208,26 @@ __global__ __launch_bounds__(pen::threads_per_pencil(L) * s_width<R>()) void k2_
     };
     auto lld = [&](int n) { return buf[n * W + p]; };
     auto lst = [&](int n, cplx<R> v) { buf[n * W + p] = v; };
+    [[maybe_unused]] mx::Bmat<PREC> bm;
+    if constexpr (PREC != PREC_VALU) bm = mx::dft16_operand<PREC>(tid & 63);
     if constexpr (MODE == S_PLAIN || MODE == S_LOAD_DF) {
-        pen::transform<R, L, TP, false, false>(gld, gst, lld, lst, tw, t, MODE == S_LOAD_DF ? +1 : a.sign);
+        const int sg = MODE == S_LOAD_DF ? +1 : a.sign;
+        if constexpr (PREC == PREC_VALU) pen::transform<R, L, TP, false, false>(gld, gst, lld, lst, tw, t, sg);
+        else mx::transform<PREC, L, TP, W, W, NT, false, false>(gld, gst, lld, lst, buf, tw, bm, tid, t, sg);
     } else if constexpr (MODE == S_CLAMP) {
         bool fired = false;
         auto cst = [&](int n, cplx<R> v) {
             if (valid) fired |= freq_clamp_bin(v, (size_t)n * a.cs_n + o * a.cs_o + col, a);
             buf[n * W + p] = v;
         };
-        pen::transform<R, L, TP, false, true>(gld, cst, lld, lst, tw, t, -1);
+        if constexpr (PREC == PREC_VALU) pen::transform<R, L, TP, false, true>(gld, cst, lld, lst, tw, t, -1);
+        else mx::transform<PREC, L, TP, W, W, NT, false, false>(gld, cst, lld, lst, buf, tw, bm, tid, t, -1);
         if (fired) a.st->freq_clamped = 1u;
         __syncthreads();
-        pen::transform<R, L, TP, true, false>(lld, gst, lld, lst, tw, t, +1);
-    } else {  // S_CERT_CHECK
+        if constexpr (PREC == PREC_VALU) pen::transform<R, L, TP, true, false>(lld, gst, lld, lst, tw, t, +1);
+        else mx::transform<PREC, L, TP, W, W, NT, true, false>(lld, gst, lld, lst, buf, tw, bm, tid, t, +1);
+    } else {
+        static_assert(PREC == PREC_VALU, "certification runs on the vector ALU");  // S_CERT_CHECK
         unsigned long long nv = 0;
         double vmax = 0.0;
         auto chk = [&](int n, cplx<R> v) {
@@ -232,24 +249,24 @@ __global__ __launch_bounds__(pen::threads_per_pencil(L) * s_width<R>()) void k2_
 // ---------------------------------------------------------------------------
 // dispatch
 // ---------------------------------------------------------------------------
-template <typename R, typename T, int MODE, int LZ>
+template <typename R, typename T, int MODE, int LZ, int PREC = PREC_VALU>
 inline void launch_zfwd_len(const ZArgs<R, T>& a, hipStream_t s) {
     const unsigned grid = (unsigned)ceil_div(a.rows, (size_t)z_rows(LZ));
-    hipLaunchKernelGGL((k2_zfwd<R, T, LZ, MODE>), dim3(grid), dim3(z_tp(LZ) * z_rows(LZ)), 0, s, a);
+    hipLaunchKernelGGL((k2_zfwd<R, T, LZ, MODE, PREC>), dim3(grid), dim3(z_tp(LZ) * z_rows(LZ)), 0, s, a);
 }
-template <typename R, typename T, int MODE, int LZ>
+template <typename R, typename T, int MODE, int LZ, int PREC = PREC_VALU>
 inline void launch_zinv_len(const ZArgs<R, T>& a, hipStream_t s) {
     const unsigned grid = (unsigned)ceil_div(a.rows, (size_t)z_rows(LZ));
-    hipLaunchKernelGGL((k2_zinv<R, T, LZ, MODE>), dim3(grid), dim3(z_tp(LZ) * z_rows(LZ)), 0, s, a);
+    hipLaunchKernelGGL((k2_zinv<R, T, LZ, MODE, PREC>), dim3(grid), dim3(z_tp(LZ) * z_rows(LZ)), 0, s, a);
 }
-template <typename R, typename T, int MODE, int L>
+template <typename R, typename T, int MODE, int L, int PREC = PREC_VALU>
 inline void launch_strided_len(SArgs<R, T> a, size_t outer, hipStream_t s) {
     constexpr int W = s_width<R>();
     a.W = W;
     a.nct = (int)ceil_div((size_t)a.ncols, (size_t)W);
     a.dNct = FastDiv((uint32_t)a.nct);
     const unsigned grid = (unsigned)(outer * (size_t)a.nct);
-    hipLaunchKernelGGL((k2_strided<R, T, L, MODE>), dim3(grid), dim3(pen::threads_per_pencil(L) * W), 0, s, a);
+    hipLaunchKernelGGL((k2_strided<R, T, L, MODE, PREC>), dim3(grid), dim3(pen::threads_per_pencil(L) * W), 0, s, a);
 }
 
 template <typename R, typename T, int MODE>
@@ -330,6 +347,62 @@ bool strided_dispatch(int mode, const SArgs<R, T>& a, size_t outer, hipStream_t 
         if constexpr ((MASK & (1u << S_LOAD_DF)) != 0) return strided_by_len<R, T, S_LOAD_DF>(a, outer, s);
     }
     return false;
+}
+
+// ---------------------------------------------------------------------------
+// MFMA split-precision variants (R = T = float): lengths with a radix-16
+// factor whose tiles fill whole waves.  The z kernels keep the VALU kernels'
+// rows per workgroup, so the per-workgroup violation slots line up.
+// ---------------------------------------------------------------------------
+#define FFCZ_MFMA_S_LENGTHS(X) X(32) X(64) X(96) X(128) X(256) X(512)
+#define FFCZ_MFMA_Z_LENGTHS(X) X(16) X(32) X(48) X(64) X(128) X(256)
+
+template <int PREC>
+bool mfma_zfwd_dispatch(int mode, const ZArgs<float, float>& a, hipStream_t s) {
+    if (!a.half || (mode != Z_PLAIN && mode != Z_ITER)) return false;
+    if (mode == Z_PLAIN && ((uintptr_t)a.in % 8) != 0) return false;
+    switch (a.Lz) {
+#define FFCZ_MZF(Lv)                                                                                      \
+    case Lv:                                                                                              \
+        if (mode == Z_ITER) launch_zfwd_len<float, float, Z_ITER, Lv, PREC>(a, s);                        \
+        else launch_zfwd_len<float, float, Z_PLAIN, Lv, PREC>(a, s);                                      \
+        return true;
+        FFCZ_MFMA_Z_LENGTHS(FFCZ_MZF)
+#undef FFCZ_MZF
+        default:
+            return false;
+    }
+}
+template <int PREC>
+bool mfma_zinv_dispatch(int mode, const ZArgs<float, float>& a, hipStream_t s) {
+    if (!a.half || (mode != Z_PLAIN && mode != Z_ITER)) return false;
+    if (mode == Z_PLAIN && ((uintptr_t)a.out % 8) != 0) return false;
+    switch (a.Lz) {
+#define FFCZ_MZI(Lv)                                                                                      \
+    case Lv:                                                                                              \
+        if (mode == Z_ITER) launch_zinv_len<float, float, Z_ITER, Lv, PREC>(a, s);                        \
+        else launch_zinv_len<float, float, Z_PLAIN, Lv, PREC>(a, s);                                      \
+        return true;
+        FFCZ_MFMA_Z_LENGTHS(FFCZ_MZI)
+#undef FFCZ_MZI
+        default:
+            return false;
+    }
+}
+template <int PREC>
+bool mfma_strided_dispatch(int mode, const SArgs<float, float>& a, size_t outer, hipStream_t s) {
+    if (mode != S_PLAIN && mode != S_CLAMP) return false;
+    switch (a.L) {
+#define FFCZ_MSL(Lv)                                                                                      \
+    case Lv:                                                                                              \
+        if (mode == S_CLAMP) launch_strided_len<float, float, S_CLAMP, Lv, PREC>(a, outer, s);            \
+        else launch_strided_len<float, float, S_PLAIN, Lv, PREC>(a, outer, s);                            \
+        return true;
+        FFCZ_MFMA_S_LENGTHS(FFCZ_MSL)
+#undef FFCZ_MSL
+        default:
+            return false;
+    }
 }
 
 }  // namespace ffcz

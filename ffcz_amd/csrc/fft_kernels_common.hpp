@@ -83,6 +83,7 @@ enum ZMode : int { Z_PLAIN = 0, Z_ITER = 1, Z_CERT = 2 };
 template <typename R, typename T>
 struct ZArgs {
     int nz, Lz, M, half, P, sn;
+    int prec;             // FftPrec (float passes only)
     FastDiv dP, dM, dNh, dNp, dNz, dFill;  // P, M, nz/2+1, M/2+1, nz, nz-(nz/2+1)
     size_t rows, Mp;
     FftPlan pl;
@@ -116,6 +117,7 @@ struct SArgs {
     FftPlan pl;
     const cplx<R>* tw;
     int L, W, wshift, ncols, nct;
+    int prec;             // FftPrec (float passes only)
     FastDiv dW, dNct;
     size_t ob, es;
     int sign;

@@ -50,11 +50,10 @@ constexpr int nsub(int L, int s) {
 // threads per pencil: ~8 elements per thread up to L = 512, ~16 beyond
 constexpr int threads_per_pencil(int L) { return L <= 1 ? 1 : (L <= 512 ? (L + 7) / 8 : (L + 15) / 16); }
 
-template <typename R, int L, int TP, int S, bool SYNC_MID, class LD, class ST>
-__device__ __forceinline__ void run_stage(const LD& ld, const ST& st, const cplx<R>* __restrict__ tw, int t,
-                                          int sign) {
-    constexpr int RAD = radix(L, S);
-    constexpr int NS = nsub(L, S);
+// One Stockham stage of radix RAD with sub-transform length NS before it.
+template <typename R, int L, int TP, int RAD, int NS, bool SYNC_MID, class LD, class ST>
+__device__ __forceinline__ void stage_rn(const LD& ld, const ST& st, const cplx<R>* __restrict__ tw, int t,
+                                         int sign) {
     constexpr int LR = L / RAD;
     constexpr int NB = (LR + TP - 1) / TP;
     constexpr int TWS = L / (NS * RAD);
@@ -88,6 +87,12 @@ __device__ __forceinline__ void run_stage(const LD& ld, const ST& st, const cplx
             for (int r = 0; r < RAD; ++r) st(base + r * NS, v[b][r]);
         }
     }
+}
+
+template <typename R, int L, int TP, int S, bool SYNC_MID, class LD, class ST>
+__device__ __forceinline__ void run_stage(const LD& ld, const ST& st, const cplx<R>* __restrict__ tw, int t,
+                                          int sign) {
+    stage_rn<R, L, TP, radix(L, S), nsub(L, S), SYNC_MID>(ld, st, tw, t, sign);
 }
 
 // Full transform of one pencil.  gld/gst: first-stage loader and last-stage

@@ -407,6 +407,14 @@ void Solver<T>::time_iteration(int reps, float* ms5) {
     FFCZ_HIP(hipStreamSynchronize(s_));
 }
 
+template <typename T>
+void Solver<T>::set_fft_precision(int prec) {
+    FFCZ_REQUIRE(prec >= PREC_VALU && prec <= PREC_FP16X3, ERR_INVALID, "unknown FFT precision variant");
+    FFCZ_REQUIRE(prec == PREC_VALU || sizeof(T) == 4, ERR_INVALID,
+                 "the MFMA split-precision FFT variants are FP32 transforms; FP64 solvers use the vector ALU");
+    f_.prec = prec;
+}
+
 template class Solver<float>;
 template class Solver<double>;
 

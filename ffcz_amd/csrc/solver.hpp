@@ -85,6 +85,9 @@ class Solver {
     hipStream_t stream() const { return s_; }
     // per-kernel timing of the iteration (for roofline reporting)
     const Fft3d<T>& fft() const { return f_; }
+    // FftPrec of the iteration's pass kernels (float solvers only)
+    void set_fft_precision(int prec);
+    int fft_precision() const { return f_.prec; }
     IterArgs<T> iter_args() const;
     QuantDesc freq_quant() const;
     QuantDesc spatial_quant() const;

@@ -16,18 +16,21 @@ def _dt(t: torch.Tensor) -> int:
     raise TypeError(f"unsupported dtype {t.dtype}")
 
 
-def fft_r2c(x: torch.Tensor) -> torch.Tensor:
+def fft_r2c(x: torch.Tensor, variant: int = 0) -> torch.Tensor:
+    """3D R2C of a device field (cuFFT layout, unnormalised); ``variant``
+    selects the FFT kernels (0 vector ALU, 1 MFMA BF16x3, 2 MFMA FP16x3)."""
     assert x.is_cuda and x.dim() == 3
     x = x.contiguous()
     nx, ny, nz = x.shape
     cdt = torch.complex64 if x.dtype == torch.float32 else torch.complex128
     out = torch.empty((nx, ny, nz // 2 + 1), dtype=cdt, device=x.device)
     torch.cuda.synchronize(x.device)
-    call("ffcz_fft_r2c", _dt(x), nx, ny, nz, x.data_ptr(), out.data_ptr(), None)
+    call("ffcz_fft_r2c_prec", _dt(x), int(variant), nx, ny, nz, x.data_ptr(), out.data_ptr(), None)
     return out
 
 
-def fft_c2r(X: torch.Tensor, nz: int) -> torch.Tensor:
+def fft_c2r(X: torch.Tensor, nz: int, variant: int = 0) -> torch.Tensor:
+    """3D C2R (unnormalised) of a device half spectrum; ``variant`` as for fft_r2c."""
     assert X.is_cuda and X.dim() == 3
     X = X.contiguous()
     nx, ny, nh = X.shape
@@ -35,5 +38,5 @@ def fft_c2r(X: torch.Tensor, nz: int) -> torch.Tensor:
     rdt = torch.float32 if X.dtype == torch.complex64 else torch.float64
     out = torch.empty((nx, ny, nz), dtype=rdt, device=X.device)
     torch.cuda.synchronize(X.device)
-    call("ffcz_fft_c2r", _dt(X), nx, ny, nz, X.data_ptr(), out.data_ptr(), None)
+    call("ffcz_fft_c2r_prec", _dt(X), int(variant), nx, ny, nz, X.data_ptr(), out.data_ptr(), None)
     return out

@@ -17,7 +17,8 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._lib import FFCZ_F32, FFCZ_F64, MODE_ABS, MODE_DIRECT, MODE_PTW, QUANT_REF16, QUANT_STEP, call
+from ._lib import (FFCZ_F32, FFCZ_F64, FFT_BF16X3, FFT_FP16X3, FFT_VALU, FFT_VARIANTS, MODE_ABS, MODE_DIRECT,  # noqa: F401
+                   MODE_PTW, QUANT_REF16, QUANT_STEP, call)
 
 
 def _np_dtype(dtype) -> np.dtype:
@@ -224,6 +225,18 @@ class ProjectionSolver:
     # -- additions: certified output ----------------------------------------
     def set_margins(self, spatial: float, freq: float):
         call("ffcz_set_margins", self._h, float(spatial), float(freq))
+
+    def set_fft_precision(self, variant):
+        """FFT variant of the iteration: FFT_VALU, FFT_BF16X3, FFT_FP16X3 or
+        one of their names ("valu-fp32", "mfma-bf16x3", "mfma-fp16x3")."""
+        if isinstance(variant, str):
+            variant = FFT_VARIANTS[variant]
+        call("ffcz_set_fft_precision", self._h, int(variant))
+
+    def fft_precision(self) -> int:
+        v = C.c_int()
+        call("ffcz_get_fft_precision", self._h, C.byref(v))
+        return v.value
 
     def set_quantizer(self, kind: int, freq_step: float = 0.0, spatial_step: float = 0.0):
         call("ffcz_set_quantizer", self._h, int(kind), float(freq_step), float(spatial_step))

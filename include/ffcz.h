@@ -129,6 +129,18 @@ int ffcz_get_edit_range(ffcz_solver* h, double* fmin, double* fmax, double* smin
 int ffcz_set_margins(ffcz_solver* h, double spatial_margin, double freq_margin);
 int ffcz_set_quantizer(ffcz_solver* h, int kind, double freq_step, double spatial_step);
 int ffcz_certify(ffcz_solver* h, int quantized, ffcz_cert* out);
+/* FFT variant of the projection iteration (an addition; the reference runs
+ * cuFFT at projection_algorithm_impl.cuh:205-214, 228-238):
+ * FFCZ_FFT_VALU   FP32 vector-ALU Stockham passes (default)
+ * FFCZ_FFT_BF16X3 radix-16 stages as DFT-matrix GEMMs on MFMA, split BF16x3
+ * FFCZ_FFT_FP16X3 the same in split FP16x3 with per-butterfly power-of-two scaling
+ * Lengths without a radix-16 factor keep the vector-ALU kernels.  Float
+ * solvers only; certification always runs in FP64 on the vector ALU. */
+#define FFCZ_FFT_VALU 0
+#define FFCZ_FFT_BF16X3 1
+#define FFCZ_FFT_FP16X3 2
+int ffcz_set_fft_precision(ffcz_solver* h, int variant);
+int ffcz_get_fft_precision(ffcz_solver* h, int* variant);
 /* process-wide tuning: force_generic=1 runs the run-time-planned FFT kernels
  * instead of the length-specialised ones; slab_mb sets the slab size of the
  * z/y pass interleave (<= 0: whole field).  Negative force_generic leaves it. */
@@ -171,6 +183,11 @@ int ffcz_huffman_zstd_decompress_u8(const uint8_t* comp, size_t comp_len, const 
  * projection_algorithm_impl.cuh:114-120, 155-161, 181-187 */
 int ffcz_fft_r2c(int dtype, size_t nx, size_t ny, size_t nz, const void* in, void* out_compact, void* stream);
 int ffcz_fft_c2r(int dtype, size_t nx, size_t ny, size_t nz, const void* in_compact, void* out, void* stream);
+/* the same transforms with an FFT variant (FFCZ_FFT_*; dtype must be f32 for the MFMA ones) */
+int ffcz_fft_r2c_prec(int dtype, int variant, size_t nx, size_t ny, size_t nz, const void* in, void* out_compact,
+                      void* stream);
+int ffcz_fft_c2r_prec(int dtype, int variant, size_t nx, size_t ny, size_t nz, const void* in_compact, void* out,
+                      void* stream);
 /* project_spatial_constraints_direct   projection_algorithm.cu:335-361 */
 int ffcz_k_project_spatial(int dtype, void* e, double eps, void* ds, size_t n, void* stream);
 /* project_frequency_constraints_all_abs projection_algorithm.cu:147-188 (compact spectrum, F bins) */
