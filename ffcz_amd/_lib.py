@@ -90,6 +90,12 @@ SIGNATURES = {
     "ffcz_get_edit_range": [_h, C.POINTER(_d), C.POINTER(_d), C.POINTER(_d), C.POINTER(_d)],
     "ffcz_set_margins": [_h, _d, _d],
     "ffcz_set_quantizer": [_h, _i, _d, _d],
+    "ffcz_batch_create": [_i, _sz, C.POINTER(_sz), C.POINTER(_h)],
+    "ffcz_batch_destroy": [_h],
+    "ffcz_batch_size": [_h, _szp],
+    "ffcz_batch_field": [_h, _sz, C.POINTER(_h)],
+    "ffcz_batch_solve": [_h, _i, C.POINTER(_vp), _i, _d, _i],
+    "ffcz_batch_status": [_h, C.POINTER(_i), C.POINTER(_d), C.POINTER(_i), _sz, C.POINTER(_i)],
     "ffcz_set_fft_precision": [_h, _i],
     "ffcz_get_fft_precision": [_h, C.POINTER(_i)],
     "ffcz_certify": [_h, _i, C.POINTER(Cert)],

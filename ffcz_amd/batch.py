@@ -1,0 +1,86 @@
+"""Python mirror of the batch C ABI: many fields of uneven shapes corrected
+together (one launch per pass and axis length over all fields, per-field
+convergence flags on the device, the iteration replayed as a HIP graph).
+
+The reference has no batch interface; each field keeps the single-field
+``ProjectionSolver`` API (initialize*, getters, extraction, certification).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import FFCZ_F32, FFCZ_F64, call
+from .solver import ProjectionSolver, _np_dtype, as_pointer
+
+
+class ProjectionBatch:
+    def __init__(self, shapes, dtype=np.float32):
+        self.dtype = _np_dtype(dtype)
+        self.shapes = [tuple(int(v) for v in s) for s in shapes]
+        if not self.shapes:
+            raise ValueError("a batch needs at least one field")
+        dims = (C.c_size_t * (3 * len(self.shapes)))(*[v for s in self.shapes for v in s])
+        h = C.c_void_p()
+        call("ffcz_batch_create", FFCZ_F32 if self.dtype == np.float32 else FFCZ_F64, len(self.shapes), dims,
+             C.byref(h))
+        self._h = h
+        self.fields = []
+        for i, s in enumerate(self.shapes):
+            fh = C.c_void_p()
+            call("ffcz_batch_field", self._h, i, C.byref(fh))
+            self.fields.append(ProjectionSolver._borrowed(fh, s, self.dtype, self))
+
+    def __len__(self):
+        return len(self.fields)
+
+    def __getitem__(self, i) -> ProjectionSolver:
+        return self.fields[i]
+
+    def solve(self, mode: int, spatial_errors, max_iterations: int, convergence_tolerance: float,
+              use_graph: bool = True):
+        """The projection loop of every field (no extraction); ``spatial_errors``
+        holds one initial error x' - x per field (numpy or torch)."""
+        if len(spatial_errors) != len(self.fields):
+            raise ValueError("one initial error per field")
+        keep = []
+        ptrs = (C.c_void_p * len(self.fields))()
+        for i, (f, e) in enumerate(zip(self.fields, spatial_errors)):
+            p, k = as_pointer(e, self.dtype)
+            if int(np.prod(k.shape)) != f.N:
+                raise ValueError(f"initial error {i} has the wrong size")
+            keep.append(k)
+            ptrs[i] = p.value
+        call("ffcz_batch_solve", self._h, int(mode), ptrs, int(max_iterations), float(convergence_tolerance),
+             int(bool(use_graph)))
+        del keep
+
+    def status(self):
+        """(iterations[], violations[], converged[], iterations the batch ran)."""
+        n = len(self.fields)
+        it = (C.c_int * n)()
+        vi = (C.c_double * n)()
+        cv = (C.c_int * n)()
+        launched = C.c_int()
+        call("ffcz_batch_status", self._h, it, vi, cv, n, C.byref(launched))
+        return list(it), list(vi), [bool(c) for c in cv], launched.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for f in self.fields:
+                f._h = None
+            call("ffcz_batch_destroy", self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover - best effort
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -28,7 +28,8 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result", "-munsafe-fp-atomics",
 ]
-SOURCES = ["fft3d.hip", "fft_fast_f32.hip", "fft_fast_f64.hip", "fft_fast_c32.hip", "fft_fast_mfma.hip", "solver.hip", "extract.hip",
+SOURCES = ["fft3d.hip", "fft_fast_f32.hip", "fft_fast_f64.hip", "fft_fast_c32.hip", "fft_fast_mfma.hip", "solver.hip",
+           "batch.hip", "extract.hip",
            "huffman.hip", "kernels.hip", "capi.cpp"]
 
 

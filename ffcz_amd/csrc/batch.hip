@@ -1,0 +1,81 @@
+// Batched projection loop (batch.hpp).
+#include <algorithm>
+
+#include "batch.hpp"
+
+namespace ffcz {
+
+template <typename T>
+Batch<T>::Batch(std::vector<Solver<T>*> fields) : f_(std::move(fields)) {
+    FFCZ_REQUIRE(!f_.empty(), ERR_INVALID, "a batch needs at least one field");
+    for (auto* p : f_) FFCZ_REQUIRE(p != nullptr, ERR_INVALID, "null field");
+    FFCZ_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+    FFCZ_HIP(hipHostMalloc((void**)&hdone_, f_.size() * sizeof(unsigned), hipHostMallocDefault));
+}
+
+template <typename T>
+Batch<T>::~Batch() {
+    release_graph();
+    batch_plan_free(plan_);
+    if (hdone_) (void)hipHostFree(hdone_);
+    if (s_) (void)hipStreamDestroy(s_);
+}
+
+template <typename T>
+void Batch<T>::release_graph() {
+    if (gexec_) (void)hipGraphExecDestroy(gexec_);
+    gexec_ = nullptr;
+}
+
+template <typename T>
+void Batch<T>::solve(int mode, const T* const* e0, int max_iter, T tol, bool use_graph) {
+    FFCZ_REQUIRE(e0 != nullptr, ERR_INVALID, "null initial-error table");
+    // the fields' own streams may still run their initialisation
+    for (auto* f : f_) FFCZ_HIP(hipStreamSynchronize(f->stream()));
+    for (size_t i = 0; i < f_.size(); ++i) f_[i]->prepare_solve(mode, e0[i], max_iter, tol, s_);
+    std::vector<const Fft3d<T>*> ffts;
+    std::vector<IterArgs<T>> args;
+    for (auto* f : f_) {
+        ffts.push_back(&f->fft());
+        args.push_back(f->iter_args());
+    }
+    release_graph();
+    batch_plan_build(plan_, ffts, args);
+    if (use_graph && max_iter > 0) {
+        hipGraph_t g = nullptr;
+        FFCZ_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
+        try {
+            batch_iteration(plan_, s_);
+        } catch (...) {
+            (void)hipStreamEndCapture(s_, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        FFCZ_HIP(hipStreamEndCapture(s_, &g));
+        const hipError_t e = hipGraphInstantiate(&gexec_, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        FFCZ_HIP(e);
+    }
+    int launched = 0, chunk = 1;
+    while (launched < max_iter) {
+        const int k = std::min(chunk, max_iter - launched);
+        for (int i = 0; i < k; ++i) {
+            if (gexec_) FFCZ_HIP(hipGraphLaunch(gexec_, s_));
+            else batch_iteration(plan_, s_);
+        }
+        launched += k;
+        batch_gather_done(plan_, hdone_, s_);
+        FFCZ_HIP(hipStreamSynchronize(s_));
+        bool all = true;
+        for (size_t i = 0; i < f_.size(); ++i) all = all && hdone_[i] != 0;
+        if (all) break;
+        chunk = std::min(chunk * 2, 16);
+    }
+    launched_ = launched;
+    for (auto* f : f_) f->finish_solve(s_);
+}
+
+template class Batch<float>;
+template class Batch<double>;
+
+}  // namespace ffcz

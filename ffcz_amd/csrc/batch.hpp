@@ -1,0 +1,44 @@
+// Many fields of uneven shapes corrected together.
+//
+// The fields are ordinary solvers (each initialised, extracted and certified
+// on its own); Batch::solve runs their projection loops in lock step: every
+// iteration is one launch per pass and axis length over all fields
+// (fft3d.hpp BatchPlan), each field gated by its own convergence state on the
+// device, and the iteration is captured once in a HIP graph that is replayed
+// between the host's convergence checks.
+#pragma once
+
+#include <vector>
+
+#include "solver.hpp"
+
+namespace ffcz {
+
+template <typename T>
+class Batch {
+   public:
+    explicit Batch(std::vector<Solver<T>*> fields);
+    ~Batch();
+    Batch(const Batch&) = delete;
+    Batch& operator=(const Batch&) = delete;
+
+    size_t size() const { return f_.size(); }
+    Solver<T>& field(size_t i) { return *f_.at(i); }
+    // solve*(e0[i], max_iter, tol) on every field (projection_algorithm_impl.cuh:772-858
+    // per field), without the trailing extraction.  use_graph: replay a captured
+    // iteration instead of launching its kernels one by one.
+    void solve(int mode, const T* const* e0, int max_iter, T tol, bool use_graph);
+    // iterations run by the last solve (the slowest field's count)
+    int launched() const { return launched_; }
+
+   private:
+    void release_graph();
+    std::vector<Solver<T>*> f_;
+    hipStream_t s_ = nullptr;
+    BatchPlan<T> plan_;
+    hipGraphExec_t gexec_ = nullptr;
+    unsigned* hdone_ = nullptr;  // pinned host copy of the done flags
+    int launched_ = 0;
+};
+
+}  // namespace ffcz

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../include/ffcz.h"
+#include "batch.hpp"
 #include "extract.hpp"
 #include "fft_fast.hpp"
 #include "huffman.hpp"
@@ -21,6 +22,14 @@ struct ffcz_solver {
     int dtype;
     std::unique_ptr<Solver<float>> f;
     std::unique_ptr<Solver<double>> d;
+    bool borrowed = false;  // a field of a batch: freed with the batch
+};
+
+struct ffcz_batch {
+    int dtype;
+    std::vector<std::unique_ptr<ffcz_solver>> fields;
+    std::unique_ptr<Batch<float>> bf;
+    std::unique_ptr<Batch<double>> bd;
 };
 
 struct ffcz_coded {
@@ -196,7 +205,78 @@ int ffcz_create(int dtype, size_t nx, size_t ny, size_t nz, ffcz_solver** out) {
 }
 
 int ffcz_destroy(ffcz_solver* h) {
-    return guard([&] { delete h; });
+    return guard([&] {
+        FFCZ_REQUIRE(h == nullptr || !h->borrowed, ERR_INVALID, "a batch field is destroyed with its batch");
+        delete h;
+    });
+}
+
+int ffcz_batch_create(int dtype, size_t nfields, const size_t* dims, ffcz_batch** out) {
+    return guard([&] {
+        FFCZ_REQUIRE(out != nullptr && dims != nullptr, ERR_INVALID, "null argument");
+        FFCZ_REQUIRE(nfields >= 1, ERR_INVALID, "a batch needs at least one field");
+        FFCZ_REQUIRE(dtype == FFCZ_F32 || dtype == FFCZ_F64, ERR_INVALID, "dtype must be FFCZ_F32 or FFCZ_F64");
+        std::unique_ptr<ffcz_batch> b(new ffcz_batch());
+        b->dtype = dtype;
+        std::vector<Solver<float>*> pf;
+        std::vector<Solver<double>*> pd;
+        for (size_t i = 0; i < nfields; ++i) {
+            std::unique_ptr<ffcz_solver> h(new ffcz_solver());
+            h->dtype = dtype;
+            h->borrowed = true;
+            if (dtype == FFCZ_F32) {
+                h->f.reset(new Solver<float>(dims[3 * i], dims[3 * i + 1], dims[3 * i + 2]));
+                pf.push_back(h->f.get());
+            } else {
+                h->d.reset(new Solver<double>(dims[3 * i], dims[3 * i + 1], dims[3 * i + 2]));
+                pd.push_back(h->d.get());
+            }
+            b->fields.push_back(std::move(h));
+        }
+        if (dtype == FFCZ_F32) b->bf.reset(new Batch<float>(pf));
+        else b->bd.reset(new Batch<double>(pd));
+        *out = b.release();
+    });
+}
+int ffcz_batch_destroy(ffcz_batch* b) {
+    return guard([&] { delete b; });
+}
+int ffcz_batch_size(ffcz_batch* b, size_t* n) {
+    return guard([&] {
+        FFCZ_REQUIRE(b != nullptr && n != nullptr, ERR_INVALID, "null argument");
+        *n = b->fields.size();
+    });
+}
+int ffcz_batch_field(ffcz_batch* b, size_t i, ffcz_solver** out) {
+    return guard([&] {
+        FFCZ_REQUIRE(b != nullptr && out != nullptr, ERR_INVALID, "null argument");
+        FFCZ_REQUIRE(i < b->fields.size(), ERR_RANGE, "field index out of range");
+        *out = b->fields[i].get();
+    });
+}
+int ffcz_batch_solve(ffcz_batch* b, int mode, const void* const* e0, int max_iter, double tol, int use_graph) {
+    return guard([&] {
+        FFCZ_REQUIRE(b != nullptr, ERR_INVALID, "null batch handle");
+        if (b->dtype == FFCZ_F32)
+            b->bf->solve(mode, reinterpret_cast<const float* const*>(e0), max_iter, (float)tol, use_graph != 0);
+        else
+            b->bd->solve(mode, reinterpret_cast<const double* const*>(e0), max_iter, tol, use_graph != 0);
+    });
+}
+int ffcz_batch_status(ffcz_batch* b, int* iterations, double* violations, int* converged, size_t cap,
+                      int* launched) {
+    return guard([&] {
+        FFCZ_REQUIRE(b != nullptr, ERR_INVALID, "null batch handle");
+        const size_t n = std::min(cap, b->fields.size());
+        for (size_t i = 0; i < n; ++i) {
+            with(b->fields[i].get(), [&](auto& s) {
+                if (iterations) iterations[i] = s.iterations();
+                if (violations) violations[i] = (double)s.violation();
+                if (converged) converged[i] = s.converged() ? 1 : 0;
+            });
+        }
+        if (launched) *launched = b->dtype == FFCZ_F32 ? b->bf->launched() : b->bd->launched();
+    });
 }
 
 #define INIT_PAIR(name, call)                                                                       \

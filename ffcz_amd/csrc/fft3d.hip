@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "fft3d.hpp"
@@ -124,13 +125,15 @@ __global__ __launch_bounds__(FFT_THREADS) void k_zfwd(ZArgs<R, T> a) {
     const size_t row0 = (size_t)blockIdx.x * a.P;
     const int P = a.P, sn = a.sn;
     const bool clamp = (MODE == Z_ITER) && a.st->iter > 0;
+    // the first iteration reads the initial error itself (e is written by the inverse pass)
+    const R* esrc = (MODE == Z_ITER && !clamp && a.e_init) ? a.e_init : a.e;
     unsigned long long nviol = 0;
     double vmax = 0.0;
     auto load = [&](size_t gi) -> R {
         if constexpr (MODE == Z_PLAIN) {
             return a.in[gi];
         } else if constexpr (MODE == Z_ITER) {
-            R v = a.e[gi];
+            R v = esrc[gi];
             if (clamp) clamp_acc(v, a.eps, &a.ds[gi]);
             return v;
         } else {
@@ -647,6 +650,7 @@ void iteration(const Fft3d<R>& f, const IterArgs<R>& ia, hipStream_t s, EventLog
         auto z = zargs<R, R>(f);
         const size_t r0 = x0 * f.ny;
         z.e = ia.e + r0 * f.nz;
+        z.e_init = ia.e_init ? ia.e_init + r0 * f.nz : nullptr;
         z.ds = ia.ds + r0 * f.nz;
         z.eps = ia.eps;
         z.st = ia.st;
@@ -699,6 +703,7 @@ template <typename R>
 void iteration_kernel(const Fft3d<R>& f, const IterArgs<R>& ia, hipStream_t s, int kind) {
     auto z = zargs<R, R>(f);
     z.e = ia.e;
+    z.e_init = ia.e_init;
     z.ds = ia.ds;
     z.eps = ia.eps;
     z.st = ia.st;
@@ -779,6 +784,229 @@ void certify(const Fft3d<double>& f, const CertArgs<T>& c, hipStream_t s) {
 }
 
 // explicit instantiations
+// ===========================================================================
+// batched iteration
+// ===========================================================================
+template <typename R>
+struct FinEntry {
+    const R* vpart;
+    size_t n;
+    DevState* st;
+};
+
+// one workgroup per field: the reduction and bookkeeping of k_finalize_iteration
+template <typename R>
+__global__ __launch_bounds__(1024) void k_finalize_batch(const FinEntry<R>* t) {
+    const FinEntry<R> e = t[blockIdx.x];
+    if (e.st->done) return;
+    R m = R(0);
+    for (size_t i = threadIdx.x; i < e.n; i += 1024) m = e.vpart[i] > m ? e.vpart[i] : m;
+    __shared__ R red[16];
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < 16; ++i) m = red[i] > m ? red[i] : m;
+        e.st->viol_bits = ordbits(red[0] > m ? red[0] : m);
+        finalize_iteration<R>(e.st);
+    }
+}
+
+__global__ void k_gather_done(DevState* const* sts, size_t n, unsigned* out) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = sts[i]->done;
+}
+
+template <typename R>
+static bool batchable(const Fft3d<R>& f) {
+    if constexpr (sizeof(R) != 4) {
+        return false;
+    } else {
+        return fast_fft_enabled() && f.prec == PREC_VALU && tuning().slab_mb.load() <= 0 && f.half &&
+               fast_z_rows(f.Lz, f.half) > 0 && fast_s_supported((int)f.ny) && fast_s_supported((int)f.nx);
+    }
+}
+
+template <typename R>
+void batch_plan_free(BatchPlan<R>& bp) {
+    if (bp.dmem) (void)hipFree(bp.dmem);
+    bp = BatchPlan<R>{};
+}
+
+template <typename R>
+void batch_plan_build(BatchPlan<R>& bp, const std::vector<const Fft3d<R>*>& fields,
+                      const std::vector<IterArgs<R>>& args) {
+    FFCZ_REQUIRE(fields.size() == args.size() && !fields.empty(), ERR_INVALID, "empty batch");
+    batch_plan_free(bp);
+    bp.nfields = fields.size();
+    // host staging of every device table; offsets are patched into pointers
+    // once the device block exists
+    std::vector<unsigned char> host;
+    auto put = [&](const void* p, size_t bytes) {
+        const size_t off = (host.size() + 255) & ~(size_t)255;
+        host.resize(off + bytes);
+        std::memcpy(host.data() + off, p, bytes);
+        return off;
+    };
+    struct Pending {
+        typename BatchPlan<R>::Group g;
+        size_t args_off, start_off;
+    };
+    std::vector<Pending> pend;
+    std::vector<size_t> batched;
+    for (size_t i = 0; i < fields.size(); ++i) {
+        if (batchable(*fields[i])) batched.push_back(i);
+        else bp.fallback.emplace_back(fields[i], args[i]);
+    }
+    if constexpr (sizeof(R) == 4) {
+        // group the batched fields by the axis length of each pass
+        auto group_by = [&](auto key) {
+            std::vector<std::pair<int, std::vector<size_t>>> gs;
+            for (size_t i : batched) {
+                const int k = key(*fields[i]);
+                auto it = std::find_if(gs.begin(), gs.end(), [&](auto& g) { return g.first == k; });
+                if (it == gs.end()) gs.push_back({k, {i}});
+                else it->second.push_back(i);
+            }
+            return gs;
+        };
+        auto zgroups = group_by([](const Fft3d<R>& f) { return f.Lz; });
+        auto ygroups = group_by([](const Fft3d<R>& f) { return (int)f.ny; });
+        auto xgroups = group_by([](const Fft3d<R>& f) { return (int)f.nx; });
+        auto add_z = [&](int pass, const std::pair<int, std::vector<size_t>>& g) {
+            std::vector<ZArgs<R, R>> tab;
+            std::vector<uint32_t> start;
+            uint64_t tiles = 0;
+            for (size_t i : g.second) {
+                const Fft3d<R>& f = *fields[i];
+                const IterArgs<R>& ia = args[i];
+                auto z = zargs<R, R>(f);
+                z.e = ia.e;
+                z.e_init = ia.e_init;
+                z.ds = ia.ds;
+                z.eps = ia.eps;
+                z.st = ia.st;
+                z.spec = ia.spec;
+                z.scale = ia.norm;
+                z.vpart = ia.vpart;
+                z.row_off = 0;
+                tab.push_back(z);
+                start.push_back((uint32_t)tiles);
+                tiles += ceil_div(f.rows, (size_t)fast_z_rows(f.Lz, f.half));
+            }
+            FFCZ_REQUIRE(tiles < (1ull << 31), ERR_INVALID, "batch too large for one launch");
+            Pending p{};
+            p.g.pass = pass;
+            p.g.L = g.first;
+            p.g.grid = (unsigned)tiles;
+            p.g.n = (int)tab.size();
+            p.args_off = put(tab.data(), tab.size() * sizeof(tab[0]));
+            p.start_off = put(start.data(), start.size() * sizeof(uint32_t));
+            pend.push_back(p);
+        };
+        auto add_s = [&](int pass, bool xaxis, const std::pair<int, std::vector<size_t>>& g) {
+            std::vector<SArgs<R, R>> tab;
+            std::vector<uint32_t> start;
+            uint64_t tiles = 0;
+            int smode = S_PLAIN;
+            const int W = fast_s_width(sizeof(R));
+            for (size_t i : g.second) {
+                const Fft3d<R>& f = *fields[i];
+                const IterArgs<R>& ia = args[i];
+                auto a = sargs<R, R>(f, ia.spec, xaxis);
+                a.st = ia.st;
+                a.sign = pass == 1 ? -1 : +1;
+                if (pass == 2) {
+                    a.fmode = ia.fmode;
+                    a.delta = ia.delta;
+                    a.delta_dense = ia.delta_dense;
+                    a.dscale = ia.dscale;
+                    a.df = ia.df;
+                    a.orig = ia.orig;
+                    a.delta_ptw = ia.delta_ptw;
+                    // one mode per launch: a field without frequency bounds still
+                    // runs the clamp kernel, whose FM_NONE projection is a no-op
+                    smode = S_CLAMP;
+                }
+                a.W = W;
+                a.nct = (int)ceil_div((size_t)a.ncols, (size_t)W);
+                a.dNct = FastDiv((uint32_t)a.nct);
+                tab.push_back(a);
+                start.push_back((uint32_t)tiles);
+                tiles += (uint64_t)(xaxis ? f.ny : f.nx) * (uint64_t)a.nct;
+            }
+            FFCZ_REQUIRE(tiles < (1ull << 31), ERR_INVALID, "batch too large for one launch");
+            Pending p{};
+            p.g.pass = pass;
+            p.g.L = g.first;
+            p.g.smode = smode;
+            p.g.grid = (unsigned)tiles;
+            p.g.n = (int)tab.size();
+            p.args_off = put(tab.data(), tab.size() * sizeof(tab[0]));
+            p.start_off = put(start.data(), start.size() * sizeof(uint32_t));
+            pend.push_back(p);
+        };
+        for (auto& g : zgroups) add_z(0, g);
+        for (auto& g : ygroups) add_s(1, false, g);
+        for (auto& g : xgroups) add_s(2, true, g);
+        for (auto& g : ygroups) add_s(3, false, g);
+        for (auto& g : zgroups) add_z(4, g);
+    }
+    std::vector<FinEntry<R>> fin;
+    for (size_t i : batched) {
+        const Fft3d<R>& f = *fields[i];
+        fin.push_back({args[i].vpart, ceil_div(f.rows, (size_t)fast_z_rows(f.Lz, f.half)), args[i].st});
+    }
+    size_t fin_off = 0;
+    if (!fin.empty()) fin_off = put(fin.data(), fin.size() * sizeof(fin[0]));
+    std::vector<DevState*> sts;
+    for (auto& a : args) sts.push_back(a.st);
+    const size_t sts_off = put(sts.data(), sts.size() * sizeof(DevState*));
+    const size_t done_off = put(std::vector<unsigned>(sts.size(), 0u).data(), sts.size() * sizeof(unsigned));
+    FFCZ_HIP(hipMalloc(&bp.dmem, host.size()));
+    FFCZ_HIP(hipMemcpy(bp.dmem, host.data(), host.size(), hipMemcpyHostToDevice));
+    for (auto& p : pend) {
+        p.g.args = bp.dmem + p.args_off;
+        p.g.start = reinterpret_cast<const uint32_t*>(bp.dmem + p.start_off);
+        bp.groups.push_back(p.g);
+    }
+    bp.nfin = (int)fin.size();
+    bp.fin = fin.empty() ? nullptr : bp.dmem + fin_off;
+    bp.sts = reinterpret_cast<DevState**>(bp.dmem + sts_off);
+    bp.done = reinterpret_cast<unsigned*>(bp.dmem + done_off);
+}
+
+template <typename R>
+void batch_iteration(const BatchPlan<R>& bp, hipStream_t s) {
+    if constexpr (sizeof(R) == 4) {
+        for (const auto& g : bp.groups) {
+            bool ok;
+            if (g.pass == 0 || g.pass == 4) {
+                BatchTab<ZArgs<R, R>> t{static_cast<const ZArgs<R, R>*>(g.args), g.start, g.n};
+                ok = g.pass == 0 ? fast_zfwd_batch(g.L, t, g.grid, s) : fast_zinv_batch(g.L, t, g.grid, s);
+            } else {
+                BatchTab<SArgs<R, R>> t{static_cast<const SArgs<R, R>*>(g.args), g.start, g.n};
+                ok = fast_strided_batch(g.smode, g.L, t, g.grid, s);
+            }
+            FFCZ_REQUIRE(ok, ERR_STATE, "batched pass without a kernel for its length");
+            FFCZ_HIP(hipGetLastError());
+        }
+        if (bp.nfin) {
+            hipLaunchKernelGGL((k_finalize_batch<R>), dim3(bp.nfin), dim3(1024), 0, s,
+                               static_cast<const FinEntry<R>*>(bp.fin));
+            FFCZ_HIP(hipGetLastError());
+        }
+    }
+    for (const auto& fb : bp.fallback) iteration(*fb.first, fb.second, s);
+}
+
+template <typename R>
+void batch_gather_done(const BatchPlan<R>& bp, unsigned* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_gather_done, dim3(1), dim3(256), 0, s, (DevState* const*)bp.sts, bp.nfields, bp.done);
+    FFCZ_HIP(hipGetLastError());
+    FFCZ_HIP(hipMemcpyAsync(out, bp.done, bp.nfields * sizeof(unsigned), hipMemcpyDeviceToHost, s));
+}
+
 template struct Fft3d<float>;
 template struct Fft3d<double>;
 template void fft_r2c<float>(const Fft3d<float>&, const float*, cplx<float>*, hipStream_t);
@@ -799,5 +1027,15 @@ template void epilogue_clamp<float>(const Fft3d<float>&, const IterArgs<float>&,
 template void epilogue_clamp<double>(const Fft3d<double>&, const IterArgs<double>&, hipStream_t);
 template void certify<float>(const Fft3d<double>&, const CertArgs<float>&, hipStream_t);
 template void certify<double>(const Fft3d<double>&, const CertArgs<double>&, hipStream_t);
+template void batch_plan_build<float>(BatchPlan<float>&, const std::vector<const Fft3d<float>*>&,
+                                      const std::vector<IterArgs<float>>&);
+template void batch_plan_build<double>(BatchPlan<double>&, const std::vector<const Fft3d<double>*>&,
+                                       const std::vector<IterArgs<double>>&);
+template void batch_plan_free<float>(BatchPlan<float>&);
+template void batch_plan_free<double>(BatchPlan<double>&);
+template void batch_iteration<float>(const BatchPlan<float>&, hipStream_t);
+template void batch_iteration<double>(const BatchPlan<double>&, hipStream_t);
+template void batch_gather_done<float>(const BatchPlan<float>&, unsigned*, hipStream_t);
+template void batch_gather_done<double>(const BatchPlan<double>&, unsigned*, hipStream_t);
 
 }  // namespace ffcz

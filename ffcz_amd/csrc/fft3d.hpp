@@ -91,6 +91,7 @@ enum FreqMode : int { FM_ABS = 0, FM_DENSE = 1, FM_PTW = 2, FM_NONE = 3 };
 template <typename R>
 struct IterArgs {
     R* e;                 // spatial iterate [N]
+    const R* e_init;      // initial error [N], read by the first iteration instead of e (optional)
     R* ds;                // spatial edit accumulator [N]
     R* df;                // frequency edit accumulator [2F] split Re|Im, reference layout
     cplx<R>* spec;        // padded half spectrum work buffer
@@ -142,6 +143,45 @@ template <typename R>
 void epilogue_clamp(const Fft3d<R>& f, const IterArgs<R>& a, hipStream_t s);
 template <typename R>
 void reset_state(DevState* st, int max_iter, double tol, hipStream_t s);
+
+// ---- one iteration over a batch of fields ----------------------------------
+// Fields whose passes all have length-specialised kernels run each pass as
+// one launch per axis length over every such field (tile -> field by a
+// prefix table in device memory); their iterations are finalised by one
+// launch with a workgroup per field.  The other fields (FP64, MFMA variants,
+// lengths without a specialised kernel, slab interleave) run iteration()
+// one by one inside the same sequence.  A field whose state is done skips
+// its tiles, so a batch keeps iterating until every field has converged.
+template <typename R>
+struct BatchPlan {
+    struct Group {
+        int pass;        // 0 z fwd, 1 y fwd, 2 x fused, 3 y inv, 4 z inv
+        int L;           // axis length (z: complex half length)
+        int smode;       // strided passes: S_PLAIN / S_CLAMP
+        unsigned grid;   // total workgroups
+        const void* args;
+        const uint32_t* start;
+        int n;
+    };
+    std::vector<Group> groups;
+    std::vector<std::pair<const Fft3d<R>*, IterArgs<R>>> fallback;
+    const void* fin = nullptr;   // finalize table of the batched fields
+    int nfin = 0;
+    DevState** sts = nullptr;    // every field's state
+    unsigned* done = nullptr;    // gathered done flags
+    size_t nfields = 0;
+    unsigned char* dmem = nullptr;
+};
+template <typename R>
+void batch_plan_build(BatchPlan<R>& bp, const std::vector<const Fft3d<R>*>& fields,
+                      const std::vector<IterArgs<R>>& args);
+template <typename R>
+void batch_plan_free(BatchPlan<R>& bp);
+template <typename R>
+void batch_iteration(const BatchPlan<R>& bp, hipStream_t s);
+// copies every field's done flag into host memory `out` (asynchronous on s)
+template <typename R>
+void batch_gather_done(const BatchPlan<R>& bp, unsigned* out, hipStream_t s);
 
 // ---- FP64 certification ----------------------------------------------------
 template <typename T>

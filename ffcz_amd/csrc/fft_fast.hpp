@@ -35,6 +35,17 @@ bool mfma_zfwd(int mode, const ZArgs<float, float>& a, hipStream_t s);
 bool mfma_zinv(int mode, const ZArgs<float, float>& a, hipStream_t s);
 bool mfma_strided(int mode, const SArgs<float, float>& a, size_t outer, hipStream_t s);
 
+// Batched launches of the vector-ALU pass kernels (float): one launch over
+// the tiles of every field in the table, all of axis length L (z: the
+// complex half length).  False when L has no specialised kernel.
+bool fast_zfwd_batch(int Lz, const BatchTab<ZArgs<float, float>>& t, unsigned grid, hipStream_t s);
+bool fast_zinv_batch(int Lz, const BatchTab<ZArgs<float, float>>& t, unsigned grid, hipStream_t s);
+bool fast_strided_batch(int mode, int L, const BatchTab<SArgs<float, float>>& t, unsigned grid, hipStream_t s);
+// whether a y/x axis length has a specialised kernel, and the columns per
+// tile of those kernels for a real type of `bytes` bytes
+bool fast_s_supported(int L);
+int fast_s_width(size_t bytes);
+
 // Process-wide tuning knobs (C ABI: ffcz_set_tuning).
 struct Tuning {
     std::atomic<int> force_generic{0};  // 1: run the generic kernels only

@@ -21,6 +21,29 @@ bool fast_strided<float, float>(int mode, const SArgs<float, float>& a, size_t o
     return strided_dispatch<float, float, (1u << S_PLAIN) | (1u << S_CLAMP)>(mode, a, outer, s);
 }
 
+bool fast_zfwd_batch(int Lz, const BatchTab<ZArgs<float, float>>& t, unsigned grid, hipStream_t s) {
+    return zfwd_batch_dispatch<float, Z_ITER>(Lz, t, grid, s);
+}
+bool fast_zinv_batch(int Lz, const BatchTab<ZArgs<float, float>>& t, unsigned grid, hipStream_t s) {
+    return zinv_batch_dispatch<float, Z_ITER>(Lz, t, grid, s);
+}
+bool fast_strided_batch(int mode, int L, const BatchTab<SArgs<float, float>>& t, unsigned grid, hipStream_t s) {
+    if (mode == S_CLAMP) return strided_batch_dispatch<float, S_CLAMP>(L, t, grid, s);
+    if (mode == S_PLAIN) return strided_batch_dispatch<float, S_PLAIN>(L, t, grid, s);
+    return false;
+}
+bool fast_s_supported(int L) {
+    switch (L) {
+#define FFCZ_SS(Lv) case Lv:
+        FFCZ_FAST_S_LENGTHS(FFCZ_SS)
+#undef FFCZ_SS
+        return true;
+        default:
+            return false;
+    }
+}
+int fast_s_width(size_t bytes) { return bytes == 4 ? s_width<float>() : s_width<double>(); }
+
 bool fast_fft_enabled() { return tuning().force_generic.load() == 0; }
 
 int fast_z_rows(int Lz, bool half) {

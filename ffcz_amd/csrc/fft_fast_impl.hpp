@@ -31,8 +31,8 @@ constexpr int z_rows(int LZ) {
 // FFT ends in LDS and the R2C split X[k] = A[k] + W_nz^k B[k] writes the
 // padded half spectrum.
 // ---------------------------------------------------------------------------
-template <typename R, typename T, int LZ, int MODE, int PREC = PREC_VALU>
-__global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ)) void k2_zfwd(ZArgs<R, T> a) {
+template <typename R, typename T, int LZ, int MODE, int PREC>
+__device__ __forceinline__ void zfwd_body(const ZArgs<R, T>& a, uint32_t blk) {
     constexpr int TP = z_tp(LZ), P = z_rows(LZ), NT = TP * P;
     constexpr int NZ = 2 * LZ, M = LZ, NH = M + 1, SN = P + 1;
     __shared__ cplx<R> tw[LZ];
@@ -44,10 +44,12 @@ __global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ)) void k2_zfwd(ZArgs<R, T> a) 
     for (int i = tid; i < NH; i += NT) tw2[i] = a.tz2[i];
     __syncthreads();
     const int t = tid % TP, p = tid / TP;
-    const size_t row0 = (size_t)blockIdx.x * P;
+    const size_t row0 = (size_t)blk * P;
     const size_t row = row0 + p;
     const bool rv = row < a.rows;
     const bool clamp = (MODE == Z_ITER) && a.st->iter > 0;
+    // the first iteration reads the initial error itself (e is written by the inverse pass)
+    const R* esrc = (MODE == Z_ITER && !clamp && a.e_init) ? a.e_init : a.e;
     unsigned long long nviol = 0;
     double vmax = 0.0;
     auto gld = [&](int n) -> cplx<R> {
@@ -55,7 +57,7 @@ __global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ)) void k2_zfwd(ZArgs<R, T> a) 
         const size_t gi = row * (size_t)NZ + 2 * (size_t)n;
         if constexpr (MODE == Z_ITER) {
             // one 2-element vector load of the iterate (gi is even)
-            cplx<R> v = *reinterpret_cast<const cplx<R>*>(a.e + gi);
+            cplx<R> v = *reinterpret_cast<const cplx<R>*>(esrc + gi);
             if (clamp) {
                 clamp_acc(v.x, a.eps, &a.ds[gi]);
                 clamp_acc(v.y, a.eps, &a.ds[gi + 1]);
@@ -122,8 +124,8 @@ __global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ)) void k2_zfwd(ZArgs<R, T> a) 
 // the real row scaled by 1/N, and ITER tracks the spatial violation and
 // finalises the iteration in the last workgroup.
 // ---------------------------------------------------------------------------
-template <typename R, typename T, int LZ, int MODE, int PREC = PREC_VALU>
-__global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ)) void k2_zinv(ZArgs<R, T> a) {
+template <typename R, typename T, int LZ, int MODE, int PREC>
+__device__ __forceinline__ void zinv_body(const ZArgs<R, T>& a, uint32_t blk) {
     constexpr int TP = z_tp(LZ), P = z_rows(LZ), NT = TP * P;
     constexpr int NZ = 2 * LZ, M = LZ, NH = M + 1, SN = P + 1;
     __shared__ cplx<R> tw[LZ];
@@ -135,7 +137,7 @@ __global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ)) void k2_zinv(ZArgs<R, T> a) 
     for (int i = tid; i < NH; i += NT) tw2[i] = a.tz2[i];
     __syncthreads();
     const int t = tid % TP, p = tid / TP;
-    const size_t row = (size_t)blockIdx.x * P + p;
+    const size_t row = (size_t)blk * P + p;
     const bool rv = row < a.rows;
     const cplx<R>* srow = a.spec + row * a.Mp;
     auto gld = [&](int n) -> cplx<R> {
@@ -172,14 +174,14 @@ __global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ)) void k2_zinv(ZArgs<R, T> a) 
         const mx::Bmat<PREC> bm = mx::dft16_operand<PREC>(tid & 63);
         mx::transform<PREC, LZ, TP, SN, P, NT, false, false>(gld, gst, lld, lst, buf, tw, bm, tid, t, +1);
     }
-    if constexpr (MODE == Z_ITER) block_max_store<R, NT>(vm, &a.vpart[a.row_off / P + blockIdx.x]);
+    if constexpr (MODE == Z_ITER) block_max_store<R, NT>(vm, &a.vpart[a.row_off / P + blk]);
 }
 
 // ---------------------------------------------------------------------------
 // y / x passes over 128-byte column tiles.
 // ---------------------------------------------------------------------------
-template <typename R, typename T, int L, int MODE, int PREC = PREC_VALU>
-__global__ __launch_bounds__(pen::threads_per_pencil(L) * s_width<R>()) void k2_strided(SArgs<R, T> a) {
+template <typename R, typename T, int L, int MODE, int PREC>
+__device__ __forceinline__ void strided_body(const SArgs<R, T>& a, uint32_t blk) {
     constexpr int TP = pen::threads_per_pencil(L), W = s_width<R>(), NT = TP * W;
     __shared__ cplx<R> tw[L];
     __shared__ cplx<R> buf[L * W];
@@ -188,8 +190,8 @@ __global__ __launch_bounds__(pen::threads_per_pencil(L) * s_width<R>()) void k2_
     for (int i = tid; i < L; i += NT) tw[i] = a.tw[i];
     __syncthreads();
     const int p = tid % W, t = tid / W;
-    const uint32_t ob_ = fdiv(blockIdx.x, a.dNct);
-    const int ct = (int)(blockIdx.x - ob_ * (uint32_t)a.nct);
+    const uint32_t ob_ = fdiv(blk, a.dNct);
+    const int ct = (int)(blk - ob_ * (uint32_t)a.nct);
     const size_t o = ob_;
     const int col = ct * W + p;
     const bool valid = col < a.ncols;
@@ -244,6 +246,53 @@ __global__ __launch_bounds__(pen::threads_per_pencil(L) * s_width<R>()) void k2_
         block_count_atomic(nv, &a.stats->freq_viol);
         block_max_atomic<double, NT>(vmax, &a.stats->freq_max_bits);
     }
+}
+
+// ---------------------------------------------------------------------------
+// kernels: one field (arguments in the kernarg segment), or a batch of fields
+// of the same axis length in one launch (workgroup -> field by the tile
+// prefix table; each field's own state gates its tiles)
+// ---------------------------------------------------------------------------
+template <typename A>
+__device__ __forceinline__ uint32_t batch_field(const BatchTab<A>& t, uint32_t blk) {
+    int lo = 0, hi = t.n - 1;  // largest f with start[f] <= blk
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (t.start[mid] <= blk) lo = mid;
+        else hi = mid - 1;
+    }
+    return (uint32_t)lo;
+}
+
+template <typename R, typename T, int LZ, int MODE, int PREC = PREC_VALU>
+__global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ)) void k2_zfwd(ZArgs<R, T> a) {
+    zfwd_body<R, T, LZ, MODE, PREC>(a, blockIdx.x);
+}
+template <typename R, typename T, int LZ, int MODE, int PREC = PREC_VALU>
+__global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ)) void k2_zinv(ZArgs<R, T> a) {
+    zinv_body<R, T, LZ, MODE, PREC>(a, blockIdx.x);
+}
+template <typename R, typename T, int L, int MODE, int PREC = PREC_VALU>
+__global__ __launch_bounds__(pen::threads_per_pencil(L) * s_width<R>()) void k2_strided(SArgs<R, T> a) {
+    strided_body<R, T, L, MODE, PREC>(a, blockIdx.x);
+}
+template <typename R, int LZ, int MODE>
+__global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ)) void k2_zfwd_b(BatchTab<ZArgs<R, R>> t) {
+    const uint32_t f = batch_field(t, blockIdx.x);
+    const ZArgs<R, R> a = t.args[f];
+    zfwd_body<R, R, LZ, MODE, PREC_VALU>(a, blockIdx.x - t.start[f]);
+}
+template <typename R, int LZ, int MODE>
+__global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ)) void k2_zinv_b(BatchTab<ZArgs<R, R>> t) {
+    const uint32_t f = batch_field(t, blockIdx.x);
+    const ZArgs<R, R> a = t.args[f];
+    zinv_body<R, R, LZ, MODE, PREC_VALU>(a, blockIdx.x - t.start[f]);
+}
+template <typename R, int L, int MODE>
+__global__ __launch_bounds__(pen::threads_per_pencil(L) * s_width<R>()) void k2_strided_b(BatchTab<SArgs<R, R>> t) {
+    const uint32_t f = batch_field(t, blockIdx.x);
+    const SArgs<R, R> a = t.args[f];
+    strided_body<R, R, L, MODE, PREC_VALU>(a, blockIdx.x - t.start[f]);
 }
 
 // ---------------------------------------------------------------------------
@@ -400,6 +449,50 @@ bool mfma_strided_dispatch(int mode, const SArgs<float, float>& a, size_t outer,
         return true;
         FFCZ_MFMA_S_LENGTHS(FFCZ_MSL)
 #undef FFCZ_MSL
+        default:
+            return false;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// batched launches (vector-ALU kernels): tiles of every field in the table
+// ---------------------------------------------------------------------------
+template <typename R, int MODE>
+bool zfwd_batch_dispatch(int Lz, const BatchTab<ZArgs<R, R>>& t, unsigned grid, hipStream_t s) {
+    switch (Lz) {
+#define FFCZ_BZF(Lv)                                                                                     \
+    case Lv:                                                                                             \
+        hipLaunchKernelGGL((k2_zfwd_b<R, Lv, MODE>), dim3(grid), dim3(z_tp(Lv) * z_rows(Lv)), 0, s, t);  \
+        return true;
+        FFCZ_FAST_Z_LENGTHS(FFCZ_BZF)
+#undef FFCZ_BZF
+        default:
+            return false;
+    }
+}
+template <typename R, int MODE>
+bool zinv_batch_dispatch(int Lz, const BatchTab<ZArgs<R, R>>& t, unsigned grid, hipStream_t s) {
+    switch (Lz) {
+#define FFCZ_BZI(Lv)                                                                                     \
+    case Lv:                                                                                             \
+        hipLaunchKernelGGL((k2_zinv_b<R, Lv, MODE>), dim3(grid), dim3(z_tp(Lv) * z_rows(Lv)), 0, s, t);  \
+        return true;
+        FFCZ_FAST_Z_LENGTHS(FFCZ_BZI)
+#undef FFCZ_BZI
+        default:
+            return false;
+    }
+}
+template <typename R, int MODE>
+bool strided_batch_dispatch(int L, const BatchTab<SArgs<R, R>>& t, unsigned grid, hipStream_t s) {
+    switch (L) {
+#define FFCZ_BSL(Lv)                                                                                      \
+    case Lv:                                                                                              \
+        hipLaunchKernelGGL((k2_strided_b<R, Lv, MODE>), dim3(grid),                                      \
+                           dim3(pen::threads_per_pencil(Lv) * s_width<R>()), 0, s, t);                    \
+        return true;
+        FFCZ_FAST_S_LENGTHS(FFCZ_BSL)
+#undef FFCZ_BSL
         default:
             return false;
     }

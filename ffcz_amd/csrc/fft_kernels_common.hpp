@@ -95,6 +95,7 @@ struct ZArgs {
     R scale;              // inverse scale
     // Z_ITER
     R* e;
+    const R* e_init;  // read instead of e by the first iteration (iter == 0), if set
     R* ds;
     R eps;
     DevState* st;
@@ -107,6 +108,15 @@ struct ZArgs {
     QuantDesc qs;
     double ceps;
     CertStats* stats;
+};
+
+// Arguments of one launch over several fields: field f owns workgroups
+// [start[f], start[f+1]).
+template <typename A>
+struct BatchTab {
+    const A* args;
+    const uint32_t* start;
+    int n;
 };
 
 enum SMode : int { S_PLAIN = 0, S_CLAMP = 1, S_CERT_CHECK = 2, S_LOAD_DF = 3 };

@@ -204,6 +204,7 @@ template <typename T>
 IterArgs<T> Solver<T>::iter_args() const {
     IterArgs<T> a{};
     a.e = e_;
+    a.e_init = e0_;
     a.ds = ds_;
     a.df = df_;
     a.spec = spec_;
@@ -222,15 +223,22 @@ IterArgs<T> Solver<T>::iter_args() const {
 }
 
 template <typename T>
-void Solver<T>::solve_loop(int mode, const T* e0, int max_iter, T tol) {
+void Solver<T>::prepare_solve(int mode, const T* e0, int max_iter, T tol, hipStream_t s) {
     FFCZ_REQUIRE(initialized_, ERR_STATE, "Solver not initialized");  // impl:778-781
     FFCZ_REQUIRE(mode == init_mode_, ERR_STATE,
                  "solve mode does not match the initialize* call (direct<->initialize, abs<->initialize_all_abs, "
                  "ptw<->initialize_all_ptw)");
     FFCZ_REQUIRE(e0 != nullptr, ERR_INVALID, "null spatial error");
-    FFCZ_HIP(hipMemcpyAsync(e_, e0, f_.N * sizeof(T), hipMemcpyDefault, s_));
-    FFCZ_HIP(hipMemcpyAsync(e0_, e_, f_.N * sizeof(T), hipMemcpyDeviceToDevice, s_));
-    reset_state<T>(st_, max_iter, (double)tol, s_);
+    // e0_ keeps the initial error for certification; the first iteration reads
+    // it directly and its inverse pass writes the iterate e_
+    FFCZ_HIP(hipMemcpyAsync(e0_, e0, f_.N * sizeof(T), hipMemcpyDefault, s));
+    if (max_iter <= 0) FFCZ_HIP(hipMemcpyAsync(e_, e0_, f_.N * sizeof(T), hipMemcpyDeviceToDevice, s));
+    reset_state<T>(st_, max_iter, (double)tol, s);
+}
+
+template <typename T>
+void Solver<T>::solve_loop(int mode, const T* e0, int max_iter, T tol) {
+    prepare_solve(mode, e0, max_iter, tol, s_);
     const IterArgs<T> a = iter_args();
     int launched = 0, chunk = 1;
     DevState h{};
@@ -243,9 +251,16 @@ void Solver<T>::solve_loop(int mode, const T* e0, int max_iter, T tol) {
         if (h.done) break;
         chunk = std::min(chunk * 2, 16);
     }
-    epilogue_clamp(f_, a, s_);
-    FFCZ_HIP(hipMemcpyAsync(&h, st_, sizeof(h), hipMemcpyDeviceToHost, s_));
-    FFCZ_HIP(hipStreamSynchronize(s_));
+    finish_solve(s_);
+}
+
+template <typename T>
+void Solver<T>::finish_solve(hipStream_t s) {
+    const IterArgs<T> a = iter_args();
+    epilogue_clamp(f_, a, s);
+    DevState h{};
+    FFCZ_HIP(hipMemcpyAsync(&h, st_, sizeof(h), hipMemcpyDeviceToHost, s));
+    FFCZ_HIP(hipStreamSynchronize(s));
     iters_ = (int)h.iter;
     viol_ = h.last_viol;
     converged_ = h.converged != 0;

@@ -37,6 +37,11 @@ class Solver {
 
     // Projection loop only (impl:791-808 / 835-852 / 879-896).
     void solve_loop(int mode, const T* e0, int max_iter, T tol);
+    // the loop in pieces, for a caller that launches the iterations itself
+    // (batches): checks + initial copy + state reset, then the epilogue and
+    // the read-back of the final state, both on stream s
+    void prepare_solve(int mode, const T* e0, int max_iter, T tol, hipStream_t s);
+    void finish_solve(hipStream_t s);
     // solve / solve_all_abs / solve_all_ptw: loop + extraction (impl:772-902).
     void solve(int mode, const T* e0, int max_iter, T tol) {
         solve_loop(mode, e0, max_iter, tol);

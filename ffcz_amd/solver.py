@@ -68,15 +68,30 @@ class ProjectionSolver:
         self.shape = (int(Nx), int(Ny), int(Nz))
         self.N = self.shape[0] * self.shape[1] * self.shape[2]
         self.F = self.shape[0] * self.shape[1] * (self.shape[2] // 2 + 1)
+        self._owner = None
         h = C.c_void_p()
         call("ffcz_create", FFCZ_F32 if self.dtype == np.float32 else FFCZ_F64,
              self.shape[0], self.shape[1], self.shape[2], C.byref(h))
         self._h = h
 
+    @classmethod
+    def _borrowed(cls, handle, shape, dtype, owner):
+        """A batch field: the handle belongs to ``owner`` (a ProjectionBatch)."""
+        self = cls.__new__(cls)
+        self.dtype = _np_dtype(dtype)
+        self._sfx = "f32" if self.dtype == np.float32 else "f64"
+        self.shape = tuple(int(v) for v in shape)
+        self.N = self.shape[0] * self.shape[1] * self.shape[2]
+        self.F = self.shape[0] * self.shape[1] * (self.shape[2] // 2 + 1)
+        self._owner = owner
+        self._h = handle
+        return self
+
     # -- lifecycle ---------------------------------------------------------
     def close(self) -> None:
         if getattr(self, "_h", None):
-            call("ffcz_destroy", self._h)
+            if getattr(self, "_owner", None) is None:
+                call("ffcz_destroy", self._h)
             self._h = None
 
     def __del__(self):  # pragma: no cover - best effort

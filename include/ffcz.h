@@ -129,6 +129,29 @@ int ffcz_get_edit_range(ffcz_solver* h, double* fmin, double* fmax, double* smin
 int ffcz_set_margins(ffcz_solver* h, double spatial_margin, double freq_margin);
 int ffcz_set_quantizer(ffcz_solver* h, int kind, double freq_step, double spatial_step);
 int ffcz_certify(ffcz_solver* h, int quantized, ffcz_cert* out);
+/* ---- batches: many fields of uneven shapes corrected together ----------
+ * No counterpart in the reference (its driver, projection_example.cu:157-265,
+ * runs one field per ProjectionSolver).  Each field is a borrowed solver
+ * handle, from ffcz_batch_field: initialise it, and after the batch solve
+ * extract, read and certify it, with the single-field calls; it is freed by
+ * ffcz_batch_destroy, never by ffcz_destroy.  ffcz_batch_solve runs every
+ * field's projection loop (solve/solve_all_abs/solve_all_ptw semantics per
+ * field, mode as for ffcz_solve_loop) with one launch per pass and axis
+ * length over all fields and per-field convergence flags on the device;
+ * use_graph=1 replays the iteration as a captured HIP graph.  dims holds
+ * (nx, ny, nz) per field; e0 one initial error (host or device) per field. */
+typedef struct ffcz_batch ffcz_batch;
+int ffcz_batch_create(int dtype, size_t nfields, const size_t* dims, ffcz_batch** out);
+int ffcz_batch_destroy(ffcz_batch* b);
+int ffcz_batch_size(ffcz_batch* b, size_t* nfields);
+int ffcz_batch_field(ffcz_batch* b, size_t i, ffcz_solver** field);
+int ffcz_batch_solve(ffcz_batch* b, int mode, const void* const* e0, int max_iter, double tol, int use_graph);
+/* per-field iteration counts, final violations and convergence flags (any
+ * may be NULL; cap entries at most); launched = iterations the batch issued
+ * (in chunks between host checks; fields already converged skip them) */
+int ffcz_batch_status(ffcz_batch* b, int* iterations, double* violations, int* converged, size_t cap,
+                      int* launched);
+
 /* FFT variant of the projection iteration (an addition; the reference runs
  * cuFFT at projection_algorithm_impl.cuh:205-214, 228-238):
  * FFCZ_FFT_VALU   FP32 vector-ALU Stockham passes (default)

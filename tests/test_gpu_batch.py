@@ -1,0 +1,153 @@
+"""Batched projection loop: many fields of uneven shapes in one sequence of
+launches (one per pass and axis length), per-field convergence flags, the
+iteration replayed as a HIP graph.
+
+Parity: each field of a batch runs the same pass kernels on the same data in
+the same order as the field solved alone, so iterate, edits and iteration
+count are bit-identical to the single-field solver (itself pinned against
+the oracle restatement of projection_algorithm_impl.cuh:772-858 in
+test_gpu_solver).  Graph replays are checked against eager launches of the
+same batch, bit for bit, over several solves with host work in between.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+import ffcz_amd
+from ffcz_amd import MODE_ABS, MODE_DIRECT, ProjectionBatch, ProjectionSolver
+from ffcz_amd._lib import FFCzError, load
+from oracle import ffcz_oracle as O
+from tests._cases import binding_case
+
+pytestmark = pytest.mark.gpu
+
+# batched lengths (60..120 and their z half-lengths) plus fields that fall back
+# to per-field launches (z half-lengths 6 and 15 have no specialised kernel)
+SHAPES = [(60, 72, 96), (72, 60, 120), (96, 96, 60), (60, 60, 60), (32, 64, 128), (6, 10, 12), (16, 20, 30)]
+
+
+def _cases(shapes, dtype, seed0=0):
+    out = []
+    for i, s in enumerate(shapes):
+        # different spreads: the fields converge after different iteration counts
+        x, e0, eps, delta = binding_case(s, seed=seed0 + i, dtype=dtype, spread=1.2 + 0.15 * i, dfrac=0.9)
+        out.append((x, e0, eps, delta))
+    return out
+
+
+def _solo(shape, case, dtype, max_iter, tol, repeat=1):
+    x, e0, eps, delta = case
+    with ProjectionSolver(*shape, dtype=dtype) as s:
+        s.initialize_all_abs(x, eps, delta)
+        for _ in range(repeat):
+            s.solve_loop(MODE_ABS, e0, max_iter, tol)
+        it, v, conv = s.status()
+        return it, conv, s.get_results(), s.get_spatial_edits_full(), s.get_freq_edits_full()
+
+
+def _batch_results(b):
+    return [(f.get_results(), f.get_spatial_edits_full(), f.get_freq_edits_full()) for f in b.fields]
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_matches_single_field_solves(gpu, use_graph):
+    dtype = np.float32
+    cases = _cases(SHAPES, dtype)
+    with ProjectionBatch(SHAPES, dtype=dtype) as b:
+        for f, (x, e0, eps, delta) in zip(b.fields, cases):
+            f.initialize_all_abs(x, eps, delta)
+        b.solve(MODE_ABS, [c[1] for c in cases], 60, 1e-7, use_graph=use_graph)
+        its, viols, convs, launched = b.status()
+        res = _batch_results(b)
+    # iterations are issued in chunks between host checks; a field past its
+    # convergence skips the rest on the device
+    assert launched >= max(its)
+    assert len(set(its)) > 1, "fields should converge at different iterations"
+    for shape, case, it, conv, (e, ds, df) in zip(SHAPES, cases, its, convs, res):
+        it1, conv1, e1, ds1, df1 = _solo(shape, case, dtype, 60, 1e-7)
+        assert (it, conv) == (it1, conv1), shape
+        assert np.array_equal(e, e1) and np.array_equal(ds, ds1) and np.array_equal(df, df1), shape
+
+
+def test_graph_replays_match_eager_over_repeated_solves(gpu):
+    """Two solves in a row (edits accumulate, impl:60) with and without the
+    graph, host reads in between: every field identical bit for bit."""
+    dtype = np.float32
+    shapes = SHAPES[:5]
+    cases = _cases(shapes, dtype, seed0=10)
+    out = {}
+    for g in (False, True):
+        with ProjectionBatch(shapes, dtype=dtype) as b:
+            for f, (x, e0, eps, delta) in zip(b.fields, cases):
+                f.initialize_all_abs(x, eps, delta)
+            b.solve(MODE_ABS, [c[1] for c in cases], 40, 1e-7, use_graph=g)
+            first = _batch_results(b)
+            b.solve(MODE_ABS, [c[1] for c in cases], 40, 1e-7, use_graph=g)
+            out[g] = (first, _batch_results(b), b.status())
+    assert out[False][2] == out[True][2]
+    for k in (0, 1):
+        for a, c in zip(out[False][k], out[True][k]):
+            for u, v in zip(a, c):
+                assert np.array_equal(u, v)
+    # the second solve accumulates like the single-field solver
+    for shape, case, (e, ds, df) in zip(shapes, cases, out[True][1]):
+        _, _, e1, ds1, df1 = _solo(shape, case, dtype, 40, 1e-7, repeat=2)
+        assert np.array_equal(e, e1) and np.array_equal(ds, ds1) and np.array_equal(df, df1)
+
+
+def test_batch_f64_against_oracle(gpu):
+    shapes = [(6, 10, 12), (12, 10, 8), (16, 20, 30)]
+    cases = _cases(shapes, np.float64, seed0=20)
+    with ProjectionBatch(shapes, dtype=np.float64) as b:
+        for f, (x, e0, eps, delta) in zip(b.fields, cases):
+            f.initialize_all_abs(x, eps, delta)
+        b.solve(MODE_ABS, [torch.from_numpy(c[1]).to(gpu) for c in cases], 50, 1e-9)
+        its, _, convs, _ = b.status()
+        for f, s, (x, e0, eps, delta), it in zip(b.fields, shapes, cases, its):
+            ref = O.solve(x, e0, eps, mode="abs", delta=delta, max_iter=50, tol=1e-9)
+            assert it == ref.iterations
+            e = f.get_results()
+            assert np.linalg.norm(e - ref.e.ravel()) <= 1e-9 * np.linalg.norm(ref.e)
+
+
+def test_batch_fields_extract_and_certify(gpu):
+    dtype = np.float32
+    shapes = SHAPES[:4]
+    cases = _cases(shapes, dtype, seed0=30)
+    with ProjectionBatch(shapes, dtype=dtype) as b:
+        for f, (x, e0, eps, delta) in zip(b.fields, cases):
+            f.initialize_all_abs(x, eps, delta)
+            f.set_margins(0.125, 0.125)
+        b.solve(MODE_ABS, [c[1] for c in cases], 200, 1e-7)
+        _, _, convs, _ = b.status()
+        assert all(convs)
+        for f in b.fields:
+            f.extract_compact_representation()
+            assert f.certify(False).ok
+            q = f.get_active_spatial_quant_edit_distances()
+            assert q.dtype == np.uint16
+
+
+def test_batch_max_iterations_zero_and_mode_mismatch(gpu):
+    shapes = [(60, 72, 96), (6, 10, 12)]
+    cases = _cases(shapes, np.float32, seed0=40)
+    with ProjectionBatch(shapes, dtype=np.float32) as b:
+        for f, (x, e0, eps, delta) in zip(b.fields, cases):
+            f.initialize_all_abs(x, eps, delta)
+        b.solve(MODE_ABS, [c[1] for c in cases], 0, 1e-7)
+        for f, c in zip(b.fields, cases):
+            assert np.array_equal(f.get_results(), c[1].ravel())
+        with pytest.raises(FFCzError):
+            b.solve(MODE_DIRECT, [c[1] for c in cases], 5, 1e-7)
+
+
+def test_batch_field_handles_are_borrowed(gpu):
+    lib = load()
+    with ProjectionBatch([(8, 8, 8), (4, 6, 8)], dtype=np.float32) as b:
+        n = C.c_size_t()
+        assert lib.ffcz_batch_size(b._h, C.byref(n)) == 0 and n.value == 2
+        assert lib.ffcz_destroy(b.fields[0]._h) != 0  # freed with the batch only
+        h = C.c_void_p()
+        assert lib.ffcz_batch_field(b._h, 2, C.byref(h)) != 0
