@@ -66,6 +66,13 @@ class ProjectionBatch:
         call("ffcz_batch_status", self._h, it, vi, cv, n, C.byref(launched))
         return list(it), list(vi), [bool(c) for c in cv], launched.value
 
+    def time_iteration(self, reps: int = 10) -> float:
+        """Mean ms of one whole batched iteration after a completed solve
+        (reset the edit accumulators afterwards)."""
+        ms = C.c_float()
+        call("ffcz_batch_time_iteration", self._h, int(reps), C.byref(ms))
+        return ms.value
+
     def close(self):
         if getattr(self, "_h", None):
             for f in self.fields:

@@ -75,6 +75,38 @@ void Batch<T>::solve(int mode, const T* const* e0, int max_iter, T tol, bool use
     for (auto* f : f_) f->finish_solve(s_);
 }
 
+template <typename T>
+float Batch<T>::time_iteration(int reps) {
+    FFCZ_REQUIRE(plan_.nfields == f_.size(), ERR_STATE, "time_iteration requires a completed batch solve");
+    FFCZ_REQUIRE(reps >= 1, ERR_INVALID, "reps must be >= 1");
+    for (auto* f : f_) reset_state<T>(f->iter_args().st, 1 << 30, -1.0, s_);  // tol < 0: never converges
+    auto one = [&] {
+        if (gexec_) FFCZ_HIP(hipGraphLaunch(gexec_, s_));
+        else batch_iteration(plan_, s_);
+    };
+    one();  // warm-up
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    FFCZ_HIP(hipEventCreateWithFlags(&e0, hipEventDisableSystemFence));
+    float t = 0;
+    try {
+        FFCZ_HIP(hipEventCreateWithFlags(&e1, hipEventDisableSystemFence));
+        FFCZ_HIP(hipEventRecord(e0, s_));
+        for (int r = 0; r < reps; ++r) one();
+        FFCZ_HIP(hipEventRecord(e1, s_));
+        FFCZ_HIP(hipEventSynchronize(e1));
+        FFCZ_HIP(hipEventElapsedTime(&t, e0, e1));
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    for (auto* f : f_) reset_state<T>(f->iter_args().st, 0, 0.0, s_);
+    FFCZ_HIP(hipStreamSynchronize(s_));
+    return t / (float)reps;
+}
+
 template class Batch<float>;
 template class Batch<double>;
 

@@ -28,8 +28,12 @@ class Batch {
     // per field), without the trailing extraction.  use_graph: replay a captured
     // iteration instead of launching its kernels one by one.
     void solve(int mode, const T* const* e0, int max_iter, T tol, bool use_graph);
-    // iterations run by the last solve (the slowest field's count)
+    // iterations issued by the last solve (chunked; at least the slowest field's count)
     int launched() const { return launched_; }
+    // mean ms of one whole batched iteration (every field iterating), timed
+    // with HIP events on the batch stream after a completed solve; the
+    // accumulators keep changing, so callers reset them afterwards
+    float time_iteration(int reps);
 
    private:
     void release_graph();

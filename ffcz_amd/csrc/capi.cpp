@@ -263,6 +263,12 @@ int ffcz_batch_solve(ffcz_batch* b, int mode, const void* const* e0, int max_ite
             b->bd->solve(mode, reinterpret_cast<const double* const*>(e0), max_iter, tol, use_graph != 0);
     });
 }
+int ffcz_batch_time_iteration(ffcz_batch* b, int reps, float* ms) {
+    return guard([&] {
+        FFCZ_REQUIRE(b != nullptr && ms != nullptr, ERR_INVALID, "null argument");
+        *ms = b->dtype == FFCZ_F32 ? b->bf->time_iteration(reps) : b->bd->time_iteration(reps);
+    });
+}
 int ffcz_batch_status(ffcz_batch* b, int* iterations, double* violations, int* converged, size_t cap,
                       int* launched) {
     return guard([&] {

@@ -139,9 +139,26 @@ def batch_shapes(seed: int = 7, count: int = 48):
     out = []
     for _ in range(count):
         shape = tuple(int(v) for v in rng.choice(dims, 3))
-        out.append(dict(shape=shape, slope=float(rng.uniform(-4, -2)), eps_rel=float(10 ** rng.uniform(-4, -2))))
+        out.append(dict(shape=shape, slope=float(rng.uniform(-4, -2)), eps_rel=float(rng.uniform(1e-4, 1e-2))))
     return out
 
 
-__all__ = ["grf", "normalize01", "uniform_quantize", "Workload", "make_config", "batch_shapes"]
+def make_batch(device="cpu", seed: int = 7, count: int = 48, specs=None) -> list:
+    """c4: the batching configuration as float32 workloads.  Field i uses the
+    slope and eps of batch_shapes; its frequency bound scales with its eps,
+    delta = 0.5 * (eps / range) * max|F(x)| (the c2 ratio of 5e-4 to 1e-3)."""
+    out = []
+    for i, sp in enumerate(specs or batch_shapes(seed, count)):
+        x = normalize01(grf(sp["shape"], sp["slope"], seed * 1000 + i, device)).to(torch.float32)
+        rng_ = float(x.max() - x.min())
+        eps = sp["eps_rel"] * rng_
+        X = torch.fft.rfftn(x.to(torch.float64), dim=(0, 1, 2))
+        delta = 0.5 * sp["eps_rel"] * float(X.abs().max())
+        del X
+        xp = uniform_quantize(x, eps)
+        out.append(Workload(f"c4[{i}]", x, xp, xp - x, eps, delta, max_iter=100, seed=seed))
+    return out
+
+
+__all__ = ["grf", "normalize01", "uniform_quantize", "Workload", "make_config", "batch_shapes", "make_batch"]
 del math

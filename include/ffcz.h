@@ -151,6 +151,9 @@ int ffcz_batch_solve(ffcz_batch* b, int mode, const void* const* e0, int max_ite
  * (in chunks between host checks; fields already converged skip them) */
 int ffcz_batch_status(ffcz_batch* b, int* iterations, double* violations, int* converged, size_t cap,
                       int* launched);
+/* mean ms of one whole batched iteration (HIP events on the batch stream),
+ * after a completed batch solve; the edit accumulators change: reset them */
+int ffcz_batch_time_iteration(ffcz_batch* b, int reps, float* ms);
 
 /* FFT variant of the projection iteration (an addition; the reference runs
  * cuFFT at projection_algorithm_impl.cuh:205-214, 228-238):
