@@ -140,11 +140,10 @@ def bench_batch(a, torch, dev):
         for f in b.fields:
             f.reset_edit_accumulators()
         b.solve(MODE_ABS, e0s, 100, 1e-7, use_graph=not a.no_graph)
-        certs = []
-        for f in b.fields:
-            f.extract_compact_representation()
-            certs.append((f.certify(False), f.certify(True)))
-        return certs
+        c1 = b.certify_all(False)
+        b.extract_all()
+        c2 = b.certify_all(True)
+        return list(zip(c1, c2))
 
     for _ in range(a.warmup):
         step()

@@ -97,6 +97,8 @@ SIGNATURES = {
     "ffcz_batch_solve": [_h, _i, C.POINTER(_vp), _i, _d, _i],
     "ffcz_batch_status": [_h, C.POINTER(_i), C.POINTER(_d), C.POINTER(_i), _sz, C.POINTER(_i)],
     "ffcz_batch_time_iteration": [_h, _i, C.POINTER(_f)],
+    "ffcz_batch_extract": [_h],
+    "ffcz_batch_certify": [_h, _i, C.POINTER(Cert), _sz],
     "ffcz_set_fft_precision": [_h, _i],
     "ffcz_get_fft_precision": [_h, C.POINTER(_i)],
     "ffcz_certify": [_h, _i, C.POINTER(Cert)],

@@ -11,8 +11,9 @@ import ctypes as C
 
 import numpy as np
 
+from . import _lib
 from ._lib import FFCZ_F32, FFCZ_F64, call
-from .solver import ProjectionSolver, _np_dtype, as_pointer
+from .solver import CertResult, ProjectionSolver, _np_dtype, as_pointer
 
 
 class ProjectionBatch:
@@ -65,6 +66,18 @@ class ProjectionBatch:
         launched = C.c_int()
         call("ffcz_batch_status", self._h, it, vi, cv, n, C.byref(launched))
         return list(it), list(vi), [bool(c) for c in cv], launched.value
+
+    def extract_all(self):
+        """extract_compact_representation of every field (overlapped)."""
+        call("ffcz_batch_extract", self._h)
+
+    def certify_all(self, quantized: bool):
+        """FP64 certification of every field (overlapped) -> [CertResult]."""
+        n = len(self.fields)
+        out = (_lib.Cert * n)()
+        call("ffcz_batch_certify", self._h, int(bool(quantized)), out, n)
+        return [CertResult(bool(r.ok), r.spatial_violations, r.freq_violations, r.spatial_max_excess,
+                           r.freq_max_excess) for r in out]
 
     def time_iteration(self, reps: int = 10) -> float:
         """Mean ms of one whole batched iteration after a completed solve

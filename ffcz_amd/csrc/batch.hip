@@ -107,6 +107,19 @@ float Batch<T>::time_iteration(int reps) {
     return t / (float)reps;
 }
 
+template <typename T>
+void Batch<T>::extract_all() {
+    for (auto* f : f_) f->extract();
+    for (auto* f : f_) f->sync_extract();
+}
+
+template <typename T>
+void Batch<T>::certify_all(bool quantized, CertResult* out) {
+    FFCZ_REQUIRE(out != nullptr, ERR_INVALID, "null result array");
+    for (auto* f : f_) f->certify_enqueue(quantized);
+    for (size_t i = 0; i < f_.size(); ++i) out[i] = f_[i]->certify_collect(quantized);
+}
+
 template class Batch<float>;
 template class Batch<double>;
 

@@ -34,6 +34,11 @@ class Batch {
     // with HIP events on the batch stream after a completed solve; the
     // accumulators keep changing, so callers reset them afterwards
     float time_iteration(int reps);
+    // extraction of every field (each on its own stream, overlapped), then
+    // one wait per field
+    void extract_all();
+    // certification of every field, overlapped the same way; out[size()]
+    void certify_all(bool quantized, CertResult* out);
 
    private:
     void release_graph();

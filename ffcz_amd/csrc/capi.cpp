@@ -263,6 +263,29 @@ int ffcz_batch_solve(ffcz_batch* b, int mode, const void* const* e0, int max_ite
             b->bd->solve(mode, reinterpret_cast<const double* const*>(e0), max_iter, tol, use_graph != 0);
     });
 }
+int ffcz_batch_extract(ffcz_batch* b) {
+    return guard([&] {
+        FFCZ_REQUIRE(b != nullptr, ERR_INVALID, "null batch handle");
+        if (b->dtype == FFCZ_F32) b->bf->extract_all();
+        else b->bd->extract_all();
+    });
+}
+int ffcz_batch_certify(ffcz_batch* b, int quantized, ffcz_cert* out, size_t cap) {
+    return guard([&] {
+        FFCZ_REQUIRE(b != nullptr && out != nullptr, ERR_INVALID, "null argument");
+        FFCZ_REQUIRE(cap >= b->fields.size(), ERR_RANGE, "result capacity below the field count");
+        std::vector<CertResult> r(b->fields.size());
+        if (b->dtype == FFCZ_F32) b->bf->certify_all(quantized != 0, r.data());
+        else b->bd->certify_all(quantized != 0, r.data());
+        for (size_t i = 0; i < r.size(); ++i) {
+            out[i].spatial_violations = r[i].spatial_violations;
+            out[i].freq_violations = r[i].freq_violations;
+            out[i].spatial_max_excess = r[i].spatial_max_excess;
+            out[i].freq_max_excess = r[i].freq_max_excess;
+            out[i].ok = r[i].ok;
+        }
+    });
+}
 int ffcz_batch_time_iteration(ffcz_batch* b, int reps, float* ms) {
     return guard([&] {
         FFCZ_REQUIRE(b != nullptr && ms != nullptr, ERR_INVALID, "null argument");

@@ -20,13 +20,6 @@ constexpr int X_ROUNDS = 4;
 constexpr int X_TILE = 256 * X_EPT * X_ROUNDS;   // 8192
 static_assert(X_TILE == EXTRACT_TILE, "tile size");
 
-struct ExtractGlobals {
-    unsigned long long count;
-    double vmin, vmax;
-    unsigned int overflow;
-    unsigned int pad;
-};
-
 size_t extract_scratch_bytes(size_t n) {
     const size_t ntiles = ceil_div(n ? n : 1, (size_t)X_TILE);
     return 256 + ntiles * (sizeof(unsigned int) + sizeof(unsigned long long) + 2 * sizeof(double));
@@ -244,16 +237,13 @@ __global__ void k_dequant_dense(const T* v, size_t n, QuantDesc q, T* out) {
 }
 
 template <typename T>
-ExtractResult extract_edits(const T* v, size_t n, QuantDesc q, uint8_t* pack, T* compact, uint16_t* quant,
-                            void* scratch, hipStream_t s) {
-    ExtractResult res;
-    if (n == 0) return res;
+static void extract_launch(const T* v, size_t n, QuantDesc q, uint8_t* pack, T* compact, uint16_t* quant,
+                           void* scratch, ExtractGlobals* g, hipStream_t s) {
     FFCZ_REQUIRE(((uintptr_t)v % 16) == 0, ERR_INVALID, "edit array must be 16-byte aligned");
     FFCZ_REQUIRE(((uintptr_t)pack % 4) == 0, ERR_INVALID, "flag buffer must be 4-byte aligned");
     const size_t ntiles = ceil_div(n, (size_t)X_TILE);
     FFCZ_REQUIRE(ntiles < (1u << 31), ERR_INVALID, "array too large for extraction");
     unsigned char* p = (unsigned char*)scratch;
-    ExtractGlobals* g = (ExtractGlobals*)p;
     unsigned long long* offsets = (unsigned long long*)(p + 256);
     double* pmin = (double*)(offsets + ntiles);
     double* pmax = pmin + ntiles;
@@ -265,6 +255,15 @@ ExtractResult extract_edits(const T* v, size_t n, QuantDesc q, uint8_t* pack, T*
     hipLaunchKernelGGL((k_tile_scatter<T>), dim3((unsigned)ntiles), dim3(256), 0, s, v, n, q, offsets, g,
                        (unsigned int*)pack, compact, quant);
     FFCZ_HIP(hipGetLastError());
+}
+
+template <typename T>
+ExtractResult extract_edits(const T* v, size_t n, QuantDesc q, uint8_t* pack, T* compact, uint16_t* quant,
+                            void* scratch, hipStream_t s) {
+    ExtractResult res;
+    if (n == 0) return res;
+    ExtractGlobals* g = (ExtractGlobals*)scratch;
+    extract_launch(v, n, q, pack, compact, quant, scratch, g, s);
     ExtractGlobals h{};
     FFCZ_HIP(hipMemcpyAsync(&h, g, sizeof(h), hipMemcpyDeviceToHost, s));
     FFCZ_HIP(hipStreamSynchronize(s));
@@ -273,6 +272,22 @@ ExtractResult extract_edits(const T* v, size_t n, QuantDesc q, uint8_t* pack, T*
     res.vmin = h.vmin;
     res.vmax = h.vmax;
     return res;
+}
+
+__global__ void k_clear_globals(ExtractGlobals* g) {
+    if (threadIdx.x == 0) *g = ExtractGlobals{};
+}
+
+template <typename T>
+void extract_edits_async(const T* v, size_t n, QuantDesc q, uint8_t* pack, T* compact, uint16_t* quant,
+                         void* scratch, ExtractGlobals* dglob, ExtractGlobals* hglob, hipStream_t s) {
+    if (n == 0) {
+        hipLaunchKernelGGL(k_clear_globals, dim3(1), dim3(64), 0, s, dglob);
+        FFCZ_HIP(hipGetLastError());
+    } else {
+        extract_launch(v, n, q, pack, compact, quant, scratch, dglob, s);
+    }
+    FFCZ_HIP(hipMemcpyAsync(hglob, dglob, sizeof(ExtractGlobals), hipMemcpyDeviceToHost, s));
 }
 
 template <typename T>
@@ -285,6 +300,10 @@ void dequantize_dense(const T* v, size_t n, QuantDesc q, T* out, hipStream_t s) 
 
 template ExtractResult extract_edits<float>(const float*, size_t, QuantDesc, uint8_t*, float*, uint16_t*, void*, hipStream_t);
 template ExtractResult extract_edits<double>(const double*, size_t, QuantDesc, uint8_t*, double*, uint16_t*, void*, hipStream_t);
+template void extract_edits_async<float>(const float*, size_t, QuantDesc, uint8_t*, float*, uint16_t*, void*,
+                                         ExtractGlobals*, ExtractGlobals*, hipStream_t);
+template void extract_edits_async<double>(const double*, size_t, QuantDesc, uint8_t*, double*, uint16_t*, void*,
+                                          ExtractGlobals*, ExtractGlobals*, hipStream_t);
 template void dequantize_dense<float>(const float*, size_t, QuantDesc, float*, hipStream_t);
 template void dequantize_dense<double>(const double*, size_t, QuantDesc, double*, hipStream_t);
 

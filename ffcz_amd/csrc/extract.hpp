@@ -15,6 +15,15 @@ struct ExtractResult {
     double vmin = 0, vmax = 0;
 };
 
+// Device-side results of one extraction (vmin, vmax adjacent: the
+// dequantisers read them through QuantDesc::dminmax).
+struct ExtractGlobals {
+    unsigned long long count;
+    double vmin, vmax;
+    unsigned int overflow;
+    unsigned int pad;
+};
+
 // Device scratch needed by extract_edits for n elements.
 size_t extract_scratch_bytes(size_t n);
 
@@ -27,6 +36,12 @@ size_t extract_scratch_bytes(size_t n);
 template <typename T>
 ExtractResult extract_edits(const T* v, size_t n, QuantDesc q, uint8_t* pack, T* compact, uint16_t* quant,
                             void* scratch, hipStream_t s);
+// The same without a host synchronisation: results land in dglob (device,
+// kept until the next extraction into it) and are copied asynchronously to
+// hglob (pinned host memory) behind the kernels on stream s.
+template <typename T>
+void extract_edits_async(const T* v, size_t n, QuantDesc q, uint8_t* pack, T* compact, uint16_t* quant,
+                         void* scratch, ExtractGlobals* dglob, ExtractGlobals* hglob, hipStream_t s);
 
 // Dense dequantised copy (projection_algorithm.cu:615-644 for Q_REF16).
 template <typename T>

@@ -44,6 +44,7 @@ enum QuantKind : int { Q_NONE = 0, Q_REF16 = 1, Q_STEP = 2 };
 struct QuantDesc {
     int kind;
     double vmin, vmax;  // Q_REF16 (cast to T inside the kernels)
+    const double* dminmax;  // Q_REF16: {vmin, vmax} in device memory, used instead when set
     double step;        // Q_STEP
     double threshold;   // Q_REF16 flag threshold (1e-7 in the reference)
 };

@@ -35,7 +35,7 @@ __device__ __forceinline__ double dequant(T v, const QuantDesc& q) {
     if (q.kind == Q_REF16) {
         // projection_algorithm.cu:629-643 in type T
         if (!(fabs(v) > (T)q.threshold)) return 0.0;
-        const T mn = (T)q.vmin, mx = (T)q.vmax;
+        const T mn = (T)(q.dminmax ? q.dminmax[0] : q.vmin), mx = (T)(q.dminmax ? q.dminmax[1] : q.vmax);
         const T qv = quant_ref16(v, mn, mx);
         const T r = qv / (T)65535 * (mx - mn) + mn;
         return (double)r;

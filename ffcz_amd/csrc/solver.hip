@@ -79,9 +79,15 @@ template <typename T>
 void Solver<T>::release() {
     if (s_) (void)hipStreamSynchronize(s_);
     void* ptrs[] = {x_, e_, e0_, ds_, df_, spec_, st_, vpart_, delta_dense_, orig_, partial_idx_, fpack_, spack_,
-                    fcomp_, scomp_, fquant_, squant_, xscratch_, cert_real_, cert_spec_, cstats_};
+                    fcomp_, scomp_, fquant_, squant_, xscratch_, cert_real_, cert_spec_, cstats_, xglob_};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    if (xhost_) (void)hipHostFree(xhost_);
+    if (chost_) (void)hipHostFree(chost_);
+    xglob_ = nullptr;
+    xhost_ = nullptr;
+    chost_ = nullptr;
+    xpending_ = false;
     x_ = e_ = e0_ = ds_ = df_ = vpart_ = nullptr;
     spec_ = nullptr;
     st_ = nullptr;
@@ -287,6 +293,7 @@ QuantDesc Solver<T>::freq_quant() const {
     q.step = qstep_f_;
     q.vmin = xf_.vmin;
     q.vmax = xf_.vmax;
+    q.dminmax = xglob_ ? &xglob_[0].vmin : nullptr;
     return q;
 }
 template <typename T>
@@ -297,6 +304,7 @@ QuantDesc Solver<T>::spatial_quant() const {
     q.step = qstep_s_;
     q.vmin = xs_.vmin;
     q.vmax = xs_.vmax;
+    q.dminmax = xglob_ ? &xglob_[1].vmin : nullptr;
     return q;
 }
 
@@ -313,15 +321,35 @@ void Solver<T>::ensure_extract_buffers() {
     unsigned char* p = nullptr;
     dalloc(&p, sb);
     xscratch_ = p;
+    dalloc(&xglob_, 2);
+    FFCZ_HIP(hipHostMalloc((void**)&xhost_, 2 * sizeof(ExtractGlobals), hipHostMallocDefault));
 }
 
 template <typename T>
 void Solver<T>::extract() {
     // extract_compact_representation, projection_algorithm_impl.cuh:371-456
     ensure_extract_buffers();
-    xf_ = extract_edits<T>(df_, 2 * f_.F, freq_quant(), fpack_, fcomp_, fquant_, xscratch_, s_);
-    xs_ = extract_edits<T>(ds_, f_.N, spatial_quant(), spack_, scomp_, squant_, xscratch_, s_);
+    sync_extract();  // a previous extraction's results are still being copied
+    QuantDesc qf = freq_quant(), qs = spatial_quant();
+    qf.dminmax = qs.dminmax = nullptr;  // the kernels compute the range themselves
+    extract_edits_async<T>(df_, 2 * f_.F, qf, fpack_, fcomp_, fquant_, xscratch_, &xglob_[0], &xhost_[0], s_);
+    extract_edits_async<T>(ds_, f_.N, qs, spack_, scomp_, squant_, xscratch_, &xglob_[1], &xhost_[1], s_);
+    xpending_ = true;
     extracted_ = true;
+}
+
+template <typename T>
+void Solver<T>::sync_extract() const {
+    if (!xpending_) return;
+    FFCZ_HIP(hipStreamSynchronize(s_));
+    xpending_ = false;
+    const ExtractGlobals hf = xhost_[0], hs = xhost_[1];
+    xf_ = ExtractResult{(size_t)hf.count, hf.vmin, hf.vmax};
+    xs_ = ExtractResult{(size_t)hs.count, hs.vmin, hs.vmax};
+    if (hf.overflow || hs.overflow) {
+        const_cast<Solver<T>*>(this)->extracted_ = false;
+        throw Error(ERR_RANGE, "quantised edit does not fit a 16-bit symbol (raise the quantiser step)");
+    }
 }
 
 template <typename T>
@@ -331,11 +359,18 @@ void Solver<T>::ensure_cert_buffers() {
     f64_->create(f_.nx, f_.ny, f_.nz);
     dalloc(&cert_real_, f_.N);
     dalloc(&cert_spec_, f64_->spec_elems());
-    dalloc(&cstats_, 1);
+    dalloc(&cstats_, 2);
+    FFCZ_HIP(hipHostMalloc((void**)&chost_, 2 * sizeof(CertStats), hipHostMallocDefault));
 }
 
 template <typename T>
 CertResult Solver<T>::certify(bool quantized) {
+    certify_enqueue(quantized);
+    return certify_collect(quantized);
+}
+
+template <typename T>
+void Solver<T>::certify_enqueue(bool quantized) {
     FFCZ_REQUIRE(solved_, ERR_STATE, "certify requires a completed solve");
     FFCZ_REQUIRE(init_mode_ == MODE_ABS || init_mode_ == MODE_DIRECT, ERR_INVALID,
                  "certification is defined for the absolute and per-bin bound modes");
@@ -349,25 +384,31 @@ CertResult Solver<T>::certify(bool quantized) {
     c.ds = ds_;
     c.df = df_;
     if (quantized) {
+        // the dequantisers read the extraction's range on the device
         c.qs = spatial_quant();
         c.qf = freq_quant();
-        c.have_df = xf_.count > 0;
     } else {
         c.qs.kind = Q_NONE;
         c.qf.kind = Q_NONE;
-        // the frequency accumulator is non-zero only if some clamp fired
-        c.have_df = df_touched_;
     }
+    // the frequency accumulator is non-zero only if some clamp fired
+    c.have_df = df_touched_;
     c.eps = (double)eps_;
     c.delta = (double)delta_;
     c.delta_dense = init_mode_ == MODE_DIRECT ? delta_dense_ : nullptr;
     c.work_real = cert_real_;
     c.work_spec = cert_spec_;
-    c.stats = cstats_;
+    c.stats = cstats_ + (quantized ? 1 : 0);
     ffcz::certify<T>(*f64_, c, s_);
-    CertStats hs{};
-    FFCZ_HIP(hipMemcpyAsync(&hs, cstats_, sizeof(hs), hipMemcpyDeviceToHost, s_));
+    FFCZ_HIP(hipMemcpyAsync(chost_ + (quantized ? 1 : 0), c.stats, sizeof(CertStats), hipMemcpyDeviceToHost, s_));
+}
+
+template <typename T>
+CertResult Solver<T>::certify_collect(bool quantized) {
+    FFCZ_REQUIRE(chost_ != nullptr, ERR_STATE, "no certification enqueued");
     FFCZ_HIP(hipStreamSynchronize(s_));
+    if (quantized) sync_extract();  // a quantiser overflow voids the quantised check
+    const CertStats hs = chost_[quantized ? 1 : 0];
     CertResult r;
     r.spatial_violations = hs.spatial_viol;
     r.freq_violations = hs.freq_viol;

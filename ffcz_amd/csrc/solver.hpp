@@ -75,12 +75,19 @@ class Solver {
     double violation() const { return viol_; }
     bool converged() const { return converged_; }
     bool extracted() const { return extracted_; }
-    size_t n_active_freq() const { return xf_.count; }
-    size_t n_active_spatial() const { return xs_.count; }
-    double freq_min() const { return xf_.vmin; }
-    double freq_max() const { return xf_.vmax; }
-    double spatial_min() const { return xs_.vmin; }
-    double spatial_max() const { return xs_.vmax; }
+    // extraction results (wait for a pending extraction first)
+    size_t n_active_freq() const { return sync_extract(), xf_.count; }
+    size_t n_active_spatial() const { return sync_extract(), xs_.count; }
+    double freq_min() const { return sync_extract(), xf_.vmin; }
+    double freq_max() const { return sync_extract(), xf_.vmax; }
+    double spatial_min() const { return sync_extract(), xs_.vmin; }
+    double spatial_max() const { return sync_extract(), xs_.vmax; }
+    // extract() only enqueues; this waits for it and checks the quantiser range
+    void sync_extract() const;
+    // certify() in two halves: enqueue the FP64 passes (slot = quantized), then
+    // wait for them and read the result; several solvers can be in flight
+    void certify_enqueue(bool quantized);
+    CertResult certify_collect(bool quantized);
     const uint8_t* d_freq_pack() const { return fpack_; }
     const uint8_t* d_spatial_pack() const { return spack_; }
     const T* d_freq_compact() const { return fcomp_; }
@@ -125,13 +132,17 @@ class Solver {
     uint16_t* fquant_ = nullptr;
     uint16_t* squant_ = nullptr;
     void* xscratch_ = nullptr;
-    ExtractResult xf_, xs_;
+    ExtractGlobals* xglob_ = nullptr;  // device results of the two extractions (freq, spatial)
+    ExtractGlobals* xhost_ = nullptr;  // pinned host copies
+    mutable ExtractResult xf_, xs_;
+    mutable bool xpending_ = false;
     bool extracted_ = false;
     // certification
     std::unique_ptr<Fft3d<double>> f64_;
     double* cert_real_ = nullptr;
     cplx<double>* cert_spec_ = nullptr;
-    CertStats* cstats_ = nullptr;
+    CertStats* cstats_ = nullptr;  // [2]: unquantised, quantised
+    CertStats* chost_ = nullptr;   // pinned host copies
     // parameters
     bool initialized_ = false;
     int init_mode_ = -1;

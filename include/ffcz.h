@@ -151,6 +151,11 @@ int ffcz_batch_solve(ffcz_batch* b, int mode, const void* const* e0, int max_ite
  * (in chunks between host checks; fields already converged skip them) */
 int ffcz_batch_status(ffcz_batch* b, int* iterations, double* violations, int* converged, size_t cap,
                       int* launched);
+/* extract_compact_representation, and certify(quantized), of every field of
+ * the batch: each field's work is queued on its own stream before any is
+ * waited for.  out receives one result per field (cap >= field count). */
+int ffcz_batch_extract(ffcz_batch* b);
+int ffcz_batch_certify(ffcz_batch* b, int quantized, ffcz_cert* out, size_t cap);
 /* mean ms of one whole batched iteration (HIP events on the batch stream),
  * after a completed batch solve; the edit accumulators change: reset them */
 int ffcz_batch_time_iteration(ffcz_batch* b, int reps, float* ms);

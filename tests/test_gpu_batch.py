@@ -151,3 +151,31 @@ def test_batch_field_handles_are_borrowed(gpu):
         assert lib.ffcz_destroy(b.fields[0]._h) != 0  # freed with the batch only
         h = C.c_void_p()
         assert lib.ffcz_batch_field(b._h, 2, C.byref(h)) != 0
+
+
+def test_batch_extract_and_certify_all_match_per_field(gpu):
+    dtype = np.float32
+    shapes = SHAPES[:5]
+    cases = _cases(shapes, dtype, seed0=50)
+    res = {}
+    for overlapped in (False, True):
+        with ProjectionBatch(shapes, dtype=dtype) as b:
+            for f, (x, e0, eps, delta) in zip(b.fields, cases):
+                f.initialize_all_abs(x, eps, delta)
+                f.set_margins(0.125, 0.125)
+            b.solve(MODE_ABS, [c[1] for c in cases], 100, 1e-7)
+            if overlapped:
+                c1 = b.certify_all(False)
+                b.extract_all()
+                c2 = b.certify_all(True)
+            else:
+                c1 = [f.certify(False) for f in b.fields]
+                for f in b.fields:
+                    f.extract_compact_representation()
+                c2 = [f.certify(True) for f in b.fields]
+            q = [(f.get_active_freq_quant_edit_distances(), f.get_spatial_flags_pack()) for f in b.fields]
+            res[overlapped] = (c1, c2, q)
+    assert res[False][0] == res[True][0] and res[False][1] == res[True][1]
+    assert all(c.ok for c in res[True][0]) and all(c.ok for c in res[True][1])
+    for (a1, b1), (a2, b2) in zip(res[False][2], res[True][2]):
+        assert np.array_equal(a1, a2) and np.array_equal(b1, b2)

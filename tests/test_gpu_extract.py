@@ -10,7 +10,7 @@ import torch
 
 import ffcz_amd
 from ffcz_amd import ProjectionSolver, QUANT_STEP
-from ffcz_amd._lib import call
+from ffcz_amd._lib import FFCzError, call
 from oracle import ffcz_oracle as O
 from tests._cases import binding_case
 
@@ -121,3 +121,41 @@ def test_quantize_dequantize_kernel(gpu):
     call("ffcz_k_quantize_dequantize", 0, dv.data_ptr(), out.data_ptr(), lo, hi, v.size, 1e-7, None)
     ref = O.quantize_and_dequantize_ref(v, np.float32(lo), np.float32(hi))
     assert np.array_equal(out.cpu().numpy(), ref)
+
+
+def test_ref16_quantized_certification_reads_device_range(gpu):
+    """certify(quantized) with the reference quantiser: the dequantisers take
+    min/max from the extraction on the device (no host round trip); counts
+    match the oracle on the dequantised edits (projection_algorithm.cu:615-644)."""
+    shape = (16, 20, 30)
+    x, e0, eps, delta = binding_case(shape, seed=9)
+    with ProjectionSolver(*shape, dtype=np.float64) as s:
+        s.initialize_all_abs(x, eps, delta)
+        s.solve_all_abs(e0, 30, 1e-12)
+        c2 = s.certify(True)
+        df = s.get_freq_edits_full()
+        ds = s.get_spatial_edits_full()
+        fmin, fmax = float(df.min()), float(df.max())
+        smin, smax = float(ds.min()), float(ds.max())
+        dq = O.reconstruct_edit(shape, e0, O.quantize_and_dequantize_ref(df, fmin, fmax),
+                                O.quantize_and_dequantize_ref(ds, smin, smax))
+        r2 = O.certify(dq, eps, delta)
+        assert c2.spatial_violations == r2["spatial_violations"]
+        assert c2.freq_violations == r2["freq_violations"]
+        assert c2.ok == r2["ok"]
+
+
+def test_step_quantizer_overflow_is_reported(gpu):
+    """Extraction only queues its kernels; a symbol beyond 16 bits surfaces as
+    an error on the next result read (getter or quantised certification)."""
+    shape = (6, 10, 12)
+    x, e0, eps, delta = binding_case(shape, seed=2)
+    with ProjectionSolver(*shape, dtype=np.float64) as s:
+        s.initialize_all_abs(x, eps, delta)
+        s.set_quantizer(QUANT_STEP, delta * 1e-9, eps * 1e-9)
+        s.solve_all_abs(e0, 20, 1e-12)  # extraction runs at the end of solve_all_abs
+        assert s.certify(False) is not None
+        with pytest.raises(FFCzError):
+            s.certify(True)
+        with pytest.raises(FFCzError):
+            s.get_active_spatial_quant_edit_distances()
