@@ -43,6 +43,7 @@ def parse():
                    help="FFT variant(s) of the iteration; 'all' runs each and reports the fastest certified one")
     p.add_argument("--no-fft-error", action="store_true")
     p.add_argument("--no-graph", action="store_true", help="c4: launch the batched iteration eagerly")
+    p.add_argument("--no-autotune", action="store_true", help="keep the default tile variants")
     p.add_argument("--cpu-sample-iters", type=int, default=1)
     p.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                    help="only run a plain device copy of BYTES (counter calibration) and exit")
@@ -278,6 +279,10 @@ def main():
     solver.set_margins(0.125, 0.125)
     solver.set_quantizer(QUANT_STEP, fstep, wl.eps / 8)
 
+    # plan building: the autotuner picks each pass kernel's tile variant
+    # (per FFT variant, before the timed region)
+    tuned = {}
+
     if a.fft == "all":
         variants = ["valu-fp32", "mfma-bf16x3", "mfma-fp16x3"] if prec == "f32" else ["valu-fp32"]
     else:
@@ -307,6 +312,9 @@ def main():
 
     def run_variant(variant):
         solver.set_fft_precision(FFT_VARIANTS[variant])
+        if not a.no_autotune:
+            tuned[variant] = solver.autotune(reps=3, timings=True)
+            print(f"[bench] {variant}: autotuned tile variants {tuned[variant]}", file=sys.stderr, flush=True)
 
         def step():
             solver.reset_edit_accumulators()
@@ -338,6 +346,7 @@ def main():
         return {
             "variant": variant, "value": value, "elapsed": elapsed, "iterations": iters, "converged": conv,
             "certified": ok, "ms": ms, "t_iter": t_iter, "dom": dom, "fft_error": fft_error(variant),
+            "tile_vars": tuned.get(variant),
             "cert_excess": {"spatial": max(max(c1.spatial_max_excess, c2.spatial_max_excess) for c1, c2 in certs),
                             "freq": max(max(c1.freq_max_excess, c2.freq_max_excess) for c1, c2 in certs)},
         }
@@ -393,6 +402,7 @@ def main():
                                     "ms_per_step": round(r["elapsed"] / a.steps * 1e3, 4),
                                     "iterations": r["iterations"], "certified": r["certified"],
                                     "fft_error_vs_fp64": r["fft_error"], "cert_max_excess": r["cert_excess"],
+                                    "tile_vars": r["tile_vars"],
                                     "iteration": iteration_summary(r)} for r in results},
         "cpu_baseline": cpu,
     }

@@ -100,6 +100,9 @@ SIGNATURES = {
     "ffcz_batch_extract": [_h],
     "ffcz_batch_certify": [_h, _i, C.POINTER(Cert), _sz],
     "ffcz_set_fft_precision": [_h, _i],
+    "ffcz_autotune": [_h, _i, _i, C.POINTER(_i), C.POINTER(_f)],
+    "ffcz_get_tile_vars": [_h, C.POINTER(_i)],
+    "ffcz_set_tile_vars": [_h, C.POINTER(_i)],
     "ffcz_get_fft_precision": [_h, C.POINTER(_i)],
     "ffcz_certify": [_h, _i, C.POINTER(Cert)],
     "ffcz_set_tuning": [_i, C.c_long],

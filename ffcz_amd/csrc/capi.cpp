@@ -481,6 +481,28 @@ int ffcz_get_fft_precision(ffcz_solver* h, int* variant) {
         with(h, [&](auto& s) { *variant = s.fft_precision(); });
     });
 }
+static void vars_out(const TileVars& v, int* out) {
+    if (!out) return;
+    const int a[7] = {v.zf, v.zi, v.y, v.x, v.czf, v.cy, v.cx};
+    std::memcpy(out, a, sizeof(a));
+}
+int ffcz_autotune(ffcz_solver* h, int reps, int use_cache, int* vars, float* ms) {
+    return guard([&] { with(h, [&](auto& s) { vars_out(s.autotune(reps, use_cache != 0, ms), vars); }); });
+}
+int ffcz_get_tile_vars(ffcz_solver* h, int* vars) {
+    return guard([&] {
+        FFCZ_REQUIRE(vars != nullptr, ERR_INVALID, "null output");
+        with(h, [&](auto& s) { vars_out(s.tile_vars(), vars); });
+    });
+}
+int ffcz_set_tile_vars(ffcz_solver* h, const int* vars) {
+    return guard([&] {
+        FFCZ_REQUIRE(vars != nullptr, ERR_INVALID, "null input");
+        with(h, [&](auto& s) {
+            s.set_tile_vars(TileVars{vars[0], vars[1], vars[2], vars[3], vars[4], vars[5], vars[6]});
+        });
+    });
+}
 int ffcz_set_quantizer(ffcz_solver* h, int kind, double fstep, double sstep) {
     return guard([&] { with(h, [&](auto& s) { s.set_quantizer(kind, fstep, sstep); }); });
 }

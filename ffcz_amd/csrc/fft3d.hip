@@ -512,13 +512,17 @@ template <typename R>
 static unsigned zgrid(const Fft3d<R>& f) { return (unsigned)ceil_div(f.rows, (size_t)f.Pz); }
 
 template <typename R, typename T, int MODE>
-static void launch_zfwd(const Fft3d<R>& f, const ZArgs<R, T>& z, hipStream_t s) {
+static void launch_zfwd(const Fft3d<R>& f, const ZArgs<R, T>& z0, hipStream_t s) {
+    ZArgs<R, T> z = z0;
+    z.var = f.prec == 0 ? f.zf_var : 0;
     if (!(fast_fft_enabled() && fast_zfwd<R, T>(MODE, z, s)))
         hipLaunchKernelGGL((k_zfwd<R, T, MODE>), dim3(zgrid(f)), dim3(FFT_THREADS), zlds(f), s, z);
     FFCZ_HIP(hipGetLastError());
 }
 template <typename R, typename T, int MODE>
-static void launch_zinv(const Fft3d<R>& f, const ZArgs<R, T>& z, hipStream_t s) {
+static void launch_zinv(const Fft3d<R>& f, const ZArgs<R, T>& z0, hipStream_t s) {
+    ZArgs<R, T> z = z0;
+    z.var = f.zinv_var();
     if (!(fast_fft_enabled() && fast_zinv<R, T>(MODE, z, s)))
         hipLaunchKernelGGL((k_zinv<R, T, MODE>), dim3(zgrid(f)), dim3(FFT_THREADS), zlds(f), s, z);
     FFCZ_HIP(hipGetLastError());
@@ -561,9 +565,11 @@ static SArgs<R, T> sargs(const Fft3d<R>& f, cplx<R>* data, bool xaxis) {
     return a;
 }
 template <typename R, typename T>
-static void launch_strided(const Fft3d<R>& f, const SArgs<R, T>& a, int mode, bool xaxis, hipStream_t s,
+static void launch_strided(const Fft3d<R>& f, const SArgs<R, T>& a0, int mode, bool xaxis, hipStream_t s,
                            size_t outer_override = 0) {
     const size_t outer = outer_override ? outer_override : (xaxis ? f.ny : f.nx);
+    SArgs<R, T> a = a0;
+    a.var = f.prec == 0 ? (xaxis ? f.x_var : f.y_var) : 0;
     if (fast_fft_enabled() && fast_strided<R, T>(mode, a, outer, s)) {
         FFCZ_HIP(hipGetLastError());
         return;
@@ -692,7 +698,7 @@ void iteration(const Fft3d<R>& f, const IterArgs<R>& ia, hipStream_t s, EventLog
         launch_zinv<R, R, Z_ITER>(f, zslab(x0, x1), s);
         mark(4);
     }
-    const int zr = fast_z_rows(f.Lz, f.half);
+    const int zr = fast_z_rows(f.Lz, f.half, f.zinv_var());
     const size_t nparts = ceil_div(f.rows, (size_t)(zr ? zr : f.Pz));
     hipLaunchKernelGGL((k_finalize_iteration<R>), dim3(1), dim3(1024), 0, s, (const R*)ia.vpart, nparts, ia.st);
     FFCZ_HIP(hipGetLastError());

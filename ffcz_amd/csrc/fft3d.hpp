@@ -62,6 +62,10 @@ struct Fft3d {
     FftPlan pz{}, py{}, px{};
     int Pz = 1, Wy = 1, Wx = 1;       // pencils per z tile, columns per y/x tile
     int prec = 0;                     // FftPrec of the FP32 pass kernels (0: vector ALU)
+    // tile variant of each pass's length-specialised kernel (autotuned; the
+    // MFMA variants and batched launches use variant 0)
+    int zf_var = 0, zi_var = 0, y_var = 0, x_var = 0;
+    int zinv_var() const { return prec == 0 ? zi_var : 0; }
     cplx<R>* d_tz = nullptr;   // W_Lz^m
     cplx<R>* d_tz2 = nullptr;  // W_nz^k, k <= nz/2 (R2C post / C2R pre)
     cplx<R>* d_ty = nullptr;

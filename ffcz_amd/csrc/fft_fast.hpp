@@ -44,7 +44,7 @@ bool fast_strided_batch(int mode, int L, const BatchTab<SArgs<float, float>>& t,
 // whether a y/x axis length has a specialised kernel, and the columns per
 // tile of those kernels for a real type of `bytes` bytes
 bool fast_s_supported(int L);
-int fast_s_width(size_t bytes);
+int fast_s_width(size_t bytes, int var = 0);
 
 // Process-wide tuning knobs (C ABI: ffcz_set_tuning).
 struct Tuning {
@@ -55,8 +55,8 @@ struct Tuning {
 };
 Tuning& tuning();
 bool fast_fft_enabled();
-// rows per workgroup of the specialised z kernels for half length Lz (0: the
-// generic kernels run)
-int fast_z_rows(int Lz, bool half);
+// rows per workgroup of the specialised z kernels for half length Lz and tile
+// variant var (0: the generic kernels run)
+int fast_z_rows(int Lz, bool half, int var = 0);
 
 }  // namespace ffcz

@@ -42,16 +42,16 @@ bool fast_s_supported(int L) {
             return false;
     }
 }
-int fast_s_width(size_t bytes) { return bytes == 4 ? s_width<float>() : s_width<double>(); }
+int fast_s_width(size_t bytes, int var) { return bytes == 4 ? s_width<float>(var) : s_width<double>(var); }
 
 bool fast_fft_enabled() { return tuning().force_generic.load() == 0; }
 
-int fast_z_rows(int Lz, bool half) {
+int fast_z_rows(int Lz, bool half, int var) {
     if (!half || !fast_fft_enabled()) return 0;
     switch (Lz) {
 #define FFCZ_ZR(Lv) \
     case Lv:        \
-        return z_rows(Lv);
+        return z_rows(Lv, var);
         FFCZ_FAST_Z_LENGTHS(FFCZ_ZR)
 #undef FFCZ_ZR
         default:

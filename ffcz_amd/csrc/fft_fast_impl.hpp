@@ -16,12 +16,15 @@ namespace ffcz {
 #define FFCZ_FAST_Z_LENGTHS(X) \
     X(4) X(8) X(16) X(30) X(32) X(36) X(48) X(60) X(64) X(75) X(90) X(120) X(125) X(128) X(150) X(256) X(300)
 
+// Tile variants (chosen per pass by the autotuner, fft3d.hpp TileVars):
+//   y/x passes: V=0 columns of 128 bytes per tile, V=1 64 bytes (half the LDS)
+//   z passes:   V=0 up to 256 threads per workgroup, V=1 up to 128
 template <typename R>
-constexpr int s_width() { return sizeof(R) == 4 ? 16 : 8; }  // 128-byte column segments
+constexpr int s_width(int V = 0) { return (sizeof(R) == 4 ? 16 : 8) >> V; }
 constexpr int z_tp(int LZ) { return pen::threads_per_pencil(LZ); }
-constexpr int z_rows(int LZ) {
+constexpr int z_rows(int LZ, int V = 0) {
     int p = 1;
-    while (p * 2 * z_tp(LZ) <= 512) p *= 2;
+    while (p * 2 * z_tp(LZ) <= (V ? 256 : 512)) p *= 2;
     return p;
 }
 
@@ -31,9 +34,9 @@ constexpr int z_rows(int LZ) {
 // FFT ends in LDS and the R2C split X[k] = A[k] + W_nz^k B[k] writes the
 // padded half spectrum.
 // ---------------------------------------------------------------------------
-template <typename R, typename T, int LZ, int MODE, int PREC>
+template <typename R, typename T, int LZ, int MODE, int PREC, int V>
 __device__ __forceinline__ void zfwd_body(const ZArgs<R, T>& a, uint32_t blk) {
-    constexpr int TP = z_tp(LZ), P = z_rows(LZ), NT = TP * P;
+    constexpr int TP = z_tp(LZ), P = z_rows(LZ, V), NT = TP * P;
     constexpr int NZ = 2 * LZ, M = LZ, NH = M + 1, SN = P + 1;
     __shared__ cplx<R> tw[LZ];
     __shared__ cplx<R> tw2[NH];
@@ -124,9 +127,9 @@ __device__ __forceinline__ void zfwd_body(const ZArgs<R, T>& a, uint32_t blk) {
 // the real row scaled by 1/N, and ITER tracks the spatial violation and
 // finalises the iteration in the last workgroup.
 // ---------------------------------------------------------------------------
-template <typename R, typename T, int LZ, int MODE, int PREC>
+template <typename R, typename T, int LZ, int MODE, int PREC, int V>
 __device__ __forceinline__ void zinv_body(const ZArgs<R, T>& a, uint32_t blk) {
-    constexpr int TP = z_tp(LZ), P = z_rows(LZ), NT = TP * P;
+    constexpr int TP = z_tp(LZ), P = z_rows(LZ, V), NT = TP * P;
     constexpr int NZ = 2 * LZ, M = LZ, NH = M + 1, SN = P + 1;
     __shared__ cplx<R> tw[LZ];
     __shared__ cplx<R> tw2[NH];
@@ -180,9 +183,9 @@ __device__ __forceinline__ void zinv_body(const ZArgs<R, T>& a, uint32_t blk) {
 // ---------------------------------------------------------------------------
 // y / x passes over 128-byte column tiles.
 // ---------------------------------------------------------------------------
-template <typename R, typename T, int L, int MODE, int PREC>
+template <typename R, typename T, int L, int MODE, int PREC, int V>
 __device__ __forceinline__ void strided_body(const SArgs<R, T>& a, uint32_t blk) {
-    constexpr int TP = pen::threads_per_pencil(L), W = s_width<R>(), NT = TP * W;
+    constexpr int TP = pen::threads_per_pencil(L), W = s_width<R>(V), NT = TP * W;
     __shared__ cplx<R> tw[L];
     __shared__ cplx<R> buf[L * W];
     if (a.st && a.st->done) return;
@@ -264,67 +267,68 @@ __device__ __forceinline__ uint32_t batch_field(const BatchTab<A>& t, uint32_t b
     return (uint32_t)lo;
 }
 
-template <typename R, typename T, int LZ, int MODE, int PREC = PREC_VALU>
-__global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ)) void k2_zfwd(ZArgs<R, T> a) {
-    zfwd_body<R, T, LZ, MODE, PREC>(a, blockIdx.x);
+template <typename R, typename T, int LZ, int MODE, int PREC = PREC_VALU, int V = 0>
+__global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ, V)) void k2_zfwd(ZArgs<R, T> a) {
+    zfwd_body<R, T, LZ, MODE, PREC, V>(a, blockIdx.x);
 }
-template <typename R, typename T, int LZ, int MODE, int PREC = PREC_VALU>
-__global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ)) void k2_zinv(ZArgs<R, T> a) {
-    zinv_body<R, T, LZ, MODE, PREC>(a, blockIdx.x);
+template <typename R, typename T, int LZ, int MODE, int PREC = PREC_VALU, int V = 0>
+__global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ, V)) void k2_zinv(ZArgs<R, T> a) {
+    zinv_body<R, T, LZ, MODE, PREC, V>(a, blockIdx.x);
 }
-template <typename R, typename T, int L, int MODE, int PREC = PREC_VALU>
-__global__ __launch_bounds__(pen::threads_per_pencil(L) * s_width<R>()) void k2_strided(SArgs<R, T> a) {
-    strided_body<R, T, L, MODE, PREC>(a, blockIdx.x);
+template <typename R, typename T, int L, int MODE, int PREC = PREC_VALU, int V = 0>
+__global__ __launch_bounds__(pen::threads_per_pencil(L) * s_width<R>(V)) void k2_strided(SArgs<R, T> a) {
+    strided_body<R, T, L, MODE, PREC, V>(a, blockIdx.x);
 }
 template <typename R, int LZ, int MODE>
 __global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ)) void k2_zfwd_b(BatchTab<ZArgs<R, R>> t) {
     const uint32_t f = batch_field(t, blockIdx.x);
     const ZArgs<R, R> a = t.args[f];
-    zfwd_body<R, R, LZ, MODE, PREC_VALU>(a, blockIdx.x - t.start[f]);
+    zfwd_body<R, R, LZ, MODE, PREC_VALU, 0>(a, blockIdx.x - t.start[f]);
 }
 template <typename R, int LZ, int MODE>
 __global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ)) void k2_zinv_b(BatchTab<ZArgs<R, R>> t) {
     const uint32_t f = batch_field(t, blockIdx.x);
     const ZArgs<R, R> a = t.args[f];
-    zinv_body<R, R, LZ, MODE, PREC_VALU>(a, blockIdx.x - t.start[f]);
+    zinv_body<R, R, LZ, MODE, PREC_VALU, 0>(a, blockIdx.x - t.start[f]);
 }
 template <typename R, int L, int MODE>
 __global__ __launch_bounds__(pen::threads_per_pencil(L) * s_width<R>()) void k2_strided_b(BatchTab<SArgs<R, R>> t) {
     const uint32_t f = batch_field(t, blockIdx.x);
     const SArgs<R, R> a = t.args[f];
-    strided_body<R, R, L, MODE, PREC_VALU>(a, blockIdx.x - t.start[f]);
+    strided_body<R, R, L, MODE, PREC_VALU, 0>(a, blockIdx.x - t.start[f]);
 }
 
 // ---------------------------------------------------------------------------
 // dispatch
 // ---------------------------------------------------------------------------
-template <typename R, typename T, int MODE, int LZ, int PREC = PREC_VALU>
+template <typename R, typename T, int MODE, int LZ, int PREC = PREC_VALU, int V = 0>
 inline void launch_zfwd_len(const ZArgs<R, T>& a, hipStream_t s) {
-    const unsigned grid = (unsigned)ceil_div(a.rows, (size_t)z_rows(LZ));
-    hipLaunchKernelGGL((k2_zfwd<R, T, LZ, MODE, PREC>), dim3(grid), dim3(z_tp(LZ) * z_rows(LZ)), 0, s, a);
+    const unsigned grid = (unsigned)ceil_div(a.rows, (size_t)z_rows(LZ, V));
+    hipLaunchKernelGGL((k2_zfwd<R, T, LZ, MODE, PREC, V>), dim3(grid), dim3(z_tp(LZ) * z_rows(LZ, V)), 0, s, a);
 }
-template <typename R, typename T, int MODE, int LZ, int PREC = PREC_VALU>
+template <typename R, typename T, int MODE, int LZ, int PREC = PREC_VALU, int V = 0>
 inline void launch_zinv_len(const ZArgs<R, T>& a, hipStream_t s) {
-    const unsigned grid = (unsigned)ceil_div(a.rows, (size_t)z_rows(LZ));
-    hipLaunchKernelGGL((k2_zinv<R, T, LZ, MODE, PREC>), dim3(grid), dim3(z_tp(LZ) * z_rows(LZ)), 0, s, a);
+    const unsigned grid = (unsigned)ceil_div(a.rows, (size_t)z_rows(LZ, V));
+    hipLaunchKernelGGL((k2_zinv<R, T, LZ, MODE, PREC, V>), dim3(grid), dim3(z_tp(LZ) * z_rows(LZ, V)), 0, s, a);
 }
-template <typename R, typename T, int MODE, int L, int PREC = PREC_VALU>
+template <typename R, typename T, int MODE, int L, int PREC = PREC_VALU, int V = 0>
 inline void launch_strided_len(SArgs<R, T> a, size_t outer, hipStream_t s) {
-    constexpr int W = s_width<R>();
+    constexpr int W = s_width<R>(V);
     a.W = W;
     a.nct = (int)ceil_div((size_t)a.ncols, (size_t)W);
     a.dNct = FastDiv((uint32_t)a.nct);
     const unsigned grid = (unsigned)(outer * (size_t)a.nct);
-    hipLaunchKernelGGL((k2_strided<R, T, L, MODE, PREC>), dim3(grid), dim3(pen::threads_per_pencil(L) * W), 0, s, a);
+    hipLaunchKernelGGL((k2_strided<R, T, L, MODE, PREC, V>), dim3(grid), dim3(pen::threads_per_pencil(L) * W), 0, s, a);
 }
 
 template <typename R, typename T, int MODE>
 bool zfwd_by_len(const ZArgs<R, T>& a, hipStream_t s) {
     if (!a.half) return false;
     switch (a.Lz) {
-#define FFCZ_ZF(Lv)                                 \
-    case Lv:                                        \
-        launch_zfwd_len<R, T, MODE, Lv>(a, s);      \
+#define FFCZ_ZF(Lv)                                                              \
+    case Lv:                                                                     \
+        if (a.var) launch_zfwd_len<R, T, MODE, Lv, PREC_VALU, 1>(a, s);          \
+        else launch_zfwd_len<R, T, MODE, Lv, PREC_VALU, 0>(a, s);                \
         return true;
         FFCZ_FAST_Z_LENGTHS(FFCZ_ZF)
 #undef FFCZ_ZF
@@ -336,9 +340,10 @@ template <typename R, typename T, int MODE>
 bool zinv_by_len(const ZArgs<R, T>& a, hipStream_t s) {
     if (!a.half) return false;
     switch (a.Lz) {
-#define FFCZ_ZI(Lv)                                 \
-    case Lv:                                        \
-        launch_zinv_len<R, T, MODE, Lv>(a, s);      \
+#define FFCZ_ZI(Lv)                                                              \
+    case Lv:                                                                     \
+        if (a.var) launch_zinv_len<R, T, MODE, Lv, PREC_VALU, 1>(a, s);          \
+        else launch_zinv_len<R, T, MODE, Lv, PREC_VALU, 0>(a, s);                \
         return true;
         FFCZ_FAST_Z_LENGTHS(FFCZ_ZI)
 #undef FFCZ_ZI
@@ -349,9 +354,10 @@ bool zinv_by_len(const ZArgs<R, T>& a, hipStream_t s) {
 template <typename R, typename T, int MODE>
 bool strided_by_len(const SArgs<R, T>& a, size_t outer, hipStream_t s) {
     switch (a.L) {
-#define FFCZ_SL(Lv)                                      \
-    case Lv:                                             \
-        launch_strided_len<R, T, MODE, Lv>(a, outer, s); \
+#define FFCZ_SL(Lv)                                                                 \
+    case Lv:                                                                        \
+        if (a.var) launch_strided_len<R, T, MODE, Lv, PREC_VALU, 1>(a, outer, s);   \
+        else launch_strided_len<R, T, MODE, Lv, PREC_VALU, 0>(a, outer, s);         \
         return true;
         FFCZ_FAST_S_LENGTHS(FFCZ_SL)
 #undef FFCZ_SL

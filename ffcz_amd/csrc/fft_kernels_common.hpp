@@ -84,6 +84,7 @@ template <typename R, typename T>
 struct ZArgs {
     int nz, Lz, M, half, P, sn;
     int prec;             // FftPrec (float passes only)
+    int var;              // tile variant of the length-specialised kernel
     FastDiv dP, dM, dNh, dNp, dNz, dFill;  // P, M, nz/2+1, M/2+1, nz, nz-(nz/2+1)
     size_t rows, Mp;
     FftPlan pl;
@@ -128,6 +129,7 @@ struct SArgs {
     const cplx<R>* tw;
     int L, W, wshift, ncols, nct;
     int prec;             // FftPrec (float passes only)
+    int var;              // tile variant of the length-specialised kernel
     FastDiv dW, dNct;
     size_t ob, es;
     int sign;

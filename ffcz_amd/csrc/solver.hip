@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <limits>
+#include <mutex>
 
 #include "solver.hpp"
 
@@ -120,7 +121,6 @@ void Solver<T>::reset_edit_accumulators() {
     hipLaunchKernelGGL((k_zero<T>), dim3(gridn(2 * f_.F)), dim3(256), 0, s_, df_, 2 * f_.F);
     hipLaunchKernelGGL((k_zero<T>), dim3(gridn(f_.N)), dim3(256), 0, s_, ds_, f_.N);
     FFCZ_HIP(hipGetLastError());
-    FFCZ_HIP(hipStreamSynchronize(s_));
 }
 
 template <typename T>
@@ -376,6 +376,13 @@ void Solver<T>::certify_enqueue(bool quantized) {
                  "certification is defined for the absolute and per-bin bound modes");
     if (quantized) FFCZ_REQUIRE(extracted_, ERR_STATE, "certify(quantized) requires extraction first");
     ensure_cert_buffers();
+    const CertArgs<T> c = cert_args(quantized);
+    ffcz::certify<T>(*f64_, c, s_);
+    FFCZ_HIP(hipMemcpyAsync(chost_ + (quantized ? 1 : 0), c.stats, sizeof(CertStats), hipMemcpyDeviceToHost, s_));
+}
+
+template <typename T>
+CertArgs<T> Solver<T>::cert_args(bool quantized) const {
     CertArgs<T> c{};
     c.nx = f_.nx;
     c.ny = f_.ny;
@@ -399,8 +406,7 @@ void Solver<T>::certify_enqueue(bool quantized) {
     c.work_real = cert_real_;
     c.work_spec = cert_spec_;
     c.stats = cstats_ + (quantized ? 1 : 0);
-    ffcz::certify<T>(*f64_, c, s_);
-    FFCZ_HIP(hipMemcpyAsync(chost_ + (quantized ? 1 : 0), c.stats, sizeof(CertStats), hipMemcpyDeviceToHost, s_));
+    return c;
 }
 
 template <typename T>
@@ -469,6 +475,156 @@ void Solver<T>::set_fft_precision(int prec) {
     FFCZ_REQUIRE(prec == PREC_VALU || sizeof(T) == 4, ERR_INVALID,
                  "the MFMA split-precision FFT variants are FP32 transforms; FP64 solvers use the vector ALU");
     f_.prec = prec;
+}
+
+// ---------------------------------------------------------------------------
+// autotuner: tile variant of every pass kernel, chosen by timing each
+// candidate on this field's own buffers; results are cached per shape, FFT
+// variant and type for the life of the process
+// ---------------------------------------------------------------------------
+namespace {
+struct TuneKey {
+    size_t bytes, nx, ny, nz;
+    int prec;
+    bool operator==(const TuneKey& o) const {
+        return bytes == o.bytes && nx == o.nx && ny == o.ny && nz == o.nz && prec == o.prec;
+    }
+};
+std::mutex g_tune_mu;
+std::vector<std::pair<TuneKey, TileVars>> g_tune_cache;
+
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    EventPair() {
+        FFCZ_HIP(hipEventCreateWithFlags(&a, hipEventDisableSystemFence));
+        if (hipEventCreateWithFlags(&b, hipEventDisableSystemFence) != hipSuccess) {
+            (void)hipEventDestroy(a);
+            throw Error(ERR_HIP, "event creation failed");
+        }
+    }
+    ~EventPair() {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+    }
+    template <class F>
+    float time(hipStream_t s, int reps, F&& fn) {
+        fn();  // warm-up
+        FFCZ_HIP(hipEventRecord(a, s));
+        for (int r = 0; r < reps; ++r) fn();
+        FFCZ_HIP(hipEventRecord(b, s));
+        FFCZ_HIP(hipEventSynchronize(b));
+        float t = 0;
+        FFCZ_HIP(hipEventElapsedTime(&t, a, b));
+        return t / (float)reps;
+    }
+};
+}  // namespace
+
+template <typename T>
+TileVars Solver<T>::tile_vars() const {
+    TileVars v{};
+    v.zf = f_.zf_var;
+    v.zi = f_.zi_var;
+    v.y = f_.y_var;
+    v.x = f_.x_var;
+    if (f64_) {
+        v.czf = f64_->zf_var;
+        v.cy = f64_->y_var;
+        v.cx = f64_->x_var;
+    }
+    return v;
+}
+
+template <typename T>
+void Solver<T>::set_tile_vars(const TileVars& v) {
+    const int* p = &v.zf;
+    for (int i = 0; i < 7; ++i) FFCZ_REQUIRE(p[i] == 0 || p[i] == 1, ERR_INVALID, "tile variants are 0 or 1");
+    f_.zf_var = v.zf;
+    f_.zi_var = v.zi;
+    f_.y_var = v.y;
+    f_.x_var = v.x;
+    ensure_cert_buffers();
+    f64_->zf_var = v.czf;
+    f64_->y_var = v.cy;
+    f64_->x_var = v.cx;
+}
+
+template <typename T>
+TileVars Solver<T>::autotune(int reps, bool use_cache, float* ms_out) {
+    FFCZ_REQUIRE(initialized_, ERR_STATE, "autotune requires an initialized solver");
+    FFCZ_REQUIRE(reps >= 1, ERR_INVALID, "reps must be >= 1");
+    const TuneKey key{sizeof(T), f_.nx, f_.ny, f_.nz, f_.prec};
+    if (use_cache && !ms_out) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        for (auto& kv : g_tune_cache)
+            if (kv.first == key) {
+                set_tile_vars(kv.second);
+                return kv.second;
+            }
+    }
+    ensure_cert_buffers();
+    // the tuner runs the passes on this solver's buffers: a zero initial error
+    // (no violations, as in a converged field), state that never stops them
+    FFCZ_HIP(hipMemsetAsync(e0_, 0, f_.N * sizeof(T), s_));
+    reset_edit_accumulators();
+    reset_state<T>(st_, 1 << 30, -1.0, s_);
+    const IterArgs<T> a = iter_args();
+    EventPair ev;
+    float ms[7 * 2] = {};
+    int* fv[4] = {&f_.zf_var, &f_.y_var, &f_.x_var, &f_.zi_var};
+    const int kinds[4] = {0, 1, 2, 4};
+    for (int p = 0; p < 4; ++p) {
+        float best = 0;
+        int bv = 0;
+        for (int v = 0; v < 2; ++v) {
+            *fv[p] = v;
+            const float t = ev.time(s_, reps, [&] { iteration_kernel(f_, a, s_, kinds[p]); });
+            ms[2 * p + v] = t;
+            if (v == 0 || t < best) {
+                best = t;
+                bv = v;
+            }
+        }
+        *fv[p] = bv;
+    }
+    reset_state<T>(st_, 0, 0.0, s_);
+    // certification passes: the whole FP64 check for each combination
+    CertArgs<T> c = cert_args(false);
+    c.have_df = false;
+    float cbest = 0;
+    int cb = 0;
+    const bool cert_defined = init_mode_ == MODE_ABS || init_mode_ == MODE_DIRECT;
+    for (int combo = 0; combo < (cert_defined ? 8 : 0); ++combo) {
+        f64_->zf_var = combo & 1;
+        f64_->y_var = (combo >> 1) & 1;
+        f64_->x_var = (combo >> 2) & 1;
+        const float t = ev.time(s_, reps, [&] { ffcz::certify<T>(*f64_, c, s_); });
+        if (combo == 0 || t < cbest) {
+            cbest = t;
+            cb = combo;
+        }
+    }
+    f64_->zf_var = cb & 1;
+    f64_->y_var = (cb >> 1) & 1;
+    f64_->x_var = (cb >> 2) & 1;
+    reset_edit_accumulators();
+    FFCZ_HIP(hipStreamSynchronize(s_));
+    solved_ = false;
+    extracted_ = false;
+    if (ms_out) {
+        for (int i = 0; i < 8; ++i) ms_out[i] = ms[i];
+        ms_out[8] = cbest;
+    }
+    const TileVars v = tile_vars();
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    bool found = false;
+    for (auto& kv : g_tune_cache)
+        if (kv.first == key) {
+            kv.second = v;
+            found = true;
+        }
+    if (!found) g_tune_cache.push_back({key, v});
+    return v;
 }
 
 template class Solver<float>;

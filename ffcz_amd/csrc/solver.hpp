@@ -12,6 +12,12 @@ namespace ffcz {
 
 enum SolveMode : int { MODE_DIRECT = 0, MODE_ABS = 1, MODE_PTW = 2 };
 
+// Tile variants (0/1) of the iteration's z forward, z inverse, y and x pass
+// kernels and of the FP64 certification's z, y and x passes.
+struct TileVars {
+    int zf, zi, y, x, czf, cy, cx;
+};
+
 struct CertResult {
     unsigned long long spatial_violations = 0;
     unsigned long long freq_violations = 0;
@@ -88,6 +94,15 @@ class Solver {
     // wait for them and read the result; several solvers can be in flight
     void certify_enqueue(bool quantized);
     CertResult certify_collect(bool quantized);
+    // Tile variant of each pass kernel (iteration and FP64 certification).
+    // autotune times every candidate on this field's buffers (after
+    // initialize*; it clobbers the iterate and resets the edit accumulators)
+    // and keeps the fastest; results are cached per shape and FFT variant.
+    // ms (optional, 9 floats): per-pass times of variants 0/1 for z forward,
+    // y, x, z inverse, then the chosen certification's time.
+    TileVars autotune(int reps, bool use_cache, float* ms = nullptr);
+    TileVars tile_vars() const;
+    void set_tile_vars(const TileVars& v);
     const uint8_t* d_freq_pack() const { return fpack_; }
     const uint8_t* d_spatial_pack() const { return spack_; }
     const T* d_freq_compact() const { return fcomp_; }
@@ -105,6 +120,7 @@ class Solver {
     QuantDesc spatial_quant() const;
 
    private:
+    CertArgs<T> cert_args(bool quantized) const;
     void release();
     void upload_x(const T* x);
     void ensure_extract_buffers();

@@ -248,6 +248,28 @@ class ProjectionSolver:
             variant = FFT_VARIANTS[variant]
         call("ffcz_set_fft_precision", self._h, int(variant))
 
+    TILE_VAR_NAMES = ("z_fwd", "z_inv", "y", "x", "cert_z", "cert_y", "cert_x")
+
+    def autotune(self, reps: int = 3, use_cache: bool = True, timings: bool = False) -> dict:
+        """Choose each pass kernel's tile variant by timing (after initialize*,
+        before solving).  Returns {pass: variant}, plus "ms" when ``timings``."""
+        v = (C.c_int * 7)()
+        ms = (C.c_float * 9)() if timings else None
+        call("ffcz_autotune", self._h, int(reps), int(bool(use_cache)), v, ms)
+        out = dict(zip(self.TILE_VAR_NAMES, list(v)))
+        if timings:
+            out["ms"] = [round(float(t), 5) for t in ms]
+        return out
+
+    def tile_vars(self) -> dict:
+        v = (C.c_int * 7)()
+        call("ffcz_get_tile_vars", self._h, v)
+        return dict(zip(self.TILE_VAR_NAMES, list(v)))
+
+    def set_tile_vars(self, vars_: dict):
+        v = (C.c_int * 7)(*[int(vars_.get(k, 0)) for k in self.TILE_VAR_NAMES])
+        call("ffcz_set_tile_vars", self._h, v)
+
     def fft_precision(self) -> int:
         v = C.c_int()
         call("ffcz_get_fft_precision", self._h, C.byref(v))

@@ -171,6 +171,21 @@ int ffcz_batch_time_iteration(ffcz_batch* b, int reps, float* ms);
 #define FFCZ_FFT_BF16X3 1
 #define FFCZ_FFT_FP16X3 2
 int ffcz_set_fft_precision(ffcz_solver* h, int variant);
+/* FFT plan autotuner (an addition; the reference takes cuFFT's plans,
+ * projection_algorithm_impl.cuh:109-131).  Every pass kernel of the
+ * iteration and of the FP64 certification has two tile variants (y/x:
+ * 128- or 64-byte column tiles; z: up to 256 or 128 threads per workgroup);
+ * ffcz_autotune times each on this field's buffers and keeps the fastest.
+ * Call it after initialize* and before solving: it overwrites the iterate
+ * and resets the edit accumulators.  use_cache=1 reuses an earlier choice for
+ * the same shape, type and FFT variant in this process.  vars[7] (optional)
+ * receives the choice: iteration z fwd, z inv, y, x; certification z, y, x.
+ * ms[9] (optional; forces a fresh tuning) receives the iteration passes'
+ * times for variants 0 and 1 (z fwd, y, x, z inv) and the chosen
+ * certification's time. */
+int ffcz_autotune(ffcz_solver* h, int reps, int use_cache, int* vars, float* ms);
+int ffcz_get_tile_vars(ffcz_solver* h, int* vars);
+int ffcz_set_tile_vars(ffcz_solver* h, const int* vars);
 int ffcz_get_fft_precision(ffcz_solver* h, int* variant);
 /* process-wide tuning: force_generic=1 runs the run-time-planned FFT kernels
  * instead of the length-specialised ones; slab_mb sets the slab size of the

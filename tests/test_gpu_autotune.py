@@ -1,0 +1,74 @@
+"""FFT plan autotuner: tile variants of every pass kernel.
+
+The variants change only how pencils are grouped into workgroups (64- or
+128-byte column tiles for y/x, 128 or 256 threads for z); each pencil's
+stages, twiddles and operation order are the same, so every variant gives
+bit-identical iterates, edits and certification counts.  That is the parity
+check here, over all 2^7 combinations on small fields and on the tuner's own
+choice."""
+import itertools
+
+import numpy as np
+import pytest
+
+from ffcz_amd import MODE_ABS, ProjectionSolver
+from ffcz_amd._lib import FFCzError
+from tests._cases import binding_case
+
+pytestmark = pytest.mark.gpu
+
+NAMES = ProjectionSolver.TILE_VAR_NAMES
+
+
+def _run(shape, dtype, case, vars_=None, tune=False):
+    x, e0, eps, delta = case
+    with ProjectionSolver(*shape, dtype=dtype) as s:
+        s.initialize_all_abs(x, eps, delta)
+        chosen = None
+        if tune:
+            chosen = s.autotune(reps=2, use_cache=False, timings=True)
+        if vars_ is not None:
+            s.set_tile_vars(vars_)
+            assert s.tile_vars() == vars_
+        s.set_margins(0.125, 0.125)
+        s.solve_loop(MODE_ABS, e0, 40, 1e-7)
+        c = s.certify(False)
+        return s.status(), s.get_results(), s.get_spatial_edits_full(), s.get_freq_edits_full(), c, chosen
+
+
+@pytest.mark.parametrize("shape,dtype", [((32, 64, 128), np.float32), ((30, 36, 50), np.float64),
+                                         ((16, 60, 120), np.float32)])
+def test_every_tile_variant_gives_identical_results(gpu, shape, dtype):
+    case = binding_case(shape, seed=3, dtype=dtype, spread=1.4, dfrac=0.9)
+    ref = _run(shape, dtype, case)
+    combos = list(itertools.product((0, 1), repeat=7))
+    # all combinations on one shape, a spread of them on the others
+    if shape != (32, 64, 128):
+        combos = combos[::9] + [(1,) * 7]
+    for combo in combos:
+        got = _run(shape, dtype, case, dict(zip(NAMES, combo)))
+        assert got[0] == ref[0], combo
+        for a, b in zip(got[1:4], ref[1:4]):
+            assert np.array_equal(a, b), combo
+        assert got[4] == ref[4], combo
+
+
+def test_autotune_choice_cache_and_errors(gpu):
+    shape, dtype = (32, 64, 128), np.float32
+    case = binding_case(shape, seed=4, dtype=dtype)
+    ref = _run(shape, dtype, case)
+    tuned = _run(shape, dtype, case, tune=True)
+    chosen = tuned[5]
+    assert set(chosen) == set(NAMES) | {"ms"} and len(chosen["ms"]) == 9
+    assert all(t > 0 for t in chosen["ms"])
+    for a, b in zip(tuned[1:4], ref[1:4]):
+        assert np.array_equal(a, b)
+    with ProjectionSolver(*shape, dtype=dtype) as s:
+        s.initialize_all_abs(case[0], case[2], case[3])
+        again = s.autotune(reps=1, use_cache=True)  # from the cache: no timing
+        assert again == {k: chosen[k] for k in NAMES}
+        with pytest.raises(FFCzError):
+            s.set_tile_vars({"z_fwd": 2})
+    with ProjectionSolver(*shape, dtype=dtype) as s:
+        with pytest.raises(FFCzError):
+            s.autotune()  # before initialize
