@@ -33,30 +33,8 @@ void Batch<T>::solve(int mode, const T* const* e0, int max_iter, T tol, bool use
     // the fields' own streams may still run their initialisation
     for (auto* f : f_) FFCZ_HIP(hipStreamSynchronize(f->stream()));
     for (size_t i = 0; i < f_.size(); ++i) f_[i]->prepare_solve(mode, e0[i], max_iter, tol, s_);
-    std::vector<const Fft3d<T>*> ffts;
-    std::vector<IterArgs<T>> args;
-    for (auto* f : f_) {
-        ffts.push_back(&f->fft());
-        args.push_back(f->iter_args());
-    }
-    release_graph();
-    batch_plan_build(plan_, ffts, args);
-    if (use_graph && max_iter > 0) {
-        hipGraph_t g = nullptr;
-        FFCZ_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
-        try {
-            batch_iteration(plan_, s_);
-        } catch (...) {
-            (void)hipStreamEndCapture(s_, &g);
-            if (g) (void)hipGraphDestroy(g);
-            throw;
-        }
-        FFCZ_HIP(hipStreamEndCapture(s_, &g));
-        const hipError_t e = hipGraphInstantiate(&gexec_, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        FFCZ_HIP(e);
-    }
-    int launched = 0, chunk = 1;
+    build(use_graph && max_iter > 0);
+    int launched = 0, chunk = 2;
     while (launched < max_iter) {
         const int k = std::min(chunk, max_iter - launched);
         for (int i = 0; i < k; ++i) {
@@ -72,13 +50,48 @@ void Batch<T>::solve(int mode, const T* const* e0, int max_iter, T tol, bool use
         chunk = std::min(chunk * 2, 16);
     }
     launched_ = launched;
+    graph_ = use_graph;
     for (auto* f : f_) f->finish_solve(s_);
+}
+
+template <typename T>
+void Batch<T>::build(bool use_graph) {
+    std::vector<const Fft3d<T>*> ffts;
+    std::vector<IterArgs<T>> args;
+    for (auto* f : f_) {
+        ffts.push_back(&f->fft());
+        args.push_back(f->iter_args());
+    }
+    release_graph();
+    batch_plan_build(plan_, ffts, args);
+    if (use_graph) {
+        hipGraph_t g = nullptr;
+        FFCZ_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
+        try {
+            batch_iteration(plan_, s_);
+        } catch (...) {
+            (void)hipStreamEndCapture(s_, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        FFCZ_HIP(hipStreamEndCapture(s_, &g));
+        const hipError_t e = hipGraphInstantiate(&gexec_, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        FFCZ_HIP(e);
+    }
 }
 
 template <typename T>
 float Batch<T>::time_iteration(int reps) {
     FFCZ_REQUIRE(plan_.nfields == f_.size(), ERR_STATE, "time_iteration requires a completed batch solve");
     FFCZ_REQUIRE(reps >= 1, ERR_INVALID, "reps must be >= 1");
+    // the solve's plan may point at the callers' initial errors: rebuild it
+    // over the fields' own buffers
+    for (auto* f : f_) {
+        f->accumulators_clobbered();
+        FFCZ_HIP(hipStreamSynchronize(f->stream()));
+    }
+    build(graph_);
     for (auto* f : f_) reset_state<T>(f->iter_args().st, 1 << 30, -1.0, s_);  // tol < 0: never converges
     auto one = [&] {
         if (gexec_) FFCZ_HIP(hipGraphLaunch(gexec_, s_));

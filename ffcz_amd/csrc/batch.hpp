@@ -42,12 +42,15 @@ class Batch {
 
    private:
     void release_graph();
+    // batched plan (and graph) over the fields' current iteration arguments
+    void build(bool use_graph);
     std::vector<Solver<T>*> f_;
     hipStream_t s_ = nullptr;
     BatchPlan<T> plan_;
     hipGraphExec_t gexec_ = nullptr;
     unsigned* hdone_ = nullptr;  // pinned host copy of the done flags
     int launched_ = 0;
+    bool graph_ = false;
 };
 
 }  // namespace ffcz

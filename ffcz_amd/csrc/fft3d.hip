@@ -125,8 +125,10 @@ __global__ __launch_bounds__(FFT_THREADS) void k_zfwd(ZArgs<R, T> a) {
     const size_t row0 = (size_t)blockIdx.x * a.P;
     const int P = a.P, sn = a.sn;
     const bool clamp = (MODE == Z_ITER) && a.st->iter > 0;
+    const bool fresh = clamp && a.ds_fresh && a.st->iter == 1;
     // the first iteration reads the initial error itself (e is written by the inverse pass)
     const R* esrc = (MODE == Z_ITER && !clamp && a.e_init) ? a.e_init : a.e;
+    R* ecopy = (MODE == Z_ITER && !clamp && a.e_init) ? a.e_copy : nullptr;
     unsigned long long nviol = 0;
     double vmax = 0.0;
     auto load = [&](size_t gi) -> R {
@@ -134,7 +136,9 @@ __global__ __launch_bounds__(FFT_THREADS) void k_zfwd(ZArgs<R, T> a) {
             return a.in[gi];
         } else if constexpr (MODE == Z_ITER) {
             R v = esrc[gi];
-            if (clamp) clamp_acc(v, a.eps, &a.ds[gi]);
+            if (ecopy) ecopy[gi] = v;
+            if (fresh) a.ds[gi] = clamp_edit(v, a.eps);
+            else if (clamp) clamp_acc(v, a.eps, &a.ds[gi]);
             return v;
         } else {
             double d = (double)a.e0[gi];
@@ -432,11 +436,16 @@ __global__ void k_reset_state(DevState* st, unsigned int max_iter, double tol) {
 }
 
 template <typename R>
-__global__ void k_epilogue_clamp(R* e, R* ds, R eps, size_t n, const DevState* st) {
+__global__ void k_epilogue_clamp(R* e, R* ds, R eps, size_t n, const DevState* st, int ds_fresh) {
     if (!(st->done && !st->converged && st->iter > 0)) return;
+    // first clamp since the accumulators were reset: store ds whole
+    const bool fresh = ds_fresh && st->iter == 1;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         R v = e[i];
-        if (v > eps || v < -eps) {
+        if (fresh) {
+            ds[i] = clamp_edit(v, eps);
+            e[i] = v;
+        } else if (v > eps || v < -eps) {
             clamp_acc(v, eps, &ds[i]);
             e[i] = v;
         }
@@ -657,6 +666,8 @@ void iteration(const Fft3d<R>& f, const IterArgs<R>& ia, hipStream_t s, EventLog
         const size_t r0 = x0 * f.ny;
         z.e = ia.e + r0 * f.nz;
         z.e_init = ia.e_init ? ia.e_init + r0 * f.nz : nullptr;
+        z.e_copy = ia.e_copy ? ia.e_copy + r0 * f.nz : nullptr;
+        z.ds_fresh = ia.ds_fresh;
         z.ds = ia.ds + r0 * f.nz;
         z.eps = ia.eps;
         z.st = ia.st;
@@ -710,6 +721,8 @@ void iteration_kernel(const Fft3d<R>& f, const IterArgs<R>& ia, hipStream_t s, i
     auto z = zargs<R, R>(f);
     z.e = ia.e;
     z.e_init = ia.e_init;
+    z.e_copy = ia.e_copy;
+    z.ds_fresh = ia.ds_fresh;
     z.ds = ia.ds;
     z.eps = ia.eps;
     z.st = ia.st;
@@ -744,7 +757,7 @@ void epilogue_clamp(const Fft3d<R>& f, const IterArgs<R>& a, hipStream_t s) {
     // a small grid: the kernel usually returns at once (converged loop)
     const unsigned g = (unsigned)std::min<size_t>(ceil_div(f.N, (size_t)256), 2048);
     hipLaunchKernelGGL((k_epilogue_clamp<R>), dim3(g), dim3(256), 0, s, a.e, a.ds, a.eps, f.N,
-                       (const DevState*)a.st);
+                       (const DevState*)a.st, a.ds_fresh);
     FFCZ_HIP(hipGetLastError());
 }
 
@@ -889,6 +902,8 @@ void batch_plan_build(BatchPlan<R>& bp, const std::vector<const Fft3d<R>*>& fiel
                 auto z = zargs<R, R>(f);
                 z.e = ia.e;
                 z.e_init = ia.e_init;
+                z.e_copy = ia.e_copy;
+                z.ds_fresh = ia.ds_fresh;
                 z.ds = ia.ds;
                 z.eps = ia.eps;
                 z.st = ia.st;

@@ -97,6 +97,10 @@ template <typename R>
 struct IterArgs {
     R* e;                 // spatial iterate [N]
     const R* e_init;      // initial error [N], read by the first iteration instead of e (optional)
+    R* e_copy;            // the first iteration stores a copy of e_init here (optional)
+    int ds_fresh;         // ds not yet written since the accumulators were reset: the first
+                          // spatial clamp (forward z pass of iteration 1, or the epilogue)
+                          // stores every element instead of adding to it
     R* ds;                // spatial edit accumulator [N]
     R* df;                // frequency edit accumulator [2F] split Re|Im, reference layout
     cplx<R>* spec;        // padded half spectrum work buffer

@@ -51,8 +51,10 @@ __device__ __forceinline__ void zfwd_body(const ZArgs<R, T>& a, uint32_t blk) {
     const size_t row = row0 + p;
     const bool rv = row < a.rows;
     const bool clamp = (MODE == Z_ITER) && a.st->iter > 0;
+    const bool fresh = clamp && a.ds_fresh && a.st->iter == 1;
     // the first iteration reads the initial error itself (e is written by the inverse pass)
     const R* esrc = (MODE == Z_ITER && !clamp && a.e_init) ? a.e_init : a.e;
+    R* ecopy = (MODE == Z_ITER && !clamp && a.e_init) ? a.e_copy : nullptr;
     unsigned long long nviol = 0;
     double vmax = 0.0;
     auto gld = [&](int n) -> cplx<R> {
@@ -61,7 +63,13 @@ __device__ __forceinline__ void zfwd_body(const ZArgs<R, T>& a, uint32_t blk) {
         if constexpr (MODE == Z_ITER) {
             // one 2-element vector load of the iterate (gi is even)
             cplx<R> v = *reinterpret_cast<const cplx<R>*>(esrc + gi);
-            if (clamp) {
+            if (ecopy) *reinterpret_cast<cplx<R>*>(ecopy + gi) = v;
+            if (fresh) {
+                cplx<R> d;
+                d.x = clamp_edit(v.x, a.eps);
+                d.y = clamp_edit(v.y, a.eps);
+                *reinterpret_cast<cplx<R>*>(a.ds + gi) = d;
+            } else if (clamp) {
                 clamp_acc(v.x, a.eps, &a.ds[gi]);
                 clamp_acc(v.y, a.eps, &a.ds[gi + 1]);
             }

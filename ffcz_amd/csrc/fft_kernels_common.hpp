@@ -20,6 +20,21 @@ __device__ __forceinline__ void clamp_acc(R& v, R d, R* acc) {
     }
 }
 
+// The same clamp when the accumulator is known to be zero: returns the edit
+// (0 where nothing is clamped) for a plain store, bit-identical to adding it to 0.
+template <typename R>
+__device__ __forceinline__ R clamp_edit(R& v, R d) {
+    R e = R(0);
+    if (v > d) {
+        e = d - v;
+        v = d;
+    } else if (v < -d) {
+        e = -d - v;
+        v = -d;
+    }
+    return e;
+}
+
 template <typename R, int NT = FFT_THREADS>
 __device__ __forceinline__ void block_max_atomic(R v, unsigned long long* target) {
     __shared__ R red[(NT + 63) / 64];
@@ -97,6 +112,8 @@ struct ZArgs {
     // Z_ITER
     R* e;
     const R* e_init;  // read instead of e by the first iteration (iter == 0), if set
+    R* e_copy;        // the first iteration also stores what it read from e_init here, if set
+    int ds_fresh;     // ds holds no edits yet: the first clamp (iter == 1) stores ds whole
     R* ds;
     R eps;
     DevState* st;

@@ -47,7 +47,10 @@ class Solver {
     // (batches): checks + initial copy + state reset, then the epilogue and
     // the read-back of the final state, both on stream s
     void prepare_solve(int mode, const T* e0, int max_iter, T tol, hipStream_t s);
-    void finish_solve(hipStream_t s);
+    // known: the final state when the caller has already read it (no extra wait)
+    void finish_solve(hipStream_t s, const DevState* known = nullptr);
+    // iterations run outside a solve (timing) leave the accumulators undefined
+    void accumulators_clobbered();
     // solve / solve_all_abs / solve_all_ptw: loop + extraction (impl:772-902).
     void solve(int mode, const T* e0, int max_iter, T tol) {
         solve_loop(mode, e0, max_iter, tol);
@@ -71,7 +74,7 @@ class Solver {
     void get_results(T* e) const;
     const T* d_e() const { return e_; }
     const T* d_df() const { return df_; }
-    const T* d_ds() const { return ds_; }
+    const T* d_ds() const { return materialize_ds(), ds_; }
     size_t N() const { return f_.N; }
     size_t F() const { return f_.F; }
     size_t nx() const { return f_.nx; }
@@ -170,7 +173,12 @@ class Solver {
     double viol_ = 0;
     bool converged_ = false;
     bool solved_ = false;
-    bool df_touched_ = false;  // a frequency clamp fired since the last accumulator reset
+    bool df_touched_ = true;   // df may be non-zero (a clamp fired, or contents unknown)
+    // ds has not been written since the accumulators were reset: zeroed
+    // lazily, or stored whole by the next solve's first spatial clamp
+    mutable bool ds_fresh_ = false;
+    const T* e_src_ = nullptr;  // device initial error read by the running solve's first iteration
+    void materialize_ds() const;
 };
 
 }  // namespace ffcz
