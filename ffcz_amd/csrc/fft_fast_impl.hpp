@@ -129,40 +129,56 @@ __device__ __forceinline__ void zfwd_body(const ZArgs<R, T>& a, uint32_t blk) {
 }
 
 // ---------------------------------------------------------------------------
-// inverse z (C2R, nz even): the first stage loads X[n] and X[M-n] straight
-// from HBM and forms Z[n] = (X[n] + conj X[M-n]) + i W^-n (X[n] - conj X[M-n])
-// (imaginary parts of the DC and Nyquist bins dropped); the last stage writes
-// the real row scaled by 1/N, and ITER tracks the spatial violation and
-// finalises the iteration in the last workgroup.
+// inverse z (C2R, nz even): the tile's rows X[0..M] are staged in LDS with
+// coalesced loads (each element read from HBM once), turned in place into
+// Z[n] = (X[n] + conj X[M-n]) + i W^-n (X[n] - conj X[M-n]) (imaginary parts
+// of the DC and Nyquist bins dropped), transformed, and the last stage writes
+// the real row scaled by 1/N; ITER keeps the workgroup's spatial violation.
 // ---------------------------------------------------------------------------
 template <typename R, typename T, int LZ, int MODE, int PREC, int V>
 __device__ __forceinline__ void zinv_body(const ZArgs<R, T>& a, uint32_t blk) {
     constexpr int TP = z_tp(LZ), P = z_rows(LZ, V), NT = TP * P;
-    constexpr int NZ = 2 * LZ, M = LZ, NH = M + 1, SN = P + 1;
+    constexpr int NZ = 2 * LZ, M = LZ, NH = M + 1, SN = P + 1, NPAIR = M / 2 + 1;
     __shared__ cplx<R> tw[LZ];
     __shared__ cplx<R> tw2[NH];
-    __shared__ cplx<R> buf[LZ * SN];
+    __shared__ cplx<R> buf[NH * SN];
     if (MODE == Z_ITER && a.st->done) return;
     const int tid = threadIdx.x;
     for (int i = tid; i < LZ; i += NT) tw[i] = a.tz[i];
     for (int i = tid; i < NH; i += NT) tw2[i] = a.tz2[i];
+    const size_t row0 = (size_t)blk * P;
+    // the tile's half-spectrum rows X[0..M] into LDS, each element loaded once
+    for (int q = tid; q < P * NH; q += NT) {
+        const int pp = q / NH;
+        const int k = q - pp * NH;
+        const size_t r2 = row0 + pp;
+        buf[k * SN + pp] = r2 < a.rows ? a.spec[r2 * a.Mp + k] : mk(R(0), R(0));
+    }
     __syncthreads();
-    const int t = tid % TP, p = tid / TP;
-    const size_t row = (size_t)blk * P + p;
-    const bool rv = row < a.rows;
-    const cplx<R>* srow = a.spec + row * a.Mp;
-    auto gld = [&](int n) -> cplx<R> {
-        if (!rv) return mk(R(0), R(0));
-        cplx<R> xk = srow[n];
-        cplx<R> xm = srow[M - n];
-        if (n == 0) {
-            xk.y = R(0);
-            xm.y = R(0);
-        }
+    // Z[n] = (X[n] + conj X[M-n]) + i W^-n (X[n] - conj X[M-n]) in place, by
+    // pairs (n, M-n); the imaginary parts of the DC and Nyquist bins are dropped
+    auto zpre = [&](cplx<R> xk, cplx<R> xm, int n) {
         const cplx<R> s = xk + conjg(xm);
         const cplx<R> d = cmul(xk - conjg(xm), conjg(tw2[n]));
         return mk(s.x - d.y, s.y + d.x);
     };
+    for (int q = tid; q < P * NPAIR; q += NT) {
+        const int pp = q / NPAIR;
+        const int n = q - pp * NPAIR;
+        const int m = M - n;
+        cplx<R> xn = buf[n * SN + pp];
+        cplx<R> xm = buf[m * SN + pp];
+        if (n == 0) {
+            xn.y = R(0);
+            xm.y = R(0);
+        }
+        buf[n * SN + pp] = zpre(xn, xm, n);
+        if (m != n && m < M) buf[m * SN + pp] = zpre(xm, xn, m);
+    }
+    __syncthreads();
+    const int t = tid % TP, p = tid / TP;
+    const size_t row = row0 + p;
+    const bool rv = row < a.rows;
     R vm = R(0);
     auto gst = [&](int n, cplx<R> z) {
         if (!rv) return;
@@ -180,10 +196,10 @@ __device__ __forceinline__ void zinv_body(const ZArgs<R, T>& a, uint32_t blk) {
     auto lld = [&](int n) { return buf[n * SN + p]; };
     auto lst = [&](int n, cplx<R> v) { buf[n * SN + p] = v; };
     if constexpr (PREC == PREC_VALU) {
-        pen::transform<R, LZ, TP, false, false>(gld, gst, lld, lst, tw, t, +1);
+        pen::transform<R, LZ, TP, true, false>(lld, gst, lld, lst, tw, t, +1);
     } else {
         const mx::Bmat<PREC> bm = mx::dft16_operand<PREC>(tid & 63);
-        mx::transform<PREC, LZ, TP, SN, P, NT, false, false>(gld, gst, lld, lst, buf, tw, bm, tid, t, +1);
+        mx::transform<PREC, LZ, TP, SN, P, NT, true, false>(lld, gst, lld, lst, buf, tw, bm, tid, t, +1);
     }
     if constexpr (MODE == Z_ITER) block_max_store<R, NT>(vm, &a.vpart[a.row_off / P + blk]);
 }

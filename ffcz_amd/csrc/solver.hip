@@ -613,31 +613,32 @@ TileVars Solver<T>::autotune(int reps, bool use_cache, float* ms_out) {
             }
     }
     ensure_cert_buffers();
-    // the tuner runs the passes on this solver's buffers: a zero initial error
-    // (no violations, as in a converged field), state that never stops them
+    // the tuner runs whole iterations on this solver's buffers (the passes
+    // share caches, so each is timed in its real sequence): a zero initial
+    // error (no violations, as in a converged field), zeroed accumulators and
+    // a state that never stops the kernels
     FFCZ_HIP(hipMemsetAsync(e0_, 0, f_.N * sizeof(T), s_));
     reset_edit_accumulators();
+    materialize_ds();
     reset_state<T>(st_, 1 << 30, -1.0, s_);
     const IterArgs<T> a = iter_args();
     EventPair ev;
-    float ms[7 * 2] = {};
+    float ms[9] = {};
+    f_.zf_var = f_.y_var = f_.x_var = f_.zi_var = 0;
+    auto t_iter = [&] { return ev.time(s_, reps, [&] { iteration(f_, a, s_); }); };
+    float best = t_iter();
+    ms[0] = best;
+    // coordinate descent over the four iteration pass kernels; an alternative
+    // must win clearly, so timing noise keeps the default
     int* fv[4] = {&f_.zf_var, &f_.y_var, &f_.x_var, &f_.zi_var};
-    const int kinds[4] = {0, 1, 2, 4};
     for (int p = 0; p < 4; ++p) {
-        float best = 0;
-        int bv = 0;
-        for (int v = 0; v < 2; ++v) {
-            *fv[p] = v;
-            const float t = ev.time(s_, reps, [&] { iteration_kernel(f_, a, s_, kinds[p]); });
-            ms[2 * p + v] = t;
-            // the alternative must win clearly: timing noise keeps the default
-            if (v == 0 || t < 0.97f * best) {
-                best = t;
-                bv = v;
-            }
-        }
-        *fv[p] = bv;
+        *fv[p] = 1;
+        const float t = t_iter();
+        ms[1 + p] = t;
+        if (t < 0.97f * best) best = t;
+        else *fv[p] = 0;
     }
+    ms[5] = best;
     reset_state<T>(st_, 0, 0.0, s_);
     // certification passes: the whole FP64 check for each combination
     CertArgs<T> c = cert_args(false);
@@ -650,6 +651,7 @@ TileVars Solver<T>::autotune(int reps, bool use_cache, float* ms_out) {
         f64_->y_var = (combo >> 1) & 1;
         f64_->x_var = (combo >> 2) & 1;
         const float t = ev.time(s_, reps, [&] { ffcz::certify<T>(*f64_, c, s_); });
+        if (combo == 0) ms[6] = t;
         if (combo == 0 || t < 0.97f * cbest) {
             cbest = t;
             cb = combo;
@@ -664,8 +666,8 @@ TileVars Solver<T>::autotune(int reps, bool use_cache, float* ms_out) {
     solved_ = false;
     extracted_ = false;
     if (ms_out) {
-        for (int i = 0; i < 8; ++i) ms_out[i] = ms[i];
-        ms_out[8] = cbest;
+        ms[7] = cbest;
+        for (int i = 0; i < 9; ++i) ms_out[i] = ms[i];
     }
     const TileVars v = tile_vars();
     std::lock_guard<std::mutex> lk(g_tune_mu);

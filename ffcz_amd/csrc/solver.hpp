@@ -101,8 +101,10 @@ class Solver {
     // autotune times every candidate on this field's buffers (after
     // initialize*; it clobbers the iterate and resets the edit accumulators)
     // and keeps the fastest; results are cached per shape and FFT variant.
-    // ms (optional, 9 floats): per-pass times of variants 0/1 for z forward,
-    // y, x, z inverse, then the chosen certification's time.
+    // ms (optional, 9 floats): whole-iteration times with the default
+    // variants, with the z forward, y, x and z inverse pass switched in turn
+    // (coordinate descent), the chosen iteration; then the certification with
+    // the default and the chosen variants; the last slot is 0.
     TileVars autotune(int reps, bool use_cache, float* ms = nullptr);
     TileVars tile_vars() const;
     void set_tile_vars(const TileVars& v);

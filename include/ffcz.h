@@ -180,9 +180,11 @@ int ffcz_set_fft_precision(ffcz_solver* h, int variant);
  * and resets the edit accumulators.  use_cache=1 reuses an earlier choice for
  * the same shape, type and FFT variant in this process.  vars[7] (optional)
  * receives the choice: iteration z fwd, z inv, y, x; certification z, y, x.
- * ms[9] (optional; forces a fresh tuning) receives the iteration passes'
- * times for variants 0 and 1 (z fwd, y, x, z inv) and the chosen
- * certification's time. */
+ * The iteration is tuned by coordinate descent over whole iterations (the
+ * passes share caches); the certification over its 8 combinations.  ms[9]
+ * (optional; forces a fresh tuning): iteration ms with the defaults, with z
+ * fwd, y, x, z inv switched in turn, with the choice; certification ms with
+ * the defaults and with the choice; 0. */
 int ffcz_autotune(ffcz_solver* h, int reps, int use_cache, int* vars, float* ms);
 int ffcz_get_tile_vars(ffcz_solver* h, int* vars);
 int ffcz_set_tile_vars(ffcz_solver* h, const int* vars);
