@@ -537,6 +537,18 @@ int ffcz_get_stream(ffcz_solver* h, void** stream) {
         with(h, [&](auto& s) { *stream = (void*)s.stream(); });
     });
 }
+int ffcz_set_pass_timing(ffcz_solver* h, int enable) {
+    return guard([&] { with(h, [&](auto& s) { s.set_pass_timing(enable != 0); }); });
+}
+int ffcz_get_pass_timing(ffcz_solver* h, float* ms6, long* iterations) {
+    return guard([&] {
+        FFCZ_REQUIRE(ms6 != nullptr, ERR_INVALID, "null output");
+        with(h, [&](auto& s) {
+            const long n = s.pass_timing(ms6);
+            if (iterations) *iterations = n;
+        });
+    });
+}
 int ffcz_time_iteration(ffcz_solver* h, int reps, float* ms5) {
     return guard([&] {
         FFCZ_REQUIRE(ms5 != nullptr, ERR_INVALID, "null output");

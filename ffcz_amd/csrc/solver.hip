@@ -279,9 +279,14 @@ void Solver<T>::solve_loop(int mode, const T* e0, int max_iter, T tol) {
     int launched = 0, chunk = 2;
     DevState h{};
     bool known = false;
+    if (pass_timing_) {
+        plog_->used = 0;
+        chunk_first_.clear();
+    }
     while (launched < max_iter) {
         const int k = std::min(chunk, max_iter - launched);
-        for (int i = 0; i < k; ++i) iteration(f_, a, s_);
+        if (pass_timing_) chunk_first_.push_back(launched + 1);
+        for (int i = 0; i < k; ++i) iteration(f_, a, s_, pass_timing_ ? plog_.get() : nullptr);
         launched += k;
         FFCZ_HIP(hipMemcpyAsync(&h, st_, sizeof(h), hipMemcpyDeviceToHost, s_));
         FFCZ_HIP(hipStreamSynchronize(s_));
@@ -289,7 +294,47 @@ void Solver<T>::solve_loop(int mode, const T* e0, int max_iter, T tol) {
         if (h.done) break;
         chunk = std::min(chunk * 2, 16);
     }
+    if (pass_timing_ && known) account_pass_times((int)h.iter);
     finish_solve(s_, known && h.done ? &h : nullptr);
+}
+
+template <typename T>
+void Solver<T>::set_pass_timing(bool enable) {
+    pass_timing_ = enable;
+    if (enable && !plog_) plog_.reset(new EventLog());
+    for (double& v : pass_ms_) v = 0.0;
+    for (long& c : pass_cnt_) c = 0;
+    pass_n_ = 0;
+}
+
+template <typename T>
+void Solver<T>::account_pass_times(int iterations) {
+    // the log holds, per launched iteration, a start mark (-1) and one mark
+    // after each launch; intervals belong to the launch that ends them
+    FFCZ_HIP(hipStreamSynchronize(s_));
+    int it = 0;
+    bool first_launch = false;
+    for (size_t i = 0; i < plog_->used; ++i) {
+        if (plog_->kinds[i] < 0) {
+            ++it;
+            first_launch = std::find(chunk_first_.begin(), chunk_first_.end(), it) != chunk_first_.end();
+            continue;
+        }
+        const bool skip = first_launch || it > iterations || i == 0;
+        first_launch = false;
+        if (skip) continue;
+        float t = 0;
+        FFCZ_HIP(hipEventElapsedTime(&t, plog_->pool[i - 1], plog_->pool[i]));
+        pass_ms_[plog_->kinds[i]] += t;
+        pass_cnt_[plog_->kinds[i]] += 1;
+    }
+    pass_n_ += iterations;
+}
+
+template <typename T>
+long Solver<T>::pass_timing(float* ms6) const {
+    for (int k = 0; k < 6; ++k) ms6[k] = pass_cnt_[k] ? (float)(pass_ms_[k] / (double)pass_cnt_[k]) : 0.0f;
+    return pass_n_;
 }
 
 template <typename T>

@@ -61,6 +61,15 @@ class Solver {
     CertResult certify(bool quantized);
     // mean ms per launch of the five iteration passes, then per whole iteration
     void time_iteration(int reps, float* ms6);
+    // Per-pass timing inside real solves: HIP events between the launches of
+    // every iteration the loop runs (fence-free), averaged over the iterations
+    // that did work.  enable resets the averages.
+    void set_pass_timing(bool enable);
+    // ms[6]: mean ms per launch of z fwd, y fwd, x fused, y inv, z inv,
+    // finalize; returns the number of iterations timed.  The first launch of
+    // each chunk is left out: the device waits for the host there, and that
+    // wait would count as kernel time.
+    long pass_timing(float* ms6) const;
 
     // margins: the loop iterates against eps*(1-ms) and delta*(1-mf); the
     // certification always checks the bounds given to initialize*.
@@ -126,6 +135,13 @@ class Solver {
 
    private:
     CertArgs<T> cert_args(bool quantized) const;
+    void account_pass_times(int iterations);
+    bool pass_timing_ = false;
+    std::unique_ptr<EventLog> plog_;
+    std::vector<int> chunk_first_;  // iterations (1-based) that start a chunk
+    double pass_ms_[6] = {};
+    long pass_cnt_[6] = {};
+    long pass_n_ = 0;
     void release();
     void upload_x(const T* x);
     void ensure_extract_buffers();

@@ -289,6 +289,19 @@ class ProjectionSolver:
         call("ffcz_get_stream", self._h, C.byref(s))
         return s.value or 0
 
+    def set_pass_timing(self, enable: bool = True):
+        """Record HIP events between the launches of every later solve's
+        iterations (resets the averages)."""
+        call("ffcz_set_pass_timing", self._h, int(bool(enable)))
+
+    def pass_timing(self):
+        """(mean ms of [z fwd, y fwd, x fused, y inv, z inv, finalize] over the
+        iterations that did work, their count)."""
+        ms = (C.c_float * 6)()
+        n = C.c_long()
+        call("ffcz_get_pass_timing", self._h, ms, C.byref(n))
+        return list(ms), n.value
+
     def time_iteration(self, reps: int = 10):
         ms = (C.c_float * 6)()
         call("ffcz_time_iteration", self._h, int(reps), ms)

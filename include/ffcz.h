@@ -202,6 +202,14 @@ int ffcz_get_stream(ffcz_solver* h, void** stream);
  * and z inverse passes (each launched `reps` times back to back), and mean ms
  * per whole iteration over `reps` back-to-back iterations */
 int ffcz_time_iteration(ffcz_solver* h, int reps, float* ms6);
+/* per-pass timing inside real solves: with enable=1 every later solve records
+ * fence-free HIP events between its launches; ms[6] = mean ms per launch of
+ * the z fwd, y fwd, x fused, y inv, z inv passes and the violation reduction
+ * over the iterations that did work (iterations receives their count).  The
+ * first launch after each host check is left out (its interval includes the
+ * device waiting for the host).  Enabling resets the averages. */
+int ffcz_set_pass_timing(ffcz_solver* h, int enable);
+int ffcz_get_pass_timing(ffcz_solver* h, float* ms6, long* iterations);
 
 /* ---- entropy coding: SimpleCudaHuffmanZstdCompressor<T>
  * compress    HuffmanZSTDCoder.cuh:248-376 (symbols: host or device memory)

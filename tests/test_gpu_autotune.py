@@ -72,3 +72,26 @@ def test_autotune_choice_cache_and_errors(gpu):
     with ProjectionSolver(*shape, dtype=dtype) as s:
         with pytest.raises(FFCzError):
             s.autotune()  # before initialize
+
+
+def test_pass_timing_inside_solves(gpu):
+    """Events between the launches of real solves: one sample per iteration
+    that did work, positive per-pass means, results unchanged."""
+    shape, dtype = (32, 64, 128), np.float32
+    case = binding_case(shape, seed=5, dtype=dtype)
+    ref = _run(shape, dtype, case)
+    x, e0, eps, delta = case
+    with ProjectionSolver(*shape, dtype=dtype) as s:
+        s.initialize_all_abs(x, eps, delta)
+        s.set_margins(0.125, 0.125)
+        s.set_pass_timing(True)
+        total = 0
+        for _ in range(2):
+            s.reset_edit_accumulators()
+            s.solve_loop(MODE_ABS, e0, 40, 1e-7)
+            total += s.status()[0]
+        ms, n = s.pass_timing()
+        assert n == total and all(t > 0 for t in ms)
+        assert np.array_equal(s.get_results(), ref[1])
+        s.set_pass_timing(False)
+        assert s.pass_timing()[1] == 0
