@@ -1,13 +1,14 @@
-// Edit extraction kernels.
+// Edit extraction kernels: one read of the edit array.
 //
-// Tiles of 8192 elements: 256 threads x 8 consecutive elements x 4 rounds.
-//  1. k_tile_stats   flag count, min and max per tile (vector loads);
-//  2. k_tile_scan    one workgroup scans the tile counts in coalesced chunks
-//                    and reduces min/max;
-//  3. k_tile_scatter forms the flags again, ranks them with a wave scan of
-//                    the per-thread counts, writes compact values and
-//                    quantised symbols in index order, and stores the flags
-//                    LSB-first (four threads' bytes combined into one word).
+// Tiles of 8192 elements: 256 threads x 8 consecutive elements x 4 rounds,
+// held in registers.  k_tile_compact forms the flags, stores them LSB-first
+// (four threads' bytes combined into one word), ranks them with wave scans,
+// finds the tile's output offset by decoupled look-back over the tiles before
+// it, and writes compact values (and step-quantised symbols) in index order.
+// k_extract_finalize reduces count, min and max; with the reference
+// quantiser, k_quant_compact then quantises the compact values (the global
+// min/max are known only after the pass).
+#include <algorithm>
 #include <cfloat>
 
 #include "extract.hpp"
@@ -21,8 +22,10 @@ constexpr int X_TILE = 256 * X_EPT * X_ROUNDS;   // 8192
 static_assert(X_TILE == EXTRACT_TILE, "tile size");
 
 size_t extract_scratch_bytes(size_t n) {
+    // [0, 64): results of the synchronous entry; [128]: tile ticket;
+    // [256, ...): tile status words, tile minima, tile maxima
     const size_t ntiles = ceil_div(n ? n : 1, (size_t)X_TILE);
-    return 256 + ntiles * (sizeof(unsigned int) + sizeof(unsigned long long) + 2 * sizeof(double));
+    return 256 + ntiles * (sizeof(unsigned long long) + 2 * sizeof(double));
 }
 
 template <typename T>
@@ -59,92 +62,159 @@ __device__ __forceinline__ void load8(const T* v, size_t i, size_t n, T (&x)[X_E
     }
 }
 
+// Tile status words of the decoupled look-back (Merrill & Garland, "Single-
+// pass Parallel Prefix Scan with Decoupled Look-back", 2016): the top two bits
+// say whether the low 62 hold the tile's own flag count (A) or the inclusive
+// prefix up to and including it (P); 0 = not yet published.
+constexpr unsigned long long ST_A = 1ull << 62;
+constexpr unsigned long long ST_P = 2ull << 62;
+constexpr unsigned long long ST_VAL = ST_A - 1;
+constexpr unsigned int LOOKBACK_SPIN_LIMIT = 1u << 22;
+
+__global__ void k_extract_init(unsigned long long* status, size_t ntiles, unsigned int* ticket, ExtractGlobals* g) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < ntiles; i += (size_t)gridDim.x * blockDim.x)
+        status[i] = 0ull;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *ticket = 0u;
+        *g = ExtractGlobals{};
+    }
+}
+
+// One pass over the edits: every workgroup takes the next tile in dispatch
+// order (a ticket, so every tile it waits for belongs to a workgroup that is
+// already running), forms its flags, packs them, counts them, publishes the
+// count, looks back over its predecessors' published counts/prefixes for its
+// output offset, then writes the compact values (and, for the step quantiser,
+// the zigzag symbols) in index order.  The look-back's wait is bounded: past
+// LOOKBACK_SPIN_LIMIT polls it records an error instead of waiting on.
 template <typename T>
-__global__ __launch_bounds__(256) void k_tile_stats(const T* v, size_t n, QuantDesc q, unsigned int* counts,
-                                                    double* pmin, double* pmax) {
-    const size_t base = (size_t)blockIdx.x * X_TILE;
-    unsigned int c = 0;
+__global__ __launch_bounds__(256) void k_tile_compact(const T* v, size_t n, QuantDesc q, unsigned long long* status,
+                                                      unsigned int* ticket, double* pmin, double* pmax,
+                                                      ExtractGlobals* g, unsigned int* pack, T* compact,
+                                                      uint16_t* quant) {
+    __shared__ unsigned int s_tile;
+    __shared__ unsigned long long s_pre;
+    __shared__ unsigned int wtot[X_ROUNDS][4];
+    __shared__ T smn[4], smx[4];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    if (tid == 0) s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const unsigned int tile = s_tile;
+    const size_t base = (size_t)tile * X_TILE;
+    T x[X_ROUNDS][X_EPT];
+    unsigned int bits[X_ROUNDS], excl[X_ROUNDS];
     T mn = (T)INFINITY, mx = (T)-INFINITY;
 #pragma unroll
     for (int r = 0; r < X_ROUNDS; ++r) {
-        const size_t i = base + ((size_t)r * 256 + threadIdx.x) * X_EPT;
+        const size_t i = base + ((size_t)r * 256 + tid) * X_EPT;
+        bits[r] = 0;
         if (i < n) {
-            T x[X_EPT];
-            load8(v, i, n, x);
+            load8(v, i, n, x[r]);
 #pragma unroll
             for (int k = 0; k < X_EPT; ++k) {
                 if (i + k < n) {
                     long long qq;
-                    c += edit_flag(x[k], q, qq) ? 1u : 0u;
-                    mn = x[k] < mn ? x[k] : mn;
-                    mx = x[k] > mx ? x[k] : mx;
+                    if (edit_flag(x[r][k], q, qq)) bits[r] |= 1u << k;
+                    mn = x[r][k] < mn ? x[r][k] : mn;
+                    mx = x[r][k] > mx ? x[r][k] : mx;
                 }
             }
         }
+        // four consecutive threads' flag bytes form one little-endian word
+        const unsigned int b1 = __shfl_down(bits[r], 1, 64), b2 = __shfl_down(bits[r], 2, 64),
+                           b3 = __shfl_down(bits[r], 3, 64);
+        if ((lane & 3) == 0 && i < n) pack[i >> 5] = bits[r] | (b1 << 8) | (b2 << 16) | (b3 << 24);
+        const unsigned int c = (unsigned int)__popc(bits[r]);
+        unsigned int xs = c;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned int y = __shfl_up(xs, o, 64);
+            if (lane >= o) xs += y;
+        }
+        excl[r] = xs - c;
+        if (lane == 63) wtot[r][w] = xs;
     }
-    __shared__ unsigned int sc[4];
-    __shared__ T smn[4], smx[4];
-    c = wave_sum(c);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         const T a = __shfl_xor(mn, o, 64), b = __shfl_xor(mx, o, 64);
         mn = a < mn ? a : mn;
         mx = b > mx ? b : mx;
     }
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        sc[w] = c;
+    if (lane == 0) {
         smn[w] = mn;
         smx[w] = mx;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned int tc = 0;
+    if (tid == 0) {
+        unsigned long long agg = 0;
         T a = smn[0], b = smx[0];
-        for (int i = 0; i < 4; ++i) {
-            tc += sc[i];
-            a = smn[i] < a ? smn[i] : a;
-            b = smx[i] > b ? smx[i] : b;
+        for (int r = 0; r < X_ROUNDS; ++r)
+            for (int k = 0; k < 4; ++k) agg += wtot[r][k];
+        for (int k = 1; k < 4; ++k) {
+            a = smn[k] < a ? smn[k] : a;
+            b = smx[k] > b ? smx[k] : b;
         }
-        counts[blockIdx.x] = tc;
-        pmin[blockIdx.x] = (double)a;
-        pmax[blockIdx.x] = (double)b;
+        pmin[tile] = (double)a;
+        pmax[tile] = (double)b;
+        unsigned long long pre = 0;
+        if (tile == 0) {
+            atomicExch(&status[0], ST_P | agg);
+        } else {
+            atomicExch(&status[tile], ST_A | agg);
+            long j = (long)tile - 1;
+            unsigned int spins = 0;
+            while (j >= 0) {
+                const unsigned long long sw = atomicAdd(&status[j], 0ull);
+                if ((sw & ~ST_VAL) == 0ull) {
+                    if (++spins > LOOKBACK_SPIN_LIMIT) {
+                        atomicOr(&g->overflow, 2u);
+                        break;
+                    }
+                    continue;
+                }
+                pre += sw & ST_VAL;
+                if ((sw & ST_P) != 0ull) break;
+                --j;
+            }
+            atomicExch(&status[tile], ST_P | (pre + agg));
+        }
+        s_pre = pre;
     }
+    __syncthreads();
+    unsigned long long run = s_pre;
+    bool ovf = false;
+#pragma unroll
+    for (int r = 0; r < X_ROUNDS; ++r) {
+        unsigned long long pos = run + excl[r];
+        for (int k = 0; k < w; ++k) pos += wtot[r][k];
+        if (bits[r]) {
+#pragma unroll
+            for (int k = 0; k < X_EPT; ++k) {
+                if (bits[r] & (1u << k)) {
+                    compact[pos] = x[r][k];
+                    if (q.kind == Q_STEP) {
+                        const unsigned long long z = zigzag64(quant_step((double)x[r][k], q.step));
+                        if (z > 65535ull) ovf = true;
+                        quant[pos] = (uint16_t)z;
+                    }
+                    ++pos;
+                }
+            }
+        }
+        run += wtot[r][0] + wtot[r][1] + wtot[r][2] + wtot[r][3];
+    }
+    if (ovf) atomicOr(&g->overflow, 1u);
 }
 
-// Exclusive scan of the tile counts in chunks of 1024 (one element per
-// thread per chunk, coalesced), plus the global min / max.
-__global__ __launch_bounds__(1024) void k_tile_scan(const unsigned int* counts, unsigned long long* offsets,
-                                                    const double* pmin, const double* pmax, size_t ntiles,
-                                                    ExtractGlobals* g) {
-    __shared__ unsigned long long wsum[16];
+// Count (the last tile's inclusive prefix), min and max over the tiles.
+__global__ __launch_bounds__(1024) void k_extract_finalize(const unsigned long long* status, const double* pmin,
+                                                           const double* pmax, size_t ntiles, ExtractGlobals* g) {
     __shared__ double wmn[16], wmx[16];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    unsigned long long run = 0;
     double mn = INFINITY, mx = -INFINITY;
-    for (size_t c0 = 0; c0 < ntiles; c0 += 1024) {
-        const size_t i = c0 + t;
-        const unsigned long long v = i < ntiles ? counts[i] : 0ull;
-        if (i < ntiles) {
-            mn = pmin[i] < mn ? pmin[i] : mn;
-            mx = pmax[i] > mx ? pmax[i] : mx;
-        }
-        unsigned long long x = v;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const unsigned long long y = __shfl_up(x, o, 64);
-            if (lane >= o) x += y;
-        }
-        if (lane == 63) wsum[w] = x;
-        __syncthreads();
-        unsigned long long woff = 0, tot = 0;
-        for (int k = 0; k < 16; ++k) {
-            woff += (k < w) ? wsum[k] : 0ull;
-            tot += wsum[k];
-        }
-        if (i < ntiles) offsets[i] = run + woff + x - v;
-        run += tot;
-        __syncthreads();
+    for (size_t i = t; i < ntiles; i += 1024) {
+        mn = pmin[i] < mn ? pmin[i] : mn;
+        mx = pmax[i] > mx ? pmax[i] : mx;
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -163,71 +233,22 @@ __global__ __launch_bounds__(1024) void k_tile_scan(const unsigned int* counts, 
             a = wmn[i] < a ? wmn[i] : a;
             b = wmx[i] > b ? wmx[i] : b;
         }
-        g->count = run;
-        g->vmin = ntiles ? a : 0.0;
-        g->vmax = ntiles ? b : 0.0;
-        g->overflow = 0;
+        const unsigned long long last = status[ntiles - 1];
+        if ((last & ST_P) == 0ull) g->overflow |= 2u;
+        g->count = last & ST_VAL;
+        g->vmin = a;
+        g->vmax = b;
     }
 }
 
+// The reference quantiser needs the global min/max: quantise the compact
+// values once the pass above has them (projection_algorithm.cu:447-455).
 template <typename T>
-__global__ __launch_bounds__(256) void k_tile_scatter(const T* v, size_t n, QuantDesc q,
-                                                      const unsigned long long* offsets, ExtractGlobals* g,
-                                                      unsigned int* pack, T* compact, uint16_t* quant) {
-    __shared__ unsigned int wtot[4];
-    const size_t base = (size_t)blockIdx.x * X_TILE;
-    unsigned long long run = offsets[blockIdx.x];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+__global__ __launch_bounds__(256) void k_quant_compact(const T* compact, uint16_t* quant, const ExtractGlobals* g) {
+    const size_t cnt = g->count;
     const T mn = (T)g->vmin, mx = (T)g->vmax;
-    bool ovf = false;
-#pragma unroll 1
-    for (int r = 0; r < X_ROUNDS; ++r) {
-        const size_t i = base + ((size_t)r * 256 + threadIdx.x) * X_EPT;
-        T x[X_EPT];
-        long long qq[X_EPT];
-        unsigned int bits = 0;
-        if (i < n) {
-            load8(v, i, n, x);
-#pragma unroll
-            for (int k = 0; k < X_EPT; ++k)
-                if (i + k < n && edit_flag(x[k], q, qq[k])) bits |= 1u << k;
-        }
-        // four consecutive threads' flag bytes form one little-endian word
-        const unsigned int b1 = __shfl_down(bits, 1, 64), b2 = __shfl_down(bits, 2, 64),
-                           b3 = __shfl_down(bits, 3, 64);
-        if ((lane & 3) == 0 && i < n) pack[i >> 5] = bits | (b1 << 8) | (b2 << 16) | (b3 << 24);
-        const unsigned int c = (unsigned int)__popc(bits);
-        unsigned int xs = c;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const unsigned int y = __shfl_up(xs, o, 64);
-            if (lane >= o) xs += y;
-        }
-        if (lane == 63) wtot[w] = xs;
-        __syncthreads();
-        unsigned long long pos = run + (xs - c);
-        for (int k = 0; k < w; ++k) pos += wtot[k];
-        const unsigned int rtot = wtot[0] + wtot[1] + wtot[2] + wtot[3];
-        if (bits) {
-#pragma unroll
-            for (int k = 0; k < X_EPT; ++k) {
-                if (bits & (1u << k)) {
-                    compact[pos] = x[k];
-                    if (q.kind == Q_STEP) {
-                        const unsigned long long z = zigzag64(qq[k]);
-                        if (z > 65535ull) ovf = true;
-                        quant[pos] = (uint16_t)z;
-                    } else {
-                        quant[pos] = (uint16_t)quant_ref16(x[k], mn, mx);
-                    }
-                    ++pos;
-                }
-            }
-        }
-        run += rtot;
-        __syncthreads();
-    }
-    if (ovf) atomicOr(&g->overflow, 1u);
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < cnt; i += (size_t)gridDim.x * blockDim.x)
+        quant[i] = (uint16_t)quant_ref16(compact[i], mn, mx);
 }
 
 template <typename T>
@@ -244,17 +265,25 @@ static void extract_launch(const T* v, size_t n, QuantDesc q, uint8_t* pack, T* 
     const size_t ntiles = ceil_div(n, (size_t)X_TILE);
     FFCZ_REQUIRE(ntiles < (1u << 31), ERR_INVALID, "array too large for extraction");
     unsigned char* p = (unsigned char*)scratch;
-    unsigned long long* offsets = (unsigned long long*)(p + 256);
-    double* pmin = (double*)(offsets + ntiles);
+    unsigned int* ticket = (unsigned int*)(p + 128);
+    unsigned long long* status = (unsigned long long*)(p + 256);
+    double* pmin = (double*)(status + ntiles);
     double* pmax = pmin + ntiles;
-    unsigned int* counts = (unsigned int*)(pmax + ntiles);
-    hipLaunchKernelGGL((k_tile_stats<T>), dim3((unsigned)ntiles), dim3(256), 0, s, v, n, q, counts, pmin, pmax);
+    const unsigned ig = (unsigned)std::min<size_t>(ceil_div(ntiles, (size_t)256), 1024);
+    hipLaunchKernelGGL(k_extract_init, dim3(ig), dim3(256), 0, s, status, ntiles, ticket, g);
     FFCZ_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, counts, offsets, pmin, pmax, ntiles, g);
+    hipLaunchKernelGGL((k_tile_compact<T>), dim3((unsigned)ntiles), dim3(256), 0, s, v, n, q, status, ticket, pmin,
+                       pmax, g, (unsigned int*)pack, compact, quant);
     FFCZ_HIP(hipGetLastError());
-    hipLaunchKernelGGL((k_tile_scatter<T>), dim3((unsigned)ntiles), dim3(256), 0, s, v, n, q, offsets, g,
-                       (unsigned int*)pack, compact, quant);
+    hipLaunchKernelGGL(k_extract_finalize, dim3(1), dim3(1024), 0, s, (const unsigned long long*)status,
+                       (const double*)pmin, (const double*)pmax, ntiles, g);
     FFCZ_HIP(hipGetLastError());
+    if (q.kind != Q_STEP) {
+        const unsigned qg = (unsigned)std::min<size_t>(ceil_div(n, (size_t)256), 4096);
+        hipLaunchKernelGGL((k_quant_compact<T>), dim3(qg), dim3(256), 0, s, (const T*)compact, quant,
+                           (const ExtractGlobals*)g);
+        FFCZ_HIP(hipGetLastError());
+    }
 }
 
 template <typename T>
@@ -267,6 +296,7 @@ ExtractResult extract_edits(const T* v, size_t n, QuantDesc q, uint8_t* pack, T*
     ExtractGlobals h{};
     FFCZ_HIP(hipMemcpyAsync(&h, g, sizeof(h), hipMemcpyDeviceToHost, s));
     FFCZ_HIP(hipStreamSynchronize(s));
+    FFCZ_REQUIRE((h.overflow & 2u) == 0, ERR_HIP, "extraction look-back did not complete");
     FFCZ_REQUIRE(h.overflow == 0, ERR_RANGE, "quantised edit does not fit a 16-bit symbol (raise the quantiser step)");
     res.count = (size_t)h.count;
     res.vmin = h.vmin;

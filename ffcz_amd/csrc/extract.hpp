@@ -20,7 +20,7 @@ struct ExtractResult {
 struct ExtractGlobals {
     unsigned long long count;
     double vmin, vmax;
-    unsigned int overflow;
+    unsigned int overflow;  // bit 0: a step-quantised symbol exceeds 16 bits; bit 1: look-back stalled
     unsigned int pad;
 };
 

@@ -439,6 +439,10 @@ void Solver<T>::sync_extract() const {
     const ExtractGlobals hf = xhost_[0], hs = xhost_[1];
     xf_ = ExtractResult{(size_t)hf.count, hf.vmin, hf.vmax};
     xs_ = ExtractResult{(size_t)hs.count, hs.vmin, hs.vmax};
+    if ((hf.overflow | hs.overflow) & 2u) {
+        const_cast<Solver<T>*>(this)->extracted_ = false;
+        throw Error(ERR_HIP, "extraction look-back did not complete");
+    }
     if (hf.overflow || hs.overflow) {
         const_cast<Solver<T>*>(this)->extracted_ = false;
         throw Error(ERR_RANGE, "quantised edit does not fit a 16-bit symbol (raise the quantiser step)");
