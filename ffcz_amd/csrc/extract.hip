@@ -1,7 +1,7 @@
 // Edit extraction kernels: one read of the edit array.
 //
-// Tiles of 8192 elements: 256 threads x 8 consecutive elements x 4 rounds,
-// held in registers.  k_tile_compact forms the flags, stores them LSB-first
+// Tiles of 256 threads x 8 consecutive elements x 4 rounds, held in
+// registers.  k_tile_compact forms the flags, stores them LSB-first
 // (four threads' bytes combined into one word), ranks them with wave scans,
 // finds the tile's output offset by decoupled look-back over the tiles before
 // it, and writes compact values (and step-quantised symbols) in index order.
@@ -17,9 +17,14 @@
 namespace ffcz {
 
 constexpr int X_EPT = 8;                          // elements per thread per round
-constexpr int X_ROUNDS = 4;
-constexpr int X_TILE = 256 * X_EPT * X_ROUNDS;   // 8192
+constexpr int X_ROUNDS = 4;                      // rounds of the smallest (FP64) tile
+constexpr int X_TILE = 256 * X_EPT * X_ROUNDS;   // 8192: scratch is sized for it
 static_assert(X_TILE == EXTRACT_TILE, "tile size");
+// rounds per tile (8192 elements; twice that for FP32 was measured slower)
+template <typename T>
+constexpr int x_rounds() { return X_ROUNDS; }
+template <typename T>
+constexpr int x_tile() { return 256 * X_EPT * x_rounds<T>(); }
 
 size_t extract_scratch_bytes(size_t n) {
     // [0, 64): results of the synchronous entry; [128]: tile ticket;
@@ -94,18 +99,23 @@ __global__ __launch_bounds__(256) void k_tile_compact(const T* v, size_t n, Quan
                                                       uint16_t* quant) {
     __shared__ unsigned int s_tile;
     __shared__ unsigned long long s_pre;
-    __shared__ unsigned int wtot[X_ROUNDS][4];
+    constexpr int RND = x_rounds<T>();
+    __shared__ unsigned int wtot[RND][4];
     __shared__ T smn[4], smx[4];
+    // the tile's active values in index order, when they fit half a tile
+    // (a full tile of values would cost occupancy)
+    constexpr unsigned int SVAL = X_TILE / 2;
+    __shared__ T sval[SVAL];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     if (tid == 0) s_tile = atomicAdd(ticket, 1u);
     __syncthreads();
     const unsigned int tile = s_tile;
-    const size_t base = (size_t)tile * X_TILE;
-    T x[X_ROUNDS][X_EPT];
-    unsigned int bits[X_ROUNDS], excl[X_ROUNDS];
+    const size_t base = (size_t)tile * x_tile<T>();
+    T x[RND][X_EPT];
+    unsigned int bits[RND], excl[RND];
     T mn = (T)INFINITY, mx = (T)-INFINITY;
 #pragma unroll
-    for (int r = 0; r < X_ROUNDS; ++r) {
+    for (int r = 0; r < RND; ++r) {
         const size_t i = base + ((size_t)r * 256 + tid) * X_EPT;
         bits[r] = 0;
         if (i < n) {
@@ -145,63 +155,91 @@ __global__ __launch_bounds__(256) void k_tile_compact(const T* v, size_t n, Quan
         smx[w] = mx;
     }
     __syncthreads();
-    if (tid == 0) {
+    if (w == 0) {
+        // tile aggregate, min and max
         unsigned long long agg = 0;
         T a = smn[0], b = smx[0];
-        for (int r = 0; r < X_ROUNDS; ++r)
+        for (int r = 0; r < RND; ++r)
             for (int k = 0; k < 4; ++k) agg += wtot[r][k];
         for (int k = 1; k < 4; ++k) {
             a = smn[k] < a ? smn[k] : a;
             b = smx[k] > b ? smx[k] : b;
         }
-        pmin[tile] = (double)a;
-        pmax[tile] = (double)b;
         unsigned long long pre = 0;
-        if (tile == 0) {
-            atomicExch(&status[0], ST_P | agg);
-        } else {
-            atomicExch(&status[tile], ST_A | agg);
-            long j = (long)tile - 1;
+        if (lane == 0) {
+            pmin[tile] = (double)a;
+            pmax[tile] = (double)b;
+            __hip_atomic_store(&status[tile], (tile == 0 ? ST_P : ST_A) | agg, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (tile > 0) {
+            // the wave reads 64 predecessors at once, nearest first; it sums
+            // them up to and including the nearest inclusive prefix (P), and
+            // polls again while any tile before that one is unpublished
+            long j = (long)tile - 1 - lane;
             unsigned int spins = 0;
-            while (j >= 0) {
-                const unsigned long long sw = atomicAdd(&status[j], 0ull);
-                if ((sw & ~ST_VAL) == 0ull) {
+            for (;;) {
+                const unsigned long long sw =
+                    j >= 0 ? __hip_atomic_load(&status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : ST_P;
+                const unsigned long long pm = __ballot((sw & ST_P) != 0ull);
+                const unsigned long long xm = __ballot((sw & ~ST_VAL) == 0ull);
+                const int fp = pm ? __builtin_ctzll(pm) : 64;
+                const unsigned long long upto = fp >= 63 ? ~0ull : ((1ull << (fp + 1)) - 1ull);
+                if (xm & upto) {
                     if (++spins > LOOKBACK_SPIN_LIMIT) {
-                        atomicOr(&g->overflow, 2u);
+                        if (lane == 0) atomicOr(&g->overflow, 2u);
                         break;
                     }
                     continue;
                 }
-                pre += sw & ST_VAL;
-                if ((sw & ST_P) != 0ull) break;
-                --j;
+                pre += wave_sum((lane <= fp && j >= 0) ? (sw & ST_VAL) : 0ull);
+                if (fp < 64) break;
+                j -= 64;
             }
-            atomicExch(&status[tile], ST_P | (pre + agg));
+            if (lane == 0)
+                __hip_atomic_store(&status[tile], ST_P | (pre + agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        s_pre = pre;
+        if (lane == 0) s_pre = pre;
     }
-    __syncthreads();
-    unsigned long long run = s_pre;
-    bool ovf = false;
+    __syncthreads();  // s_pre is read by every wave below
+    // active values to their tile-local rank; a tile whose values fit half a
+    // tile stages them in LDS and copies its contiguous output range
+    // [s_pre, s_pre + count) with coalesced stores, a denser one stores them
+    // from registers
+    unsigned int total = 0;
 #pragma unroll
-    for (int r = 0; r < X_ROUNDS; ++r) {
-        unsigned long long pos = run + excl[r];
+    for (int r = 0; r < RND; ++r) total += wtot[r][0] + wtot[r][1] + wtot[r][2] + wtot[r][3];
+    const bool staged = total <= SVAL;
+    const unsigned long long out0 = s_pre;
+    bool ovf = false;
+    auto emit = [&](unsigned long long at, T val) {
+        compact[at] = val;
+        if (q.kind == Q_STEP) {
+            const unsigned long long z = zigzag64(quant_step((double)val, q.step));
+            if (z > 65535ull) ovf = true;
+            quant[at] = (uint16_t)z;
+        }
+    };
+    unsigned int run = 0;
+#pragma unroll
+    for (int r = 0; r < RND; ++r) {
+        unsigned int pos = run + excl[r];
         for (int k = 0; k < w; ++k) pos += wtot[r][k];
         if (bits[r]) {
 #pragma unroll
             for (int k = 0; k < X_EPT; ++k) {
                 if (bits[r] & (1u << k)) {
-                    compact[pos] = x[r][k];
-                    if (q.kind == Q_STEP) {
-                        const unsigned long long z = zigzag64(quant_step((double)x[r][k], q.step));
-                        if (z > 65535ull) ovf = true;
-                        quant[pos] = (uint16_t)z;
-                    }
+                    if (staged) sval[pos] = x[r][k];
+                    else emit(out0 + pos, x[r][k]);
                     ++pos;
                 }
             }
         }
         run += wtot[r][0] + wtot[r][1] + wtot[r][2] + wtot[r][3];
+    }
+    if (staged) {
+        __syncthreads();
+        for (unsigned int i = tid; i < total; i += 256) emit(out0 + i, sval[i]);
     }
     if (ovf) atomicOr(&g->overflow, 1u);
 }
@@ -262,7 +300,7 @@ static void extract_launch(const T* v, size_t n, QuantDesc q, uint8_t* pack, T* 
                            void* scratch, ExtractGlobals* g, hipStream_t s) {
     FFCZ_REQUIRE(((uintptr_t)v % 16) == 0, ERR_INVALID, "edit array must be 16-byte aligned");
     FFCZ_REQUIRE(((uintptr_t)pack % 4) == 0, ERR_INVALID, "flag buffer must be 4-byte aligned");
-    const size_t ntiles = ceil_div(n, (size_t)X_TILE);
+    const size_t ntiles = ceil_div(n, (size_t)x_tile<T>());
     FFCZ_REQUIRE(ntiles < (1u << 31), ERR_INVALID, "array too large for extraction");
     unsigned char* p = (unsigned char*)scratch;
     unsigned int* ticket = (unsigned int*)(p + 128);
