@@ -105,8 +105,7 @@ __device__ __forceinline__ void zfwd_body(const ZArgs<R, T>& a, uint32_t blk) {
     if constexpr (PREC == PREC_VALU) {
         pen::transform<R, LZ, TP, false, true>(gld, lst, lld, lst, tw, t, -1);
     } else {
-        const mx::Bmat<PREC> bm = mx::dft16_operand<PREC>(tid & 63);
-        mx::transform<PREC, LZ, TP, SN, P, NT, false, true>(gld, lst, lld, lst, buf, tw, bm, tid, t, -1);
+        mx::transform<PREC, LZ, TP, SN, P, NT, false, true>(gld, lst, lld, lst, buf, tw, tid, t, -1);
     }
     if constexpr (MODE == Z_CERT) {
         block_count_atomic(nviol, &a.stats->spatial_viol);
@@ -198,8 +197,7 @@ __device__ __forceinline__ void zinv_body(const ZArgs<R, T>& a, uint32_t blk) {
     if constexpr (PREC == PREC_VALU) {
         pen::transform<R, LZ, TP, true, false>(lld, gst, lld, lst, tw, t, +1);
     } else {
-        const mx::Bmat<PREC> bm = mx::dft16_operand<PREC>(tid & 63);
-        mx::transform<PREC, LZ, TP, SN, P, NT, true, false>(lld, gst, lld, lst, buf, tw, bm, tid, t, +1);
+        mx::transform<PREC, LZ, TP, SN, P, NT, true, false>(lld, gst, lld, lst, buf, tw, tid, t, +1);
     }
     if constexpr (MODE == Z_ITER) block_max_store<R, NT>(vm, &a.vpart[a.row_off / P + blk]);
 }
@@ -237,12 +235,10 @@ __device__ __forceinline__ void strided_body(const SArgs<R, T>& a, uint32_t blk)
     };
     auto lld = [&](int n) { return buf[n * W + p]; };
     auto lst = [&](int n, cplx<R> v) { buf[n * W + p] = v; };
-    [[maybe_unused]] mx::Bmat<PREC> bm;
-    if constexpr (PREC != PREC_VALU) bm = mx::dft16_operand<PREC>(tid & 63);
     if constexpr (MODE == S_PLAIN || MODE == S_LOAD_DF) {
         const int sg = MODE == S_LOAD_DF ? +1 : a.sign;
         if constexpr (PREC == PREC_VALU) pen::transform<R, L, TP, false, false>(gld, gst, lld, lst, tw, t, sg);
-        else mx::transform<PREC, L, TP, W, W, NT, false, false>(gld, gst, lld, lst, buf, tw, bm, tid, t, sg);
+        else mx::transform<PREC, L, TP, W, W, NT, false, false>(gld, gst, lld, lst, buf, tw, tid, t, sg);
     } else if constexpr (MODE == S_CLAMP) {
         bool fired = false;
         auto cst = [&](int n, cplx<R> v) {
@@ -250,11 +246,11 @@ __device__ __forceinline__ void strided_body(const SArgs<R, T>& a, uint32_t blk)
             buf[n * W + p] = v;
         };
         if constexpr (PREC == PREC_VALU) pen::transform<R, L, TP, false, true>(gld, cst, lld, lst, tw, t, -1);
-        else mx::transform<PREC, L, TP, W, W, NT, false, false>(gld, cst, lld, lst, buf, tw, bm, tid, t, -1);
+        else mx::transform<PREC, L, TP, W, W, NT, false, false>(gld, cst, lld, lst, buf, tw, tid, t, -1);
         if (fired) a.st->freq_clamped = 1u;
         __syncthreads();
         if constexpr (PREC == PREC_VALU) pen::transform<R, L, TP, true, false>(lld, gst, lld, lst, tw, t, +1);
-        else mx::transform<PREC, L, TP, W, W, NT, true, false>(lld, gst, lld, lst, buf, tw, bm, tid, t, +1);
+        else mx::transform<PREC, L, TP, W, W, NT, true, false>(lld, gst, lld, lst, buf, tw, tid, t, +1);
     } else {
         static_assert(PREC == PREC_VALU, "certification runs on the vector ALU");  // S_CERT_CHECK
         unsigned long long nv = 0;
@@ -299,8 +295,12 @@ template <typename R, typename T, int LZ, int MODE, int PREC = PREC_VALU, int V 
 __global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ, V)) void k2_zinv(ZArgs<R, T> a) {
     zinv_body<R, T, LZ, MODE, PREC, V>(a, blockIdx.x);
 }
+// The plain MFMA variants ask for 8 waves per SIMD (at most 64 VGPRs): two
+// 1024-thread tiles per CU, like the vector-ALU kernels.  (The fused clamp
+// kernel would spill under that bound.)
 template <typename R, typename T, int L, int MODE, int PREC = PREC_VALU, int V = 0>
-__global__ __launch_bounds__(pen::threads_per_pencil(L) * s_width<R>(V)) void k2_strided(SArgs<R, T> a) {
+__global__ __launch_bounds__(pen::threads_per_pencil(L) * s_width<R>(V),
+                             (PREC != PREC_VALU && MODE == S_PLAIN) ? 8 : 1) void k2_strided(SArgs<R, T> a) {
     strided_body<R, T, L, MODE, PREC, V>(a, blockIdx.x);
 }
 template <typename R, int LZ, int MODE>

@@ -102,6 +102,16 @@ __device__ __forceinline__ void split(float x, typename Op<PREC>::E& h, typename
     l = (E)(x - (float)h);
 }
 
+// The workgroup's copy of every lane's DFT-matrix slice (hi/lo for both output
+// halves, 4 KB): written once per transform by the first wave, read by each
+// stage just before its MFMAs.  Keeping it in LDS rather than registers holds
+// the kernels to two 1024-thread workgroups per CU.
+template <int PREC>
+__device__ __forceinline__ typename Op<PREC>::V* b_table() {
+    __shared__ typename Op<PREC>::V tab[4][64];
+    return &tab[0][0];
+}
+
 // This lane's slice of the forward 16-point DFT matrix (both output halves).
 template <int PREC>
 __device__ __forceinline__ Bmat<PREC> dft16_operand(int lane) {
@@ -132,8 +142,7 @@ __device__ __forceinline__ Bmat<PREC> dft16_operand(int lane) {
 // stage) over the tile buf[n*SW + p], p < NPEN, n < L, run by NT threads.
 // The caller barriers before (inputs visible) and after (outputs visible).
 template <int PREC, int L, int NS, int SW, int NPEN, int NT>
-__device__ __forceinline__ void r16_stage(cplx<float>* buf, const cplx<float>* __restrict__ tw, const Bmat<PREC>& bm,
-                                          int tid, bool inv) {
+__device__ __forceinline__ void r16_stage(cplx<float>* buf, const cplx<float>* __restrict__ tw, int tid, bool inv) {
     using O = Op<PREC>;
     using V = typename O::V;
     constexpr int LR = L / 16;
@@ -192,6 +201,12 @@ __device__ __forceinline__ void r16_stage(cplx<float>* buf, const cplx<float>* _
         }
     }
     __syncthreads();
+    const V* bt = b_table<PREC>();
+    Bmat<PREC> bm;
+    bm.hi[0] = bt[0 * 64 + lane];
+    bm.hi[1] = bt[1 * 64 + lane];
+    bm.lo[0] = bt[2 * 64 + lane];
+    bm.lo[1] = bt[3 * 64 + lane];
     float* fb = reinterpret_cast<float*>(buf);
     const int col = lane & 15, cp = col & 1, m0 = col >> 1;
 #pragma unroll
@@ -240,12 +255,11 @@ __device__ __forceinline__ void valu_stages(const GLD& gld, const LLD& lld, cons
 }
 
 template <int PREC, int L, int REST, int S, int SW, int NPEN, int NT>
-__device__ __forceinline__ void mfma_stages(cplx<float>* buf, const cplx<float>* __restrict__ tw, const Bmat<PREC>& bm,
-                                            int tid, bool inv) {
+__device__ __forceinline__ void mfma_stages(cplx<float>* buf, const cplx<float>* __restrict__ tw, int tid, bool inv) {
     if constexpr (S < n16(L)) {
-        r16_stage<PREC, L, REST * pow16(S), SW, NPEN, NT>(buf, tw, bm, tid, inv);
+        r16_stage<PREC, L, REST * pow16(S), SW, NPEN, NT>(buf, tw, tid, inv);
         __syncthreads();
-        mfma_stages<PREC, L, REST, S + 1, SW, NPEN, NT>(buf, tw, bm, tid, inv);
+        mfma_stages<PREC, L, REST, S + 1, SW, NPEN, NT>(buf, tw, tid, inv);
     }
 }
 
@@ -258,17 +272,26 @@ __device__ __forceinline__ void mfma_stages(cplx<float>* buf, const cplx<float>*
 template <int PREC, int L, int TP, int SW, int NPEN, int NT, bool IN_BUF, bool OUT_BUF, class GLD, class GST,
           class LLD, class LST>
 __device__ __forceinline__ void transform(const GLD& gld, const GST& gst, const LLD& lld, const LST& lst,
-                                          cplx<float>* buf, const cplx<float>* __restrict__ tw, const Bmat<PREC>& bm,
-                                          int tid, int t, int sign) {
+                                          cplx<float>* buf, const cplx<float>* __restrict__ tw, int tid, int t,
+                                          int sign) {
     static_assert(PREC != PREC_VALU && n16(L) > 0, "MFMA transform needs a radix-16 factor");
     constexpr int REST = rest_of(L);
+    if (tid < 64) {
+        // read only after the barrier that ends the first MFMA stage's loads
+        const Bmat<PREC> bm = dft16_operand<PREC>(tid);
+        typename Op<PREC>::V* bt = b_table<PREC>();
+        bt[0 * 64 + tid] = bm.hi[0];
+        bt[1 * 64 + tid] = bm.hi[1];
+        bt[2 * 64 + tid] = bm.lo[0];
+        bt[3 * 64 + tid] = bm.lo[1];
+    }
     if constexpr (REST > 1) {
         valu_stages<float, L, TP, REST, 0, IN_BUF>(gld, lld, lst, tw, t, sign);
     } else if constexpr (!IN_BUF) {
         for (int n = t; n < L; n += TP) lst(n, gld(n));
         __syncthreads();
     }
-    mfma_stages<PREC, L, REST, 0, SW, NPEN, NT>(buf, tw, bm, tid, sign > 0);
+    mfma_stages<PREC, L, REST, 0, SW, NPEN, NT>(buf, tw, tid, sign > 0);
     if constexpr (!OUT_BUF) {
         for (int n = t; n < L; n += TP) gst(n, lld(n));
     }
