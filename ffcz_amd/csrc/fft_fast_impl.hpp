@@ -22,6 +22,10 @@ namespace ffcz {
 template <typename R>
 constexpr int s_width(int V = 0) { return (sizeof(R) == 4 ? 16 : 8) >> V; }
 constexpr int z_tp(int LZ) { return pen::threads_per_pencil(LZ); }
+// threads per pencil of the y/x kernels: FP64 pencils keep ~8 elements per
+// thread at every length (more threads in flight for the long FP64 axes)
+template <typename R>
+constexpr int s_tp(int L) { return sizeof(R) == 8 ? (L + 7) / 8 : pen::threads_per_pencil(L); }
 constexpr int z_rows(int LZ, int V = 0) {
     int p = 1;
     while (p * 2 * z_tp(LZ) <= (V ? 256 : 512)) p *= 2;
@@ -207,7 +211,7 @@ __device__ __forceinline__ void zinv_body(const ZArgs<R, T>& a, uint32_t blk) {
 // ---------------------------------------------------------------------------
 template <typename R, typename T, int L, int MODE, int PREC, int V>
 __device__ __forceinline__ void strided_body(const SArgs<R, T>& a, uint32_t blk) {
-    constexpr int TP = pen::threads_per_pencil(L), W = s_width<R>(V), NT = TP * W;
+    constexpr int TP = s_tp<R>(L), W = s_width<R>(V), NT = TP * W;
     __shared__ cplx<R> tw[L];
     __shared__ cplx<R> buf[L * W];
     if (a.st && a.st->done) return;
@@ -299,7 +303,7 @@ __global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ, V)) void k2_zinv(ZArgs<R, T> 
 // 1024-thread tiles per CU, like the vector-ALU kernels.  (The fused clamp
 // kernel would spill under that bound.)
 template <typename R, typename T, int L, int MODE, int PREC = PREC_VALU, int V = 0>
-__global__ __launch_bounds__(pen::threads_per_pencil(L) * s_width<R>(V),
+__global__ __launch_bounds__(s_tp<R>(L) * s_width<R>(V),
                              (PREC != PREC_VALU && MODE == S_PLAIN) ? 8 : 1) void k2_strided(SArgs<R, T> a) {
     strided_body<R, T, L, MODE, PREC, V>(a, blockIdx.x);
 }
@@ -316,7 +320,7 @@ __global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ)) void k2_zinv_b(BatchTab<ZArg
     zinv_body<R, R, LZ, MODE, PREC_VALU, 0>(a, blockIdx.x - t.start[f]);
 }
 template <typename R, int L, int MODE>
-__global__ __launch_bounds__(pen::threads_per_pencil(L) * s_width<R>()) void k2_strided_b(BatchTab<SArgs<R, R>> t) {
+__global__ __launch_bounds__(s_tp<R>(L) * s_width<R>()) void k2_strided_b(BatchTab<SArgs<R, R>> t) {
     const uint32_t f = batch_field(t, blockIdx.x);
     const SArgs<R, R> a = t.args[f];
     strided_body<R, R, L, MODE, PREC_VALU, 0>(a, blockIdx.x - t.start[f]);
@@ -342,7 +346,7 @@ inline void launch_strided_len(SArgs<R, T> a, size_t outer, hipStream_t s) {
     a.nct = (int)ceil_div((size_t)a.ncols, (size_t)W);
     a.dNct = FastDiv((uint32_t)a.nct);
     const unsigned grid = (unsigned)(outer * (size_t)a.nct);
-    hipLaunchKernelGGL((k2_strided<R, T, L, MODE, PREC, V>), dim3(grid), dim3(pen::threads_per_pencil(L) * W), 0, s, a);
+    hipLaunchKernelGGL((k2_strided<R, T, L, MODE, PREC, V>), dim3(grid), dim3(s_tp<R>(L) * W), 0, s, a);
 }
 
 template <typename R, typename T, int MODE>
@@ -519,7 +523,7 @@ bool strided_batch_dispatch(int L, const BatchTab<SArgs<R, R>>& t, unsigned grid
 #define FFCZ_BSL(Lv)                                                                                      \
     case Lv:                                                                                              \
         hipLaunchKernelGGL((k2_strided_b<R, Lv, MODE>), dim3(grid),                                      \
-                           dim3(pen::threads_per_pencil(Lv) * s_width<R>()), 0, s, t);                    \
+                           dim3(s_tp<R>(Lv) * s_width<R>()), 0, s, t);                                    \
         return true;
         FFCZ_FAST_S_LENGTHS(FFCZ_BSL)
 #undef FFCZ_BSL
