@@ -17,6 +17,7 @@ template <typename T>
 Batch<T>::~Batch() {
     release_graph();
     batch_plan_free(plan_);
+    cert_tables_.release();
     if (hdone_) (void)hipHostFree(hdone_);
     if (s_) (void)hipStreamDestroy(s_);
 }
@@ -129,8 +130,19 @@ void Batch<T>::extract_all() {
 template <typename T>
 void Batch<T>::certify_all(bool quantized, CertResult* out) {
     FFCZ_REQUIRE(out != nullptr, ERR_INVALID, "null result array");
-    for (auto* f : f_) f->certify_enqueue(quantized);
-    for (size_t i = 0; i < f_.size(); ++i) out[i] = f_[i]->certify_collect(quantized);
+    // every field's FP64 passes as one launch per pass and axis length on the
+    // batch stream, after the fields' own streams (extraction) have drained
+    std::vector<const Fft3d<double>*> ffts;
+    std::vector<CertArgs<T>> args;
+    for (auto* f : f_) args.push_back(f->certify_prepare(quantized));
+    for (auto* f : f_) {
+        FFCZ_HIP(hipStreamSynchronize(f->stream()));
+        ffts.push_back(&f->cert_fft());
+    }
+    certify_batch<T>(ffts, args, cert_tables_, s_);
+    for (auto* f : f_) f->certify_copy_stats(quantized, s_);
+    FFCZ_HIP(hipStreamSynchronize(s_));
+    for (size_t i = 0; i < f_.size(); ++i) out[i] = f_[i]->certify_read(quantized);
 }
 
 template class Batch<float>;

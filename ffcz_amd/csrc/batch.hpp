@@ -51,6 +51,7 @@ class Batch {
     unsigned* hdone_ = nullptr;  // pinned host copy of the done flags
     int launched_ = 0;
     bool graph_ = false;
+    DevBuffer cert_tables_;
 };
 
 }  // namespace ffcz

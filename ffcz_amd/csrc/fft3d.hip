@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "fft3d.hpp"
@@ -1028,6 +1029,180 @@ void batch_gather_done(const BatchPlan<R>& bp, unsigned* out, hipStream_t s) {
     FFCZ_HIP(hipMemcpyAsync(out, bp.done, bp.nfields * sizeof(unsigned), hipMemcpyDeviceToHost, s));
 }
 
+// ===========================================================================
+// batched certification
+// ===========================================================================
+void DevBuffer::ensure(size_t n) {
+    if (n <= cap) return;
+    release();
+    FFCZ_HIP(hipMalloc(&p, n));
+    cap = n;
+}
+void DevBuffer::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
+__global__ void k_reset_stats_many(CertStats* const* st, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) *st[i] = CertStats{};
+}
+
+template <typename T>
+void certify_batch(const std::vector<const Fft3d<double>*>& fs, const std::vector<CertArgs<T>>& cs,
+                   DevBuffer& tables, hipStream_t s) {
+    FFCZ_REQUIRE(fs.size() == cs.size(), ERR_INVALID, "certification batch mismatch");
+    std::vector<size_t> bat;
+    std::vector<size_t> rest;
+    for (size_t i = 0; i < fs.size(); ++i) {
+        const Fft3d<double>& f = *fs[i];
+        const bool ok = std::is_same<T, float>::value && fast_fft_enabled() && f.half &&
+                        fast_z_rows(f.Lz, f.half, 0) > 0 && fast_s_supported((int)f.ny) &&
+                        fast_s_supported((int)f.nx);
+        (ok ? bat : rest).push_back(i);
+    }
+    if constexpr (std::is_same<T, float>::value) {
+        if (!bat.empty()) {
+            std::vector<unsigned char> host;
+            auto put = [&](const void* p, size_t bytes) {
+                const size_t off = (host.size() + 255) & ~(size_t)255;
+                host.resize(off + bytes);
+                std::memcpy(host.data() + off, p, bytes);
+                return off;
+            };
+            struct G {
+                int kind;  // 0 z fwd cert, 1 z inv plain, 2 strided
+                int smode, L;
+                unsigned grid;
+                size_t args_off, start_off;
+                int n;
+            };
+            std::vector<G> gs;
+            const int W = fast_s_width(sizeof(double), 0);
+            // one launch per axis length over the fields in `which`; make(i) builds field i's arguments
+            auto add_z = [&](int kind, const std::vector<size_t>& which, auto make) {
+                std::vector<std::pair<int, std::vector<size_t>>> byL;
+                for (size_t i : which) {
+                    const int L = fs[i]->Lz;
+                    auto it = std::find_if(byL.begin(), byL.end(), [&](auto& g) { return g.first == L; });
+                    if (it == byL.end()) byL.push_back({L, {i}});
+                    else it->second.push_back(i);
+                }
+                for (auto& g : byL) {
+                    std::vector<ZArgs<double, T>> tab;
+                    std::vector<uint32_t> start;
+                    uint64_t tiles = 0;
+                    for (size_t i : g.second) {
+                        tab.push_back(make(i));
+                        start.push_back((uint32_t)tiles);
+                        tiles += ceil_div(fs[i]->rows, (size_t)fast_z_rows(fs[i]->Lz, true, 0));
+                    }
+                    FFCZ_REQUIRE(tiles < (1ull << 31), ERR_INVALID, "batch too large for one launch");
+                    gs.push_back({kind, 0, g.first, (unsigned)tiles, put(tab.data(), tab.size() * sizeof(tab[0])),
+                                  put(start.data(), start.size() * sizeof(uint32_t)), (int)tab.size()});
+                }
+            };
+            auto add_s = [&](int smode, bool xaxis, const std::vector<size_t>& which, auto make) {
+                std::vector<std::pair<int, std::vector<size_t>>> byL;
+                for (size_t i : which) {
+                    const int L = (int)(xaxis ? fs[i]->nx : fs[i]->ny);
+                    auto it = std::find_if(byL.begin(), byL.end(), [&](auto& g) { return g.first == L; });
+                    if (it == byL.end()) byL.push_back({L, {i}});
+                    else it->second.push_back(i);
+                }
+                for (auto& g : byL) {
+                    std::vector<SArgs<double, T>> tab;
+                    std::vector<uint32_t> start;
+                    uint64_t tiles = 0;
+                    for (size_t i : g.second) {
+                        SArgs<double, T> a = make(i);
+                        a.W = W;
+                        a.nct = (int)ceil_div((size_t)a.ncols, (size_t)W);
+                        a.dNct = FastDiv((uint32_t)a.nct);
+                        tab.push_back(a);
+                        start.push_back((uint32_t)tiles);
+                        tiles += (uint64_t)(xaxis ? fs[i]->ny : fs[i]->nx) * (uint64_t)a.nct;
+                    }
+                    FFCZ_REQUIRE(tiles < (1ull << 31), ERR_INVALID, "batch too large for one launch");
+                    gs.push_back({2, smode, g.first, (unsigned)tiles, put(tab.data(), tab.size() * sizeof(tab[0])),
+                                  put(start.data(), start.size() * sizeof(uint32_t)), (int)tab.size()});
+                }
+            };
+            std::vector<size_t> withdf;
+            for (size_t i : bat)
+                if (cs[i].have_df) withdf.push_back(i);
+            // C2R(deq df)/N of the fields with frequency edits: x (loads df), y, z inverse
+            add_s(S_LOAD_DF, true, withdf, [&](size_t i) {
+                auto x = sargs<double, T>(*fs[i], cs[i].work_spec, true);
+                x.sign = +1;
+                x.dfsrc = cs[i].df;
+                x.qf = cs[i].qf;
+                return x;
+            });
+            add_s(S_PLAIN, false, withdf, [&](size_t i) {
+                auto y = sargs<double, T>(*fs[i], cs[i].work_spec, false);
+                y.sign = +1;
+                return y;
+            });
+            add_z(1, withdf, [&](size_t i) {
+                auto z = zargs<double, T>(*fs[i]);
+                z.spec = cs[i].work_spec;
+                z.out = cs[i].work_real;
+                z.scale = 1.0 / (double)fs[i]->N;
+                return z;
+            });
+            // forward FP64 transform of x_hat - x with the spatial check, then the frequency check
+            add_z(0, bat, [&](size_t i) {
+                auto z = zargs<double, T>(*fs[i]);
+                z.spec = cs[i].work_spec;
+                z.e0 = cs[i].e0;
+                z.cds = cs[i].ds;
+                z.dfi = cs[i].have_df ? cs[i].work_real : nullptr;
+                z.qs = cs[i].qs;
+                z.ceps = cs[i].eps;
+                z.stats = cs[i].stats;
+                return z;
+            });
+            add_s(S_PLAIN, false, bat, [&](size_t i) {
+                auto y = sargs<double, T>(*fs[i], cs[i].work_spec, false);
+                y.sign = -1;
+                return y;
+            });
+            add_s(S_CERT_CHECK, true, bat, [&](size_t i) {
+                auto x = sargs<double, T>(*fs[i], cs[i].work_spec, true);
+                x.cdelta = cs[i].delta;
+                x.cdelta_dense = cs[i].delta_dense;
+                x.stats = cs[i].stats;
+                return x;
+            });
+            std::vector<CertStats*> sts;
+            for (size_t i : bat) sts.push_back(cs[i].stats);
+            const size_t sts_off = put(sts.data(), sts.size() * sizeof(CertStats*));
+            tables.ensure(host.size());
+            FFCZ_HIP(hipMemcpyAsync(tables.p, host.data(), host.size(), hipMemcpyHostToDevice, s));
+            FFCZ_HIP(hipStreamSynchronize(s));  // the host staging goes out of scope
+            hipLaunchKernelGGL(k_reset_stats_many, dim3(1), dim3(256), 0, s,
+                               (CertStats* const*)(tables.p + sts_off), (int)sts.size());
+            FFCZ_HIP(hipGetLastError());
+            for (const G& g : gs) {
+                bool ok;
+                if (g.kind == 2) {
+                    BatchTab<SArgs<double, T>> t{(const SArgs<double, T>*)(tables.p + g.args_off),
+                                                 (const uint32_t*)(tables.p + g.start_off), g.n};
+                    ok = fast_cert_strided_batch(g.smode, g.L, t, g.grid, s);
+                } else {
+                    BatchTab<ZArgs<double, T>> t{(const ZArgs<double, T>*)(tables.p + g.args_off),
+                                                 (const uint32_t*)(tables.p + g.start_off), g.n};
+                    ok = g.kind == 0 ? fast_cert_zfwd_batch(g.L, t, g.grid, s) : fast_cert_zinv_batch(g.L, t, g.grid, s);
+                }
+                FFCZ_REQUIRE(ok, ERR_STATE, "batched certification pass without a kernel for its length");
+                FFCZ_HIP(hipGetLastError());
+            }
+        }
+    }
+    for (size_t i : rest) certify<T>(*fs[i], cs[i], s);
+}
+
 template struct Fft3d<float>;
 template struct Fft3d<double>;
 template void fft_r2c<float>(const Fft3d<float>&, const float*, cplx<float>*, hipStream_t);
@@ -1048,6 +1223,10 @@ template void epilogue_clamp<float>(const Fft3d<float>&, const IterArgs<float>&,
 template void epilogue_clamp<double>(const Fft3d<double>&, const IterArgs<double>&, hipStream_t);
 template void certify<float>(const Fft3d<double>&, const CertArgs<float>&, hipStream_t);
 template void certify<double>(const Fft3d<double>&, const CertArgs<double>&, hipStream_t);
+template void certify_batch<float>(const std::vector<const Fft3d<double>*>&, const std::vector<CertArgs<float>>&,
+                                   DevBuffer&, hipStream_t);
+template void certify_batch<double>(const std::vector<const Fft3d<double>*>&, const std::vector<CertArgs<double>>&,
+                                    DevBuffer&, hipStream_t);
 template void batch_plan_build<float>(BatchPlan<float>&, const std::vector<const Fft3d<float>*>&,
                                       const std::vector<IterArgs<float>>&);
 template void batch_plan_build<double>(BatchPlan<double>&, const std::vector<const Fft3d<double>*>&,

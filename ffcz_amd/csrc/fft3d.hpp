@@ -211,4 +211,18 @@ struct CertArgs {
 template <typename T>
 void certify(const Fft3d<double>& f64, const CertArgs<T>& a, hipStream_t s);
 
+// A device buffer that only grows (tables of batched launches).
+struct DevBuffer {
+    unsigned char* p = nullptr;
+    size_t cap = 0;
+    void ensure(size_t n);
+    void release();
+};
+// The certification of many fields with one launch per pass and axis length
+// (float fields with length-specialised kernels; the others run certify()
+// one by one on the same stream).  Results land in each field's stats.
+template <typename T>
+void certify_batch(const std::vector<const Fft3d<double>*>& f64s, const std::vector<CertArgs<T>>& args,
+                   DevBuffer& tables, hipStream_t s);
+
 }  // namespace ffcz

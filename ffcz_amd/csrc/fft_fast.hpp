@@ -41,6 +41,13 @@ bool mfma_strided(int mode, const SArgs<float, float>& a, size_t outer, hipStrea
 bool fast_zfwd_batch(int Lz, const BatchTab<ZArgs<float, float>>& t, unsigned grid, hipStream_t s);
 bool fast_zinv_batch(int Lz, const BatchTab<ZArgs<float, float>>& t, unsigned grid, hipStream_t s);
 bool fast_strided_batch(int mode, int L, const BatchTab<SArgs<float, float>>& t, unsigned grid, hipStream_t s);
+// Batched FP64 certification passes over float fields (fft_fast_c32.hip):
+// z forward in Z_CERT mode, z inverse in Z_PLAIN mode, y/x in S_PLAIN,
+// S_CERT_CHECK or S_LOAD_DF mode.
+bool fast_cert_zfwd_batch(int Lz, const BatchTab<ZArgs<double, float>>& t, unsigned grid, hipStream_t s);
+bool fast_cert_zinv_batch(int Lz, const BatchTab<ZArgs<double, float>>& t, unsigned grid, hipStream_t s);
+bool fast_cert_strided_batch(int mode, int L, const BatchTab<SArgs<double, float>>& t, unsigned grid,
+                             hipStream_t s);
 // whether a y/x axis length has a specialised kernel, and the columns per
 // tile of those kernels for a real type of `bytes` bytes
 bool fast_s_supported(int L);

@@ -18,4 +18,19 @@ bool fast_strided<double, float>(int mode, const SArgs<double, float>& a, size_t
     return strided_dispatch<double, float, (1u << S_PLAIN) | (1u << S_CERT_CHECK) | (1u << S_LOAD_DF)>(mode, a, outer, s);
 }
 
+// batched certification passes (FP64 transforms of float edits)
+bool fast_cert_zfwd_batch(int Lz, const BatchTab<ZArgs<double, float>>& t, unsigned grid, hipStream_t s) {
+    return zfwd_batch_dispatch<double, float, Z_CERT>(Lz, t, grid, s);
+}
+bool fast_cert_zinv_batch(int Lz, const BatchTab<ZArgs<double, float>>& t, unsigned grid, hipStream_t s) {
+    return zinv_batch_dispatch<double, float, Z_PLAIN>(Lz, t, grid, s);
+}
+bool fast_cert_strided_batch(int mode, int L, const BatchTab<SArgs<double, float>>& t, unsigned grid,
+                             hipStream_t s) {
+    if (mode == S_PLAIN) return strided_batch_dispatch<double, float, S_PLAIN>(L, t, grid, s);
+    if (mode == S_CERT_CHECK) return strided_batch_dispatch<double, float, S_CERT_CHECK>(L, t, grid, s);
+    if (mode == S_LOAD_DF) return strided_batch_dispatch<double, float, S_LOAD_DF>(L, t, grid, s);
+    return false;
+}
+
 }  // namespace ffcz

@@ -22,14 +22,14 @@ bool fast_strided<float, float>(int mode, const SArgs<float, float>& a, size_t o
 }
 
 bool fast_zfwd_batch(int Lz, const BatchTab<ZArgs<float, float>>& t, unsigned grid, hipStream_t s) {
-    return zfwd_batch_dispatch<float, Z_ITER>(Lz, t, grid, s);
+    return zfwd_batch_dispatch<float, float, Z_ITER>(Lz, t, grid, s);
 }
 bool fast_zinv_batch(int Lz, const BatchTab<ZArgs<float, float>>& t, unsigned grid, hipStream_t s) {
-    return zinv_batch_dispatch<float, Z_ITER>(Lz, t, grid, s);
+    return zinv_batch_dispatch<float, float, Z_ITER>(Lz, t, grid, s);
 }
 bool fast_strided_batch(int mode, int L, const BatchTab<SArgs<float, float>>& t, unsigned grid, hipStream_t s) {
-    if (mode == S_CLAMP) return strided_batch_dispatch<float, S_CLAMP>(L, t, grid, s);
-    if (mode == S_PLAIN) return strided_batch_dispatch<float, S_PLAIN>(L, t, grid, s);
+    if (mode == S_CLAMP) return strided_batch_dispatch<float, float, S_CLAMP>(L, t, grid, s);
+    if (mode == S_PLAIN) return strided_batch_dispatch<float, float, S_PLAIN>(L, t, grid, s);
     return false;
 }
 bool fast_s_supported(int L) {

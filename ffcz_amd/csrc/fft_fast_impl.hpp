@@ -307,23 +307,23 @@ __global__ __launch_bounds__(s_tp<R>(L) * s_width<R>(V),
                              (PREC != PREC_VALU && MODE == S_PLAIN) ? 8 : 1) void k2_strided(SArgs<R, T> a) {
     strided_body<R, T, L, MODE, PREC, V>(a, blockIdx.x);
 }
-template <typename R, int LZ, int MODE>
-__global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ)) void k2_zfwd_b(BatchTab<ZArgs<R, R>> t) {
+template <typename R, typename T, int LZ, int MODE>
+__global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ)) void k2_zfwd_b(BatchTab<ZArgs<R, T>> t) {
     const uint32_t f = batch_field(t, blockIdx.x);
-    const ZArgs<R, R> a = t.args[f];
-    zfwd_body<R, R, LZ, MODE, PREC_VALU, 0>(a, blockIdx.x - t.start[f]);
+    const ZArgs<R, T> a = t.args[f];
+    zfwd_body<R, T, LZ, MODE, PREC_VALU, 0>(a, blockIdx.x - t.start[f]);
 }
-template <typename R, int LZ, int MODE>
-__global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ)) void k2_zinv_b(BatchTab<ZArgs<R, R>> t) {
+template <typename R, typename T, int LZ, int MODE>
+__global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ)) void k2_zinv_b(BatchTab<ZArgs<R, T>> t) {
     const uint32_t f = batch_field(t, blockIdx.x);
-    const ZArgs<R, R> a = t.args[f];
-    zinv_body<R, R, LZ, MODE, PREC_VALU, 0>(a, blockIdx.x - t.start[f]);
+    const ZArgs<R, T> a = t.args[f];
+    zinv_body<R, T, LZ, MODE, PREC_VALU, 0>(a, blockIdx.x - t.start[f]);
 }
-template <typename R, int L, int MODE>
-__global__ __launch_bounds__(s_tp<R>(L) * s_width<R>()) void k2_strided_b(BatchTab<SArgs<R, R>> t) {
+template <typename R, typename T, int L, int MODE>
+__global__ __launch_bounds__(s_tp<R>(L) * s_width<R>()) void k2_strided_b(BatchTab<SArgs<R, T>> t) {
     const uint32_t f = batch_field(t, blockIdx.x);
-    const SArgs<R, R> a = t.args[f];
-    strided_body<R, R, L, MODE, PREC_VALU, 0>(a, blockIdx.x - t.start[f]);
+    const SArgs<R, T> a = t.args[f];
+    strided_body<R, T, L, MODE, PREC_VALU, 0>(a, blockIdx.x - t.start[f]);
 }
 
 // ---------------------------------------------------------------------------
@@ -491,12 +491,12 @@ bool mfma_strided_dispatch(int mode, const SArgs<float, float>& a, size_t outer,
 // ---------------------------------------------------------------------------
 // batched launches (vector-ALU kernels): tiles of every field in the table
 // ---------------------------------------------------------------------------
-template <typename R, int MODE>
-bool zfwd_batch_dispatch(int Lz, const BatchTab<ZArgs<R, R>>& t, unsigned grid, hipStream_t s) {
+template <typename R, typename T, int MODE>
+bool zfwd_batch_dispatch(int Lz, const BatchTab<ZArgs<R, T>>& t, unsigned grid, hipStream_t s) {
     switch (Lz) {
-#define FFCZ_BZF(Lv)                                                                                     \
-    case Lv:                                                                                             \
-        hipLaunchKernelGGL((k2_zfwd_b<R, Lv, MODE>), dim3(grid), dim3(z_tp(Lv) * z_rows(Lv)), 0, s, t);  \
+#define FFCZ_BZF(Lv)                                                                                        \
+    case Lv:                                                                                                \
+        hipLaunchKernelGGL((k2_zfwd_b<R, T, Lv, MODE>), dim3(grid), dim3(z_tp(Lv) * z_rows(Lv)), 0, s, t);  \
         return true;
         FFCZ_FAST_Z_LENGTHS(FFCZ_BZF)
 #undef FFCZ_BZF
@@ -504,12 +504,12 @@ bool zfwd_batch_dispatch(int Lz, const BatchTab<ZArgs<R, R>>& t, unsigned grid, 
             return false;
     }
 }
-template <typename R, int MODE>
-bool zinv_batch_dispatch(int Lz, const BatchTab<ZArgs<R, R>>& t, unsigned grid, hipStream_t s) {
+template <typename R, typename T, int MODE>
+bool zinv_batch_dispatch(int Lz, const BatchTab<ZArgs<R, T>>& t, unsigned grid, hipStream_t s) {
     switch (Lz) {
-#define FFCZ_BZI(Lv)                                                                                     \
-    case Lv:                                                                                             \
-        hipLaunchKernelGGL((k2_zinv_b<R, Lv, MODE>), dim3(grid), dim3(z_tp(Lv) * z_rows(Lv)), 0, s, t);  \
+#define FFCZ_BZI(Lv)                                                                                        \
+    case Lv:                                                                                                \
+        hipLaunchKernelGGL((k2_zinv_b<R, T, Lv, MODE>), dim3(grid), dim3(z_tp(Lv) * z_rows(Lv)), 0, s, t);  \
         return true;
         FFCZ_FAST_Z_LENGTHS(FFCZ_BZI)
 #undef FFCZ_BZI
@@ -517,12 +517,12 @@ bool zinv_batch_dispatch(int Lz, const BatchTab<ZArgs<R, R>>& t, unsigned grid, 
             return false;
     }
 }
-template <typename R, int MODE>
-bool strided_batch_dispatch(int L, const BatchTab<SArgs<R, R>>& t, unsigned grid, hipStream_t s) {
+template <typename R, typename T, int MODE>
+bool strided_batch_dispatch(int L, const BatchTab<SArgs<R, T>>& t, unsigned grid, hipStream_t s) {
     switch (L) {
 #define FFCZ_BSL(Lv)                                                                                      \
     case Lv:                                                                                              \
-        hipLaunchKernelGGL((k2_strided_b<R, Lv, MODE>), dim3(grid),                                      \
+        hipLaunchKernelGGL((k2_strided_b<R, T, Lv, MODE>), dim3(grid),                                   \
                            dim3(s_tp<R>(Lv) * s_width<R>()), 0, s, t);                                    \
         return true;
         FFCZ_FAST_S_LENGTHS(FFCZ_BSL)

@@ -467,15 +467,26 @@ CertResult Solver<T>::certify(bool quantized) {
 }
 
 template <typename T>
-void Solver<T>::certify_enqueue(bool quantized) {
+CertArgs<T> Solver<T>::certify_prepare(bool quantized) {
     FFCZ_REQUIRE(solved_, ERR_STATE, "certify requires a completed solve");
     FFCZ_REQUIRE(init_mode_ == MODE_ABS || init_mode_ == MODE_DIRECT, ERR_INVALID,
                  "certification is defined for the absolute and per-bin bound modes");
     if (quantized) FFCZ_REQUIRE(extracted_, ERR_STATE, "certify(quantized) requires extraction first");
     ensure_cert_buffers();
-    const CertArgs<T> c = cert_args(quantized);
+    return cert_args(quantized);
+}
+
+template <typename T>
+void Solver<T>::certify_enqueue(bool quantized) {
+    const CertArgs<T> c = certify_prepare(quantized);
     ffcz::certify<T>(*f64_, c, s_);
-    FFCZ_HIP(hipMemcpyAsync(chost_ + (quantized ? 1 : 0), c.stats, sizeof(CertStats), hipMemcpyDeviceToHost, s_));
+    certify_copy_stats(quantized, s_);
+}
+
+template <typename T>
+void Solver<T>::certify_copy_stats(bool quantized, hipStream_t s) {
+    FFCZ_HIP(hipMemcpyAsync(chost_ + (quantized ? 1 : 0), cstats_ + (quantized ? 1 : 0), sizeof(CertStats),
+                            hipMemcpyDeviceToHost, s));
 }
 
 template <typename T>
@@ -511,6 +522,12 @@ template <typename T>
 CertResult Solver<T>::certify_collect(bool quantized) {
     FFCZ_REQUIRE(chost_ != nullptr, ERR_STATE, "no certification enqueued");
     FFCZ_HIP(hipStreamSynchronize(s_));
+    return certify_read(quantized);
+}
+
+template <typename T>
+CertResult Solver<T>::certify_read(bool quantized) {
+    FFCZ_REQUIRE(chost_ != nullptr, ERR_STATE, "no certification enqueued");
     if (quantized) sync_extract();  // a quantiser overflow voids the quantised check
     const CertStats hs = chost_[quantized ? 1 : 0];
     CertResult r;

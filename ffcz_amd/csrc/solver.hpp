@@ -106,6 +106,13 @@ class Solver {
     // wait for them and read the result; several solvers can be in flight
     void certify_enqueue(bool quantized);
     CertResult certify_collect(bool quantized);
+    // pieces for a caller that queues the FP64 passes of many solvers itself
+    // (batches): checks and arguments; the FP64 plan; the copy of the
+    // counters to the host on stream s; reading them once s has finished
+    CertArgs<T> certify_prepare(bool quantized);
+    const Fft3d<double>& cert_fft() const { return *f64_; }
+    void certify_copy_stats(bool quantized, hipStream_t s);
+    CertResult certify_read(bool quantized);
     // Tile variant of each pass kernel (iteration and FP64 certification).
     // autotune times every candidate on this field's buffers (after
     // initialize*; it clobbers the iterate and resets the edit accumulators)
