@@ -10,7 +10,7 @@ Batch<T>::Batch(std::vector<Solver<T>*> fields) : f_(std::move(fields)) {
     FFCZ_REQUIRE(!f_.empty(), ERR_INVALID, "a batch needs at least one field");
     for (auto* p : f_) FFCZ_REQUIRE(p != nullptr, ERR_INVALID, "null field");
     FFCZ_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
-    FFCZ_HIP(hipHostMalloc((void**)&hdone_, f_.size() * sizeof(unsigned), hipHostMallocDefault));
+    FFCZ_HIP(hipHostMalloc((void**)&hstate_, f_.size() * sizeof(DevState), hipHostMallocDefault));
 }
 
 template <typename T>
@@ -18,7 +18,7 @@ Batch<T>::~Batch() {
     release_graph();
     batch_plan_free(plan_);
     cert_tables_.release();
-    if (hdone_) (void)hipHostFree(hdone_);
+    if (hstate_) (void)hipHostFree(hstate_);
     if (s_) (void)hipStreamDestroy(s_);
 }
 
@@ -43,16 +43,17 @@ void Batch<T>::solve(int mode, const T* const* e0, int max_iter, T tol, bool use
             else batch_iteration(plan_, s_);
         }
         launched += k;
-        batch_gather_done(plan_, hdone_, s_);
+        batch_gather_states(plan_, hstate_, s_);
         FFCZ_HIP(hipStreamSynchronize(s_));
         bool all = true;
-        for (size_t i = 0; i < f_.size(); ++i) all = all && hdone_[i] != 0;
+        for (size_t i = 0; i < f_.size(); ++i) all = all && hstate_[i].done != 0;
         if (all) break;
         chunk = std::min(chunk * 2, 16);
     }
     launched_ = launched;
     graph_ = use_graph;
-    for (auto* f : f_) f->finish_solve(s_);
+    // every state was read at the last check (the epilogue leaves them as they are)
+    for (size_t i = 0; i < f_.size(); ++i) f_[i]->finish_solve(s_, max_iter > 0 ? &hstate_[i] : nullptr);
 }
 
 template <typename T>

@@ -48,7 +48,7 @@ class Batch {
     hipStream_t s_ = nullptr;
     BatchPlan<T> plan_;
     hipGraphExec_t gexec_ = nullptr;
-    unsigned* hdone_ = nullptr;  // pinned host copy of the done flags
+    DevState* hstate_ = nullptr;  // pinned host copies of the fields' states
     int launched_ = 0;
     bool graph_ = false;
     DevBuffer cert_tables_;

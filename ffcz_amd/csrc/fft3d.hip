@@ -832,9 +832,9 @@ __global__ __launch_bounds__(1024) void k_finalize_batch(const FinEntry<R>* t) {
     }
 }
 
-__global__ void k_gather_done(DevState* const* sts, size_t n, unsigned* out) {
+__global__ void k_gather_states(DevState* const* sts, size_t n, DevState* out) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        out[i] = sts[i]->done;
+        out[i] = *sts[i];
 }
 
 template <typename R>
@@ -984,7 +984,7 @@ void batch_plan_build(BatchPlan<R>& bp, const std::vector<const Fft3d<R>*>& fiel
     std::vector<DevState*> sts;
     for (auto& a : args) sts.push_back(a.st);
     const size_t sts_off = put(sts.data(), sts.size() * sizeof(DevState*));
-    const size_t done_off = put(std::vector<unsigned>(sts.size(), 0u).data(), sts.size() * sizeof(unsigned));
+    const size_t done_off = put(std::vector<DevState>(sts.size()).data(), sts.size() * sizeof(DevState));
     FFCZ_HIP(hipMalloc(&bp.dmem, host.size()));
     FFCZ_HIP(hipMemcpy(bp.dmem, host.data(), host.size(), hipMemcpyHostToDevice));
     for (auto& p : pend) {
@@ -995,7 +995,7 @@ void batch_plan_build(BatchPlan<R>& bp, const std::vector<const Fft3d<R>*>& fiel
     bp.nfin = (int)fin.size();
     bp.fin = fin.empty() ? nullptr : bp.dmem + fin_off;
     bp.sts = reinterpret_cast<DevState**>(bp.dmem + sts_off);
-    bp.done = reinterpret_cast<unsigned*>(bp.dmem + done_off);
+    bp.gathered = reinterpret_cast<DevState*>(bp.dmem + done_off);
 }
 
 template <typename R>
@@ -1023,10 +1023,11 @@ void batch_iteration(const BatchPlan<R>& bp, hipStream_t s) {
 }
 
 template <typename R>
-void batch_gather_done(const BatchPlan<R>& bp, unsigned* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_gather_done, dim3(1), dim3(256), 0, s, (DevState* const*)bp.sts, bp.nfields, bp.done);
+void batch_gather_states(const BatchPlan<R>& bp, DevState* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_gather_states, dim3(1), dim3(256), 0, s, (DevState* const*)bp.sts, bp.nfields,
+                       bp.gathered);
     FFCZ_HIP(hipGetLastError());
-    FFCZ_HIP(hipMemcpyAsync(out, bp.done, bp.nfields * sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    FFCZ_HIP(hipMemcpyAsync(out, bp.gathered, bp.nfields * sizeof(DevState), hipMemcpyDeviceToHost, s));
 }
 
 // ===========================================================================
@@ -1235,7 +1236,7 @@ template void batch_plan_free<float>(BatchPlan<float>&);
 template void batch_plan_free<double>(BatchPlan<double>&);
 template void batch_iteration<float>(const BatchPlan<float>&, hipStream_t);
 template void batch_iteration<double>(const BatchPlan<double>&, hipStream_t);
-template void batch_gather_done<float>(const BatchPlan<float>&, unsigned*, hipStream_t);
-template void batch_gather_done<double>(const BatchPlan<double>&, unsigned*, hipStream_t);
+template void batch_gather_states<float>(const BatchPlan<float>&, DevState*, hipStream_t);
+template void batch_gather_states<double>(const BatchPlan<double>&, DevState*, hipStream_t);
 
 }  // namespace ffcz

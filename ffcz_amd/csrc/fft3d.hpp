@@ -177,7 +177,7 @@ struct BatchPlan {
     const void* fin = nullptr;   // finalize table of the batched fields
     int nfin = 0;
     DevState** sts = nullptr;    // every field's state
-    unsigned* done = nullptr;    // gathered done flags
+    DevState* gathered = nullptr;  // gathered copies of the states
     size_t nfields = 0;
     unsigned char* dmem = nullptr;
 };
@@ -188,9 +188,9 @@ template <typename R>
 void batch_plan_free(BatchPlan<R>& bp);
 template <typename R>
 void batch_iteration(const BatchPlan<R>& bp, hipStream_t s);
-// copies every field's done flag into host memory `out` (asynchronous on s)
+// copies every field's state into host memory `out` (asynchronous on s)
 template <typename R>
-void batch_gather_done(const BatchPlan<R>& bp, unsigned* out, hipStream_t s);
+void batch_gather_states(const BatchPlan<R>& bp, DevState* out, hipStream_t s);
 
 // ---- FP64 certification ----------------------------------------------------
 template <typename T>
