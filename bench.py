@@ -6,7 +6,8 @@ A step is one certified pass of the hot path over the synthetic field, inputs
 resident in HBM: reset the edit accumulators, the projection loop to
 convergence (iterating against bounds shrunk by the certification margins),
 extraction with the step quantiser (eps_s/8, delta/8), then the FP64
-certification of the unquantised and of the dequantised edits.  ``value`` is
+certification of the unquantised and of the dequantised edits (one paired
+sequence that reads x'-x and the edits once).  ``value`` is
 certified GB/s = steps * N * sizeof(T) / elapsed (0 if any certification fails).
 
 The iteration runs on each FFT variant in turn (FP32 fields): the plain FP32
@@ -320,9 +321,8 @@ def main():
             solver.reset_edit_accumulators()
             solver.solve_loop(mode, wl.e0, wl.max_iter, 1e-7)
             solver.extract_compact_representation()
-            c1 = solver.certify(False)
-            c2 = solver.certify(True)
-            return c1, c2
+            # FP64 certification of the unquantised and of the dequantised edits (paired)
+            return solver.certify_both()
 
         for _ in range(a.warmup):
             step()

@@ -472,6 +472,22 @@ int ffcz_set_margins(ffcz_solver* h, double sm, double fm) {
         with(h, [&](auto& s) { s.set_margins(sm, fm); });
     });
 }
+int ffcz_certify_both(ffcz_solver* h, ffcz_cert* out) {
+    return guard([&] {
+        FFCZ_REQUIRE(out != nullptr, ERR_INVALID, "null result");
+        with(h, [&](auto& s) {
+            CertResult r[2];
+            s.certify_both(r);
+            for (int i = 0; i < 2; ++i) {
+                out[i].spatial_violations = r[i].spatial_violations;
+                out[i].freq_violations = r[i].freq_violations;
+                out[i].spatial_max_excess = r[i].spatial_max_excess;
+                out[i].freq_max_excess = r[i].freq_max_excess;
+                out[i].ok = r[i].ok;
+            }
+        });
+    });
+}
 int ffcz_set_fft_precision(ffcz_solver* h, int variant) {
     return guard([&] { with(h, [&](auto& s) { s.set_fft_precision(variant); }); });
 }

@@ -1030,6 +1030,76 @@ void batch_gather_states(const BatchPlan<R>& bp, DevState* out, hipStream_t s) {
     FFCZ_HIP(hipMemcpyAsync(out, bp.gathered, bp.nfields * sizeof(DevState), hipMemcpyDeviceToHost, s));
 }
 
+template <typename T>
+static const double* cert_df_field(const Fft3d<double>& f, const CertArgs<T>& c, hipStream_t s) {
+    // C2R(deq df)/N in FP64: x pass loads the split accumulator
+    if (!c.have_df) return nullptr;
+    auto x = sargs<double, T>(f, c.work_spec, true);
+    x.sign = +1;
+    x.dfsrc = c.df;
+    x.qf = c.qf;
+    launch_strided(f, x, S_LOAD_DF, true, s);
+    auto y = sargs<double, T>(f, c.work_spec, false);
+    y.sign = +1;
+    launch_strided(f, y, S_PLAIN, false, s);
+    auto z = zargs<double, T>(f);
+    z.spec = c.work_spec;
+    z.out = c.work_real;
+    z.scale = 1.0 / (double)f.N;
+    launch_zinv<double, T, Z_PLAIN>(f, z, s);
+    return c.work_real;
+}
+
+template <typename T>
+static ZArgs<double, T> cert_zargs(const Fft3d<double>& f, const CertArgs<T>& c, const double* dfi) {
+    auto z = zargs<double, T>(f);
+    z.spec = c.work_spec;
+    z.e0 = c.e0;
+    z.cds = c.ds;
+    z.dfi = dfi;
+    z.qs = c.qs;
+    z.ceps = c.eps;
+    z.stats = c.stats;
+    return z;
+}
+
+template <typename T>
+static void cert_xcheck(const Fft3d<double>& f, const CertArgs<T>& c, hipStream_t s) {
+    auto x = sargs<double, T>(f, c.work_spec, true);
+    x.cdelta = c.delta;
+    x.cdelta_dense = c.delta_dense;
+    x.stats = c.stats;
+    launch_strided(f, x, S_CERT_CHECK, true, s);
+}
+
+template <typename T>
+void certify_pair(const Fft3d<double>& f, const CertArgs<T>& a, const CertArgs<T>& b, hipStream_t s) {
+    hipLaunchKernelGGL(k_reset_stats, dim3(1), dim3(64), 0, s, a.stats);
+    hipLaunchKernelGGL(k_reset_stats, dim3(1), dim3(64), 0, s, b.stats);
+    FFCZ_HIP(hipGetLastError());
+    const double* dfa = cert_df_field(f, a, s);
+    const double* dfb = cert_df_field(f, b, s);
+    const ZArgs<double, T> za = cert_zargs(f, a, dfa), zb = cert_zargs(f, b, dfb);
+    if (!(fast_fft_enabled() && f.zf_var == 0 && fast_cert_zfwd_pair(za, zb, s))) {
+        launch_zfwd<double, T, Z_CERT>(f, za, s);
+        launch_zfwd<double, T, Z_CERT>(f, zb, s);
+    }
+    FFCZ_HIP(hipGetLastError());
+    if (b.work_spec == a.work_spec + f.spec_elems()) {
+        auto y = sargs<double, T>(f, a.work_spec, false);
+        y.sign = -1;
+        launch_strided(f, y, S_PLAIN, false, s, 2 * f.nx);  // both spectra: 2 nx planes
+    } else {
+        for (const CertArgs<T>* c : {&a, &b}) {
+            auto y = sargs<double, T>(f, c->work_spec, false);
+            y.sign = -1;
+            launch_strided(f, y, S_PLAIN, false, s);
+        }
+    }
+    cert_xcheck(f, a, s);
+    cert_xcheck(f, b, s);
+}
+
 // ===========================================================================
 // batched certification
 // ===========================================================================
@@ -1224,6 +1294,9 @@ template void epilogue_clamp<float>(const Fft3d<float>&, const IterArgs<float>&,
 template void epilogue_clamp<double>(const Fft3d<double>&, const IterArgs<double>&, hipStream_t);
 template void certify<float>(const Fft3d<double>&, const CertArgs<float>&, hipStream_t);
 template void certify<double>(const Fft3d<double>&, const CertArgs<double>&, hipStream_t);
+template void certify_pair<float>(const Fft3d<double>&, const CertArgs<float>&, const CertArgs<float>&, hipStream_t);
+template void certify_pair<double>(const Fft3d<double>&, const CertArgs<double>&, const CertArgs<double>&,
+                                   hipStream_t);
 template void certify_batch<float>(const std::vector<const Fft3d<double>*>&, const std::vector<CertArgs<float>>&,
                                    DevBuffer&, hipStream_t);
 template void certify_batch<double>(const std::vector<const Fft3d<double>*>&, const std::vector<CertArgs<double>>&,

@@ -211,6 +211,13 @@ struct CertArgs {
 template <typename T>
 void certify(const Fft3d<double>& f64, const CertArgs<T>& a, hipStream_t s);
 
+// Both certifications of one field (unquantised a, quantised b) in one
+// sequence: one forward z launch for both (the second read of x'-x and the
+// spatial edits comes from cache), one y launch over both spectra when b's
+// spectrum directly follows a's, then the two frequency checks.
+template <typename T>
+void certify_pair(const Fft3d<double>& f64, const CertArgs<T>& a, const CertArgs<T>& b, hipStream_t s);
+
 // A device buffer that only grows (tables of batched launches).
 struct DevBuffer {
     unsigned char* p = nullptr;

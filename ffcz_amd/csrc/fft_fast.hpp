@@ -48,6 +48,10 @@ bool fast_cert_zfwd_batch(int Lz, const BatchTab<ZArgs<double, float>>& t, unsig
 bool fast_cert_zinv_batch(int Lz, const BatchTab<ZArgs<double, float>>& t, unsigned grid, hipStream_t s);
 bool fast_cert_strided_batch(int mode, int L, const BatchTab<SArgs<double, float>>& t, unsigned grid,
                              hipStream_t s);
+// Both certification z passes (unquantised a, quantised b) in one launch;
+// false when the length has no specialised kernel.
+bool fast_cert_zfwd_pair(const ZArgs<double, float>& a, const ZArgs<double, float>& b, hipStream_t s);
+bool fast_cert_zfwd_pair(const ZArgs<double, double>& a, const ZArgs<double, double>& b, hipStream_t s);
 // whether a y/x axis length has a specialised kernel, and the columns per
 // tile of those kernels for a real type of `bytes` bytes
 bool fast_s_supported(int L);

@@ -18,6 +18,10 @@ bool fast_strided<double, float>(int mode, const SArgs<double, float>& a, size_t
     return strided_dispatch<double, float, (1u << S_PLAIN) | (1u << S_CERT_CHECK) | (1u << S_LOAD_DF)>(mode, a, outer, s);
 }
 
+bool fast_cert_zfwd_pair(const ZArgs<double, float>& a, const ZArgs<double, float>& b, hipStream_t s) {
+    return zfwd_pair_dispatch<double, float, Z_CERT>(a, b, s);
+}
+
 // batched certification passes (FP64 transforms of float edits)
 bool fast_cert_zfwd_batch(int Lz, const BatchTab<ZArgs<double, float>>& t, unsigned grid, hipStream_t s) {
     return zfwd_batch_dispatch<double, float, Z_CERT>(Lz, t, grid, s);

@@ -18,4 +18,8 @@ bool fast_strided<double, double>(int mode, const SArgs<double, double>& a, size
     return strided_dispatch<double, double, (1u << S_PLAIN) | (1u << S_CLAMP) | (1u << S_CERT_CHECK) | (1u << S_LOAD_DF)>(mode, a, outer, s);
 }
 
+bool fast_cert_zfwd_pair(const ZArgs<double, double>& a, const ZArgs<double, double>& b, hipStream_t s) {
+    return zfwd_pair_dispatch<double, double, Z_CERT>(a, b, s);
+}
+
 }  // namespace ffcz

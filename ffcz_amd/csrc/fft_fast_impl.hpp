@@ -307,6 +307,14 @@ __global__ __launch_bounds__(s_tp<R>(L) * s_width<R>(V),
                              (PREC != PREC_VALU && MODE == S_PLAIN) ? 8 : 1) void k2_strided(SArgs<R, T> a) {
     strided_body<R, T, L, MODE, PREC, V>(a, blockIdx.x);
 }
+// Two forward z passes over the same rows in one workgroup (the paired
+// certification): the second re-reads the rows the first just read, from cache.
+template <typename R, typename T, int LZ, int MODE>
+__global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ)) void k2_zfwd_pair(ZArgs<R, T> a, ZArgs<R, T> b) {
+    zfwd_body<R, T, LZ, MODE, PREC_VALU, 0>(a, blockIdx.x);
+    __syncthreads();
+    zfwd_body<R, T, LZ, MODE, PREC_VALU, 0>(b, blockIdx.x);
+}
 template <typename R, typename T, int LZ, int MODE>
 __global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ)) void k2_zfwd_b(BatchTab<ZArgs<R, T>> t) {
     const uint32_t f = batch_field(t, blockIdx.x);
@@ -483,6 +491,22 @@ bool mfma_strided_dispatch(int mode, const SArgs<float, float>& a, size_t outer,
         return true;
         FFCZ_MFMA_S_LENGTHS(FFCZ_MSL)
 #undef FFCZ_MSL
+        default:
+            return false;
+    }
+}
+
+template <typename R, typename T, int MODE>
+bool zfwd_pair_dispatch(const ZArgs<R, T>& a, const ZArgs<R, T>& b, hipStream_t s) {
+    if (!a.half) return false;
+    switch (a.Lz) {
+#define FFCZ_PZF(Lv)                                                                                  \
+    case Lv:                                                                                          \
+        hipLaunchKernelGGL((k2_zfwd_pair<R, T, Lv, MODE>), dim3((unsigned)ceil_div(a.rows, (size_t)z_rows(Lv))), \
+                           dim3(z_tp(Lv) * z_rows(Lv)), 0, s, a, b);                                  \
+        return true;
+        FFCZ_FAST_Z_LENGTHS(FFCZ_PZF)
+#undef FFCZ_PZF
         default:
             return false;
     }

@@ -80,7 +80,8 @@ template <typename T>
 void Solver<T>::release() {
     if (s_) (void)hipStreamSynchronize(s_);
     void* ptrs[] = {x_, e_, e0_, ds_, df_, spec_, st_, vpart_, delta_dense_, orig_, partial_idx_, fpack_, spack_,
-                    fcomp_, scomp_, fquant_, squant_, xscratch_, cert_real_, cert_spec_, cstats_, xglob_};
+                    fcomp_, scomp_, fquant_, squant_, xscratch_, cert_real_, cert_spec_, cstats_, xglob_,
+                    cert_real2_};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (xhost_) (void)hipHostFree(xhost_);
@@ -100,6 +101,7 @@ void Solver<T>::release() {
     fquant_ = squant_ = nullptr;
     xscratch_ = nullptr;
     cert_real_ = nullptr;
+    cert_real2_ = nullptr;
     cert_spec_ = nullptr;
     cstats_ = nullptr;
     if (f64_) f64_->destroy();
@@ -455,7 +457,7 @@ void Solver<T>::ensure_cert_buffers() {
     f64_.reset(new Fft3d<double>());
     f64_->create(f_.nx, f_.ny, f_.nz);
     dalloc(&cert_real_, f_.N);
-    dalloc(&cert_spec_, f64_->spec_elems());
+    dalloc(&cert_spec_, 2 * f64_->spec_elems());
     dalloc(&cstats_, 2);
     FFCZ_HIP(hipHostMalloc((void**)&chost_, 2 * sizeof(CertStats), hipHostMallocDefault));
 }
@@ -481,6 +483,24 @@ void Solver<T>::certify_enqueue(bool quantized) {
     const CertArgs<T> c = certify_prepare(quantized);
     ffcz::certify<T>(*f64_, c, s_);
     certify_copy_stats(quantized, s_);
+}
+
+template <typename T>
+void Solver<T>::certify_both(CertResult* out) {
+    FFCZ_REQUIRE(out != nullptr, ERR_INVALID, "null result");
+    CertArgs<T> a = certify_prepare(false);
+    CertArgs<T> b = certify_prepare(true);
+    b.work_spec = cert_spec_ + f64_->spec_elems();
+    if (b.have_df) {
+        if (!cert_real2_) dalloc(&cert_real2_, f_.N);
+        b.work_real = cert_real2_;
+    }
+    ffcz::certify_pair<T>(*f64_, a, b, s_);
+    certify_copy_stats(false, s_);
+    certify_copy_stats(true, s_);
+    FFCZ_HIP(hipStreamSynchronize(s_));
+    out[0] = certify_read(false);
+    out[1] = certify_read(true);
 }
 
 template <typename T>

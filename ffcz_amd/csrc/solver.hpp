@@ -59,6 +59,9 @@ class Solver {
     void extract();
     void reset_edit_accumulators();
     CertResult certify(bool quantized);
+    // both certifications (unquantised [0], quantised [1]) in one paired
+    // sequence; requires the extraction
+    void certify_both(CertResult* out2);
     // mean ms per launch of the five iteration passes, then per whole iteration
     void time_iteration(int reps, float* ms6);
     // Per-pass timing inside real solves: HIP events between the launches of
@@ -184,7 +187,8 @@ class Solver {
     // certification
     std::unique_ptr<Fft3d<double>> f64_;
     double* cert_real_ = nullptr;
-    cplx<double>* cert_spec_ = nullptr;
+    cplx<double>* cert_spec_ = nullptr;   // two FP64 spectra (the paired certification uses both)
+    double* cert_real2_ = nullptr;         // second C2R work field (paired, frequency edits present)
     CertStats* cstats_ = nullptr;  // [2]: unquantised, quantised
     CertStats* chost_ = nullptr;   // pinned host copies
     // parameters

@@ -284,6 +284,14 @@ class ProjectionSolver:
         return CertResult(bool(r.ok), r.spatial_violations, r.freq_violations, r.spatial_max_excess,
                           r.freq_max_excess)
 
+    def certify_both(self):
+        """(unquantised, quantised) certifications as one paired FP64 sequence
+        (after extract_compact_representation)."""
+        r = (_lib.Cert * 2)()
+        call("ffcz_certify_both", self._h, r)
+        return tuple(CertResult(bool(c.ok), c.spatial_violations, c.freq_violations, c.spatial_max_excess,
+                                c.freq_max_excess) for c in r)
+
     def stream_handle(self) -> int:
         s = C.c_void_p()
         call("ffcz_get_stream", self._h, C.byref(s))

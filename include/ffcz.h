@@ -129,6 +129,9 @@ int ffcz_get_edit_range(ffcz_solver* h, double* fmin, double* fmax, double* smin
 int ffcz_set_margins(ffcz_solver* h, double spatial_margin, double freq_margin);
 int ffcz_set_quantizer(ffcz_solver* h, int kind, double freq_step, double spatial_step);
 int ffcz_certify(ffcz_solver* h, int quantized, ffcz_cert* out);
+/* both certifications after extraction, out[0] unquantised and out[1]
+ * quantised, as one paired FP64 sequence (x'-x and the edits read once) */
+int ffcz_certify_both(ffcz_solver* h, ffcz_cert* out2);
 /* ---- batches: many fields of uneven shapes corrected together ----------
  * No counterpart in the reference (its driver, projection_example.cu:157-265,
  * runs one field per ProjectionSolver).  Each field is a borrowed solver

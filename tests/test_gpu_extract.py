@@ -9,7 +9,7 @@ import pytest
 import torch
 
 import ffcz_amd
-from ffcz_amd import ProjectionSolver, QUANT_STEP
+from ffcz_amd import MODE_ABS, ProjectionSolver, QUANT_STEP
 from ffcz_amd._lib import FFCzError, call
 from oracle import ffcz_oracle as O
 from tests._cases import binding_case
@@ -159,3 +159,27 @@ def test_step_quantizer_overflow_is_reported(gpu):
             s.certify(True)
         with pytest.raises(FFCzError):
             s.get_active_spatial_quant_edit_distances()
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("shape", [(16, 20, 30), (32, 64, 128), (6, 10, 12), (9, 15, 25)])
+@pytest.mark.parametrize("quant", ["ref16", "step"])
+def test_certify_both_equals_the_two_certifications(gpu, shape, dtype, quant):
+    """The paired sequence (one z launch for both, one y launch over both
+    spectra) gives exactly the counts and maximal excesses of certify(False)
+    and certify(True); frequency edits present (the C2R of df runs)."""
+    from ffcz_amd import QUANT_REF16
+    x, e0, eps, delta = binding_case(shape, seed=11, dtype=dtype, spread=1.4, dfrac=0.8)
+    with ProjectionSolver(*shape, dtype=dtype) as s:
+        s.initialize_all_abs(x, eps, delta)
+        s.set_margins(0.125, 0.125)
+        if quant == "step":
+            s.set_quantizer(QUANT_STEP, delta / 8, eps / 8)
+        else:
+            s.set_quantizer(QUANT_REF16)
+        s.solve_loop(MODE_ABS, e0, 60, 1e-9)
+        s.extract_compact_representation()
+        pair = s.certify_both()
+        single = (s.certify(False), s.certify(True))
+        assert pair == single
+        assert any(v.freq_violations or v.spatial_violations or v.ok for v in pair)
