@@ -21,6 +21,8 @@ Prints one JSON line.  ``--gpus`` > 1 is rejected: this build runs on one GPU.
 from __future__ import annotations
 
 import argparse
+import contextlib
+import gc
 import json
 import os
 import sys
@@ -29,6 +31,19 @@ import time
 import numpy as np
 
 HBM_PEAK_GBPS = 8000.0
+
+
+@contextlib.contextmanager
+def no_gc():
+    """The timed region without Python's cyclic garbage collector (as timeit
+    does): a full collection of the interpreter's objects (torch's included)
+    takes tens of milliseconds and would land in whichever step triggers it."""
+    gc.collect()
+    gc.disable()
+    try:
+        yield
+    finally:
+        gc.enable()
 
 
 def parse():
@@ -150,12 +165,13 @@ def bench_batch(a, torch, dev):
     for _ in range(a.warmup):
         step()
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
     allc = []
-    for _ in range(a.steps):
-        allc.extend(step())
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+    with no_gc():
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            allc.extend(step())
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
     its, viols, convs, launched = b.status()
     ok = all(c1.ok and c2.ok for c1, c2 in allc)
     value = (a.steps * N_tot * 4 / elapsed / 1e9) if ok else 0.0
@@ -329,12 +345,13 @@ def main():
         torch.cuda.synchronize()
         # per-pass HIP events between the launches of the timed steps' iterations
         solver.set_pass_timing(True)
-        t0 = time.perf_counter()
         certs = []
-        for _ in range(a.steps):
-            certs.append(step())
-        torch.cuda.synchronize()
-        elapsed = time.perf_counter() - t0
+        with no_gc():
+            t0 = time.perf_counter()
+            for _ in range(a.steps):
+                certs.append(step())
+            torch.cuda.synchronize()
+            elapsed = time.perf_counter() - t0
         ms6, n_timed = solver.pass_timing()
         solver.set_pass_timing(False)
         iters, viol, conv = solver.status()

@@ -68,11 +68,13 @@ class ProjectionBatch:
         return list(it), list(vi), [bool(c) for c in cv], launched.value
 
     def extract_all(self):
-        """extract_compact_representation of every field (overlapped)."""
+        """extract_compact_representation of every field: one launch of each
+        extraction kernel over all fields' edit arrays."""
         call("ffcz_batch_extract", self._h)
 
     def certify_all(self, quantized: bool):
-        """FP64 certification of every field (overlapped) -> [CertResult]."""
+        """FP64 certification of every field (one launch per pass and axis
+        length) -> [CertResult]."""
         n = len(self.fields)
         out = (_lib.Cert * n)()
         call("ffcz_batch_certify", self._h, int(bool(quantized)), out, n)

@@ -1,5 +1,6 @@
 // Batched projection loop (batch.hpp).
 #include <algorithm>
+#include <exception>
 
 #include "batch.hpp"
 
@@ -11,6 +12,7 @@ Batch<T>::Batch(std::vector<Solver<T>*> fields) : f_(std::move(fields)) {
     for (auto* p : f_) FFCZ_REQUIRE(p != nullptr, ERR_INVALID, "null field");
     FFCZ_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
     FFCZ_HIP(hipHostMalloc((void**)&hstate_, f_.size() * sizeof(DevState), hipHostMallocDefault));
+    FFCZ_HIP(hipHostMalloc((void**)&cert_host_, f_.size() * sizeof(CertStats), hipHostMallocDefault));
 }
 
 template <typename T>
@@ -18,7 +20,9 @@ Batch<T>::~Batch() {
     release_graph();
     batch_plan_free(plan_);
     cert_tables_.release();
+    xwork_.release();
     if (hstate_) (void)hipHostFree(hstate_);
+    if (cert_host_) (void)hipHostFree(cert_host_);
     if (s_) (void)hipStreamDestroy(s_);
 }
 
@@ -33,8 +37,9 @@ void Batch<T>::solve(int mode, const T* const* e0, int max_iter, T tol, bool use
     FFCZ_REQUIRE(e0 != nullptr, ERR_INVALID, "null initial-error table");
     // the fields' own streams may still run their initialisation
     for (auto* f : f_) FFCZ_HIP(hipStreamSynchronize(f->stream()));
-    for (size_t i = 0; i < f_.size(); ++i) f_[i]->prepare_solve(mode, e0[i], max_iter, tol, s_);
+    for (size_t i = 0; i < f_.size(); ++i) f_[i]->prepare_solve(mode, e0[i], max_iter, tol, s_, false);
     build(use_graph && max_iter > 0);
+    batch_reset_states(plan_, max_iter, (double)tol, s_);
     int launched = 0, chunk = 2;
     while (launched < max_iter) {
         const int k = std::min(chunk, max_iter - launched);
@@ -58,6 +63,8 @@ void Batch<T>::solve(int mode, const T* const* e0, int max_iter, T tol, bool use
 
 template <typename T>
 void Batch<T>::build(bool use_graph) {
+    // earlier work on the batch stream may still read the plan and graph replaced here
+    FFCZ_HIP(hipStreamSynchronize(s_));
     std::vector<const Fft3d<T>*> ffts;
     std::vector<IterArgs<T>> args;
     for (auto* f : f_) {
@@ -124,8 +131,23 @@ float Batch<T>::time_iteration(int reps) {
 
 template <typename T>
 void Batch<T>::extract_all() {
-    for (auto* f : f_) f->extract();
-    for (auto* f : f_) f->sync_extract();
+    // every field's two edit arrays in one launch of each extraction kernel
+    // on the batch stream, after the fields' own streams have drained
+    std::vector<ExtractSeg<T>> segs;
+    for (auto* f : f_) FFCZ_HIP(hipStreamSynchronize(f->stream()));
+    for (auto* f : f_) f->extract_segments(segs, s_);
+    const ExtractGlobals* h = extract_edits_batch<T>(segs, xwork_, s_);
+    FFCZ_HIP(hipStreamSynchronize(s_));
+    // every field takes its results; the first field's error is reported
+    std::exception_ptr first;
+    for (size_t i = 0; i < f_.size(); ++i) {
+        try {
+            f_[i]->extract_adopt(h[2 * i], h[2 * i + 1]);
+        } catch (...) {
+            if (!first) first = std::current_exception();
+        }
+    }
+    if (first) std::rethrow_exception(first);
 }
 
 template <typename T>
@@ -140,10 +162,9 @@ void Batch<T>::certify_all(bool quantized, CertResult* out) {
         FFCZ_HIP(hipStreamSynchronize(f->stream()));
         ffts.push_back(&f->cert_fft());
     }
-    certify_batch<T>(ffts, args, cert_tables_, s_);
-    for (auto* f : f_) f->certify_copy_stats(quantized, s_);
+    certify_batch<T>(ffts, args, cert_tables_, s_, cert_host_);
     FFCZ_HIP(hipStreamSynchronize(s_));
-    for (size_t i = 0; i < f_.size(); ++i) out[i] = f_[i]->certify_read(quantized);
+    for (size_t i = 0; i < f_.size(); ++i) out[i] = f_[i]->certify_adopt(quantized, cert_host_[i]);
 }
 
 template class Batch<float>;

@@ -34,10 +34,11 @@ class Batch {
     // with HIP events on the batch stream after a completed solve; the
     // accumulators keep changing, so callers reset them afterwards
     float time_iteration(int reps);
-    // extraction of every field (each on its own stream, overlapped), then
-    // one wait per field
+    // extraction of every field: one launch of each extraction kernel over
+    // all fields' edit arrays on the batch stream, then one wait
     void extract_all();
-    // certification of every field, overlapped the same way; out[size()]
+    // certification of every field (one launch per pass and axis length);
+    // out[size()]
     void certify_all(bool quantized, CertResult* out);
 
    private:
@@ -49,9 +50,11 @@ class Batch {
     BatchPlan<T> plan_;
     hipGraphExec_t gexec_ = nullptr;
     DevState* hstate_ = nullptr;  // pinned host copies of the fields' states
+    CertStats* cert_host_ = nullptr;  // pinned host copies of the fields' certification stats
     int launched_ = 0;
     bool graph_ = false;
     DevBuffer cert_tables_;
+    ExtractBatchWork xwork_;
 };
 
 }  // namespace ffcz

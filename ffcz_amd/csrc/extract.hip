@@ -10,6 +10,7 @@
 // min/max are known only after the pass).
 #include <algorithm>
 #include <cfloat>
+#include <cstring>
 
 #include "extract.hpp"
 #include "quant.hpp"
@@ -92,12 +93,12 @@ __global__ void k_extract_init(unsigned long long* status, size_t ntiles, unsign
 // output offset, then writes the compact values (and, for the step quantiser,
 // the zigzag symbols) in index order.  The look-back's wait is bounded: past
 // LOOKBACK_SPIN_LIMIT polls it records an error instead of waiting on.
+// `tile` numbers the tile within its edit array; status, pmin and pmax point
+// at that array's tile 0.
 template <typename T>
-__global__ __launch_bounds__(256) void k_tile_compact(const T* v, size_t n, QuantDesc q, unsigned long long* status,
-                                                      unsigned int* ticket, double* pmin, double* pmax,
-                                                      ExtractGlobals* g, unsigned int* pack, T* compact,
-                                                      uint16_t* quant) {
-    __shared__ unsigned int s_tile;
+__device__ __forceinline__ void tile_compact(const unsigned int tile, const T* v, size_t n, const QuantDesc& q,
+                                             unsigned long long* status, double* pmin, double* pmax,
+                                             ExtractGlobals* g, unsigned int* pack, T* compact, uint16_t* quant) {
     __shared__ unsigned long long s_pre;
     constexpr int RND = x_rounds<T>();
     __shared__ unsigned int wtot[RND][4];
@@ -107,9 +108,6 @@ __global__ __launch_bounds__(256) void k_tile_compact(const T* v, size_t n, Quan
     constexpr unsigned int SVAL = X_TILE / 2;
     __shared__ T sval[SVAL];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    if (tid == 0) s_tile = atomicAdd(ticket, 1u);
-    __syncthreads();
-    const unsigned int tile = s_tile;
     const size_t base = (size_t)tile * x_tile<T>();
     T x[RND][X_EPT];
     unsigned int bits[RND], excl[RND];
@@ -244,9 +242,48 @@ __global__ __launch_bounds__(256) void k_tile_compact(const T* v, size_t n, Quan
     if (ovf) atomicOr(&g->overflow, 1u);
 }
 
-// Count (the last tile's inclusive prefix), min and max over the tiles.
-__global__ __launch_bounds__(1024) void k_extract_finalize(const unsigned long long* status, const double* pmin,
-                                                           const double* pmax, size_t ntiles, ExtractGlobals* g) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_tile_compact(const T* v, size_t n, QuantDesc q, unsigned long long* status,
+                                                      unsigned int* ticket, double* pmin, double* pmax,
+                                                      ExtractGlobals* g, unsigned int* pack, T* compact,
+                                                      uint16_t* quant) {
+    __shared__ unsigned int s_tile;
+    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    tile_compact<T>(s_tile, v, n, q, status, pmin, pmax, g, pack, compact, quant);
+}
+
+// Many edit arrays (segments) in one launch: the tickets number the tiles of
+// segment 0, then those of segment 1, and so on, so a tile's predecessors in
+// its own segment hold smaller tickets and belong to workgroups already
+// running, as in the single-array pass.
+template <typename T>
+__global__ __launch_bounds__(256) void k_tile_compact_b(const ExtractSeg<T>* segs, int nseg,
+                                                        unsigned long long* status, unsigned int* ticket,
+                                                        double* pmin, double* pmax) {
+    __shared__ unsigned int s_tile;
+    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const unsigned int t = s_tile;
+    // the last segment whose first tile is <= t (segments without tiles share
+    // their first tile with the next one and are passed over)
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        const int m = (lo + hi + 1) >> 1;
+        if (segs[m].tile0 <= t) lo = m;
+        else hi = m - 1;
+    }
+    const ExtractSeg<T>& sg = segs[lo];
+    const unsigned int b = sg.tile0;
+    tile_compact<T>(t - b, sg.v, sg.n, sg.q, status + b, pmin + b, pmax + b, sg.g, sg.pack, sg.compact, sg.quant);
+}
+
+// Count (the last tile's inclusive prefix), min and max over the tiles of one
+// array, by a workgroup of 1024 threads; a second copy of the results goes to
+// g2 when it is set.
+__device__ __forceinline__ void finalize_tiles(const unsigned long long* status, const double* pmin,
+                                               const double* pmax, size_t ntiles, ExtractGlobals* g,
+                                               ExtractGlobals* g2) {
     __shared__ double wmn[16], wmx[16];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     double mn = INFINITY, mx = -INFINITY;
@@ -271,11 +308,44 @@ __global__ __launch_bounds__(1024) void k_extract_finalize(const unsigned long l
             a = wmn[i] < a ? wmn[i] : a;
             b = wmx[i] > b ? wmx[i] : b;
         }
-        const unsigned long long last = status[ntiles - 1];
-        if ((last & ST_P) == 0ull) g->overflow |= 2u;
-        g->count = last & ST_VAL;
-        g->vmin = a;
-        g->vmax = b;
+        ExtractGlobals r = *g;
+        if (ntiles == 0) {
+            r.count = 0;
+            r.vmin = r.vmax = 0.0;
+        } else {
+            const unsigned long long last = status[ntiles - 1];
+            if ((last & ST_P) == 0ull) r.overflow |= 2u;
+            r.count = last & ST_VAL;
+            r.vmin = a;
+            r.vmax = b;
+        }
+        *g = r;
+        if (g2) *g2 = r;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_extract_finalize(const unsigned long long* status, const double* pmin,
+                                                           const double* pmax, size_t ntiles, ExtractGlobals* g) {
+    finalize_tiles(status, pmin, pmax, ntiles, g, nullptr);
+}
+
+// One workgroup per segment; every segment's results also go to out[segment].
+template <typename T>
+__global__ __launch_bounds__(1024) void k_extract_finalize_b(const ExtractSeg<T>* segs, int nseg,
+                                                             const unsigned long long* status, const double* pmin,
+                                                             const double* pmax, size_t ntot, ExtractGlobals* out) {
+    const int k = blockIdx.x;
+    const size_t b = segs[k].tile0, e = k + 1 < nseg ? segs[k + 1].tile0 : ntot;
+    finalize_tiles(status + b, pmin + b, pmax + b, e - b, segs[k].g, out + k);
+}
+
+__global__ void k_extract_init_b(ExtractGlobals* const* gs, int nseg, unsigned long long* status, size_t ntot,
+                                 unsigned int* ticket) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < ntot; i += (size_t)gridDim.x * blockDim.x)
+        status[i] = 0ull;
+    if (blockIdx.x == 0) {
+        if (threadIdx.x == 0) *ticket = 0u;
+        for (int k = threadIdx.x; k < nseg; k += blockDim.x) *gs[k] = ExtractGlobals{};
     }
 }
 
@@ -287,6 +357,18 @@ __global__ __launch_bounds__(256) void k_quant_compact(const T* compact, uint16_
     const T mn = (T)g->vmin, mx = (T)g->vmax;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < cnt; i += (size_t)gridDim.x * blockDim.x)
         quant[i] = (uint16_t)quant_ref16(compact[i], mn, mx);
+}
+
+// The same for the segments with the reference quantiser (grid.y = segment).
+template <typename T>
+__global__ __launch_bounds__(256) void k_quant_compact_b(const ExtractSeg<T>* segs) {
+    const ExtractSeg<T>& sg = segs[blockIdx.y];
+    if (sg.q.kind == Q_STEP) return;
+    const ExtractGlobals* g = sg.g;
+    const size_t cnt = g->count;
+    const T mn = (T)g->vmin, mx = (T)g->vmax;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < cnt; i += (size_t)gridDim.x * blockDim.x)
+        sg.quant[i] = (uint16_t)quant_ref16(sg.compact[i], mn, mx);
 }
 
 template <typename T>
@@ -358,6 +440,88 @@ void extract_edits_async(const T* v, size_t n, QuantDesc q, uint8_t* pack, T* co
     FFCZ_HIP(hipMemcpyAsync(hglob, dglob, sizeof(ExtractGlobals), hipMemcpyDeviceToHost, s));
 }
 
+void ExtractBatchWork::release() {
+    if (dev) (void)hipFree(dev);
+    if (host) (void)hipHostFree(host);
+    if (done) (void)hipEventDestroy(done);
+    dev = host = nullptr;
+    dev_cap = host_cap = 0;
+    done = nullptr;
+}
+
+template <typename T>
+const ExtractGlobals* extract_edits_batch(std::vector<ExtractSeg<T>>& segs, ExtractBatchWork& w, hipStream_t s) {
+    const int nseg = (int)segs.size();
+    FFCZ_REQUIRE(nseg > 0, ERR_INVALID, "no edit arrays to extract");
+    size_t ntot = 0, maxn = 0;
+    bool ref = false;
+    for (ExtractSeg<T>& sg : segs) {
+        FFCZ_REQUIRE(((uintptr_t)sg.v % 16) == 0, ERR_INVALID, "edit array must be 16-byte aligned");
+        FFCZ_REQUIRE(((uintptr_t)sg.pack % 4) == 0, ERR_INVALID, "flag buffer must be 4-byte aligned");
+        sg.tile0 = (unsigned int)ntot;
+        ntot += ceil_div(sg.n, (size_t)x_tile<T>());
+        FFCZ_REQUIRE(ntot < (1u << 31), ERR_INVALID, "batch too large for extraction");
+        maxn = std::max(maxn, sg.n);
+        ref = ref || (sg.q.kind != Q_STEP && sg.n > 0);
+    }
+    auto r256 = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    // device: segment table | result pointers | ticket | status, pmin, pmax | results
+    // pinned host: segment table | result pointers | results
+    const size_t tab_b = r256(nseg * sizeof(ExtractSeg<T>)), gs_b = r256(nseg * sizeof(ExtractGlobals*));
+    const size_t ticket_off = tab_b + gs_b, status_off = ticket_off + 256;
+    const size_t out_off = r256(status_off + ntot * (sizeof(unsigned long long) + 2 * sizeof(double)));
+    const size_t dev_b = out_off + nseg * sizeof(ExtractGlobals), host_b = tab_b + gs_b + nseg * sizeof(ExtractGlobals);
+    // the staging area and results are reused: the previous batch must be done with them
+    if (w.done) FFCZ_HIP(hipEventSynchronize(w.done));
+    else FFCZ_HIP(hipEventCreateWithFlags(&w.done, hipEventDisableTiming));
+    if (dev_b > w.dev_cap) {
+        if (w.dev) FFCZ_HIP(hipFree(w.dev));
+        w.dev = nullptr;
+        w.dev_cap = 0;
+        FFCZ_HIP(hipMalloc((void**)&w.dev, dev_b));
+        w.dev_cap = dev_b;
+    }
+    if (host_b > w.host_cap) {
+        if (w.host) FFCZ_HIP(hipHostFree(w.host));
+        w.host = nullptr;
+        w.host_cap = 0;
+        FFCZ_HIP(hipHostMalloc((void**)&w.host, host_b, hipHostMallocDefault));
+        w.host_cap = host_b;
+    }
+    std::memcpy(w.host, segs.data(), nseg * sizeof(ExtractSeg<T>));
+    ExtractGlobals** hg = (ExtractGlobals**)(w.host + tab_b);
+    for (int k = 0; k < nseg; ++k) hg[k] = segs[k].g;
+    FFCZ_HIP(hipMemcpyAsync(w.dev, w.host, tab_b + gs_b, hipMemcpyHostToDevice, s));
+    const ExtractSeg<T>* dsegs = (const ExtractSeg<T>*)w.dev;
+    ExtractGlobals* const* dgs = (ExtractGlobals* const*)(w.dev + tab_b);
+    unsigned int* ticket = (unsigned int*)(w.dev + ticket_off);
+    unsigned long long* status = (unsigned long long*)(w.dev + status_off);
+    double* pmin = (double*)(status + ntot);
+    double* pmax = pmin + ntot;
+    ExtractGlobals* dout = (ExtractGlobals*)(w.dev + out_off);
+    ExtractGlobals* hout = (ExtractGlobals*)(w.host + tab_b + gs_b);
+    const unsigned ig = (unsigned)std::min<size_t>(std::max<size_t>(ceil_div(ntot, (size_t)256), 1), 1024);
+    hipLaunchKernelGGL(k_extract_init_b, dim3(ig), dim3(256), 0, s, dgs, nseg, status, ntot, ticket);
+    FFCZ_HIP(hipGetLastError());
+    if (ntot > 0) {
+        hipLaunchKernelGGL((k_tile_compact_b<T>), dim3((unsigned)ntot), dim3(256), 0, s, dsegs, nseg, status, ticket,
+                           pmin, pmax);
+        FFCZ_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL((k_extract_finalize_b<T>), dim3((unsigned)nseg), dim3(1024), 0, s, dsegs, nseg,
+                       (const unsigned long long*)status, (const double*)pmin, (const double*)pmax, ntot, dout);
+    FFCZ_HIP(hipGetLastError());
+    if (ref) {
+        const unsigned qx = (unsigned)std::min<size_t>(ceil_div(maxn, (size_t)256), 256);
+        FFCZ_REQUIRE(nseg < 65536, ERR_INVALID, "too many edit arrays in one batch");
+        hipLaunchKernelGGL((k_quant_compact_b<T>), dim3(qx, (unsigned)nseg), dim3(256), 0, s, dsegs);
+        FFCZ_HIP(hipGetLastError());
+    }
+    FFCZ_HIP(hipMemcpyAsync(hout, dout, nseg * sizeof(ExtractGlobals), hipMemcpyDeviceToHost, s));
+    FFCZ_HIP(hipEventRecord(w.done, s));
+    return hout;
+}
+
 template <typename T>
 void dequantize_dense(const T* v, size_t n, QuantDesc q, T* out, hipStream_t s) {
     size_t gsz = ceil_div(n ? n : 1, (size_t)256);
@@ -372,6 +536,10 @@ template void extract_edits_async<float>(const float*, size_t, QuantDesc, uint8_
                                          ExtractGlobals*, ExtractGlobals*, hipStream_t);
 template void extract_edits_async<double>(const double*, size_t, QuantDesc, uint8_t*, double*, uint16_t*, void*,
                                           ExtractGlobals*, ExtractGlobals*, hipStream_t);
+template const ExtractGlobals* extract_edits_batch<float>(std::vector<ExtractSeg<float>>&, ExtractBatchWork&,
+                                                         hipStream_t);
+template const ExtractGlobals* extract_edits_batch<double>(std::vector<ExtractSeg<double>>&, ExtractBatchWork&,
+                                                          hipStream_t);
 template void dequantize_dense<float>(const float*, size_t, QuantDesc, float*, hipStream_t);
 template void dequantize_dense<double>(const double*, size_t, QuantDesc, double*, hipStream_t);
 

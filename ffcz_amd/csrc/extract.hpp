@@ -4,6 +4,8 @@
 // projection_algorithm.cu:363-550).
 #pragma once
 
+#include <vector>
+
 #include "fft3d.hpp"
 
 namespace ffcz {
@@ -42,6 +44,37 @@ ExtractResult extract_edits(const T* v, size_t n, QuantDesc q, uint8_t* pack, T*
 template <typename T>
 void extract_edits_async(const T* v, size_t n, QuantDesc q, uint8_t* pack, T* compact, uint16_t* quant,
                          void* scratch, ExtractGlobals* dglob, ExtractGlobals* hglob, hipStream_t s);
+
+// One edit array of a batched extraction, with its quantiser and outputs.
+template <typename T>
+struct ExtractSeg {
+    const T* v;
+    size_t n;
+    QuantDesc q;
+    unsigned int* pack;  // the flag bytes, written four at a time
+    T* compact;
+    uint16_t* quant;
+    ExtractGlobals* g;   // device results (zeroed first)
+    unsigned int tile0;  // first tile in the batch's numbering (set by extract_edits_batch)
+};
+
+// Device scratch and pinned staging of batched extractions, grown as needed
+// and reused (a call waits for the previous one's results to be copied out).
+struct ExtractBatchWork {
+    unsigned char* dev = nullptr;
+    size_t dev_cap = 0;
+    unsigned char* host = nullptr;
+    size_t host_cap = 0;
+    hipEvent_t done = nullptr;
+    void release();
+};
+
+// Every segment extracted as by extract_edits_async, with one launch of each
+// kernel for the whole batch on stream s.  Returns pinned host copies of the
+// segments' results, in segment order, valid once s has drained (and until
+// the next call with the same workspace).
+template <typename T>
+const ExtractGlobals* extract_edits_batch(std::vector<ExtractSeg<T>>& segs, ExtractBatchWork& w, hipStream_t s);
 
 // Dense dequantised copy (projection_algorithm.cu:615-644 for Q_REF16).
 template <typename T>

@@ -421,19 +421,26 @@ __global__ __launch_bounds__(1024) void k_finalize_iteration(const R* part, size
     }
 }
 
+__device__ __forceinline__ void reset_one(DevState* st, unsigned int max_iter, double tol) {
+    st->done = max_iter == 0 ? 1u : 0u;
+    st->converged = 0;
+    st->freq_clamped = 0;
+    st->iter = 0;
+    st->max_iter = max_iter;
+    st->blocks_done = 0;
+    st->viol_bits = 0;
+    st->tol = tol;
+    st->tol_f = (float)tol;
+    st->last_viol = 0.0;
+}
+
 __global__ void k_reset_state(DevState* st, unsigned int max_iter, double tol) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        st->done = max_iter == 0 ? 1u : 0u;
-        st->converged = 0;
-        st->freq_clamped = 0;
-        st->iter = 0;
-        st->max_iter = max_iter;
-        st->blocks_done = 0;
-        st->viol_bits = 0;
-        st->tol = tol;
-        st->tol_f = (float)tol;
-        st->last_viol = 0.0;
-    }
+    if (threadIdx.x == 0 && blockIdx.x == 0) reset_one(st, max_iter, tol);
+}
+
+__global__ void k_reset_states(DevState* const* sts, size_t n, unsigned int max_iter, double tol) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        reset_one(sts[i], max_iter, tol);
 }
 
 template <typename R>
@@ -857,7 +864,11 @@ template <typename R>
 void batch_plan_build(BatchPlan<R>& bp, const std::vector<const Fft3d<R>*>& fields,
                       const std::vector<IterArgs<R>>& args) {
     FFCZ_REQUIRE(fields.size() == args.size() && !fields.empty(), ERR_INVALID, "empty batch");
-    batch_plan_free(bp);
+    unsigned char* keep = bp.dmem;
+    const size_t keep_cap = bp.dcap;
+    bp = BatchPlan<R>{};
+    bp.dmem = keep;
+    bp.dcap = keep_cap;
     bp.nfields = fields.size();
     // host staging of every device table; offsets are patched into pointers
     // once the device block exists
@@ -985,7 +996,13 @@ void batch_plan_build(BatchPlan<R>& bp, const std::vector<const Fft3d<R>*>& fiel
     for (auto& a : args) sts.push_back(a.st);
     const size_t sts_off = put(sts.data(), sts.size() * sizeof(DevState*));
     const size_t done_off = put(std::vector<DevState>(sts.size()).data(), sts.size() * sizeof(DevState));
-    FFCZ_HIP(hipMalloc(&bp.dmem, host.size()));
+    if (host.size() > bp.dcap) {
+        if (bp.dmem) FFCZ_HIP(hipFree(bp.dmem));
+        bp.dmem = nullptr;
+        bp.dcap = 0;
+        FFCZ_HIP(hipMalloc(&bp.dmem, host.size()));
+        bp.dcap = host.size();
+    }
     FFCZ_HIP(hipMemcpy(bp.dmem, host.data(), host.size(), hipMemcpyHostToDevice));
     for (auto& p : pend) {
         p.g.args = bp.dmem + p.args_off;
@@ -1028,6 +1045,13 @@ void batch_gather_states(const BatchPlan<R>& bp, DevState* out, hipStream_t s) {
                        bp.gathered);
     FFCZ_HIP(hipGetLastError());
     FFCZ_HIP(hipMemcpyAsync(out, bp.gathered, bp.nfields * sizeof(DevState), hipMemcpyDeviceToHost, s));
+}
+
+template <typename R>
+void batch_reset_states(const BatchPlan<R>& bp, int max_iter, double tol, hipStream_t s) {
+    hipLaunchKernelGGL(k_reset_states, dim3(1), dim3(256), 0, s, (DevState* const*)bp.sts, bp.nfields,
+                       (unsigned)(max_iter < 0 ? 0 : max_iter), tol);
+    FFCZ_HIP(hipGetLastError());
 }
 
 template <typename T>
@@ -1119,9 +1143,13 @@ __global__ void k_reset_stats_many(CertStats* const* st, int n) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) *st[i] = CertStats{};
 }
 
+__global__ void k_gather_stats(CertStats* const* st, int n, CertStats* out) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[i] = *st[i];
+}
+
 template <typename T>
 void certify_batch(const std::vector<const Fft3d<double>*>& fs, const std::vector<CertArgs<T>>& cs,
-                   DevBuffer& tables, hipStream_t s) {
+                   DevBuffer& tables, hipStream_t s, CertStats* host_out) {
     FFCZ_REQUIRE(fs.size() == cs.size(), ERR_INVALID, "certification batch mismatch");
     std::vector<size_t> bat;
     std::vector<size_t> rest;
@@ -1132,15 +1160,22 @@ void certify_batch(const std::vector<const Fft3d<double>*>& fs, const std::vecto
                         fast_s_supported((int)f.nx);
         (ok ? bat : rest).push_back(i);
     }
+    std::vector<unsigned char> host;
+    auto put = [&](const void* p, size_t bytes) {
+        const size_t off = (host.size() + 255) & ~(size_t)255;
+        host.resize(off + bytes);
+        std::memcpy(host.data() + off, p, bytes);
+        return off;
+    };
+    // every field's stats: reset by the batched passes' first launch, gathered after the last
+    std::vector<CertStats*> all_st;
+    for (const CertArgs<T>& c : cs) all_st.push_back(c.stats);
+    const size_t all_off = put(all_st.data(), all_st.size() * sizeof(CertStats*));
+    const size_t out_off = (host.size() + 255) & ~(size_t)255;
+    host.resize(out_off + all_st.size() * sizeof(CertStats));
+    bool uploaded = false;
     if constexpr (std::is_same<T, float>::value) {
         if (!bat.empty()) {
-            std::vector<unsigned char> host;
-            auto put = [&](const void* p, size_t bytes) {
-                const size_t off = (host.size() + 255) & ~(size_t)255;
-                host.resize(off + bytes);
-                std::memcpy(host.data() + off, p, bytes);
-                return off;
-            };
             struct G {
                 int kind;  // 0 z fwd cert, 1 z inv plain, 2 strided
                 int smode, L;
@@ -1246,14 +1281,12 @@ void certify_batch(const std::vector<const Fft3d<double>*>& fs, const std::vecto
                 x.stats = cs[i].stats;
                 return x;
             });
-            std::vector<CertStats*> sts;
-            for (size_t i : bat) sts.push_back(cs[i].stats);
-            const size_t sts_off = put(sts.data(), sts.size() * sizeof(CertStats*));
             tables.ensure(host.size());
             FFCZ_HIP(hipMemcpyAsync(tables.p, host.data(), host.size(), hipMemcpyHostToDevice, s));
             FFCZ_HIP(hipStreamSynchronize(s));  // the host staging goes out of scope
+            uploaded = true;
             hipLaunchKernelGGL(k_reset_stats_many, dim3(1), dim3(256), 0, s,
-                               (CertStats* const*)(tables.p + sts_off), (int)sts.size());
+                               (CertStats* const*)(tables.p + all_off), (int)all_st.size());
             FFCZ_HIP(hipGetLastError());
             for (const G& g : gs) {
                 bool ok;
@@ -1272,6 +1305,19 @@ void certify_batch(const std::vector<const Fft3d<double>*>& fs, const std::vecto
         }
     }
     for (size_t i : rest) certify<T>(*fs[i], cs[i], s);
+    if (host_out && !cs.empty()) {
+        if (!uploaded) {
+            // no batched field: the stats pointers go up now
+            tables.ensure(host.size());
+            FFCZ_HIP(hipMemcpyAsync(tables.p, host.data(), host.size(), hipMemcpyHostToDevice, s));
+            FFCZ_HIP(hipStreamSynchronize(s));
+        }
+        CertStats* dout = (CertStats*)(tables.p + out_off);
+        hipLaunchKernelGGL(k_gather_stats, dim3(1), dim3(256), 0, s, (CertStats* const*)(tables.p + all_off),
+                           (int)all_st.size(), dout);
+        FFCZ_HIP(hipGetLastError());
+        FFCZ_HIP(hipMemcpyAsync(host_out, dout, all_st.size() * sizeof(CertStats), hipMemcpyDeviceToHost, s));
+    }
 }
 
 template struct Fft3d<float>;
@@ -1298,9 +1344,9 @@ template void certify_pair<float>(const Fft3d<double>&, const CertArgs<float>&, 
 template void certify_pair<double>(const Fft3d<double>&, const CertArgs<double>&, const CertArgs<double>&,
                                    hipStream_t);
 template void certify_batch<float>(const std::vector<const Fft3d<double>*>&, const std::vector<CertArgs<float>>&,
-                                   DevBuffer&, hipStream_t);
+                                   DevBuffer&, hipStream_t, CertStats*);
 template void certify_batch<double>(const std::vector<const Fft3d<double>*>&, const std::vector<CertArgs<double>>&,
-                                    DevBuffer&, hipStream_t);
+                                    DevBuffer&, hipStream_t, CertStats*);
 template void batch_plan_build<float>(BatchPlan<float>&, const std::vector<const Fft3d<float>*>&,
                                       const std::vector<IterArgs<float>>&);
 template void batch_plan_build<double>(BatchPlan<double>&, const std::vector<const Fft3d<double>*>&,
@@ -1311,5 +1357,7 @@ template void batch_iteration<float>(const BatchPlan<float>&, hipStream_t);
 template void batch_iteration<double>(const BatchPlan<double>&, hipStream_t);
 template void batch_gather_states<float>(const BatchPlan<float>&, DevState*, hipStream_t);
 template void batch_gather_states<double>(const BatchPlan<double>&, DevState*, hipStream_t);
+template void batch_reset_states<float>(const BatchPlan<float>&, int, double, hipStream_t);
+template void batch_reset_states<double>(const BatchPlan<double>&, int, double, hipStream_t);
 
 }  // namespace ffcz

@@ -180,7 +180,10 @@ struct BatchPlan {
     DevState* gathered = nullptr;  // gathered copies of the states
     size_t nfields = 0;
     unsigned char* dmem = nullptr;
+    size_t dcap = 0;
 };
+// Builds (or rebuilds) the plan; its device tables are reused when they fit,
+// so no work that reads the previous plan may still be pending.
 template <typename R>
 void batch_plan_build(BatchPlan<R>& bp, const std::vector<const Fft3d<R>*>& fields,
                       const std::vector<IterArgs<R>>& args);
@@ -191,6 +194,9 @@ void batch_iteration(const BatchPlan<R>& bp, hipStream_t s);
 // copies every field's state into host memory `out` (asynchronous on s)
 template <typename R>
 void batch_gather_states(const BatchPlan<R>& bp, DevState* out, hipStream_t s);
+// every field's state reset for a solve of at most max_iter iterations (one launch)
+template <typename R>
+void batch_reset_states(const BatchPlan<R>& bp, int max_iter, double tol, hipStream_t s);
 
 // ---- FP64 certification ----------------------------------------------------
 template <typename T>
@@ -227,9 +233,11 @@ struct DevBuffer {
 };
 // The certification of many fields with one launch per pass and axis length
 // (float fields with length-specialised kernels; the others run certify()
-// one by one on the same stream).  Results land in each field's stats.
+// one by one on the same stream).  Results land in each field's stats and,
+// when host_out is set (pinned, one entry per field), are gathered and copied
+// there behind the passes.
 template <typename T>
 void certify_batch(const std::vector<const Fft3d<double>*>& f64s, const std::vector<CertArgs<T>>& args,
-                   DevBuffer& tables, hipStream_t s);
+                   DevBuffer& tables, hipStream_t s, CertStats* host_out = nullptr);
 
 }  // namespace ffcz

@@ -136,9 +136,9 @@ void Solver<T>::accumulators_clobbered() {
 }
 
 template <typename T>
-void Solver<T>::materialize_ds() const {
+void Solver<T>::materialize_ds(hipStream_t s) const {
     if (!ds_fresh_) return;
-    hipLaunchKernelGGL((k_zero<T>), dim3(gridn(f_.N)), dim3(256), 0, s_, ds_, f_.N);
+    hipLaunchKernelGGL((k_zero<T>), dim3(gridn(f_.N)), dim3(256), 0, s ? s : s_, ds_, f_.N);
     FFCZ_HIP(hipGetLastError());
     ds_fresh_ = false;
 }
@@ -251,7 +251,7 @@ IterArgs<T> Solver<T>::iter_args() const {
 }
 
 template <typename T>
-void Solver<T>::prepare_solve(int mode, const T* e0, int max_iter, T tol, hipStream_t s) {
+void Solver<T>::prepare_solve(int mode, const T* e0, int max_iter, T tol, hipStream_t s, bool reset) {
     FFCZ_REQUIRE(initialized_, ERR_STATE, "Solver not initialized");  // impl:778-781
     FFCZ_REQUIRE(mode == init_mode_, ERR_STATE,
                  "solve mode does not match the initialize* call (direct<->initialize, abs<->initialize_all_abs, "
@@ -269,7 +269,7 @@ void Solver<T>::prepare_solve(int mode, const T* e0, int max_iter, T tol, hipStr
         (void)hipGetLastError();  // a host pointer is not an error here
     if (!e_src_) FFCZ_HIP(hipMemcpyAsync(e0_, e0, f_.N * sizeof(T), hipMemcpyDefault, s));
     if (max_iter <= 0) FFCZ_HIP(hipMemcpyAsync(e_, e0_, f_.N * sizeof(T), hipMemcpyDeviceToDevice, s));
-    reset_state<T>(st_, max_iter, (double)tol, s);
+    if (reset) reset_state<T>(st_, max_iter, (double)tol, s);
 }
 
 template <typename T>
@@ -342,7 +342,9 @@ long Solver<T>::pass_timing(float* ms6) const {
 template <typename T>
 void Solver<T>::finish_solve(hipStream_t s, const DevState* known) {
     const IterArgs<T> a = iter_args();
-    epilogue_clamp(f_, a, s);
+    // the clamp applies after an unconverged loop only: a known final state
+    // that says otherwise needs no launch
+    if (!known || (known->done && !known->converged && known->iter > 0)) epilogue_clamp(f_, a, s);
     DevState h{};
     if (known) {
         h = *known;  // the epilogue leaves the state as it is
@@ -434,11 +436,36 @@ void Solver<T>::extract() {
 }
 
 template <typename T>
+void Solver<T>::extract_segments(std::vector<ExtractSeg<T>>& out, hipStream_t s) {
+    ensure_extract_buffers();
+    sync_extract();
+    materialize_ds(s);
+    QuantDesc qf = freq_quant(), qs = spatial_quant();
+    qf.dminmax = qs.dminmax = nullptr;
+    out.push_back(ExtractSeg<T>{df_, 2 * f_.F, qf, (unsigned int*)fpack_, fcomp_, fquant_, &xglob_[0], 0u});
+    out.push_back(ExtractSeg<T>{ds_, f_.N, qs, (unsigned int*)spack_, scomp_, squant_, &xglob_[1], 0u});
+    extracted_ = false;  // until the results are taken over
+}
+
+template <typename T>
+void Solver<T>::extract_adopt(const ExtractGlobals& hf, const ExtractGlobals& hs) {
+    xhost_[0] = hf;
+    xhost_[1] = hs;
+    xpending_ = false;
+    extracted_ = true;
+    take_extract_results(hf, hs);
+}
+
+template <typename T>
 void Solver<T>::sync_extract() const {
     if (!xpending_) return;
     FFCZ_HIP(hipStreamSynchronize(s_));
     xpending_ = false;
-    const ExtractGlobals hf = xhost_[0], hs = xhost_[1];
+    take_extract_results(xhost_[0], xhost_[1]);
+}
+
+template <typename T>
+void Solver<T>::take_extract_results(const ExtractGlobals& hf, const ExtractGlobals& hs) const {
     xf_ = ExtractResult{(size_t)hf.count, hf.vmin, hf.vmax};
     xs_ = ExtractResult{(size_t)hs.count, hs.vmin, hs.vmax};
     if ((hf.overflow | hs.overflow) & 2u) {
@@ -542,6 +569,13 @@ template <typename T>
 CertResult Solver<T>::certify_collect(bool quantized) {
     FFCZ_REQUIRE(chost_ != nullptr, ERR_STATE, "no certification enqueued");
     FFCZ_HIP(hipStreamSynchronize(s_));
+    return certify_read(quantized);
+}
+
+template <typename T>
+CertResult Solver<T>::certify_adopt(bool quantized, const CertStats& st) {
+    FFCZ_REQUIRE(chost_ != nullptr, ERR_STATE, "no certification enqueued");
+    chost_[quantized ? 1 : 0] = st;
     return certify_read(quantized);
 }
 

@@ -44,9 +44,10 @@ class Solver {
     // Projection loop only (impl:791-808 / 835-852 / 879-896).
     void solve_loop(int mode, const T* e0, int max_iter, T tol);
     // the loop in pieces, for a caller that launches the iterations itself
-    // (batches): checks + initial copy + state reset, then the epilogue and
-    // the read-back of the final state, both on stream s
-    void prepare_solve(int mode, const T* e0, int max_iter, T tol, hipStream_t s);
+    // (batches): checks + initial copy + state reset (left to the caller
+    // when reset is false), then the epilogue and the read-back of the final
+    // state, both on stream s
+    void prepare_solve(int mode, const T* e0, int max_iter, T tol, hipStream_t s, bool reset = true);
     // known: the final state when the caller has already read it (no extra wait)
     void finish_solve(hipStream_t s, const DevState* known = nullptr);
     // iterations run outside a solve (timing) leave the accumulators undefined
@@ -57,6 +58,11 @@ class Solver {
         extract();
     }
     void extract();
+    // A batch's extraction: the field's two edit arrays (frequency, then
+    // spatial) as segments of one launch on stream s, and that launch's
+    // results taken over afterwards.
+    void extract_segments(std::vector<ExtractSeg<T>>& out, hipStream_t s);
+    void extract_adopt(const ExtractGlobals& hf, const ExtractGlobals& hs);
     void reset_edit_accumulators();
     CertResult certify(bool quantized);
     // both certifications (unquantised [0], quantised [1]) in one paired
@@ -116,6 +122,8 @@ class Solver {
     const Fft3d<double>& cert_fft() const { return *f64_; }
     void certify_copy_stats(bool quantized, hipStream_t s);
     CertResult certify_read(bool quantized);
+    // a batch's certification: the stats its launch gathered for this field
+    CertResult certify_adopt(bool quantized, const CertStats& st);
     // Tile variant of each pass kernel (iteration and FP64 certification).
     // autotune times every candidate on this field's buffers (after
     // initialize*; it clobbers the iterate and resets the edit accumulators)
@@ -207,7 +215,8 @@ class Solver {
     // lazily, or stored whole by the next solve's first spatial clamp
     mutable bool ds_fresh_ = false;
     const T* e_src_ = nullptr;  // device initial error read by the running solve's first iteration
-    void materialize_ds() const;
+    void materialize_ds(hipStream_t s = nullptr) const;
+    void take_extract_results(const ExtractGlobals& hf, const ExtractGlobals& hs) const;
 };
 
 }  // namespace ffcz

@@ -155,8 +155,11 @@ int ffcz_batch_solve(ffcz_batch* b, int mode, const void* const* e0, int max_ite
 int ffcz_batch_status(ffcz_batch* b, int* iterations, double* violations, int* converged, size_t cap,
                       int* launched);
 /* extract_compact_representation, and certify(quantized), of every field of
- * the batch: each field's work is queued on its own stream before any is
- * waited for.  out receives one result per field (cap >= field count). */
+ * the batch, with one launch of each kernel (extraction) or of each pass and
+ * axis length (certification) over all fields, and one wait.  A field whose
+ * quantised edits overflow keeps no extraction; the others keep theirs and
+ * the first error is returned.  out receives one result per field
+ * (cap >= field count). */
 int ffcz_batch_extract(ffcz_batch* b);
 int ffcz_batch_certify(ffcz_batch* b, int quantized, ffcz_cert* out, size_t cap);
 /* mean ms of one whole batched iteration (HIP events on the batch stream),

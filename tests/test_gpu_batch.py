@@ -182,3 +182,57 @@ def test_batch_extract_and_certify_all_match_per_field(gpu):
     assert all(c.ok for c in res[True][0]) and all(c.ok for c in res[True][1])
     for (a1, b1), (a2, b2) in zip(res[False][2], res[True][2]):
         assert np.array_equal(a1, a2) and np.array_equal(b1, b2)
+
+
+def _extraction(f):
+    return (f.get_freq_flags_pack(), f.get_spatial_flags_pack(), f.get_active_freq_edit_distances(),
+            f.get_active_spatial_edit_distances(), f.get_active_freq_quant_edit_distances(),
+            f.get_active_spatial_quant_edit_distances(), f.get_min_freq_edit(), f.get_max_freq_edit(),
+            f.get_min_spatial_edit(), f.get_max_spatial_edit())
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_batched_extraction_matches_per_field(gpu, dtype):
+    """extract_all runs every field's two edit arrays through one launch of
+    each extraction kernel (tiles of all arrays under one ticket counter):
+    flags, compact values, symbols, counts and ranges equal each field's own
+    extraction, with the reference quantiser and the step quantiser mixed in
+    one batch, and again on a second call (reused workspace)."""
+    from ffcz_amd import QUANT_REF16, QUANT_STEP
+    shapes = SHAPES
+    cases = _cases(shapes, dtype, seed0=60)
+    with ProjectionBatch(shapes, dtype=dtype) as b:
+        for k, (f, (x, e0, eps, delta)) in enumerate(zip(b.fields, cases)):
+            f.initialize_all_abs(x, eps, delta)
+            if k % 2:
+                f.set_quantizer(QUANT_STEP, float(delta) / 8, float(eps) / 8)
+        b.solve(MODE_ABS, [c[1] for c in cases], 60, 1e-7)
+        for rep in range(2):
+            b.extract_all()
+            got = [_extraction(f) for f in b.fields]
+            for f in b.fields:
+                f.extract_compact_representation()
+            ref = [_extraction(f) for f in b.fields]
+            for shape, g, r in zip(shapes, got, ref):
+                for u, v in zip(g, r):
+                    assert np.array_equal(u, v), (shape, rep)
+            assert any(len(g[2]) > 0 for g in got) and any(len(g[3]) > 0 for g in got)
+
+
+def test_batched_extraction_reports_overflow_per_field(gpu):
+    """A step too fine for 16-bit symbols in one field: extract_all raises
+    the range error and the other fields keep their results."""
+    from ffcz_amd import QUANT_STEP
+    shapes = SHAPES[:3]
+    cases = _cases(shapes, np.float32, seed0=70)
+    with ProjectionBatch(shapes, dtype=np.float32) as b:
+        for f, (x, e0, eps, delta) in zip(b.fields, cases):
+            f.initialize_all_abs(x, eps, delta)
+            f.set_margins(0.125, 0.125)  # room for the 16-bit quantiser
+        b.fields[1].set_quantizer(QUANT_STEP, float(cases[1][3]) * 1e-9, float(cases[1][2]) * 1e-9)
+        b.solve(MODE_ABS, [c[1] for c in cases], 200, 1e-7)
+        with pytest.raises(FFCzError):
+            b.extract_all()
+        assert b.fields[0].certify(True).ok and b.fields[2].certify(True).ok
+        with pytest.raises(FFCzError):
+            b.fields[1].certify(True)
