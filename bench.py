@@ -66,15 +66,18 @@ def parse():
     return p.parse_args()
 
 
-def kernel_bytes(shape, s):
+def kernel_bytes(shape, s, fresh_frac=0.0):
     """Compulsory HBM bytes per launch of the five iteration kernels (the
-    accumulators are read/written only where a clamp fires)."""
+    accumulators are read/written only where a clamp fires).  The z forward
+    pass of the second iteration applies the first spatial clamp since the
+    accumulators were reset and stores the spatial edits whole (N*s bytes
+    more); fresh_frac is the share of the timed z forward launches that did."""
     nx, ny, nz = shape
     N = nx * ny * nz
     F = nx * ny * (nz // 2 + 1)
     cs = 2 * s
     return {
-        "z_fwd": N * s + F * cs,
+        "z_fwd": N * s + F * cs + int(round(fresh_frac * N * s)),
         "y_fwd": 2 * F * cs,
         "x_fused": 2 * F * cs,
         "y_inv": 2 * F * cs,
@@ -323,7 +326,8 @@ def main():
         return {k: float(f"{v:.3e}") for k, v in out.items()}
 
     names = ["z_fwd", "y_fwd", "x_fused", "y_inv", "z_inv"]
-    kb = kernel_bytes(shape, s_in)
+    # per iteration: the method's count, four real arrays and four half spectra
+    # (independent of how many passes the FFT takes)
     C = 2 * (shape[0] * shape[1] * (shape[2] // 2 + 1)) * s_in
     B_iter = 4 * N * s_in + 4 * C
 
@@ -353,6 +357,7 @@ def main():
             torch.cuda.synchronize()
             elapsed = time.perf_counter() - t0
         ms6, n_timed = solver.pass_timing()
+        n_zfwd = solver.pass_counts()[0]
         solver.set_pass_timing(False)
         iters, viol, conv = solver.status()
         ok = all(c1.ok and c2.ok for c1, c2 in certs)
@@ -361,9 +366,13 @@ def main():
               f"certified={ok}; passes timed over {n_timed} iterations", file=sys.stderr, flush=True)
         ms, t_iter = ms6[:5], sum(ms6) * 1e-3
         dom = int(np.argmax(ms))
+        # every solve's second iteration is timed (only the first launch after
+        # each host check is left out) and its z forward pass stores ds whole
+        fresh = min(1.0, a.steps / n_zfwd) if (iters >= 2 and n_zfwd > 0) else 0.0
         return {
             "variant": variant, "value": value, "elapsed": elapsed, "iterations": iters, "converged": conv,
             "certified": ok, "ms": ms, "t_iter": t_iter, "dom": dom, "fft_error": fft_error(variant),
+            "fresh_frac": fresh,
             "tile_vars": tuned.get(variant),
             "cert_excess": {"spatial": max(max(c1.spatial_max_excess, c2.spatial_max_excess) for c1, c2 in certs),
                             "freq": max(max(c1.freq_max_excess, c2.freq_max_excess) for c1, c2 in certs)},
@@ -374,6 +383,7 @@ def main():
     best = max(results, key=lambda r: (r["value"], r["variant"] == "valu-fp32"))
     ms, t_iter, dom = best["ms"], best["t_iter"], best["dom"]
     dname = names[dom]
+    kb = kernel_bytes(shape, s_in, best["fresh_frac"])
     achieved = kb[dname] / (ms[dom] * 1e-3) / 1e9
     traffic = read_traffic(dname, a.config)
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",

@@ -112,6 +112,7 @@ SIGNATURES = {
     "ffcz_time_iteration": [_h, _i, C.POINTER(_f)],
     "ffcz_set_pass_timing": [_h, _i],
     "ffcz_get_pass_timing": [_h, C.POINTER(_f), C.POINTER(C.c_long)],
+    "ffcz_get_pass_counts": [_h, C.POINTER(C.c_long)],
     "ffcz_huffman_zstd_compress_u16": [_vp, _sz, _i, C.POINTER(_h)],
     "ffcz_huffman_zstd_compress_u8": [_vp, _sz, _i, C.POINTER(_h)],
     "ffcz_huffman_encode_u16": [_vp, _sz, C.POINTER(_h)],

@@ -565,6 +565,12 @@ int ffcz_get_pass_timing(ffcz_solver* h, float* ms6, long* iterations) {
         });
     });
 }
+int ffcz_get_pass_counts(ffcz_solver* h, long* n6) {
+    return guard([&] {
+        FFCZ_REQUIRE(n6 != nullptr, ERR_INVALID, "null output");
+        with(h, [&](auto& s) { s.pass_counts(n6); });
+    });
+}
 int ffcz_time_iteration(ffcz_solver* h, int reps, float* ms5) {
     return guard([&] {
         FFCZ_REQUIRE(ms5 != nullptr, ERR_INVALID, "null output");

@@ -43,13 +43,23 @@ static FftPlan make_plan(int L) {
 }
 
 template <typename R>
-static cplx<R>* upload_table(int L, int count) {
+static cplx<R>* upload_table(int L, int count, bool half_mirror = false) {
     std::vector<cplx<R>> h((size_t)count);
     const double two_pi = 6.283185307179586476925286766559;
     for (int m = 0; m < count; ++m) {
         double a = -two_pi * (double)m / (double)L;
         h[m].x = (R)std::cos(a);
         h[m].y = (R)std::sin(a);
+    }
+    if (half_mirror) {
+        // W_L^(M-k) = -conj(W_L^k) with M = L/2: the upper half is stored as
+        // the exact mirror of the lower, so kernels that keep only k <= M/2
+        // and derive the rest see the same values as the table
+        const int M = L / 2;
+        for (int m = M / 2 + 1; m < count; ++m) {
+            h[m].x = -h[M - m].x;
+            h[m].y = h[M - m].y;
+        }
     }
     cplx<R>* d = nullptr;
     FFCZ_HIP(hipMalloc(&d, sizeof(cplx<R>) * (size_t)count));
@@ -93,7 +103,7 @@ void Fft3d<R>::create(size_t nx_, size_t ny_, size_t nz_) {
     Wx = pickW(nx);
     FFCZ_REQUIRE(Wy >= 1 && Wx >= 1, ERR_INVALID, "x/y axis too long for one LDS tile");
     d_tz = upload_table<R>(Lz, Lz);
-    d_tz2 = upload_table<R>((int)nz, (int)(nz / 2 + 1));
+    d_tz2 = upload_table<R>((int)nz, (int)(nz / 2 + 1), nz % 2 == 0);
     d_ty = upload_table<R>((int)ny, (int)ny);
     d_tx = upload_table<R>((int)nx, (int)nx);
 }

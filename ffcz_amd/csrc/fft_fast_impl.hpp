@@ -32,6 +32,27 @@ constexpr int z_rows(int LZ, int V = 0) {
     return p;
 }
 
+// The vector-ALU z tiles are row-major with a pad slot after every 8
+// elements: a radix-8 stage's stores (positions 8t + r of thread t) and
+// loads (t + L/8 r) then fall on distinct LDS banks across a row's threads.
+// The stride holds X[0..LZ] (the C2R input's Nyquist bin included).
+constexpr int zrows8_stride(int LZ) { return LZ + LZ / 8 + 1; }
+__device__ __forceinline__ int zrows8_idx(int n, int p, int rs) { return p * rs + n + (n >> 3); }
+
+// W_nz^k for k <= M = nz/2 from the stored half k <= M/2:
+// W^(M-k) = -conj(W^k) (the host table is mirrored the same way)
+template <typename R, int M>
+__device__ __forceinline__ cplx<R> half_twiddle(const cplx<R>* tw2, int k) {
+    if (k <= M / 2) return tw2[k];
+    const cplx<R> w = tw2[M - k];
+    return mk(-w.x, w.y);
+}
+
+// the z passes of half length 256 in the MFMA variants: both radix-16 stages
+// on row batches over a row-major tile (mx::rows_transform)
+template <int PREC, int LZ>
+constexpr bool ZROWS = PREC != PREC_VALU && mx::n16(LZ) == 2 && mx::rest_of(LZ) == 1;
+
 // ---------------------------------------------------------------------------
 // forward z (R2C, nz even): loads fuse the deferred spatial clamp (ITER) or
 // the FP64 reconstruction and spatial check (CERT); the half-length complex
@@ -42,13 +63,21 @@ template <typename R, typename T, int LZ, int MODE, int PREC, int V>
 __device__ __forceinline__ void zfwd_body(const ZArgs<R, T>& a, uint32_t blk) {
     constexpr int TP = z_tp(LZ), P = z_rows(LZ, V), NT = TP * P;
     constexpr int NZ = 2 * LZ, M = LZ, NH = M + 1, SN = P + 1;
+    // tile layouts: ZL_ROWS8 (vector ALU), mx rows (the 16 x 16 MFMA
+    // transform), columns n*SN + p (other MFMA lengths)
+    constexpr bool ROWS = ZROWS<PREC, LZ>, COLS = PREC != PREC_VALU && !ROWS;
+    constexpr int RS = ROWS ? mx::rows_stride<LZ>() : zrows8_stride(LZ);
     __shared__ cplx<R> tw[LZ];
-    __shared__ cplx<R> tw2[NH];
-    __shared__ cplx<R> buf[LZ * SN];
+    __shared__ cplx<R> tw2[M / 2 + 1];
+    __shared__ cplx<R> buf[COLS ? LZ * SN : P * RS];
+    auto bidx = [&](int n, int pp) {
+        return COLS ? n * SN + pp : ROWS ? mx::rows_idx(n, pp, RS) : zrows8_idx(n, pp, RS);
+    };
+    auto tw2v = [&](int k) { return half_twiddle<R, M>(tw2, k); };
     if (MODE == Z_ITER && a.st->done) return;
     const int tid = threadIdx.x;
     for (int i = tid; i < LZ; i += NT) tw[i] = a.tz[i];
-    for (int i = tid; i < NH; i += NT) tw2[i] = a.tz2[i];
+    for (int i = tid; i <= M / 2; i += NT) tw2[i] = a.tz2[i];
     __syncthreads();
     const int t = tid % TP, p = tid / TP;
     const size_t row0 = (size_t)blk * P;
@@ -61,9 +90,11 @@ __device__ __forceinline__ void zfwd_body(const ZArgs<R, T>& a, uint32_t blk) {
     R* ecopy = (MODE == Z_ITER && !clamp && a.e_init) ? a.e_copy : nullptr;
     unsigned long long nviol = 0;
     double vmax = 0.0;
-    auto gld = [&](int n) -> cplx<R> {
-        if (!rv) return mk(R(0), R(0));
-        const size_t gi = row * (size_t)NZ + 2 * (size_t)n;
+    // element pair n of row rw (ok: rw is in range), with the pass's fused
+    // element-wise work
+    auto gldr = [&](int n, size_t rw, bool ok) -> cplx<R> {
+        if (!ok) return mk(R(0), R(0));
+        const size_t gi = rw * (size_t)NZ + 2 * (size_t)n;
         if constexpr (MODE == Z_ITER) {
             // one 2-element vector load of the iterate (gi is even)
             cplx<R> v = *reinterpret_cast<const cplx<R>*>(esrc + gi);
@@ -104,12 +135,26 @@ __device__ __forceinline__ void zfwd_body(const ZArgs<R, T>& a, uint32_t blk) {
             return mk((R)d0, (R)d1);
         }
     };
-    auto lld = [&](int n) { return buf[n * SN + p]; };
-    auto lst = [&](int n, cplx<R> v) { buf[n * SN + p] = v; };
+    auto gld = [&](int n) { return gldr(n, row, rv); };
+    auto lld = [&](int n) { return buf[bidx(n, p)]; };
+    auto lst = [&](int n, cplx<R> v) { buf[bidx(n, p)] = v; };
     if constexpr (PREC == PREC_VALU) {
         pen::transform<R, LZ, TP, false, true>(gld, lst, lld, lst, tw, t, -1);
+    } else if constexpr (ROWS) {
+        // the first stage loads 16 adjacent pairs of a row per MFMA batch
+        auto gldp = [&](int n, int pp) {
+            const size_t rw = row0 + pp;
+            return gldr(n, rw, rw < a.rows);
+        };
+        auto bst = [&](int n, int pp, cplx<float> v) { buf[mx::rows_idx(n, pp, RS)] = v; };
+        mx::rows_transform<PREC, LZ, P, NT>(gldp, bst, buf, tw, tid, -1);
     } else {
-        mx::transform<PREC, LZ, TP, SN, P, NT, false, true>(gld, lst, lld, lst, buf, tw, tid, t, -1);
+        // rows are contiguous along n: a coalesced copy into the tile first
+        for (int n = t; n < LZ; n += TP) lst(n, gld(n));
+        __syncthreads();
+        auto none_ld = [](int, int) { return mk(0.0f, 0.0f); };
+        auto none_st = [](int, int, cplx<float>) {};
+        mx::transform<PREC, LZ, TP, SN, P, NT, true, true, false>(none_ld, none_st, buf, tw, tid, p, t, -1);
     }
     if constexpr (MODE == Z_CERT) {
         block_count_atomic(nviol, &a.stats->spatial_viol);
@@ -123,11 +168,11 @@ __device__ __forceinline__ void zfwd_body(const ZArgs<R, T>& a, uint32_t blk) {
         if (r2 >= a.rows) continue;
         const int ik = (k == M) ? 0 : k;
         const int im = (k == 0) ? 0 : M - k;
-        const cplx<R> zk = buf[ik * SN + pp];
-        const cplx<R> zm = buf[im * SN + pp];
+        const cplx<R> zk = buf[bidx(ik, pp)];
+        const cplx<R> zm = buf[bidx(im, pp)];
         const cplx<R> A = mk(R(0.5) * (zk.x + zm.x), R(0.5) * (zk.y - zm.y));
         const cplx<R> B = mk(R(0.5) * (zk.y + zm.y), R(0.5) * (zm.x - zk.x));
-        a.spec[r2 * a.Mp + k] = A + cmul(tw2[k], B);
+        a.spec[r2 * a.Mp + k] = A + cmul(tw2v(k), B);
     }
 }
 
@@ -142,50 +187,58 @@ template <typename R, typename T, int LZ, int MODE, int PREC, int V>
 __device__ __forceinline__ void zinv_body(const ZArgs<R, T>& a, uint32_t blk) {
     constexpr int TP = z_tp(LZ), P = z_rows(LZ, V), NT = TP * P;
     constexpr int NZ = 2 * LZ, M = LZ, NH = M + 1, SN = P + 1, NPAIR = M / 2 + 1;
+    constexpr bool ROWS = ZROWS<PREC, LZ>, COLS = PREC != PREC_VALU && !ROWS;
+    // the row strides hold X[0..M] with the pad slots
+    constexpr int RS = ROWS ? mx::rows_stride<LZ>() : zrows8_stride(LZ);
     __shared__ cplx<R> tw[LZ];
-    __shared__ cplx<R> tw2[NH];
-    __shared__ cplx<R> buf[NH * SN];
+    __shared__ cplx<R> tw2[M / 2 + 1];
+    __shared__ cplx<R> buf[COLS ? NH * SN : P * RS];
+    auto bidx = [&](int n, int pp) {
+        return COLS ? n * SN + pp : ROWS ? mx::rows_idx(n, pp, RS) : zrows8_idx(n, pp, RS);
+    };
+    auto tw2v = [&](int k) { return half_twiddle<R, M>(tw2, k); };
     if (MODE == Z_ITER && a.st->done) return;
     const int tid = threadIdx.x;
     for (int i = tid; i < LZ; i += NT) tw[i] = a.tz[i];
-    for (int i = tid; i < NH; i += NT) tw2[i] = a.tz2[i];
+    for (int i = tid; i <= M / 2; i += NT) tw2[i] = a.tz2[i];
     const size_t row0 = (size_t)blk * P;
     // the tile's half-spectrum rows X[0..M] into LDS, each element loaded once
     for (int q = tid; q < P * NH; q += NT) {
         const int pp = q / NH;
         const int k = q - pp * NH;
         const size_t r2 = row0 + pp;
-        buf[k * SN + pp] = r2 < a.rows ? a.spec[r2 * a.Mp + k] : mk(R(0), R(0));
+        buf[bidx(k, pp)] = r2 < a.rows ? a.spec[r2 * a.Mp + k] : mk(R(0), R(0));
     }
     __syncthreads();
     // Z[n] = (X[n] + conj X[M-n]) + i W^-n (X[n] - conj X[M-n]) in place, by
     // pairs (n, M-n); the imaginary parts of the DC and Nyquist bins are dropped
     auto zpre = [&](cplx<R> xk, cplx<R> xm, int n) {
         const cplx<R> s = xk + conjg(xm);
-        const cplx<R> d = cmul(xk - conjg(xm), conjg(tw2[n]));
+        const cplx<R> d = cmul(xk - conjg(xm), conjg(tw2v(n)));
         return mk(s.x - d.y, s.y + d.x);
     };
     for (int q = tid; q < P * NPAIR; q += NT) {
         const int pp = q / NPAIR;
         const int n = q - pp * NPAIR;
         const int m = M - n;
-        cplx<R> xn = buf[n * SN + pp];
-        cplx<R> xm = buf[m * SN + pp];
+        cplx<R> xn = buf[bidx(n, pp)];
+        cplx<R> xm = buf[bidx(m, pp)];
         if (n == 0) {
             xn.y = R(0);
             xm.y = R(0);
         }
-        buf[n * SN + pp] = zpre(xn, xm, n);
-        if (m != n && m < M) buf[m * SN + pp] = zpre(xm, xn, m);
+        buf[bidx(n, pp)] = zpre(xn, xm, n);
+        if (m != n && m < M) buf[bidx(m, pp)] = zpre(xm, xn, m);
     }
     __syncthreads();
     const int t = tid % TP, p = tid / TP;
     const size_t row = row0 + p;
     const bool rv = row < a.rows;
     R vm = R(0);
-    auto gst = [&](int n, cplx<R> z) {
-        if (!rv) return;
-        const size_t gi = row * (size_t)NZ + 2 * (size_t)n;
+    // output pair n of row rw (ok: rw is in range); ITER keeps the violation
+    auto gstr = [&](int n, size_t rw, bool ok, cplx<R> z) {
+        if (!ok) return;
+        const size_t gi = rw * (size_t)NZ + 2 * (size_t)n;
         const R v0 = z.x * a.scale, v1 = z.y * a.scale;
         if constexpr (MODE == Z_ITER) {
             *reinterpret_cast<cplx<R>*>(a.e + gi) = mk(v0, v1);
@@ -196,12 +249,30 @@ __device__ __forceinline__ void zinv_body(const ZArgs<R, T>& a, uint32_t blk) {
             *reinterpret_cast<cplx<R>*>(a.out + gi) = mk(v0, v1);
         }
     };
-    auto lld = [&](int n) { return buf[n * SN + p]; };
-    auto lst = [&](int n, cplx<R> v) { buf[n * SN + p] = v; };
+    auto gst = [&](int n, cplx<R> z) { gstr(n, row, rv, z); };
+    auto lld = [&](int n) { return buf[bidx(n, p)]; };
+    auto lst = [&](int n, cplx<R> v) { buf[bidx(n, p)] = v; };
     if constexpr (PREC == PREC_VALU) {
         pen::transform<R, LZ, TP, true, false>(lld, gst, lld, lst, tw, t, +1);
+    } else if constexpr (ROWS) {
+        // the second stage stores 16 adjacent pairs of a row per MFMA batch
+        auto bld = [&](int n, int pp) { return buf[mx::rows_idx(n, pp, RS)]; };
+        auto gstp = [&](int n, int pp, cplx<float> z) {
+            const size_t rw = row0 + pp;
+            gstr(n, rw, rw < a.rows, z);
+        };
+        mx::rows_transform<PREC, LZ, P, NT>(bld, gstp, buf, tw, tid, +1);
     } else {
-        mx::transform<PREC, LZ, TP, SN, P, NT, true, false>(lld, gst, lld, lst, buf, tw, tid, t, +1);
+        // a vector-ALU last stage stores the thread's own row (coalesced); a
+        // radix-16 last stage leaves the tile in LDS for a coalesced copy out
+        constexpr bool COPY_OUT = mx::rest_of(LZ) == 1;
+        auto none_ld = [](int, int) { return mk(0.0f, 0.0f); };
+        auto gstp = [&](int n, int, cplx<float> z) { gst(n, z); };
+        mx::transform<PREC, LZ, TP, SN, P, NT, true, COPY_OUT, false>(none_ld, gstp, buf, tw, tid, p, t, +1);
+        if constexpr (COPY_OUT) {
+            __syncthreads();
+            for (int n = t; n < LZ; n += TP) gst(n, lld(n));
+        }
     }
     if constexpr (MODE == Z_ITER) block_max_store<R, NT>(vm, &a.vpart[a.row_off / P + blk]);
 }
@@ -239,22 +310,38 @@ __device__ __forceinline__ void strided_body(const SArgs<R, T>& a, uint32_t blk)
     };
     auto lld = [&](int n) { return buf[n * W + p]; };
     auto lst = [&](int n, cplx<R> v) { buf[n * W + p] = v; };
+    // element n of column pp of the tile (the radix-16 stages serve
+    // butterflies of any column of the tile, 16 adjacent columns at a time)
+    auto gldp = [&](int n, int pp) -> cplx<R> {
+        const int c = ct * W + pp;
+        if (c >= a.ncols) return mk(R(0), R(0));
+        return a.data[o * a.ob + c + (size_t)n * a.es];
+    };
+    auto gstp = [&](int n, int pp, cplx<R> v) {
+        const int c = ct * W + pp;
+        if (c < a.ncols) a.data[o * a.ob + c + (size_t)n * a.es] = v;
+    };
     if constexpr (MODE == S_PLAIN || MODE == S_LOAD_DF) {
         const int sg = MODE == S_LOAD_DF ? +1 : a.sign;
         if constexpr (PREC == PREC_VALU) pen::transform<R, L, TP, false, false>(gld, gst, lld, lst, tw, t, sg);
-        else mx::transform<PREC, L, TP, W, W, NT, false, false>(gld, gst, lld, lst, buf, tw, tid, t, sg);
+        else mx::transform<PREC, L, TP, W, W, NT, false, false, false>(gldp, gstp, buf, tw, tid, p, t, sg);
     } else if constexpr (MODE == S_CLAMP) {
         bool fired = false;
         auto cst = [&](int n, cplx<R> v) {
             if (valid) fired |= freq_clamp_bin(v, (size_t)n * a.cs_n + o * a.cs_o + col, a);
             buf[n * W + p] = v;
         };
+        auto cstp = [&](int n, int pp, cplx<R> v) {
+            const int c = ct * W + pp;
+            if (c < a.ncols) fired |= freq_clamp_bin(v, (size_t)n * a.cs_n + o * a.cs_o + c, a);
+            buf[n * W + pp] = v;
+        };
         if constexpr (PREC == PREC_VALU) pen::transform<R, L, TP, false, true>(gld, cst, lld, lst, tw, t, -1);
-        else mx::transform<PREC, L, TP, W, W, NT, false, false>(gld, cst, lld, lst, buf, tw, tid, t, -1);
+        else mx::transform<PREC, L, TP, W, W, NT, false, false, true>(gldp, cstp, buf, tw, tid, p, t, -1);
         if (fired) a.st->freq_clamped = 1u;
         __syncthreads();
         if constexpr (PREC == PREC_VALU) pen::transform<R, L, TP, true, false>(lld, gst, lld, lst, tw, t, +1);
-        else mx::transform<PREC, L, TP, W, W, NT, true, false>(lld, gst, lld, lst, buf, tw, tid, t, +1);
+        else mx::transform<PREC, L, TP, W, W, NT, true, false, false>(gldp, gstp, buf, tw, tid, p, t, +1);
     } else {
         static_assert(PREC == PREC_VALU, "certification runs on the vector ALU");  // S_CERT_CHECK
         unsigned long long nv = 0;
@@ -299,12 +386,13 @@ template <typename R, typename T, int LZ, int MODE, int PREC = PREC_VALU, int V 
 __global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ, V)) void k2_zinv(ZArgs<R, T> a) {
     zinv_body<R, T, LZ, MODE, PREC, V>(a, blockIdx.x);
 }
-// The plain MFMA variants ask for 8 waves per SIMD (at most 64 VGPRs): two
-// 1024-thread tiles per CU, like the vector-ALU kernels.  (The fused clamp
-// kernel would spill under that bound.)
+// The MFMA variants ask for 8 waves per SIMD (at most 64 VGPRs): two
+// 1024-thread tiles per CU, like the vector-ALU kernels.  The fused clamp
+// kernel spills a few registers under that bound and still runs faster than
+// at one tile per CU (0.41 against 0.53 ms at 512^3).
 template <typename R, typename T, int L, int MODE, int PREC = PREC_VALU, int V = 0>
 __global__ __launch_bounds__(s_tp<R>(L) * s_width<R>(V),
-                             (PREC != PREC_VALU && MODE == S_PLAIN) ? 8 : 1) void k2_strided(SArgs<R, T> a) {
+                             (PREC != PREC_VALU) ? 8 : 1) void k2_strided(SArgs<R, T> a) {
     strided_body<R, T, L, MODE, PREC, V>(a, blockIdx.x);
 }
 // Two forward z passes over the same rows in one workgroup (the paired

@@ -13,7 +13,10 @@
 // the device): lane l holds A[l%16][8*(l/16) + 0..7], B[8*(l/16) + 0..7][l%16]
 // and D[4*(l/16) + 0..3][l%16].  With the interleaved K order above, lane l
 // owns inputs r = 4*(l/16) .. +3 of butterfly l%16, so each complex input is
-// read from LDS exactly once.
+// read from LDS exactly once.  The same registers are the lane's slices of
+// B^T (as the first operand) and A^T (as the second), so the instruction
+// computes Y^T with lane l receiving two whole outputs of butterfly l%16,
+// stored to LDS as complex values (16 lanes write 128 contiguous bytes).
 //
 // Precision variants (FP32 accumulation throughout):
 //   PREC_BF16X3  x = hi + lo, B = Bhi + Blo in bfloat16; Y = hi*Bhi + lo*Bhi + hi*Blo
@@ -138,17 +141,25 @@ __device__ __forceinline__ Bmat<PREC> dft16_operand(int lane) {
     return b;
 }
 
-// One in-place radix-16 Stockham stage (sub-transform length NS before the
-// stage) over the tile buf[n*SW + p], p < NPEN, n < L, run by NT threads.
-// The caller barriers before (inputs visible) and after (outputs visible).
-template <int PREC, int L, int NS, int SW, int NPEN, int NT>
-__device__ __forceinline__ void r16_stage(cplx<float>* buf, const cplx<float>* __restrict__ tw, int tid, bool inv) {
+// One radix-16 Stockham stage (sub-transform length NS before the stage) over
+// the tile's NPEN pencils of length L, run by NT threads.  The caller
+// barriers before (inputs visible) and after (outputs visible).
+// ld(n, p) reads element n of pencil p (from LDS or straight from HBM),
+// st(n, p, v) stores output n of pencil p.  The 16 butterflies of an MFMA
+// batch are 16 adjacent pencils at one position (ROWS false: pencils are the
+// tile's columns, adjacent in memory) or 16 adjacent positions of one pencil
+// (ROWS true: pencils are rows, contiguous in memory).  The stage barriers
+// between its loads and its stores, so ld and st may address the same buffer.
+template <int PREC, int L, int NS, int NPEN, int NT, bool ROWS, class LD, class ST>
+__device__ __forceinline__ void r16_stage(const LD& ld, const ST& st, const cplx<float>* __restrict__ tw, int tid,
+                                          bool inv) {
     using O = Op<PREC>;
     using V = typename O::V;
     constexpr int LR = L / 16;
     constexpr int TWS = L / (NS * 16);
     constexpr int NBF = NPEN * LR;
     static_assert(NBF % 16 == 0, "butterflies must fill whole MFMA batches");
+    static_assert(!ROWS || LR % 16 == 0, "row batches need whole groups of 16 butterflies per row");
     static_assert(NT % 64 == 0, "MFMA stages need whole waves");
     constexpr int NQ = NBF / 16;
     constexpr int NW = NT / 64;
@@ -163,13 +174,13 @@ __device__ __forceinline__ void r16_stage(cplx<float>* buf, const cplx<float>* _
         unsc[qq] = 1.0f;
         if ((NQ % NW) != 0 && q >= NQ) continue;  // uniform per wave
         const int b = q * 16 + i;
-        const int p = b % NPEN, j = b / NPEN;
+        const int p = ROWS ? b / LR : b % NPEN, j = ROWS ? b % LR : b / NPEN;
         const int k = j % NS;
         float v[8];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int r = 4 * g + u;
-            cplx<float> x = buf[(j + r * LR) * SW + p];
+            cplx<float> x = ld(j + r * LR, p);
             if (inv) x.y = -x.y;
             if constexpr (NS > 1) {
                 if (r > 0) x = cmul(x, tw[k * r * TWS]);
@@ -207,77 +218,96 @@ __device__ __forceinline__ void r16_stage(cplx<float>* buf, const cplx<float>* _
     bm.hi[1] = bt[1 * 64 + lane];
     bm.lo[0] = bt[2 * 64 + lane];
     bm.lo[1] = bt[3 * 64 + lane];
-    float* fb = reinterpret_cast<float*>(buf);
-    const int col = lane & 15, cp = col & 1, m0 = col >> 1;
 #pragma unroll
     for (int qq = 0; qq < QPW; ++qq) {
         const int q = wave + qq * NW;
         if ((NQ % NW) != 0 && q >= NQ) continue;
+        // Y^T = B^T A^T: the lane's DFT slice is its slice of the first
+        // operand B^T and its butterfly slice that of A^T, so the product's
+        // lane layout gives lane l the outputs 2(l/16), 2(l/16)+1 (+8 jb) of
+        // its own butterfly l%16 as whole complex values (D^T rows 4(l/16)..+3
+        // = Re, Im, Re, Im)
         f32x4 d[2];
 #pragma unroll
         for (int jb = 0; jb < 2; ++jb) {
             f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
-            c = O::mma(alo[qq], bm.hi[jb], c);
-            c = O::mma(ahi[qq], bm.lo[jb], c);
-            d[jb] = O::mma(ahi[qq], bm.hi[jb], c);
+            c = O::mma(bm.hi[jb], alo[qq], c);
+            c = O::mma(bm.lo[jb], ahi[qq], c);
+            d[jb] = O::mma(bm.hi[jb], ahi[qq], c);
         }
+        const float us = PREC == PREC_FP16X3 ? unsc[qq] : 1.0f;  // this lane's butterfly's scale
+        const int b = q * 16 + i;
+        const int p = ROWS ? b / LR : b % NPEN, j = ROWS ? b % LR : b / NPEN;
+        const int k = j % NS;
+        const int base = (j - k) * 16 + k;
 #pragma unroll
-        for (int rr = 0; rr < 4; ++rr) {
-            const int ib = 4 * g + rr;
-            float us = 1.0f;
-            if constexpr (PREC == PREC_FP16X3) us = __shfl(unsc[qq], ib);
-            const int b = q * 16 + ib;
-            const int p = b % NPEN, j = b / NPEN;
-            const int k = j % NS;
-            const int base = (j - k) * 16 + k;
+        for (int jb = 0; jb < 2; ++jb) {
 #pragma unroll
-            for (int jb = 0; jb < 2; ++jb) {
-                const int m = m0 + 8 * jb;
-                float val = d[jb][rr] * us;
-                if (inv && cp) val = -val;
-                fb[2 * ((base + m * NS) * SW + p) + cp] = val;
+            for (int h = 0; h < 2; ++h) {
+                const int m = 2 * g + h + 8 * jb;
+                cplx<float> v = mk(d[jb][2 * h] * us, d[jb][2 * h + 1] * us);
+                if (inv) v.y = -v.y;
+                st(base + m * NS, p, v);
             }
         }
     }
 }
 
-template <typename R, int L, int TP, int REST, int S, bool FIRST_IN_BUF, class GLD, class LLD, class LST>
-__device__ __forceinline__ void valu_stages(const GLD& gld, const LLD& lld, const LST& lst,
-                                            const cplx<R>* __restrict__ tw, int t, int sign) {
+// The stages after the radix-16 ones: Stockham stages of the remaining
+// factors REST on the vector ALU, with sub-transform lengths NSM (the product
+// of the radix-16 stages before them) times their own.  The last one stores
+// through gst (GST_LDS: gst itself writes buf, so the stage barriers between
+// its reads and its writes); OUT_BUF: the last one writes buf.
+template <int L, int TP, int REST, int NSM, int S, bool OUT_BUF, bool GST_LDS, class GST, class LLD, class LST>
+__device__ __forceinline__ void valu_tail(const GST& gst, const LLD& lld, const LST& lst,
+                                          const cplx<float>* __restrict__ tw, int t, int sign) {
     constexpr int NSTR = pen::nstages(REST);
     if constexpr (S < NSTR) {
-        constexpr int RAD = pen::radix(REST, S), NS = pen::nsub(REST, S);
-        if constexpr (S == 0) pen::stage_rn<R, L, TP, RAD, NS, FIRST_IN_BUF>(gld, lst, tw, t, sign);
-        else pen::stage_rn<R, L, TP, RAD, NS, true>(lld, lst, tw, t, sign);
-        __syncthreads();
-        valu_stages<R, L, TP, REST, S + 1, FIRST_IN_BUF>(gld, lld, lst, tw, t, sign);
+        constexpr int RAD = pen::radix(REST, S), NS = NSM * pen::nsub(REST, S);
+        if constexpr (S == NSTR - 1) {
+            if constexpr (OUT_BUF) pen::stage_rn<float, L, TP, RAD, NS, true>(lld, lst, tw, t, sign);
+            else pen::stage_rn<float, L, TP, RAD, NS, GST_LDS>(lld, gst, tw, t, sign);
+        } else {
+            pen::stage_rn<float, L, TP, RAD, NS, true>(lld, lst, tw, t, sign);
+            __syncthreads();
+            valu_tail<L, TP, REST, NSM, S + 1, OUT_BUF, GST_LDS>(gst, lld, lst, tw, t, sign);
+        }
     }
 }
 
-template <int PREC, int L, int REST, int S, int SW, int NPEN, int NT>
-__device__ __forceinline__ void mfma_stages(cplx<float>* buf, const cplx<float>* __restrict__ tw, int tid, bool inv) {
-    if constexpr (S < n16(L)) {
-        r16_stage<PREC, L, REST * pow16(S), SW, NPEN, NT>(buf, tw, tid, inv);
-        __syncthreads();
-        mfma_stages<PREC, L, REST, S + 1, SW, NPEN, NT>(buf, tw, tid, inv);
+template <int PREC, int L, int S, int SW, int NPEN, int NT, bool IN_BUF, bool OUT_BUF, class GLDP, class GSTP>
+__device__ __forceinline__ void mfma_stages(cplx<float>* buf, const cplx<float>* __restrict__ tw, int tid, bool inv,
+                                            const GLDP& gldp, const GSTP& gstp) {
+    constexpr int NM = n16(L);
+    if constexpr (S < NM) {
+        constexpr bool LAST = S == NM - 1 && rest_of(L) == 1;
+        auto bld = [&](int n, int p) { return buf[n * SW + p]; };
+        auto bst = [&](int n, int p, cplx<float> v) { buf[n * SW + p] = v; };
+        constexpr bool FROM_G = S == 0 && !IN_BUF, TO_G = LAST && !OUT_BUF;
+        if constexpr (FROM_G && TO_G) r16_stage<PREC, L, pow16(S), NPEN, NT, false>(gldp, gstp, tw, tid, inv);
+        else if constexpr (FROM_G) r16_stage<PREC, L, pow16(S), NPEN, NT, false>(gldp, bst, tw, tid, inv);
+        else if constexpr (TO_G) r16_stage<PREC, L, pow16(S), NPEN, NT, false>(bld, gstp, tw, tid, inv);
+        else r16_stage<PREC, L, pow16(S), NPEN, NT, false>(bld, bst, tw, tid, inv);
+        if constexpr (!LAST) __syncthreads();
+        mfma_stages<PREC, L, S + 1, SW, NPEN, NT, IN_BUF, OUT_BUF>(buf, tw, tid, inv, gldp, gstp);
     }
 }
 
-// Full transform of the tile's pencils: the VALU stages of the non-16 factors
-// (or a copy into LDS), the radix-16 stages on MFMA, then a copy out through
-// gst.  IN_BUF: the input already sits in buf (gld reads it).  OUT_BUF: the
-// caller wants the result in buf itself (gst is not called).  Same calling
-// convention as pen::transform: t is the thread's lane within its pencil,
-// tid its workgroup thread index; no trailing barrier after the copy out.
-template <int PREC, int L, int TP, int SW, int NPEN, int NT, bool IN_BUF, bool OUT_BUF, class GLD, class GST,
-          class LLD, class LST>
-__device__ __forceinline__ void transform(const GLD& gld, const GST& gst, const LLD& lld, const LST& lst,
-                                          cplx<float>* buf, const cplx<float>* __restrict__ tw, int tid, int t,
-                                          int sign) {
+// Full transform of the tile's NPEN pencils buf[n*SW + p] by NT threads: the
+// radix-16 stages first on the matrix cores (the first one loads straight
+// from HBM through gldp(n, p) unless IN_BUF; a radix-16 last stage stores
+// through gstp(n, p, v) unless OUT_BUF), then the vector-ALU stages of the
+// remaining factors by thread t of the TP threads of pencil pth, the last of
+// them storing through gstp(n, pth, v) (or into buf when OUT_BUF).  GST_LDS:
+// gstp writes buf itself.  No trailing barrier, like pen::transform.
+template <int PREC, int L, int TP, int SW, int NPEN, int NT, bool IN_BUF, bool OUT_BUF, bool GST_LDS, class GLDP,
+          class GSTP>
+__device__ __forceinline__ void transform(const GLDP& gldp, const GSTP& gstp, cplx<float>* buf,
+                                          const cplx<float>* __restrict__ tw, int tid, int pth, int t, int sign) {
     static_assert(PREC != PREC_VALU && n16(L) > 0, "MFMA transform needs a radix-16 factor");
     constexpr int REST = rest_of(L);
     if (tid < 64) {
-        // read only after the barrier that ends the first MFMA stage's loads
+        // read by the first stage after its loads' barrier
         const Bmat<PREC> bm = dft16_operand<PREC>(tid);
         typename Op<PREC>::V* bt = b_table<PREC>();
         bt[0 * 64 + tid] = bm.hi[0];
@@ -285,16 +315,43 @@ __device__ __forceinline__ void transform(const GLD& gld, const GST& gst, const 
         bt[2 * 64 + tid] = bm.lo[0];
         bt[3 * 64 + tid] = bm.lo[1];
     }
+    mfma_stages<PREC, L, 0, SW, NPEN, NT, IN_BUF, OUT_BUF>(buf, tw, tid, sign > 0, gldp, gstp);
     if constexpr (REST > 1) {
-        valu_stages<float, L, TP, REST, 0, IN_BUF>(gld, lld, lst, tw, t, sign);
-    } else if constexpr (!IN_BUF) {
-        for (int n = t; n < L; n += TP) lst(n, gld(n));
-        __syncthreads();
+        auto lld = [&](int n) { return buf[n * SW + pth]; };
+        auto lst = [&](int n, cplx<float> v) { buf[n * SW + pth] = v; };
+        auto gst = [&](int n, cplx<float> v) { gstp(n, pth, v); };
+        valu_tail<L, TP, REST, pow16(n16(L)), 0, OUT_BUF, GST_LDS>(gst, lld, lst, tw, t, sign);
     }
-    mfma_stages<PREC, L, REST, 0, SW, NPEN, NT>(buf, tw, tid, sign > 0);
-    if constexpr (!OUT_BUF) {
-        for (int n = t; n < L; n += TP) gst(n, lld(n));
+}
+
+// The transform of length 256 = 16 x 16 over the rows of a tile: both stages
+// on the matrix cores with row batches (16 adjacent positions of a row per
+// MFMA batch), the exchange between them in LDS in the row-major layout
+// rows_idx (one pad slot after every 16 elements, row stride RS).  ld0 feeds
+// the first stage (straight from HBM, or from buf in that layout); st1 takes
+// the second stage's outputs (to HBM, or into buf).  No trailing barrier.
+template <int LZ>
+__host__ __device__ constexpr int rows_stride() { return LZ + LZ / 16 + 1; }
+__device__ __forceinline__ int rows_idx(int n, int p, int rs) { return p * rs + n + (n >> 4); }
+
+template <int PREC, int LZ, int NPEN, int NT, class LD0, class ST1>
+__device__ __forceinline__ void rows_transform(const LD0& ld0, const ST1& st1, cplx<float>* buf,
+                                               const cplx<float>* __restrict__ tw, int tid, int sign) {
+    static_assert(PREC != PREC_VALU && n16(LZ) == 2 && rest_of(LZ) == 1, "rows_transform is the 16 x 16 transform");
+    constexpr int RS = rows_stride<LZ>();
+    if (tid < 64) {
+        const Bmat<PREC> bm = dft16_operand<PREC>(tid);
+        typename Op<PREC>::V* bt = b_table<PREC>();
+        bt[0 * 64 + tid] = bm.hi[0];
+        bt[1 * 64 + tid] = bm.hi[1];
+        bt[2 * 64 + tid] = bm.lo[0];
+        bt[3 * 64 + tid] = bm.lo[1];
     }
+    auto bld = [&](int n, int p) { return buf[rows_idx(n, p, RS)]; };
+    auto bst = [&](int n, int p, cplx<float> v) { buf[rows_idx(n, p, RS)] = v; };
+    r16_stage<PREC, LZ, 1, NPEN, NT, true>(ld0, bst, tw, tid, sign > 0);
+    __syncthreads();
+    r16_stage<PREC, LZ, 16, NPEN, NT, true>(bld, st1, tw, tid, sign > 0);
 }
 
 }  // namespace mx

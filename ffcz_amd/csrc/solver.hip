@@ -334,6 +334,11 @@ void Solver<T>::account_pass_times(int iterations) {
 }
 
 template <typename T>
+void Solver<T>::pass_counts(long* n6) const {
+    for (int k = 0; k < 6; ++k) n6[k] = pass_cnt_[k];
+}
+
+template <typename T>
 long Solver<T>::pass_timing(float* ms6) const {
     for (int k = 0; k < 6; ++k) ms6[k] = pass_cnt_[k] ? (float)(pass_ms_[k] / (double)pass_cnt_[k]) : 0.0f;
     return pass_n_;

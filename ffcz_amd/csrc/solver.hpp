@@ -79,6 +79,8 @@ class Solver {
     // each chunk is left out: the device waits for the host there, and that
     // wait would count as kernel time.
     long pass_timing(float* ms6) const;
+    // n6[6]: how many launches of each of those passes were timed
+    void pass_counts(long* n6) const;
 
     // margins: the loop iterates against eps*(1-ms) and delta*(1-mf); the
     // certification always checks the bounds given to initialize*.

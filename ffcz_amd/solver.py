@@ -310,6 +310,12 @@ class ProjectionSolver:
         call("ffcz_get_pass_timing", self._h, ms, C.byref(n))
         return list(ms), n.value
 
+    def pass_counts(self):
+        """Timed launches behind each of pass_timing's six means."""
+        n = (C.c_long * 6)()
+        call("ffcz_get_pass_counts", self._h, n)
+        return list(n)
+
     def time_iteration(self, reps: int = 10):
         ms = (C.c_float * 6)()
         call("ffcz_time_iteration", self._h, int(reps), ms)

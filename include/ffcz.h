@@ -216,6 +216,8 @@ int ffcz_time_iteration(ffcz_solver* h, int reps, float* ms6);
  * device waiting for the host).  Enabling resets the averages. */
 int ffcz_set_pass_timing(ffcz_solver* h, int enable);
 int ffcz_get_pass_timing(ffcz_solver* h, float* ms6, long* iterations);
+/* n6[6]: the number of timed launches behind each of those six means */
+int ffcz_get_pass_counts(ffcz_solver* h, long* n6);
 
 /* ---- entropy coding: SimpleCudaHuffmanZstdCompressor<T>
  * compress    HuffmanZSTDCoder.cuh:248-376 (symbols: host or device memory)
