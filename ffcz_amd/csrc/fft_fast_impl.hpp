@@ -383,7 +383,7 @@ __global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ, V)) void k2_zfwd(ZArgs<R, T> 
     zfwd_body<R, T, LZ, MODE, PREC, V>(a, blockIdx.x);
 }
 template <typename R, typename T, int LZ, int MODE, int PREC = PREC_VALU, int V = 0>
-__global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ, V)) void k2_zinv(ZArgs<R, T> a) {
+__global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ, V), PREC != PREC_VALU ? 8 : 1) void k2_zinv(ZArgs<R, T> a) {
     zinv_body<R, T, LZ, MODE, PREC, V>(a, blockIdx.x);
 }
 // The MFMA variants ask for 8 waves per SIMD (at most 64 VGPRs): two

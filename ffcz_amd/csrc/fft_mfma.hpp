@@ -327,12 +327,16 @@ __device__ __forceinline__ void transform(const GLDP& gldp, const GSTP& gstp, cp
 // The transform of length 256 = 16 x 16 over the rows of a tile: both stages
 // on the matrix cores with row batches (16 adjacent positions of a row per
 // MFMA batch), the exchange between them in LDS in the row-major layout
-// rows_idx (one pad slot after every 16 elements, row stride RS).  ld0 feeds
-// the first stage (straight from HBM, or from buf in that layout); st1 takes
-// the second stage's outputs (to HBM, or into buf).  No trailing barrier.
+// rows_idx (row stride RS, room for X[0..LZ]).  Within each group of 16
+// positions the slot is XOR-swizzled by the group index, so the first
+// stage's stores (16j + m over lanes j), the second stage's loads and stores
+// (j + 16r over lanes j) and the R2C/C2R passes over adjacent positions all
+// fall on distinct LDS banks, without pad slots.  ld0 feeds the first stage
+// (straight from HBM, or from buf in that layout); st1 takes the second
+// stage's outputs (to HBM, or into buf).  No trailing barrier.
 template <int LZ>
-__host__ __device__ constexpr int rows_stride() { return LZ + LZ / 16 + 1; }
-__device__ __forceinline__ int rows_idx(int n, int p, int rs) { return p * rs + n + (n >> 4); }
+__host__ __device__ constexpr int rows_stride() { return LZ + 1; }
+__device__ __forceinline__ int rows_idx(int n, int p, int rs) { return p * rs + (n ^ ((n >> 4) & 15)); }
 
 template <int PREC, int LZ, int NPEN, int NT, class LD0, class ST1>
 __device__ __forceinline__ void rows_transform(const LD0& ld0, const ST1& st1, cplx<float>* buf,
