@@ -341,7 +341,7 @@ __device__ __forceinline__ void strided_body(const SArgs<R, T>& a, uint32_t blk)
         if (fired) a.st->freq_clamped = 1u;
         __syncthreads();
         if constexpr (PREC == PREC_VALU) pen::transform<R, L, TP, true, false>(lld, gst, lld, lst, tw, t, +1);
-        else mx::transform<PREC, L, TP, W, W, NT, true, false, false>(gldp, gstp, buf, tw, tid, p, t, +1);
+        else mx::transform<PREC, L, TP, W, W, NT, true, false, false, false>(gldp, gstp, buf, tw, tid, p, t, +1);
     } else {
         static_assert(PREC == PREC_VALU, "certification runs on the vector ALU");  // S_CERT_CHECK
         unsigned long long nv = 0;
@@ -389,10 +389,12 @@ __global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ, V), PREC != PREC_VALU ? 8 : 1
 // The MFMA variants ask for 8 waves per SIMD (at most 64 VGPRs): two
 // 1024-thread tiles per CU, like the vector-ALU kernels.  The fused clamp
 // kernel spills a few registers under that bound and still runs faster than
-// at one tile per CU (0.41 against 0.53 ms at 512^3).
+// at one tile per CU (0.41 against 0.53 ms at 512^3).  The half-width tiles
+// (V=1, 512 threads) ask for 6 waves per SIMD (80 VGPRs, no spills): three
+// tiles per CU.
 template <typename R, typename T, int L, int MODE, int PREC = PREC_VALU, int V = 0>
 __global__ __launch_bounds__(s_tp<R>(L) * s_width<R>(V),
-                             (PREC != PREC_VALU) ? 8 : 1) void k2_strided(SArgs<R, T> a) {
+                             (PREC != PREC_VALU) ? (V == 0 ? 8 : 6) : 1) void k2_strided(SArgs<R, T> a) {
     strided_body<R, T, L, MODE, PREC, V>(a, blockIdx.x);
 }
 // Two forward z passes over the same rows in one workgroup (the paired
@@ -536,6 +538,12 @@ bool strided_dispatch(int mode, const SArgs<R, T>& a, size_t outer, hipStream_t 
 #define FFCZ_MFMA_S_LENGTHS(X) X(32) X(64) X(96) X(128) X(256) X(512)
 #define FFCZ_MFMA_Z_LENGTHS(X) X(16) X(32) X(48) X(64) X(128) X(256)
 
+// the half-size tile variant (V=1) where its workgroup is whole waves
+template <int LZ>
+constexpr bool mfma_z_v1() { return (z_tp(LZ) * z_rows(LZ, 1)) % 64 == 0; }
+template <int L>
+constexpr bool mfma_s_v1() { return (s_tp<float>(L) * s_width<float>(1)) % 64 == 0; }
+
 template <int PREC>
 bool mfma_zfwd_dispatch(int mode, const ZArgs<float, float>& a, hipStream_t s) {
     if (!a.half || (mode != Z_PLAIN && mode != Z_ITER)) return false;
@@ -543,6 +551,13 @@ bool mfma_zfwd_dispatch(int mode, const ZArgs<float, float>& a, hipStream_t s) {
     switch (a.Lz) {
 #define FFCZ_MZF(Lv)                                                                                      \
     case Lv:                                                                                              \
+        if constexpr (mfma_z_v1<Lv>()) {                                                                  \
+            if (a.var) {                                                                                  \
+                if (mode == Z_ITER) launch_zfwd_len<float, float, Z_ITER, Lv, PREC, 1>(a, s);             \
+                else launch_zfwd_len<float, float, Z_PLAIN, Lv, PREC, 1>(a, s);                           \
+                return true;                                                                              \
+            }                                                                                             \
+        }                                                                                                 \
         if (mode == Z_ITER) launch_zfwd_len<float, float, Z_ITER, Lv, PREC>(a, s);                        \
         else launch_zfwd_len<float, float, Z_PLAIN, Lv, PREC>(a, s);                                      \
         return true;
@@ -559,6 +574,13 @@ bool mfma_zinv_dispatch(int mode, const ZArgs<float, float>& a, hipStream_t s) {
     switch (a.Lz) {
 #define FFCZ_MZI(Lv)                                                                                      \
     case Lv:                                                                                              \
+        if constexpr (mfma_z_v1<Lv>()) {                                                                  \
+            if (a.var) {                                                                                  \
+                if (mode == Z_ITER) launch_zinv_len<float, float, Z_ITER, Lv, PREC, 1>(a, s);             \
+                else launch_zinv_len<float, float, Z_PLAIN, Lv, PREC, 1>(a, s);                           \
+                return true;                                                                              \
+            }                                                                                             \
+        }                                                                                                 \
         if (mode == Z_ITER) launch_zinv_len<float, float, Z_ITER, Lv, PREC>(a, s);                        \
         else launch_zinv_len<float, float, Z_PLAIN, Lv, PREC>(a, s);                                      \
         return true;
@@ -574,6 +596,13 @@ bool mfma_strided_dispatch(int mode, const SArgs<float, float>& a, size_t outer,
     switch (a.L) {
 #define FFCZ_MSL(Lv)                                                                                      \
     case Lv:                                                                                              \
+        if constexpr (mfma_s_v1<Lv>()) {                                                                  \
+            if (a.var) {                                                                                  \
+                if (mode == S_CLAMP) launch_strided_len<float, float, S_CLAMP, Lv, PREC, 1>(a, outer, s); \
+                else launch_strided_len<float, float, S_PLAIN, Lv, PREC, 1>(a, outer, s);                 \
+                return true;                                                                              \
+            }                                                                                             \
+        }                                                                                                 \
         if (mode == S_CLAMP) launch_strided_len<float, float, S_CLAMP, Lv, PREC>(a, outer, s);            \
         else launch_strided_len<float, float, S_PLAIN, Lv, PREC>(a, outer, s);                            \
         return true;

@@ -300,13 +300,15 @@ __device__ __forceinline__ void mfma_stages(cplx<float>* buf, const cplx<float>*
 // remaining factors by thread t of the TP threads of pencil pth, the last of
 // them storing through gstp(n, pth, v) (or into buf when OUT_BUF).  GST_LDS:
 // gstp writes buf itself.  No trailing barrier, like pen::transform.
-template <int PREC, int L, int TP, int SW, int NPEN, int NT, bool IN_BUF, bool OUT_BUF, bool GST_LDS, class GLDP,
-          class GSTP>
+// TABLE: write the DFT-matrix table first (false: an earlier transform of
+// the same kernel left it in LDS).
+template <int PREC, int L, int TP, int SW, int NPEN, int NT, bool IN_BUF, bool OUT_BUF, bool GST_LDS,
+          bool TABLE = true, class GLDP, class GSTP>
 __device__ __forceinline__ void transform(const GLDP& gldp, const GSTP& gstp, cplx<float>* buf,
                                           const cplx<float>* __restrict__ tw, int tid, int pth, int t, int sign) {
     static_assert(PREC != PREC_VALU && n16(L) > 0, "MFMA transform needs a radix-16 factor");
     constexpr int REST = rest_of(L);
-    if (tid < 64) {
+    if (TABLE && tid < 64) {
         // read by the first stage after its loads' barrier
         const Bmat<PREC> bm = dft16_operand<PREC>(tid);
         typename Op<PREC>::V* bt = b_table<PREC>();

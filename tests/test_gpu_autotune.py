@@ -11,7 +11,7 @@ import itertools
 import numpy as np
 import pytest
 
-from ffcz_amd import MODE_ABS, ProjectionSolver
+from ffcz_amd import FFT_BF16X3, FFT_FP16X3, MODE_ABS, ProjectionSolver
 from ffcz_amd._lib import FFCzError
 from tests._cases import binding_case
 
@@ -20,10 +20,12 @@ pytestmark = pytest.mark.gpu
 NAMES = ProjectionSolver.TILE_VAR_NAMES
 
 
-def _run(shape, dtype, case, vars_=None, tune=False):
+def _run(shape, dtype, case, vars_=None, tune=False, fft=None):
     x, e0, eps, delta = case
     with ProjectionSolver(*shape, dtype=dtype) as s:
         s.initialize_all_abs(x, eps, delta)
+        if fft is not None:
+            s.set_fft_precision(fft)
         chosen = None
         if tune:
             chosen = s.autotune(reps=2, use_cache=False, timings=True)
@@ -51,6 +53,24 @@ def test_every_tile_variant_gives_identical_results(gpu, shape, dtype):
         for a, b in zip(got[1:4], ref[1:4]):
             assert np.array_equal(a, b), combo
         assert got[4] == ref[4], combo
+
+
+@pytest.mark.parametrize("fft", [FFT_BF16X3, FFT_FP16X3])
+@pytest.mark.parametrize("shape", [(32, 64, 512), (16, 128, 256), (64, 32, 96)])
+def test_mfma_tile_variants_give_identical_results(gpu, shape, fft):
+    """The MFMA kernels' tile variants regroup the radix-16 batches (16
+    butterflies of adjacent columns or row positions per instruction) without
+    changing any butterfly's operands: bit-identical to the default tiles.
+    The shapes cover the 16 x 16 row transform of the z passes (nz = 512),
+    radix-16 stages with vector-ALU remainders and the strided passes."""
+    dtype = np.float32
+    case = binding_case(shape, seed=6, dtype=dtype, spread=1.4, dfrac=0.9)
+    ref = _run(shape, dtype, case, fft=fft)
+    for combo in [(1,) * 7, (1, 0, 1, 0, 0, 0, 0), (0, 1, 0, 1, 0, 0, 0)]:
+        got = _run(shape, dtype, case, dict(zip(NAMES, combo)), fft=fft)
+        assert got[0] == ref[0], combo
+        for a, b in zip(got[1:4], ref[1:4]):
+            assert np.array_equal(a, b), combo
 
 
 def test_autotune_choice_cache_and_errors(gpu):

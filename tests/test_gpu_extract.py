@@ -179,7 +179,15 @@ def test_certify_both_equals_the_two_certifications(gpu, shape, dtype, quant):
             s.set_quantizer(QUANT_REF16)
         s.solve_loop(MODE_ABS, e0, 60, 1e-9)
         s.extract_compact_representation()
+        # called with the extraction still pending
         pair = s.certify_both()
+        got = (s.get_spatial_flags_pack(), s.get_freq_flags_pack(), s.get_active_spatial_quant_edit_distances(),
+               s.get_active_freq_quant_edit_distances(), s.get_active_spatial_edit_distances())
         single = (s.certify(False), s.certify(True))
         assert pair == single
         assert any(v.freq_violations or v.spatial_violations or v.ok for v in pair)
+        # and that extraction is the one made on its own
+        s.extract_compact_representation()
+        again = (s.get_spatial_flags_pack(), s.get_freq_flags_pack(), s.get_active_spatial_quant_edit_distances(),
+                 s.get_active_freq_quant_edit_distances(), s.get_active_spatial_edit_distances())
+        assert all(np.array_equal(a, b) for a, b in zip(got, again))
