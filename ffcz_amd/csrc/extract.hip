@@ -35,12 +35,8 @@ size_t extract_scratch_bytes(size_t n) {
 }
 
 template <typename T>
-__device__ __forceinline__ bool edit_flag(T v, const QuantDesc& q, long long& r) {
-    if (q.kind == Q_STEP) {
-        r = quant_step((double)v, q.step);
-        return r != 0;
-    }
-    r = 0;
+__device__ __forceinline__ bool edit_flag(T v, const QuantDesc& q) {
+    if (q.kind == Q_STEP) return quant_step_f((double)v, q.step) != 0.0;
     return fabs(v) > (T)q.threshold;  // projection_algorithm.cu:377
 }
 
@@ -121,8 +117,7 @@ __device__ __forceinline__ void tile_compact(const unsigned int tile, const T* v
 #pragma unroll
             for (int k = 0; k < X_EPT; ++k) {
                 if (i + k < n) {
-                    long long qq;
-                    if (edit_flag(x[r][k], q, qq)) bits[r] |= 1u << k;
+                    if (edit_flag(x[r][k], q)) bits[r] |= 1u << k;
                     mn = x[r][k] < mn ? x[r][k] : mn;
                     mx = x[r][k] > mx ? x[r][k] : mx;
                 }

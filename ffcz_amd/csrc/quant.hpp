@@ -15,13 +15,15 @@ __device__ __forceinline__ T quant_ref16(T v, T mn, T mx) {
     return n > (T)0.5 ? ceil(s) : floor(s);
 }
 
-// Step quantiser: llround(v/step) in FP64, ties away from zero.
-__device__ __forceinline__ long long quant_step(double v, double step) {
+// Step quantiser: llround(v/step) in FP64, ties away from zero, as an
+// integer-valued double (|n| <= 4e15 < 2^53, so every value is exact).
+__device__ __forceinline__ double quant_step_f(double v, double step) {
     double r = v / step;
     const double lim = 4.0e15;
     r = r > lim ? lim : (r < -lim ? -lim : r);
-    return r < 0.0 ? -(long long)floor(-r + 0.5) : (long long)floor(r + 0.5);
+    return r < 0.0 ? -floor(-r + 0.5) : floor(r + 0.5);
 }
+__device__ __forceinline__ long long quant_step(double v, double step) { return (long long)quant_step_f(v, step); }
 
 __device__ __forceinline__ unsigned long long zigzag64(long long q) {
     return ((unsigned long long)q << 1) ^ (unsigned long long)(q >> 63);
@@ -40,7 +42,7 @@ __device__ __forceinline__ double dequant(T v, const QuantDesc& q) {
         const T r = qv / (T)65535 * (mx - mn) + mn;
         return (double)r;
     }
-    return (double)quant_step((double)v, q.step) * q.step;
+    return quant_step_f((double)v, q.step) * q.step;  // no integer round trip
 }
 
 }  // namespace ffcz
