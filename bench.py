@@ -63,6 +63,9 @@ def parse():
     p.add_argument("--cpu-sample-iters", type=int, default=1)
     p.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                    help="only run a plain device copy of BYTES (counter calibration) and exit")
+    p.add_argument("--hbm-ceilings", type=int, default=0, metavar="BYTES",
+                   help="only time plain write, copy and two-read-one-write kernels over arrays of BYTES "
+                        "(the HBM rates simple streaming kernels reach) and exit")
     return p.parse_args()
 
 
@@ -255,6 +258,30 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
     import torch
+
+    if a.hbm_ceilings:
+        n = a.hbm_ceilings // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        y = torch.ones_like(x)
+        z = torch.empty_like(x)
+
+        def rate(fn, moved, reps=20):
+            fn()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                fn()
+            e1.record()
+            torch.cuda.synchronize()
+            return round(moved * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9, 1)
+
+        B = n * 4
+        print(json.dumps({"hbm_ceilings_bytes": B, "unit": "GB/s",
+                          "write": rate(lambda: x.fill_(3.0), B),
+                          "copy": rate(lambda: y.copy_(x), 2 * B),
+                          "two_reads_one_write": rate(lambda: torch.add(x, y, out=z), 3 * B),
+                          "peak": HBM_PEAK_GBPS}))
+        return
 
     if a.copy_calibration:
         n = a.copy_calibration // 4
