@@ -36,7 +36,7 @@ size_t extract_scratch_bytes(size_t n) {
 
 template <typename T>
 __device__ __forceinline__ bool edit_flag(T v, const QuantDesc& q) {
-    if (q.kind == Q_STEP) return quant_step_f((double)v, q.step) != 0.0;
+    if (q.kind == Q_STEP) return quant_step_nonzero((double)v, q.step);
     return fabs(v) > (T)q.threshold;  // projection_algorithm.cu:377
 }
 
@@ -183,6 +183,7 @@ __device__ __forceinline__ void tile_compact(const unsigned int tile, const T* v
                         if (lane == 0) atomicOr(&g->overflow, 2u);
                         break;
                     }
+                    __builtin_amdgcn_s_sleep(2);  // fewer polls while predecessors are still running
                     continue;
                 }
                 pre += wave_sum((lane <= fp && j >= 0) ? (sw & ST_VAL) : 0ull);

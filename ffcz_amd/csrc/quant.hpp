@@ -25,6 +25,17 @@ __device__ __forceinline__ double quant_step_f(double v, double step) {
 }
 __device__ __forceinline__ long long quant_step(double v, double step) { return (long long)quant_step_f(v, step); }
 
+// quant_step_f(v, step) != 0 without the division where the answer is
+// clear: |v/step| < 1/2 rounds to 0, above it to at least 1.  The relative
+// margin 1e-5 is far wider than the rounding of the thresholds and of the
+// quotient, so only |v| within it of step/2 takes the quantiser itself.
+__device__ __forceinline__ bool quant_step_nonzero(double v, double step) {
+    const double a = fabs(v);
+    if (a < step * 0.49999) return false;
+    if (a > step * 0.50001) return true;
+    return quant_step_f(v, step) != 0.0;
+}
+
 __device__ __forceinline__ unsigned long long zigzag64(long long q) {
     return ((unsigned long long)q << 1) ^ (unsigned long long)(q >> 63);
 }
@@ -42,6 +53,8 @@ __device__ __forceinline__ double dequant(T v, const QuantDesc& q) {
         const T r = qv / (T)65535 * (mx - mn) + mn;
         return (double)r;
     }
+    // edits below half a step dequantise to zero (with the quantiser's sign)
+    if (!quant_step_nonzero((double)v, q.step)) return v < (T)0 ? -0.0 : 0.0;
     return quant_step_f((double)v, q.step) * q.step;  // no integer round trip
 }
 
