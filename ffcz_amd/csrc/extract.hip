@@ -34,11 +34,18 @@ size_t extract_scratch_bytes(size_t n) {
     return 256 + ntiles * (sizeof(unsigned long long) + 2 * sizeof(double));
 }
 
+// The edit's flag where comparisons decide it: 1 active, 0 inactive, 2 for a
+// step-quantised value within 1e-5 (relative) of half a step, which the
+// quantiser itself decides (edit_flag_exact).
 template <typename T>
-__device__ __forceinline__ bool edit_flag(T v, const QuantDesc& q) {
-    if (q.kind == Q_STEP) return quant_step_nonzero((double)v, q.step);
-    return fabs(v) > (T)q.threshold;  // projection_algorithm.cu:377
+__device__ __forceinline__ unsigned int edit_flag3(T v, const QuantDesc& q, double lo, double hi) {
+    if (q.kind == Q_STEP) {
+        const double a = fabs((double)v);
+        return a < lo ? 0u : (a > hi ? 1u : 2u);
+    }
+    return fabs(v) > (T)q.threshold ? 1u : 0u;  // projection_algorithm.cu:377
 }
+__device__ __noinline__ bool edit_flag_exact(double v, double step) { return quant_step_f(v, step) != 0.0; }
 
 // 8 consecutive elements starting at i (a multiple of 8); lanes past n get 0
 template <typename T>
@@ -99,8 +106,8 @@ __device__ __forceinline__ void tile_compact(const unsigned int tile, const T* v
     constexpr int RND = x_rounds<T>();
     __shared__ unsigned int wtot[RND][4];
     __shared__ T smn[4], smx[4];
-    // the tile's active values in index order, when they fit half a tile
-    // (a full tile of values would cost occupancy)
+    // the tile's active values in index order, half a tile at a time (a
+    // full tile of values would cost occupancy)
     constexpr unsigned int SVAL = X_TILE / 2;
     __shared__ T sval[SVAL];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -108,19 +115,32 @@ __device__ __forceinline__ void tile_compact(const unsigned int tile, const T* v
     T x[RND][X_EPT];
     unsigned int bits[RND], excl[RND];
     T mn = (T)INFINITY, mx = (T)-INFINITY;
+    // the step quantiser's flag by comparisons, away from half a step (the
+    // margin is far wider than the rounding of the thresholds and quotients)
+    const double flo = q.step * 0.49999, fhi = q.step * 0.50001;
 #pragma unroll
     for (int r = 0; r < RND; ++r) {
         const size_t i = base + ((size_t)r * 256 + tid) * X_EPT;
         bits[r] = 0;
         if (i < n) {
             load8(v, i, n, x[r]);
+            unsigned int unsure = 0;
 #pragma unroll
             for (int k = 0; k < X_EPT; ++k) {
                 if (i + k < n) {
-                    if (edit_flag(x[r][k], q)) bits[r] |= 1u << k;
+                    const unsigned int f = edit_flag3(x[r][k], q, flo, fhi);
+                    bits[r] |= (f & 1u) << k;
+                    unsure |= (f >> 1) << k;
                     mn = x[r][k] < mn ? x[r][k] : mn;
                     mx = x[r][k] > mx ? x[r][k] : mx;
                 }
+            }
+            // the rare values near half a step: the quantiser decides (one
+            // copy of the division, the value read again from memory)
+            while (unsure) {
+                const int k = __builtin_ctz(unsure);
+                unsure &= unsure - 1u;
+                if (edit_flag_exact((double)v[i + k], q.step)) bits[r] |= 1u << k;
             }
         }
         // four consecutive threads' flag bytes form one little-endian word
@@ -196,50 +216,69 @@ __device__ __forceinline__ void tile_compact(const unsigned int tile, const T* v
         if (lane == 0) s_pre = pre;
     }
     __syncthreads();  // s_pre is read by every wave below
-    // active values to their tile-local rank; a tile whose values fit half a
-    // tile stages them in LDS and copies its contiguous output range
-    // [s_pre, s_pre + count) with coalesced stores, a denser one stores them
-    // from registers
+    // active values to their tile-local rank, staged in LDS half a tile at a
+    // time (one chunk, or two for a tile more than half active), then the
+    // contiguous output range [s_pre, s_pre + count) is written with
+    // coalesced stores
     unsigned int total = 0;
 #pragma unroll
     for (int r = 0; r < RND; ++r) total += wtot[r][0] + wtot[r][1] + wtot[r][2] + wtot[r][3];
-    const bool staged = total <= SVAL;
     const unsigned long long out0 = s_pre;
     bool ovf = false;
-    auto emit = [&](unsigned long long at, T val) {
-        compact[at] = val;
-        if (q.kind == Q_STEP) {
-            const unsigned long long z = zigzag64(quant_step((double)val, q.step));
-            if (z > 65535ull) ovf = true;
-            quant[at] = (uint16_t)z;
+    unsigned int rank0[RND];  // this thread's first rank in each round
+    {
+        unsigned int run = 0;
+#pragma unroll
+        for (int r = 0; r < RND; ++r) {
+            unsigned int pos = run + excl[r];
+            for (int k = 0; k < w; ++k) pos += wtot[r][k];
+            rank0[r] = pos;
+            run += wtot[r][0] + wtot[r][1] + wtot[r][2] + wtot[r][3];
         }
-    };
-    unsigned int run = 0;
+    }
+    for (unsigned int lo = 0; lo < total; lo += SVAL) {
+        if (lo > 0) __syncthreads();  // the previous chunk has been copied out
 #pragma unroll
-    for (int r = 0; r < RND; ++r) {
-        unsigned int pos = run + excl[r];
-        for (int k = 0; k < w; ++k) pos += wtot[r][k];
-        if (bits[r]) {
+        for (int r = 0; r < RND; ++r) {
+            unsigned int pos = rank0[r];
+            if (bits[r]) {
 #pragma unroll
-            for (int k = 0; k < X_EPT; ++k) {
-                if (bits[r] & (1u << k)) {
-                    if (staged) sval[pos] = x[r][k];
-                    else emit(out0 + pos, x[r][k]);
-                    ++pos;
+                for (int k = 0; k < X_EPT; ++k) {
+                    if (bits[r] & (1u << k)) {
+                        if (pos - lo < SVAL) sval[pos - lo] = x[r][k];  // pos >= lo (unsigned wrap otherwise)
+                        ++pos;
+                    }
                 }
             }
         }
-        run += wtot[r][0] + wtot[r][1] + wtot[r][2] + wtot[r][3];
-    }
-    if (staged) {
         __syncthreads();
-        for (unsigned int i = tid; i < total; i += 256) emit(out0 + i, sval[i]);
+        const unsigned int cnt = min(total - lo, SVAL);
+        for (unsigned int i = tid; i < cnt; i += 256) {
+            const unsigned long long at = out0 + lo + i;
+            const T val = sval[i];
+            compact[at] = val;
+            if (q.kind == Q_STEP) {
+                const unsigned long long z = zigzag64(quant_step((double)val, q.step));
+                if (z > 65535ull) ovf = true;
+                quant[at] = (uint16_t)z;
+            }
+        }
     }
     if (ovf) atomicOr(&g->overflow, 1u);
 }
 
+// tiles per CU: six for FP32 edits (at most 80 VGPRs), four for FP64; the
+// batched kernel's segment lookup needs a few more registers (five)
 template <typename T>
-__global__ __launch_bounds__(256) void k_tile_compact(const T* v, size_t n, QuantDesc q, unsigned long long* status,
+constexpr int x_waves(bool batched = false) { return sizeof(T) == 4 ? (batched ? 5 : 6) : 4; }
+
+// Each workgroup takes one ticket (an atomic on one counter) for one tile.
+// Those atomics serialise workgroup starts: a bare 32 KB-per-workgroup read
+// drops from 6.0 to 2.5 TB/s when each workgroup first takes a ticket, which
+// is the rate of this pass.  Several tiles per ticket would chain each
+// workgroup's first tile to the previous workgroup's last one.
+template <typename T>
+__global__ __launch_bounds__(256, x_waves<T>()) void k_tile_compact(const T* v, size_t n, QuantDesc q, unsigned long long* status,
                                                       unsigned int* ticket, double* pmin, double* pmax,
                                                       ExtractGlobals* g, unsigned int* pack, T* compact,
                                                       uint16_t* quant) {
@@ -254,7 +293,7 @@ __global__ __launch_bounds__(256) void k_tile_compact(const T* v, size_t n, Quan
 // its own segment hold smaller tickets and belong to workgroups already
 // running, as in the single-array pass.
 template <typename T>
-__global__ __launch_bounds__(256) void k_tile_compact_b(const ExtractSeg<T>* segs, int nseg,
+__global__ __launch_bounds__(256, x_waves<T>(true)) void k_tile_compact_b(const ExtractSeg<T>* segs, int nseg,
                                                         unsigned long long* status, unsigned int* ticket,
                                                         double* pmin, double* pmax) {
     __shared__ unsigned int s_tile;
