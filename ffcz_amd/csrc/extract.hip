@@ -273,10 +273,8 @@ template <typename T>
 constexpr int x_waves(bool batched = false) { return sizeof(T) == 4 ? (batched ? 5 : 6) : 4; }
 
 // Each workgroup takes one ticket (an atomic on one counter) for one tile.
-// Those atomics serialise workgroup starts: a bare 32 KB-per-workgroup read
-// drops from 6.0 to 2.5 TB/s when each workgroup first takes a ticket, which
-// is the rate of this pass.  Several tiles per ticket would chain each
-// workgroup's first tile to the previous workgroup's last one.
+// Several tiles per ticket would chain each workgroup's first tile to the
+// previous workgroup's last one.
 template <typename T>
 __global__ __launch_bounds__(256, x_waves<T>()) void k_tile_compact(const T* v, size_t n, QuantDesc q, unsigned long long* status,
                                                       unsigned int* ticket, double* pmin, double* pmax,
