@@ -90,14 +90,38 @@ __device__ __forceinline__ void zfwd_body(const ZArgs<R, T>& a, uint32_t blk) {
     R* ecopy = (MODE == Z_ITER && !clamp && a.e_init) ? a.e_copy : nullptr;
     unsigned long long nviol = 0;
     double vmax = 0.0;
-    // element pair n of row rw (ok: rw is in range), with the pass's fused
-    // element-wise work
-    auto gldr = [&](int n, size_t rw, bool ok) -> cplx<R> {
-        if (!ok) return mk(R(0), R(0));
+    // The loads of element pair n of row rw (ok: rw is in range) and the
+    // pass's fused element-wise work on them, kept apart so a thread can
+    // issue all of its first-stage loads before any of that work (its stores
+    // to ds / e0 could alias later loads for all the compiler knows, and its
+    // branches would keep each load waiting for the previous element).
+    struct Raw {
+        cplx<T> e, s;      // iterate (ITER: e, in s's place unused) / e0 and ds (CERT)
+        cplx<double> f;    // C2R(df)/N (CERT with frequency edits)
+    };
+    auto rawr = [&](int n, size_t rw, bool ok) -> Raw {
+        Raw w{};
+        if (!ok) return w;
         const size_t gi = rw * (size_t)NZ + 2 * (size_t)n;
         if constexpr (MODE == Z_ITER) {
             // one 2-element vector load of the iterate (gi is even)
-            cplx<R> v = *reinterpret_cast<const cplx<R>*>(esrc + gi);
+            const cplx<R> v = *reinterpret_cast<const cplx<R>*>(esrc + gi);
+            w.e = mk((T)v.x, (T)v.y);
+        } else if constexpr (MODE == Z_PLAIN) {
+            const cplx<R> v = *reinterpret_cast<const cplx<R>*>(a.in + gi);
+            w.f = mk((double)v.x, (double)v.y);
+        } else {
+            w.e = *reinterpret_cast<const cplx<T>*>(a.e0 + gi);
+            w.s = *reinterpret_cast<const cplx<T>*>(a.cds + gi);
+            if (a.dfi) w.f = *reinterpret_cast<const cplx<double>*>(a.dfi + gi);
+        }
+        return w;
+    };
+    auto applyr = [&](int n, size_t rw, bool ok, const Raw& w) -> cplx<R> {
+        if (!ok) return mk(R(0), R(0));
+        const size_t gi = rw * (size_t)NZ + 2 * (size_t)n;
+        if constexpr (MODE == Z_ITER) {
+            cplx<R> v = mk((R)w.e.x, (R)w.e.y);
             if (ecopy) *reinterpret_cast<cplx<R>*>(ecopy + gi) = v;
             if (fresh) {
                 cplx<R> d;
@@ -110,19 +134,16 @@ __device__ __forceinline__ void zfwd_body(const ZArgs<R, T>& a, uint32_t blk) {
             }
             return v;
         } else if constexpr (MODE == Z_PLAIN) {
-            return *reinterpret_cast<const cplx<R>*>(a.in + gi);
+            return mk((R)w.f.x, (R)w.f.y);
         } else {
             // x_hat - x = e0 + C2R(df)/N + ds, element pairs as vectors
-            const cplx<T> e2 = *reinterpret_cast<const cplx<T>*>(a.e0 + gi);
-            const cplx<T> s2 = *reinterpret_cast<const cplx<T>*>(a.cds + gi);
-            double d0 = (double)e2.x, d1 = (double)e2.y;
+            double d0 = (double)w.e.x, d1 = (double)w.e.y;
             if (a.dfi) {
-                const cplx<double> f2 = *reinterpret_cast<const cplx<double>*>(a.dfi + gi);
-                d0 += f2.x;
-                d1 += f2.y;
+                d0 += w.f.x;
+                d1 += w.f.y;
             }
-            d0 += dequant(s2.x, a.qs);
-            d1 += dequant(s2.y, a.qs);
+            d0 += dequant(w.s.x, a.qs);
+            d1 += dequant(w.s.y, a.qs);
             const double x0 = fabs(d0) - a.ceps, x1 = fabs(d1) - a.ceps;
             if (x0 > 0.0) {
                 ++nviol;
@@ -135,11 +156,25 @@ __device__ __forceinline__ void zfwd_body(const ZArgs<R, T>& a, uint32_t blk) {
             return mk((R)d0, (R)d1);
         }
     };
+    auto gldr = [&](int n, size_t rw, bool ok) { return applyr(n, rw, ok, rawr(n, rw, ok)); };
     auto gld = [&](int n) { return gldr(n, row, rv); };
     auto lld = [&](int n) { return buf[bidx(n, p)]; };
     auto lst = [&](int n, cplx<R> v) { buf[bidx(n, p)] = v; };
-    if constexpr (PREC == PREC_VALU) {
+    if constexpr (PREC == PREC_VALU && MODE == Z_PLAIN) {
         pen::transform<R, LZ, TP, false, true>(gld, lst, lld, lst, tw, t, -1);
+    } else if constexpr (PREC == PREC_VALU) {
+        // every first-stage load of the thread first, then the fused work as
+        // the stage consumes them
+        constexpr int NB0 = pen::first_nb<LZ, TP>(), R0 = pen::radix(LZ, 0);
+        Raw raw[NB0][R0];
+#pragma unroll
+        for (int b = 0; b < NB0; ++b) {
+            const bool has = pen::first_has<LZ, TP>(t, b);
+#pragma unroll
+            for (int r = 0; r < R0; ++r) raw[b][r] = rawr(pen::first_elem<LZ>(t, b, r, TP), row, rv && has);
+        }
+        auto gld_br = [&](int n, int b, int r) { return applyr(n, row, rv, raw[b][r]); };
+        pen::transform<R, LZ, TP, false, true, true>(gld_br, lst, lld, lst, tw, t, -1);
     } else if constexpr (ROWS) {
         // the first stage loads 16 adjacent pairs of a row per MFMA batch
         auto gldp = [&](int n, int pp) {
@@ -202,12 +237,26 @@ __device__ __forceinline__ void zinv_body(const ZArgs<R, T>& a, uint32_t blk) {
     for (int i = tid; i < LZ; i += NT) tw[i] = a.tz[i];
     for (int i = tid; i <= M / 2; i += NT) tw2[i] = a.tz2[i];
     const size_t row0 = (size_t)blk * P;
-    // the tile's half-spectrum rows X[0..M] into LDS, each element loaded once
-    for (int q = tid; q < P * NH; q += NT) {
-        const int pp = q / NH;
-        const int k = q - pp * NH;
-        const size_t r2 = row0 + pp;
-        buf[bidx(k, pp)] = r2 < a.rows ? a.spec[r2 * a.Mp + k] : mk(R(0), R(0));
+    // the tile's half-spectrum rows X[0..M] into LDS, each element loaded
+    // once: every load of the thread is issued before the first LDS store
+    // (a load-store loop would keep one load in flight per thread)
+    {
+        constexpr int NQ = (P * NH + NT - 1) / NT;
+        cplx<R> xv[NQ];
+#pragma unroll
+        for (int u = 0; u < NQ; ++u) {
+            const int q = tid + u * NT;
+            const int pp = q / NH;
+            const int k = q - pp * NH;
+            const size_t r2 = row0 + pp;
+            xv[u] = (q < P * NH && r2 < a.rows) ? a.spec[r2 * a.Mp + k] : mk(R(0), R(0));
+        }
+#pragma unroll
+        for (int u = 0; u < NQ; ++u) {
+            const int q = tid + u * NT;
+            const int pp = q / NH;
+            if (q < P * NH) buf[bidx(q - pp * NH, pp)] = xv[u];
+        }
     }
     __syncthreads();
     // Z[n] = (X[n] + conj X[M-n]) + i W^-n (X[n] - conj X[M-n]) in place, by

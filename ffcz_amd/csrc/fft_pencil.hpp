@@ -51,7 +51,10 @@ constexpr int nsub(int L, int s) {
 constexpr int threads_per_pencil(int L) { return L <= 1 ? 1 : (L <= 512 ? (L + 7) / 8 : (L + 15) / 16); }
 
 // One Stockham stage of radix RAD with sub-transform length NS before it.
-template <typename R, int L, int TP, int RAD, int NS, bool SYNC_MID, class LD, class ST>
+// BR: the loader also takes the (butterfly, input) position (b, r) of the
+// element, both compile-time constants after unrolling, so a caller can serve
+// the stage from values it loaded beforehand.
+template <typename R, int L, int TP, int RAD, int NS, bool SYNC_MID, bool BR = false, class LD, class ST>
 __device__ __forceinline__ void stage_rn(const LD& ld, const ST& st, const cplx<R>* __restrict__ tw, int t,
                                          int sign) {
     constexpr int LR = L / RAD;
@@ -63,7 +66,10 @@ __device__ __forceinline__ void stage_rn(const LD& ld, const ST& st, const cplx<
         const int j = t + b * TP;
         if ((LR % TP == 0) || j < LR) {
 #pragma unroll
-            for (int r = 0; r < RAD; ++r) v[b][r] = ld(j + r * LR);
+            for (int r = 0; r < RAD; ++r) {
+                if constexpr (BR) v[b][r] = ld(j + r * LR, b, r);
+                else v[b][r] = ld(j + r * LR);
+            }
             if constexpr (NS > 1) {
                 const int k = j % NS;
 #pragma unroll
@@ -89,10 +95,25 @@ __device__ __forceinline__ void stage_rn(const LD& ld, const ST& st, const cplx<
     }
 }
 
-template <typename R, int L, int TP, int S, bool SYNC_MID, class LD, class ST>
+template <typename R, int L, int TP, int S, bool SYNC_MID, bool BR = false, class LD, class ST>
 __device__ __forceinline__ void run_stage(const LD& ld, const ST& st, const cplx<R>* __restrict__ tw, int t,
                                           int sign) {
-    stage_rn<R, L, TP, radix(L, S), nsub(L, S), SYNC_MID>(ld, st, tw, t, sign);
+    stage_rn<R, L, TP, radix(L, S), nsub(L, S), SYNC_MID, BR>(ld, st, tw, t, sign);
+}
+
+// The first stage's loads of thread t: position (b, r) is element
+// t + b*TP + r*L/R0 (R0 the first radix) when b < first_nb<L, TP>() and that
+// element's butterfly exists (first_has<L, TP>(t, b)).
+template <int L, int TP>
+constexpr int first_nb() { return (L / radix(L, 0) + TP - 1) / TP; }
+template <int L, int TP>
+__device__ __forceinline__ bool first_has(int t, int b) {
+    constexpr int LR = L / radix(L, 0);
+    return (LR % TP == 0) || t + b * TP < LR;
+}
+template <int L>
+__device__ __forceinline__ int first_elem(int t, int b, int r, int TP) {
+    return t + b * TP + r * (L / radix(L, 0));
 }
 
 // Full transform of one pencil.  gld/gst: first-stage loader and last-stage
@@ -100,14 +121,18 @@ __device__ __forceinline__ void run_stage(const LD& ld, const ST& st, const cplx
 // gld / gst themselves address the exchange buffer (so the stage must barrier
 // between its reads and its writes).  Returns after the last stage's stores
 // without a trailing barrier.
-template <typename R, int L, int TP, bool IN_LDS, bool OUT_LDS, int S = 0, class GLD, class GST, class LLD, class LST>
+// BR0: the first stage's loader takes (n, b, r) (stage_rn's BR).
+template <typename R, int L, int TP, bool IN_LDS, bool OUT_LDS, bool BR0 = false, int S = 0, class GLD, class GST,
+          class LLD, class LST>
 __device__ __forceinline__ void transform(const GLD& gld, const GST& gst, const LLD& lld, const LST& lst,
                                           const cplx<R>* __restrict__ tw, int t, int sign) {
     constexpr int NST = nstages(L);
     static_assert(NST >= 0, "length must be 2/3/5-smooth");
     if constexpr (NST == 0) {
         if (t == 0) {
-            const cplx<R> v = gld(0);
+            cplx<R> v;
+            if constexpr (BR0) v = gld(0, 0, 0);
+            else v = gld(0);
             if constexpr (IN_LDS && OUT_LDS) __syncthreads();
             gst(0, v);
         } else if constexpr (IN_LDS && OUT_LDS) {
@@ -116,9 +141,9 @@ __device__ __forceinline__ void transform(const GLD& gld, const GST& gst, const 
     } else if constexpr (S < NST) {
         constexpr bool FIRST = (S == 0), LAST = (S == NST - 1);
         if constexpr (FIRST && LAST) {
-            run_stage<R, L, TP, S, IN_LDS && OUT_LDS>(gld, gst, tw, t, sign);
+            run_stage<R, L, TP, S, IN_LDS && OUT_LDS, BR0>(gld, gst, tw, t, sign);
         } else if constexpr (FIRST) {
-            run_stage<R, L, TP, S, IN_LDS>(gld, lst, tw, t, sign);
+            run_stage<R, L, TP, S, IN_LDS, BR0>(gld, lst, tw, t, sign);
             __syncthreads();
         } else if constexpr (LAST) {
             run_stage<R, L, TP, S, OUT_LDS>(lld, gst, tw, t, sign);
@@ -126,7 +151,7 @@ __device__ __forceinline__ void transform(const GLD& gld, const GST& gst, const 
             run_stage<R, L, TP, S, true>(lld, lst, tw, t, sign);
             __syncthreads();
         }
-        transform<R, L, TP, IN_LDS, OUT_LDS, S + 1>(gld, gst, lld, lst, tw, t, sign);
+        transform<R, L, TP, IN_LDS, OUT_LDS, BR0, S + 1>(gld, gst, lld, lst, tw, t, sign);
     }
 }
 
