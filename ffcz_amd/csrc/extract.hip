@@ -118,12 +118,19 @@ __device__ __forceinline__ void tile_compact(const unsigned int tile, const T* v
     // the step quantiser's flag by comparisons, away from half a step (the
     // margin is far wider than the rounding of the thresholds and quotients)
     const double flo = q.step * 0.49999, fhi = q.step * 0.50001;
+    // every round's loads first: the flag words stored below could alias
+    // the edits for all the compiler knows, which would hold each round's
+    // loads back until the previous round's stores
+#pragma unroll
+    for (int r = 0; r < RND; ++r) {
+        const size_t i = base + ((size_t)r * 256 + tid) * X_EPT;
+        if (i < n) load8(v, i, n, x[r]);
+    }
 #pragma unroll
     for (int r = 0; r < RND; ++r) {
         const size_t i = base + ((size_t)r * 256 + tid) * X_EPT;
         bits[r] = 0;
         if (i < n) {
-            load8(v, i, n, x[r]);
             unsigned int unsure = 0;
 #pragma unroll
             for (int k = 0; k < X_EPT; ++k) {
