@@ -176,13 +176,22 @@ __device__ __forceinline__ void zfwd_body(const ZArgs<R, T>& a, uint32_t blk) {
         auto gld_br = [&](int n, int b, int r) { return applyr(n, row, rv, raw[b][r]); };
         pen::transform<R, LZ, TP, false, true, true>(gld_br, lst, lld, lst, tw, t, -1);
     } else if constexpr (ROWS) {
-        // the first stage loads 16 adjacent pairs of a row per MFMA batch
-        auto gldp = [&](int n, int pp) {
+        // the first stage loads 16 adjacent pairs of a row per MFMA batch,
+        // all of the thread's loads before the fused work
+        auto pre = [&](int n, int pp) {
             const size_t rw = row0 + pp;
-            return gldr(n, rw, rw < a.rows);
+            return rawr(n, rw, rw < a.rows);
         };
+        auto fin = [&](int n, int pp, const Raw& w) {
+            const size_t rw = row0 + pp;
+            return applyr(n, rw, rw < a.rows, w);
+        };
+        struct {
+            decltype(pre)& pre;
+            decltype(fin)& fin;
+        } ld0{pre, fin};
         auto bst = [&](int n, int pp, cplx<float> v) { buf[mx::rows_idx(n, pp, RS)] = v; };
-        mx::rows_transform<PREC, LZ, P, NT>(gldp, bst, buf, tw, tid, -1);
+        mx::rows_transform<PREC, LZ, P, NT, true>(ld0, bst, buf, tw, tid, -1);
     } else {
         // rows are contiguous along n: a coalesced copy into the tile first
         for (int n = t; n < LZ; n += TP) lst(n, gld(n));

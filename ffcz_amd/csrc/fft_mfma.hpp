@@ -150,7 +150,11 @@ __device__ __forceinline__ Bmat<PREC> dft16_operand(int lane) {
 // tile's columns, adjacent in memory) or 16 adjacent positions of one pencil
 // (ROWS true: pencils are rows, contiguous in memory).  The stage barriers
 // between its loads and its stores, so ld and st may address the same buffer.
-template <int PREC, int L, int NS, int NPEN, int NT, bool ROWS, class LD, class ST>
+// TWO: ld is a pair {pre, fin}: pre(n, p) issues the element's loads and
+// returns them raw, fin(n, p, raw) does its element-wise work; every pre of
+// the thread runs before the first fin (the work's stores would otherwise hold
+// later loads back).
+template <int PREC, int L, int NS, int NPEN, int NT, bool ROWS, bool TWO = false, class LD, class ST>
 __device__ __forceinline__ void r16_stage(const LD& ld, const ST& st, const cplx<float>* __restrict__ tw, int tid,
                                           bool inv) {
     using O = Op<PREC>;
@@ -168,6 +172,27 @@ __device__ __forceinline__ void r16_stage(const LD& ld, const ST& st, const cplx
     const int i = lane & 15, g = lane >> 4;
     V ahi[QPW], alo[QPW];
     float unsc[QPW];
+    auto raw_of = [&]() {
+        if constexpr (TWO) {
+            using RawT = decltype(ld.pre(0, 0));
+            struct Arr {
+                RawT w[QPW][4];
+            } a{};
+#pragma unroll
+            for (int qq = 0; qq < QPW; ++qq) {
+                const int q = wave + qq * NW;
+                if ((NQ % NW) != 0 && q >= NQ) continue;
+                const int b = q * 16 + i;
+                const int p = ROWS ? b / LR : b % NPEN, j = ROWS ? b % LR : b / NPEN;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) a.w[qq][u] = ld.pre(j + (4 * g + u) * LR, p);
+            }
+            return a;
+        } else {
+            return 0;
+        }
+    };
+    const auto raw = raw_of();
 #pragma unroll
     for (int qq = 0; qq < QPW; ++qq) {
         const int q = wave + qq * NW;
@@ -180,7 +205,9 @@ __device__ __forceinline__ void r16_stage(const LD& ld, const ST& st, const cplx
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int r = 4 * g + u;
-            cplx<float> x = ld(j + r * LR, p);
+            cplx<float> x;
+            if constexpr (TWO) x = ld.fin(j + r * LR, p, raw.w[qq][u]);
+            else x = ld(j + r * LR, p);
             if (inv) x.y = -x.y;
             if constexpr (NS > 1) {
                 if (r > 0) x = cmul(x, tw[k * r * TWS]);
@@ -340,7 +367,8 @@ template <int LZ>
 __host__ __device__ constexpr int rows_stride() { return LZ + 1; }
 __device__ __forceinline__ int rows_idx(int n, int p, int rs) { return p * rs + (n ^ ((n >> 4) & 15)); }
 
-template <int PREC, int LZ, int NPEN, int NT, class LD0, class ST1>
+// TWO0: ld0 is a {pre, fin} pair (r16_stage's TWO).
+template <int PREC, int LZ, int NPEN, int NT, bool TWO0 = false, class LD0, class ST1>
 __device__ __forceinline__ void rows_transform(const LD0& ld0, const ST1& st1, cplx<float>* buf,
                                                const cplx<float>* __restrict__ tw, int tid, int sign) {
     static_assert(PREC != PREC_VALU && n16(LZ) == 2 && rest_of(LZ) == 1, "rows_transform is the 16 x 16 transform");
@@ -355,7 +383,7 @@ __device__ __forceinline__ void rows_transform(const LD0& ld0, const ST1& st1, c
     }
     auto bld = [&](int n, int p) { return buf[rows_idx(n, p, RS)]; };
     auto bst = [&](int n, int p, cplx<float> v) { buf[rows_idx(n, p, RS)] = v; };
-    r16_stage<PREC, LZ, 1, NPEN, NT, true>(ld0, bst, tw, tid, sign > 0);
+    r16_stage<PREC, LZ, 1, NPEN, NT, true, TWO0>(ld0, bst, tw, tid, sign > 0);
     __syncthreads();
     r16_stage<PREC, LZ, 16, NPEN, NT, true>(bld, st1, tw, tid, sign > 0);
 }
