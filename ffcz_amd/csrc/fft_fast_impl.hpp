@@ -39,6 +39,19 @@ constexpr int z_rows(int LZ, int V = 0) {
 constexpr int zrows8_stride(int LZ) { return LZ + LZ / 8 + 1; }
 __device__ __forceinline__ int zrows8_idx(int n, int p, int rs) { return p * rs + n + (n >> 3); }
 
+// Element o of a tile that starts at base.  The byte offset is formed in
+// 32 bits (a tile spans far less than 4 GiB), so the access is a uniform
+// base plus a 32-bit per-lane offset instead of 64-bit address arithmetic
+// per element.
+template <typename E>
+__device__ __forceinline__ E& tile_at(E* base, uint32_t o) {
+    return *reinterpret_cast<E*>(reinterpret_cast<char*>(base) + o * (uint32_t)sizeof(E));
+}
+template <typename E>
+__device__ __forceinline__ const E& tile_at(const E* base, uint32_t o) {
+    return *reinterpret_cast<const E*>(reinterpret_cast<const char*>(base) + o * (uint32_t)sizeof(E));
+}
+
 // W_nz^k for k <= M = nz/2 from the stored half k <= M/2:
 // W^(M-k) = -conj(W^k) (the host table is mirrored the same way)
 template <typename R, int M>
@@ -81,8 +94,10 @@ __device__ __forceinline__ void zfwd_body(const ZArgs<R, T>& a, uint32_t blk) {
     __syncthreads();
     const int t = tid % TP, p = tid / TP;
     const size_t row0 = (size_t)blk * P;
-    const size_t row = row0 + p;
-    const bool rv = row < a.rows;
+    const bool rv = row0 + p < a.rows;
+    // the tile's first element: the per-tile bases are uniform, the offsets
+    // of the tile's elements from them 32-bit (tile_at)
+    const size_t t0 = row0 * (size_t)NZ;
     const bool clamp = (MODE == Z_ITER) && a.st->iter > 0;
     const bool fresh = clamp && a.ds_fresh && a.st->iter == 1;
     // the first iteration reads the initial error itself (e is written by the inverse pass)
@@ -90,47 +105,47 @@ __device__ __forceinline__ void zfwd_body(const ZArgs<R, T>& a, uint32_t blk) {
     R* ecopy = (MODE == Z_ITER && !clamp && a.e_init) ? a.e_copy : nullptr;
     unsigned long long nviol = 0;
     double vmax = 0.0;
-    // The loads of element pair n of row rw (ok: rw is in range) and the
-    // pass's fused element-wise work on them, kept apart so a thread can
-    // issue all of its first-stage loads before any of that work (its stores
-    // to ds / e0 could alias later loads for all the compiler knows, and its
-    // branches would keep each load waiting for the previous element).
+    // The loads of element pair n of tile row pp (ok: the row is in range)
+    // and the pass's fused element-wise work on them, kept apart so a thread
+    // can issue all of its first-stage loads before any of that work (its
+    // stores to ds / e0 could alias later loads for all the compiler knows,
+    // and its branches would keep each load waiting for the previous element).
     struct Raw {
         cplx<T> e, s;      // iterate (ITER: e, in s's place unused) / e0 and ds (CERT)
         cplx<double> f;    // C2R(df)/N (CERT with frequency edits)
     };
-    auto rawr = [&](int n, size_t rw, bool ok) -> Raw {
+    auto rawr = [&](int n, int pp, bool ok) -> Raw {
         Raw w{};
         if (!ok) return w;
-        const size_t gi = rw * (size_t)NZ + 2 * (size_t)n;
+        const uint32_t o2 = (uint32_t)(pp * LZ + n);  // element pairs as vectors
         if constexpr (MODE == Z_ITER) {
-            // one 2-element vector load of the iterate (gi is even)
-            const cplx<R> v = *reinterpret_cast<const cplx<R>*>(esrc + gi);
+            const cplx<R> v = tile_at(reinterpret_cast<const cplx<R>*>(esrc + t0), o2);
             w.e = mk((T)v.x, (T)v.y);
         } else if constexpr (MODE == Z_PLAIN) {
-            const cplx<R> v = *reinterpret_cast<const cplx<R>*>(a.in + gi);
+            const cplx<R> v = tile_at(reinterpret_cast<const cplx<R>*>(a.in + t0), o2);
             w.f = mk((double)v.x, (double)v.y);
         } else {
-            w.e = *reinterpret_cast<const cplx<T>*>(a.e0 + gi);
-            w.s = *reinterpret_cast<const cplx<T>*>(a.cds + gi);
-            if (a.dfi) w.f = *reinterpret_cast<const cplx<double>*>(a.dfi + gi);
+            w.e = tile_at(reinterpret_cast<const cplx<T>*>(a.e0 + t0), o2);
+            w.s = tile_at(reinterpret_cast<const cplx<T>*>(a.cds + t0), o2);
+            if (a.dfi) w.f = tile_at(reinterpret_cast<const cplx<double>*>(a.dfi + t0), o2);
         }
         return w;
     };
-    auto applyr = [&](int n, size_t rw, bool ok, const Raw& w) -> cplx<R> {
+    auto applyr = [&](int n, int pp, bool ok, const Raw& w) -> cplx<R> {
         if (!ok) return mk(R(0), R(0));
-        const size_t gi = rw * (size_t)NZ + 2 * (size_t)n;
+        const uint32_t o2 = (uint32_t)(pp * LZ + n);  // pair index within the tile
         if constexpr (MODE == Z_ITER) {
             cplx<R> v = mk((R)w.e.x, (R)w.e.y);
-            if (ecopy) *reinterpret_cast<cplx<R>*>(ecopy + gi) = v;
+            if (ecopy) tile_at(reinterpret_cast<cplx<R>*>(ecopy + t0), o2) = v;
             if (fresh) {
                 cplx<R> d;
                 d.x = clamp_edit(v.x, a.eps);
                 d.y = clamp_edit(v.y, a.eps);
-                *reinterpret_cast<cplx<R>*>(a.ds + gi) = d;
+                tile_at(reinterpret_cast<cplx<R>*>(a.ds + t0), o2) = d;
             } else if (clamp) {
-                clamp_acc(v.x, a.eps, &a.ds[gi]);
-                clamp_acc(v.y, a.eps, &a.ds[gi + 1]);
+                R* dsp = &tile_at(reinterpret_cast<cplx<R>*>(a.ds + t0), o2).x;
+                clamp_acc(v.x, a.eps, dsp);
+                clamp_acc(v.y, a.eps, dsp + 1);
             }
             return v;
         } else if constexpr (MODE == Z_PLAIN) {
@@ -145,19 +160,12 @@ __device__ __forceinline__ void zfwd_body(const ZArgs<R, T>& a, uint32_t blk) {
             d0 += dequant(w.s.x, a.qs);
             d1 += dequant(w.s.y, a.qs);
             const double x0 = fabs(d0) - a.ceps, x1 = fabs(d1) - a.ceps;
-            if (x0 > 0.0) {
-                ++nviol;
-                vmax = x0 > vmax ? x0 : vmax;
-            }
-            if (x1 > 0.0) {
-                ++nviol;
-                vmax = x1 > vmax ? x1 : vmax;
-            }
+            nviol += (x0 > 0.0 ? 1u : 0u) + (x1 > 0.0 ? 1u : 0u);
+            vmax = fmax(vmax, fmax(x0, x1));
             return mk((R)d0, (R)d1);
         }
     };
-    auto gldr = [&](int n, size_t rw, bool ok) { return applyr(n, rw, ok, rawr(n, rw, ok)); };
-    auto gld = [&](int n) { return gldr(n, row, rv); };
+    auto gld = [&](int n) { return applyr(n, p, rv, rawr(n, p, rv)); };
     auto lld = [&](int n) { return buf[bidx(n, p)]; };
     auto lst = [&](int n, cplx<R> v) { buf[bidx(n, p)] = v; };
     if constexpr (PREC == PREC_VALU && MODE == Z_PLAIN) {
@@ -171,21 +179,15 @@ __device__ __forceinline__ void zfwd_body(const ZArgs<R, T>& a, uint32_t blk) {
         for (int b = 0; b < NB0; ++b) {
             const bool has = pen::first_has<LZ, TP>(t, b);
 #pragma unroll
-            for (int r = 0; r < R0; ++r) raw[b][r] = rawr(pen::first_elem<LZ>(t, b, r, TP), row, rv && has);
+            for (int r = 0; r < R0; ++r) raw[b][r] = rawr(pen::first_elem<LZ>(t, b, r, TP), p, rv && has);
         }
-        auto gld_br = [&](int n, int b, int r) { return applyr(n, row, rv, raw[b][r]); };
+        auto gld_br = [&](int n, int b, int r) { return applyr(n, p, rv, raw[b][r]); };
         pen::transform<R, LZ, TP, false, true, true>(gld_br, lst, lld, lst, tw, t, -1);
     } else if constexpr (ROWS) {
         // the first stage loads 16 adjacent pairs of a row per MFMA batch,
         // all of the thread's loads before the fused work
-        auto pre = [&](int n, int pp) {
-            const size_t rw = row0 + pp;
-            return rawr(n, rw, rw < a.rows);
-        };
-        auto fin = [&](int n, int pp, const Raw& w) {
-            const size_t rw = row0 + pp;
-            return applyr(n, rw, rw < a.rows, w);
-        };
+        auto pre = [&](int n, int pp) { return rawr(n, pp, row0 + pp < a.rows); };
+        auto fin = [&](int n, int pp, const Raw& w) { return applyr(n, pp, row0 + pp < a.rows, w); };
         struct {
             decltype(pre)& pre;
             decltype(fin)& fin;
@@ -205,18 +207,19 @@ __device__ __forceinline__ void zfwd_body(const ZArgs<R, T>& a, uint32_t blk) {
         block_max_atomic<double, NT>(vmax, &a.stats->spatial_max_bits);
     }
     __syncthreads();
+    cplx<R>* spec_t = a.spec + row0 * a.Mp;
+    const uint32_t mp = (uint32_t)a.Mp;
     for (int q = tid; q < P * NH; q += NT) {
         const int pp = q / NH;
         const int k = q - pp * NH;
-        const size_t r2 = row0 + pp;
-        if (r2 >= a.rows) continue;
+        if (row0 + pp >= a.rows) continue;
         const int ik = (k == M) ? 0 : k;
         const int im = (k == 0) ? 0 : M - k;
         const cplx<R> zk = buf[bidx(ik, pp)];
         const cplx<R> zm = buf[bidx(im, pp)];
         const cplx<R> A = mk(R(0.5) * (zk.x + zm.x), R(0.5) * (zk.y - zm.y));
         const cplx<R> B = mk(R(0.5) * (zk.y + zm.y), R(0.5) * (zm.x - zk.x));
-        a.spec[r2 * a.Mp + k] = A + cmul(tw2v(k), B);
+        tile_at(spec_t, (uint32_t)pp * mp + (uint32_t)k) = A + cmul(tw2v(k), B);
     }
 }
 
@@ -251,14 +254,16 @@ __device__ __forceinline__ void zinv_body(const ZArgs<R, T>& a, uint32_t blk) {
     // (a load-store loop would keep one load in flight per thread)
     {
         constexpr int NQ = (P * NH + NT - 1) / NT;
+        const cplx<R>* spec_t = a.spec + row0 * a.Mp;
+        const uint32_t mp = (uint32_t)a.Mp;
         cplx<R> xv[NQ];
 #pragma unroll
         for (int u = 0; u < NQ; ++u) {
             const int q = tid + u * NT;
             const int pp = q / NH;
             const int k = q - pp * NH;
-            const size_t r2 = row0 + pp;
-            xv[u] = (q < P * NH && r2 < a.rows) ? a.spec[r2 * a.Mp + k] : mk(R(0), R(0));
+            xv[u] = (q < P * NH && row0 + pp < a.rows) ? tile_at(spec_t, (uint32_t)pp * mp + (uint32_t)k)
+                                                       : mk(R(0), R(0));
         }
 #pragma unroll
         for (int u = 0; u < NQ; ++u) {
@@ -290,24 +295,25 @@ __device__ __forceinline__ void zinv_body(const ZArgs<R, T>& a, uint32_t blk) {
     }
     __syncthreads();
     const int t = tid % TP, p = tid / TP;
-    const size_t row = row0 + p;
-    const bool rv = row < a.rows;
+    const bool rv = row0 + p < a.rows;
+    // the tile's first element: uniform bases, 32-bit offsets (tile_at)
+    const size_t t0 = row0 * (size_t)NZ;
     R vm = R(0);
-    // output pair n of row rw (ok: rw is in range); ITER keeps the violation
-    auto gstr = [&](int n, size_t rw, bool ok, cplx<R> z) {
+    // output pair n of tile row pp (ok: the row is in range); ITER keeps the violation
+    auto gstr = [&](int n, int pp, bool ok, cplx<R> z) {
         if (!ok) return;
-        const size_t gi = rw * (size_t)NZ + 2 * (size_t)n;
+        const uint32_t o2 = (uint32_t)(pp * LZ + n);
         const R v0 = z.x * a.scale, v1 = z.y * a.scale;
         if constexpr (MODE == Z_ITER) {
-            *reinterpret_cast<cplx<R>*>(a.e + gi) = mk(v0, v1);
-            const R a0 = fabs(v0), a1 = fabs(v1);
-            if (a0 > a.eps) vm = fmax(vm, a0 - a.eps);
-            if (a1 > a.eps) vm = fmax(vm, a1 - a.eps);
+            tile_at(reinterpret_cast<cplx<R>*>(a.e + t0), o2) = mk(v0, v1);
+            // max(|v| - eps, 0) without branches: vm >= 0 absorbs the
+            // non-positive excesses and fmax ignores NaN, as the test did
+            vm = fmax(vm, fmax(fabs(v0) - a.eps, fabs(v1) - a.eps));
         } else {
-            *reinterpret_cast<cplx<R>*>(a.out + gi) = mk(v0, v1);
+            tile_at(reinterpret_cast<cplx<R>*>(a.out + t0), o2) = mk(v0, v1);
         }
     };
-    auto gst = [&](int n, cplx<R> z) { gstr(n, row, rv, z); };
+    auto gst = [&](int n, cplx<R> z) { gstr(n, p, rv, z); };
     auto lld = [&](int n) { return buf[bidx(n, p)]; };
     auto lst = [&](int n, cplx<R> v) { buf[bidx(n, p)] = v; };
     if constexpr (PREC == PREC_VALU) {
@@ -315,10 +321,7 @@ __device__ __forceinline__ void zinv_body(const ZArgs<R, T>& a, uint32_t blk) {
     } else if constexpr (ROWS) {
         // the second stage stores 16 adjacent pairs of a row per MFMA batch
         auto bld = [&](int n, int pp) { return buf[mx::rows_idx(n, pp, RS)]; };
-        auto gstp = [&](int n, int pp, cplx<float> z) {
-            const size_t rw = row0 + pp;
-            gstr(n, rw, rw < a.rows, z);
-        };
+        auto gstp = [&](int n, int pp, cplx<float> z) { gstr(n, pp, row0 + pp < a.rows, z); };
         mx::rows_transform<PREC, LZ, P, NT>(bld, gstp, buf, tw, tid, +1);
     } else {
         // a vector-ALU last stage stores the thread's own row (coalesced); a

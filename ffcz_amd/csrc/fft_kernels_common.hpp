@@ -10,28 +10,23 @@ namespace ffcz {
 template <typename R>
 __device__ __forceinline__ void clamp_acc(R& v, R d, R* acc) {
     // projection_algorithm.cu:161-172 / :351-360: the edit is formed first
-    // and then accumulated (a single writer per address).
-    if (v > d) {
-        *acc += d - v;
-        v = d;
-    } else if (v < -d) {
-        *acc += -d - v;
-        v = -d;
-    }
+    // and then accumulated (a single writer per address).  The clamped value
+    // is selected without branches (d - v above the box, -d - v below it);
+    // only the accumulator's read-modify-write is conditional.
+    const bool hi = v > d, lo = v < -d;
+    const R c = hi ? d : (lo ? -d : v);
+    if (hi || lo) *acc += c - v;
+    v = c;
 }
 
 // The same clamp when the accumulator is known to be zero: returns the edit
 // (0 where nothing is clamped) for a plain store, bit-identical to adding it to 0.
 template <typename R>
 __device__ __forceinline__ R clamp_edit(R& v, R d) {
-    R e = R(0);
-    if (v > d) {
-        e = d - v;
-        v = d;
-    } else if (v < -d) {
-        e = -d - v;
-        v = -d;
-    }
+    const bool hi = v > d, lo = v < -d;
+    const R c = hi ? d : (lo ? -d : v);
+    const R e = (hi || lo) ? c - v : R(0);
+    v = c;
     return e;
 }
 
