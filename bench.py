@@ -15,6 +15,8 @@ vector-ALU passes and the radix-16 stages as DFT-matrix GEMMs on MFMA in split
 BF16x3 and in scaled split FP16x3.  ``value`` is the fastest certified
 variant; ``variants`` lists every variant's certified GB/s, iteration count,
 per-pass times and forward-FFT error against the FP64 transform of the field.
+``huffman`` times the entropy coder on the last solve's four symbol streams
+(outside the timed steps), with the oracle encoder beside it.
 
 Prints one JSON line.  ``--gpus`` > 1 is rejected: this build runs on one GPU.
 """
@@ -61,6 +63,7 @@ def parse():
     p.add_argument("--no-graph", action="store_true", help="c4: launch the batched iteration eagerly")
     p.add_argument("--no-autotune", action="store_true", help="keep the default tile variants")
     p.add_argument("--cpu-sample-iters", type=int, default=1)
+    p.add_argument("--no-huffman", action="store_true", help="skip the Huffman encoding measurement")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                    help="only run a plain device copy of BYTES (counter calibration) and exit")
     p.add_argument("--hbm-ceilings", type=int, default=0, metavar="BYTES",
@@ -135,6 +138,71 @@ def cpu_baseline(wl, gpu_iters, sample_iters, cores):
                   f"{'x'.join(map(str, shape))} field (numpy/scipy pocketfft, workers={cores}); "
                   f"t_iter={t_iter:.3f}s t_cert={t_cert:.3f}s, scaled to {gpu_iters} iteration(s) + 2 certifications",
     }
+
+
+def huffman_bench(solver, torch, dev, reps=3, cpu_sample=2_000_000):
+    """Huffman encoding (HuffmanZSTDCoder.cuh:48-357) of the four symbol
+    streams the reference driver codes (projection_example.cu:202-246): the
+    quantised frequency and spatial edits (u16) and the packed flags (u8) of
+    the last solve, symbols resident in HBM.  Rate = input symbol bytes per
+    second of the whole encode call (histogram, host codebook, bit emission,
+    bitstream back on the host; ZSTD is host libzstd and left out).  The
+    device bitstream of the stream with the most coded bits is checked bit
+    for bit against the oracle encoder, which is also timed on a bounded
+    sample of it."""
+    import ctypes as C
+
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from ffcz_amd._lib import call
+    from ffcz_amd.entropy import _collect
+    from oracle import ffcz_oracle as O
+
+    streams = {
+        "freq_quant_u16": solver.get_active_freq_quant_edit_distances(),
+        "spatial_quant_u16": solver.get_active_spatial_quant_edit_distances(),
+        "freq_flags_u8": solver.get_freq_flags_pack(),
+        "spatial_flags_u8": solver.get_spatial_flags_pack(),
+    }
+    per, tot_bytes, tot_s = {}, 0, 0.0
+    big, big_res = None, None
+    for name, arr in streams.items():
+        if arr.size == 0:
+            per[name] = {"symbols": 0}
+            continue
+        k = "u16" if arr.dtype == np.uint16 else "u8"
+        dev_arr = torch.from_numpy(arr.view(np.int16) if k == "u16" else arr).to(dev)
+        torch.cuda.synchronize()
+        times, res = [], None
+        for r in range(reps + 1):
+            h = C.c_void_p()
+            t0 = time.perf_counter()
+            call(f"ffcz_huffman_encode_{k}", C.c_void_p(dev_arr.data_ptr()), arr.size, C.byref(h))
+            dt = time.perf_counter() - t0
+            res = _collect(h)
+            if r:  # the first call is a warm-up
+                times.append(dt)
+        t = float(np.median(times))
+        nbytes = arr.size * arr.dtype.itemsize
+        per[name] = {"symbols": int(arr.size), "alphabet": int(res.symbols.size), "bits": int(res.num_bits),
+                     "ms": round(t * 1e3, 3), "GBps": round(nbytes / t / 1e9, 3)}
+        tot_bytes += nbytes
+        tot_s += t
+        if big is None or res.num_bits > big_res.num_bits:
+            big, big_res = arr, res
+        del dev_arr
+    out = {"unit": "GB/s of input symbols", "GBps": round(tot_bytes / tot_s / 1e9, 3) if tot_s else None,
+           "streams": per}
+    if big is not None:
+        ref_bits, _ = O.huffman_encode(big)
+        out["bit_exact_vs_oracle"] = bool(np.array_equal(np.frombuffer(big_res.bitstream, np.uint8), ref_bits))
+        sample = big[:cpu_sample]
+        t0 = time.perf_counter()
+        O.huffman_encode(sample)
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"GBps": round(sample.size * sample.dtype.itemsize / tc / 1e9, 4), "cores": 1,
+                               "kind": "port", "sample": f"first {sample.size} symbols of the stream with the most coded bits "
+                               "(numpy oracle encoder)"}
+    return out
 
 
 def bench_batch(a, torch, dev):
@@ -407,6 +475,8 @@ def main():
 
     results = [run_variant(v) for v in variants]
     del X64
+    # entropy coding of the last solve's symbol streams (after the timed steps)
+    huff = None if a.no_huffman else huffman_bench(solver, torch, dev)
     best = max(results, key=lambda r: (r["value"], r["variant"] == "valu-fp32"))
     ms, t_iter, dom = best["ms"], best["t_iter"], best["dom"]
     dname = names[dom]
@@ -461,6 +531,7 @@ def main():
                                     "tile_vars": r["tile_vars"],
                                     "iteration": iteration_summary(r)} for r in results},
         "cpu_baseline": cpu,
+        "huffman": huff,
     }
     print(json.dumps(out))
     solver.close()

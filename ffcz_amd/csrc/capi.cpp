@@ -714,6 +714,7 @@ int ffcz_k_extract(int dtype, const void* v, size_t n, int kind, double step, ui
         QuantDesc q{};
         q.kind = kind;
         q.step = step;
+        q.inv_step = step > 0 ? 1.0 / step : 0.0;
         q.threshold = 1e-7;
         void* scratch = nullptr;
         FFCZ_HIP(hipMalloc(&scratch, extract_scratch_bytes(n)));

@@ -265,7 +265,7 @@ __device__ __forceinline__ void tile_compact(const unsigned int tile, const T* v
             const T val = sval[i];
             compact[at] = val;
             if (q.kind == Q_STEP) {
-                const unsigned long long z = zigzag64(quant_step((double)val, q.step));
+                const unsigned long long z = zigzag64((long long)quant_step_fi((double)val, q.step, q.inv_step));
                 if (z > 65535ull) ovf = true;
                 quant[at] = (uint16_t)z;
             }

@@ -46,6 +46,7 @@ struct QuantDesc {
     double vmin, vmax;  // Q_REF16 (cast to T inside the kernels)
     const double* dminmax;  // Q_REF16: {vmin, vmax} in device memory, used instead when set
     double step;        // Q_STEP
+    double inv_step;    // Q_STEP: 1/step (0: every quotient by division)
     double threshold;   // Q_REF16 flag threshold (1e-7 in the reference)
 };
 

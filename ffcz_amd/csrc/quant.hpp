@@ -25,6 +25,23 @@ __device__ __forceinline__ double quant_step_f(double v, double step) {
 }
 __device__ __forceinline__ long long quant_step(double v, double step) { return (long long)quant_step_f(v, step); }
 
+// quant_step_f(v, step) with the quotient taken through inv = 1/step (a
+// multiply instead of an FP64 division) where that cannot change the result.
+// r = v*inv is within 1.5*2^-52 (relative) of the rounded quotient v/step, so
+// below 2^24 in magnitude the two differ by less than 6e-9; farther than 1e-6
+// from a half-integer both then round to the same integer (the addition of
+// 1/2 is exact to 2^-29 there).  Near half-integers, for large or non-finite
+// values, and without a reciprocal (inv <= 0) the division decides.
+__device__ __forceinline__ double quant_step_fi(double v, double step, double inv) {
+    const double r = v * inv;
+    const double a = fabs(r);
+    if (inv > 0.0 && a < 16777216.0 && fabs((a - floor(a)) - 0.5) > 1e-6) {
+        const double n = floor(a + 0.5);
+        return r < 0.0 ? -n : n;
+    }
+    return quant_step_f(v, step);
+}
+
 // quant_step_f(v, step) != 0 without the division where the answer is
 // clear: |v/step| < 1/2 rounds to 0, above it to at least 1.  The relative
 // margin 1e-5 is far wider than the rounding of the thresholds and of the
@@ -55,7 +72,7 @@ __device__ __forceinline__ double dequant(T v, const QuantDesc& q) {
     }
     // edits below half a step dequantise to zero (with the quantiser's sign)
     if (!quant_step_nonzero((double)v, q.step)) return v < (T)0 ? -0.0 : 0.0;
-    return quant_step_f((double)v, q.step) * q.step;  // no integer round trip
+    return quant_step_fi((double)v, q.step, q.inv_step) * q.step;  // no integer round trip
 }
 
 }  // namespace ffcz

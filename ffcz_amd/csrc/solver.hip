@@ -392,6 +392,7 @@ QuantDesc Solver<T>::freq_quant() const {
     q.kind = qkind_;
     q.threshold = 1e-7;  // projection_algorithm_impl.cuh:389
     q.step = qstep_f_;
+    q.inv_step = qstep_f_ > 0 ? 1.0 / qstep_f_ : 0.0;
     q.vmin = xf_.vmin;
     q.vmax = xf_.vmax;
     q.dminmax = xglob_ ? &xglob_[0].vmin : nullptr;
@@ -403,6 +404,7 @@ QuantDesc Solver<T>::spatial_quant() const {
     q.kind = qkind_;
     q.threshold = 1e-7;  // projection_algorithm_impl.cuh:398
     q.step = qstep_s_;
+    q.inv_step = qstep_s_ > 0 ? 1.0 / qstep_s_ : 0.0;
     q.vmin = xs_.vmin;
     q.vmax = xs_.vmax;
     q.dminmax = xglob_ ? &xglob_[1].vmin : nullptr;

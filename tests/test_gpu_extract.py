@@ -41,6 +41,34 @@ def test_extraction_bit_exact(gpu, shape, dtype):
             assert ref.flags.any()
 
 
+@pytest.mark.parametrize("dtype, dt", [(np.float64, ffcz_amd._lib.FFCZ_F64), (np.float32, ffcz_amd._lib.FFCZ_F32)])
+def test_kernel_extract_step_symbols_near_half_steps(gpu, dtype, dt):
+    # values at and around (k + 1/2) steps, where the quotient's rounding
+    # decides the symbol: the device's reciprocal form must give the
+    # division's symbols (llround(v/step), ties away from zero)
+    step = 1.25e-5
+    rng = np.random.default_rng(3)
+    k = rng.integers(0, 30000, 4000).astype(np.float64)
+    offs = np.array([0.0, 1e-16, -1e-16, 1e-13, -1e-13, 1e-9, -1e-9, 1e-6, -1e-6, 1e-3])
+    near = ((k[:, None] + 0.5) * (1.0 + offs[None, :]) * step).ravel()
+    v = np.concatenate([near, -near, rng.standard_normal(50000) * step * 100]).astype(dtype)
+    n = v.size
+    dv = torch.from_numpy(v).to(gpu)
+    pack = torch.zeros(((n + 63) // 64) * 8, dtype=torch.uint8, device=gpu)
+    comp = torch.zeros(n, dtype=dv.dtype, device=gpu)
+    quant = torch.zeros(n, dtype=torch.int16, device=gpu)
+    cnt, lo, hi = C.c_size_t(), C.c_double(), C.c_double()
+    torch.cuda.synchronize()
+    call("ffcz_k_extract", dt, dv.data_ptr(), n, 2, step, pack.data_ptr(), comp.data_ptr(), quant.data_ptr(),
+         C.byref(cnt), C.byref(lo), C.byref(hi), None)
+    q = O.quantize_step(v, step)
+    act = q != 0
+    kk = cnt.value
+    assert kk == int(act.sum())
+    assert np.array_equal(comp.cpu().numpy()[:kk], v[act])
+    assert np.array_equal(quant.cpu().numpy().view(np.uint16)[:kk], O.zigzag(q[act]).astype(np.uint16))
+
+
 @pytest.mark.parametrize("n", [1, 63, 64, 65, 4095, 4096, 4097, 100_000, 1_000_003])
 def test_kernel_extract_sizes(gpu, n):
     rng = np.random.default_rng(n)

@@ -336,3 +336,38 @@ def test_certify_brute_force():
     assert r["freq_violations"] == int((np.maximum(np.abs(D.real), np.abs(D.imag)) > delta).sum())
     assert not r["ok"]
     assert O.certify(d * 0.5, 1e-3, 1e9)["ok"]
+
+
+# ---------------------------------------------------------------------------
+# step quantiser: the device's reciprocal form decides only where it must
+# agree with the division (restated here in the same IEEE double operations)
+# ---------------------------------------------------------------------------
+def _quant_step_reciprocal(v, step):
+    v = np.asarray(v, dtype=np.float64)
+    inv = np.float64(1.0) / np.float64(step)
+    r = v * inv
+    a = np.abs(r)
+    with np.errstate(invalid="ignore"):
+        fast = (a < 16777216.0) & (np.abs((a - np.floor(a)) - 0.5) > 1e-6)
+    n = np.floor(a + 0.5)
+    return fast, np.where(r < 0.0, -n, n)
+
+
+@pytest.mark.parametrize("step", [1e-5, 1.25e-5, 3.0e-3 / 7, 0.1, 6716.3 / 8])
+def test_step_quantiser_reciprocal_agrees_with_division(step):
+    rng = np.random.default_rng(11)
+    k = rng.integers(0, 1 << 23, 20000).astype(np.float64)
+    # half-integer neighbourhoods at every scale of relative offset, and
+    # plain random values of both signs
+    offs = np.array([0.0, 1e-16, -1e-16, 1e-13, -1e-13, 1e-10, -1e-10, 1e-7, -1e-7, 2e-6, -2e-6, 1e-3])
+    near = ((k[:, None] + 0.5) * (1.0 + offs[None, :]) * step).ravel()
+    rand = rng.standard_normal(200000) * step * rng.choice([1.0, 10.0, 1e4, 1e7], 200000)
+    v = np.concatenate([near, -near, rand, [0.0, -0.0, 0.49 * step, 0.51 * step]])
+    fast, q_fast = _quant_step_reciprocal(v, step)
+    q_div = O.quantize_step(v, step).astype(np.float64)
+    # the multiply decides almost every ordinary value below 2^24 steps
+    moderate = np.abs(rand) < 1e6 * step
+    assert fast[2 * near.size: 2 * near.size + rand.size][moderate].mean() > 0.99
+    assert np.array_equal(q_fast[fast], q_div[fast])
+    # the values the margin sends to the division include the exact half-integers
+    assert not fast[: near.size].reshape(k.size, offs.size)[:, 0].any()
