@@ -16,7 +16,9 @@ BF16x3 and in scaled split FP16x3.  ``value`` is the fastest certified
 variant; ``variants`` lists every variant's certified GB/s, iteration count,
 per-pass times and forward-FFT error against the FP64 transform of the field.
 ``huffman`` times the entropy coder on the last solve's four symbol streams
-(outside the timed steps), with the oracle encoder beside it.
+(outside the timed steps), with the oracle encoder beside it; ``container``
+reports the size of the self-describing container of those streams and the
+time to decode it back into x_hat on the GPU.
 
 Prints one JSON line.  ``--gpus`` > 1 is rejected: this build runs on one GPU.
 """
@@ -63,7 +65,8 @@ def parse():
     p.add_argument("--no-graph", action="store_true", help="c4: launch the batched iteration eagerly")
     p.add_argument("--no-autotune", action="store_true", help="keep the default tile variants")
     p.add_argument("--cpu-sample-iters", type=int, default=1)
-    p.add_argument("--no-huffman", action="store_true", help="skip the Huffman encoding measurement")
+    p.add_argument("--no-huffman", action="store_true",
+                   help="skip the Huffman encoding and container measurements")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                    help="only run a plain device copy of BYTES (counter calibration) and exit")
     p.add_argument("--hbm-ceilings", type=int, default=0, metavar="BYTES",
@@ -203,6 +206,33 @@ def huffman_bench(solver, torch, dev, reps=3, cpu_sample=2_000_000):
                                "kind": "port", "sample": f"first {sample.size} symbols of the stream with the most coded bits "
                                "(numpy oracle encoder)"}
     return out
+
+
+def container_bench(solver, wl, torch, reps=3, level=3):
+    """The container of the last solve's coded edits (ZSTD level 3): its size
+    against the field's N*s bytes, the time to pack it, and the time to decode
+    it back into x_hat = x' + C2R(deq df)/N + deq(ds) with x' and x_hat in HBM."""
+    from ffcz_amd import container
+
+    t0 = time.perf_counter()
+    buf = container.pack(solver, zstd_level=level)
+    t_pack = time.perf_counter() - t0
+    xp = (wl.x + wl.e0).contiguous()
+    xh = torch.empty_like(xp)
+    times = []
+    for r in range(reps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        container.decode(buf, xp, out=xh)
+        if r:
+            times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    nbytes = xp.numel() * xp.element_size()
+    err = float((xh.double() - wl.x.double()).abs().max())
+    del xp, xh
+    return {"bytes": len(buf), "ratio": round(nbytes / len(buf), 2), "zstd_level": level,
+            "pack_ms": round(t_pack * 1e3, 2), "decode_ms": round(t * 1e3, 2),
+            "decode_GBps": round(nbytes / t / 1e9, 3), "max_abs_error": err}
 
 
 def bench_batch(a, torch, dev):
@@ -477,6 +507,7 @@ def main():
     del X64
     # entropy coding of the last solve's symbol streams (after the timed steps)
     huff = None if a.no_huffman else huffman_bench(solver, torch, dev)
+    cont = None if a.no_huffman else container_bench(solver, wl, torch)
     best = max(results, key=lambda r: (r["value"], r["variant"] == "valu-fp32"))
     ms, t_iter, dom = best["ms"], best["t_iter"], best["dom"]
     dname = names[dom]
@@ -532,6 +563,7 @@ def main():
                                     "iteration": iteration_summary(r)} for r in results},
         "cpu_baseline": cpu,
         "huffman": huff,
+        "container": cont,
     }
     print(json.dumps(out))
     solver.close()

@@ -14,10 +14,11 @@ from .entropy import (  # noqa: F401
 )
 from .fft import fft_r2c, fft_c2r  # noqa: F401
 from .batch import ProjectionBatch  # noqa: F401
+from . import container  # noqa: F401
 
 __all__ = [
     "ProjectionSolver", "ProjectionBatch", "CertResult", "FFCzError", "load", "fft_r2c", "fft_c2r",
     "cuda_huffman_zstd_compress", "cuda_huffman_zstd_decompress", "huffman_encode", "CompressionResult",
     "MODE_ABS", "MODE_DIRECT", "MODE_PTW", "QUANT_REF16", "QUANT_STEP",
-    "FFT_VALU", "FFT_BF16X3", "FFT_FP16X3", "FFT_VARIANTS",
+    "FFT_VALU", "FFT_BF16X3", "FFT_FP16X3", "FFT_VARIANTS", "container",
 ]

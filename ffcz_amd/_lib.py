@@ -124,6 +124,12 @@ SIGNATURES = {
     "ffcz_coded_free": [_h],
     "ffcz_huffman_zstd_decompress_u16": [_vp, _sz, _vp, _vp, _vp, _sz, _sz, _sz, _vp, _szp],
     "ffcz_huffman_zstd_decompress_u8": [_vp, _sz, _vp, _vp, _vp, _sz, _sz, _sz, _vp, _szp],
+    "ffcz_container_pack": [_h, _i, C.POINTER(_h)],
+    "ffcz_blob_size": [_h, _szp],
+    "ffcz_blob_copy": [_h, _vp, _sz],
+    "ffcz_blob_free": [_h],
+    "ffcz_container_info": [_vp, _sz, C.POINTER(_i), _szp, C.POINTER(_d), C.POINTER(_d), C.POINTER(_i), _szp],
+    "ffcz_container_decode": [_vp, _sz, _vp, _vp, _vp],
     "ffcz_fft_r2c": [_i, _sz, _sz, _sz, _vp, _vp, _vp],
     "ffcz_fft_c2r": [_i, _sz, _sz, _sz, _vp, _vp, _vp],
     "ffcz_fft_r2c_prec": [_i, _i, _sz, _sz, _sz, _vp, _vp, _vp],
@@ -134,7 +140,7 @@ SIGNATURES = {
     "ffcz_k_quantize_dequantize": [_i, _vp, _vp, _d, _d, _sz, _d, _vp],
     "ffcz_k_extract": [_i, _vp, _sz, _i, _d, _vp, _vp, _vp, _szp, C.POINTER(_d), C.POINTER(_d), _vp],
 }
-_RESTYPES = {"ffcz_last_error": C.c_char_p, "ffcz_coded_free": None}
+_RESTYPES = {"ffcz_last_error": C.c_char_p, "ffcz_coded_free": None, "ffcz_blob_free": None}
 
 _lib = None
 

@@ -29,7 +29,7 @@ FLAGS = [
     "-Wno-unused-result", "-munsafe-fp-atomics",
 ]
 SOURCES = ["fft3d.hip", "fft_fast_f32.hip", "fft_fast_f64.hip", "fft_fast_c32.hip", "fft_fast_mfma.hip", "solver.hip",
-           "batch.hip", "extract.hip",
+           "batch.hip", "extract.hip", "container.hip",
            "huffman.hip", "kernels.hip", "capi.cpp"]
 
 

@@ -8,6 +8,7 @@
 
 #include "../../include/ffcz.h"
 #include "batch.hpp"
+#include "container.hpp"
 #include "extract.hpp"
 #include "fft_fast.hpp"
 #include "huffman.hpp"
@@ -36,6 +37,10 @@ struct ffcz_coded {
     HuffmanCoded h;
     std::vector<uint8_t> zdata;
     bool zstd = false;
+};
+
+struct ffcz_blob {
+    std::vector<uint8_t> bytes;
 };
 
 template <typename F>
@@ -631,6 +636,51 @@ int ffcz_coded_table(const ffcz_coded* c, uint32_t* symbols, uint32_t* codes, in
     });
 }
 void ffcz_coded_free(ffcz_coded* c) { delete c; }
+
+int ffcz_container_pack(ffcz_solver* h, int zstd_level, ffcz_blob** out) {
+    return guard([&] {
+        FFCZ_REQUIRE(out != nullptr, ERR_INVALID, "null output");
+        auto b = std::make_unique<ffcz_blob>();
+        with(h, [&](auto& s) { b->bytes = container_pack(s, zstd_level); });
+        *out = b.release();
+    });
+}
+int ffcz_blob_size(const ffcz_blob* b, size_t* len) {
+    return guard([&] {
+        FFCZ_REQUIRE(b != nullptr && len != nullptr, ERR_INVALID, "null argument");
+        *len = b->bytes.size();
+    });
+}
+int ffcz_blob_copy(const ffcz_blob* b, uint8_t* out, size_t cap) {
+    return guard([&] {
+        FFCZ_REQUIRE(b != nullptr && out != nullptr, ERR_INVALID, "null argument");
+        FFCZ_REQUIRE(cap >= b->bytes.size(), ERR_RANGE, "output capacity too small");
+        std::memcpy(out, b->bytes.data(), b->bytes.size());
+    });
+}
+void ffcz_blob_free(ffcz_blob* b) { delete b; }
+int ffcz_container_info(const uint8_t* buf, size_t len, int* dtype, size_t* dims, double* eps, double* delta,
+                        int* quantizer, size_t* n_active) {
+    return guard([&] {
+        const ContainerInfo in = container_info(buf, len);
+        if (dtype) *dtype = in.dtype;
+        if (dims) {
+            dims[0] = in.nx;
+            dims[1] = in.ny;
+            dims[2] = in.nz;
+        }
+        if (eps) *eps = in.eps;
+        if (delta) *delta = in.delta;
+        if (quantizer) *quantizer = in.qkind;
+        if (n_active) {
+            n_active[0] = in.n_freq;
+            n_active[1] = in.n_spatial;
+        }
+    });
+}
+int ffcz_container_decode(const uint8_t* buf, size_t len, const void* x_prime, void* x_hat, void* stream) {
+    return guard([&] { container_decode(buf, len, x_prime, x_hat, as_stream(stream)); });
+}
 
 int ffcz_huffman_zstd_decompress_u16(const uint8_t* comp, size_t clen, const uint32_t* sym, const uint32_t* code,
                                      const int32_t* len, size_t nsym, size_t ne, size_t bb, uint16_t* out,

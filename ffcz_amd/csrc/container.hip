@@ -1,0 +1,541 @@
+// Container of one solve's coded edits, and the GPU reconstruction.
+//
+// Layout (little-endian; offsets in bytes):
+//   header (128):
+//     0 "FFCZ"   4 u32 version (1)   8 u32 dtype (0 float, 1 double)   12 u32 mode
+//     16 u64 nx  24 u64 ny  32 u64 nz
+//     40 f64 eps  48 f64 delta (the abs / ptw bound; NaN for per-bin bounds)
+//     56 u32 quantiser (1 reference 16-bit, 2 step)  60 u32 streams (4)
+//     64 f64 frequency step  72 f64 spatial step  (step quantiser)
+//     80 f64 frequency min, max  96 f64 spatial min, max  (the reference quantiser's ranges)
+//     112 u64 active frequency edits  120 u64 active spatial edits
+//   four streams, in the order the reference driver codes them
+//   (projection_example.cu:202-246): frequency flags (u8 bytes of the
+//   LSB-first flag pack, ceil(2F/8)), spatial flags (ceil(N/8)), frequency
+//   symbols (u16), spatial symbols (u16).  Each stream:
+//     u32 symbol width (1, 2)  u32 zstd (payload is ZSTD of the bitstream)
+//     u64 symbols  u64 coded bits  u64 bitstream bytes  u64 payload bytes
+//     u32 table size k  u32 chunk (symbols per index entry, 1024)
+//     u32 symbol[k]  u32 code[k]  u8 length[k]   (the encoder's own codes)
+//     u16 coded bits of each chunk of symbols[ceil(symbols/chunk)]
+//     payload
+// The bitstream is the reference encoder's (HuffmanZSTDCoder.cuh:307-357);
+// the chunk index is side information that lets a decoder start anywhere.
+#include <cmath>
+#include <cstring>
+
+#include "container.hpp"
+#include "huffman.hpp"
+
+namespace ffcz {
+
+namespace {
+
+constexpr char MAGIC[4] = {'F', 'F', 'C', 'Z'};
+constexpr uint32_t VERSION = 1;
+constexpr size_t HDR_BYTES = 128;
+constexpr uint32_t CHUNK = 1024;  // 1024 codes of <= 32 bits: a chunk's bit count fits 16 bits
+
+struct Writer {
+    std::vector<uint8_t>& b;
+    template <typename V>
+    void put(V v) {
+        const size_t o = b.size();
+        b.resize(o + sizeof(V));
+        std::memcpy(b.data() + o, &v, sizeof(V));
+    }
+    void bytes(const void* p, size_t n) {
+        const size_t o = b.size();
+        b.resize(o + n);
+        if (n) std::memcpy(b.data() + o, p, n);
+    }
+};
+
+struct Reader {
+    const uint8_t* p;
+    size_t n;
+    size_t o = 0;
+    const uint8_t* take(size_t k) {
+        FFCZ_REQUIRE(o <= n && n - o >= k, ERR_INVALID, "truncated container");
+        const uint8_t* q = p + o;
+        o += k;
+        return q;
+    }
+    template <typename V>
+    V get() {
+        V v;
+        std::memcpy(&v, take(sizeof(V)), sizeof(V));
+        return v;
+    }
+};
+
+// device memory owned for the length of one call
+struct DBuf {
+    void* p = nullptr;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() {
+        if (p) (void)hipFree(p);
+    }
+    template <typename V>
+    V* alloc(size_t n) {
+        FFCZ_HIP(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(V)));
+        return static_cast<V*>(p);
+    }
+};
+
+// Waits for the stream when it goes out of scope, so that kernels still
+// reading call-owned buffers (an exception's way out included) finish before
+// the buffers declared ahead of it are freed.
+struct StreamWait {
+    hipStream_t s;
+    ~StreamWait() { (void)hipStreamSynchronize(s); }
+};
+
+bool on_device(const void* p) {
+    hipPointerAttribute_t a{};
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();  // a host pointer is not an error here
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+// coded bits of each CHUNK-symbol chunk (one workgroup per chunk)
+template <typename S>
+__global__ __launch_bounds__(256) void k_chunk_bits(const S* d, size_t n, const uint8_t* lens, uint16_t* out) {
+    __shared__ unsigned int red[256];
+    const size_t c0 = (size_t)blockIdx.x * CHUNK;
+    unsigned int b = 0;
+    for (unsigned int i = threadIdx.x; i < CHUNK; i += 256) {
+        const size_t j = c0 + i;
+        if (j < n) b += lens[d[j]];
+    }
+    red[threadIdx.x] = b;
+    __syncthreads();
+    for (unsigned int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = (uint16_t)red[0];
+}
+
+template <typename S>
+void pack_stream(Writer& w, const S* d, size_t n, int level, hipStream_t s) {
+    w.put<uint32_t>((uint32_t)sizeof(S));
+    if (n == 0) {
+        w.put<uint32_t>(0);
+        for (int i = 0; i < 4; ++i) w.put<uint64_t>(0);
+        w.put<uint32_t>(0);
+        w.put<uint32_t>(CHUNK);
+        return;
+    }
+    const HuffmanCoded h = huffman_encode_device<S>(d, n, s);
+    // the chunk index from the code lengths of the symbols in place
+    constexpr size_t NB = (size_t)1 << (8 * sizeof(S));
+    std::vector<uint8_t> hl(NB, 0);
+    for (size_t k = 0; k < h.table.sym.size(); ++k) hl[h.table.sym[k]] = (uint8_t)h.table.len[k];
+    const size_t nchunks = ceil_div(n, (size_t)CHUNK);
+    std::vector<uint16_t> cbits(nchunks);
+    {
+        DBuf dl, dc;
+        const StreamWait wait{s};
+        uint8_t* lens = dl.alloc<uint8_t>(NB);
+        uint16_t* cb = dc.alloc<uint16_t>(nchunks);
+        FFCZ_HIP(hipMemcpyAsync(lens, hl.data(), NB, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL((k_chunk_bits<S>), dim3((unsigned)nchunks), dim3(256), 0, s, d, n, lens, cb);
+        FFCZ_HIP(hipGetLastError());
+        FFCZ_HIP(hipMemcpyAsync(cbits.data(), cb, nchunks * sizeof(uint16_t), hipMemcpyDeviceToHost, s));
+        FFCZ_HIP(hipStreamSynchronize(s));
+    }
+    const bool z = level > 0;
+    const std::vector<uint8_t> payload = z ? zstd_compress(h.bitstream.data(), h.bitstream.size(), level) : h.bitstream;
+    w.put<uint32_t>(z ? 1u : 0u);
+    w.put<uint64_t>(n);
+    w.put<uint64_t>(h.nbits);
+    w.put<uint64_t>(h.bitstream.size());
+    w.put<uint64_t>(payload.size());
+    const size_t k = h.table.sym.size();
+    w.put<uint32_t>((uint32_t)k);
+    w.put<uint32_t>(CHUNK);
+    w.bytes(h.table.sym.data(), k * sizeof(uint32_t));
+    w.bytes(h.table.code.data(), k * sizeof(uint32_t));
+    for (size_t i = 0; i < k; ++i) w.put<uint8_t>((uint8_t)h.table.len[i]);
+    w.bytes(cbits.data(), nchunks * sizeof(uint16_t));
+    w.bytes(payload.data(), payload.size());
+}
+
+struct Stream {
+    uint32_t width = 0, zstd = 0, chunk = 0;
+    uint64_t nsym = 0, nbits = 0, bsbytes = 0, plbytes = 0;
+    HuffmanTable t;
+    std::vector<uint16_t> cbits;
+    const uint8_t* payload = nullptr;
+};
+
+Stream read_stream(Reader& r, uint32_t width) {
+    Stream st;
+    st.width = r.get<uint32_t>();
+    FFCZ_REQUIRE(st.width == width, ERR_INVALID, "container stream has the wrong symbol width");
+    st.zstd = r.get<uint32_t>();
+    st.nsym = r.get<uint64_t>();
+    st.nbits = r.get<uint64_t>();
+    st.bsbytes = r.get<uint64_t>();
+    st.plbytes = r.get<uint64_t>();
+    const uint32_t k = r.get<uint32_t>();
+    st.chunk = r.get<uint32_t>();
+    FFCZ_REQUIRE(st.zstd <= 1 && st.chunk == CHUNK, ERR_INVALID, "unsupported container stream");
+    FFCZ_REQUIRE(st.nsym <= ((uint64_t)1 << 44), ERR_INVALID, "container stream too large");
+    FFCZ_REQUIRE(st.bsbytes == (st.nbits + 7) / 8 && st.nbits <= 32 * st.nsym && st.nbits >= st.nsym, ERR_INVALID,
+                 "inconsistent container stream sizes");
+    FFCZ_REQUIRE((st.nsym == 0) == (k == 0) && k <= ((uint64_t)1 << (8 * width)), ERR_INVALID,
+                 "bad container code table size");
+    const uint8_t* sym = r.take((size_t)k * 4);
+    const uint8_t* code = r.take((size_t)k * 4);
+    const uint8_t* len = r.take(k);
+    st.t.sym.resize(k);
+    st.t.code.resize(k);
+    st.t.len.resize(k);
+    if (k) {
+        std::memcpy(st.t.sym.data(), sym, (size_t)k * 4);
+        std::memcpy(st.t.code.data(), code, (size_t)k * 4);
+    }
+    for (uint32_t i = 0; i < k; ++i) {
+        st.t.len[i] = len[i];
+        FFCZ_REQUIRE(len[i] >= 1 && len[i] <= 32, ERR_INVALID, "bad container code length");
+        FFCZ_REQUIRE(len[i] == 32 || (st.t.code[i] >> len[i]) == 0, ERR_INVALID, "bad container code");
+        FFCZ_REQUIRE(st.t.sym[i] < ((uint64_t)1 << (8 * width)), ERR_INVALID, "bad container symbol");
+    }
+    const size_t nchunks = ceil_div((size_t)st.nsym, (size_t)CHUNK);
+    const uint8_t* cb = r.take(nchunks * sizeof(uint16_t));
+    st.cbits.resize(nchunks);
+    if (nchunks) std::memcpy(st.cbits.data(), cb, nchunks * sizeof(uint16_t));
+    uint64_t tot = 0;
+    for (uint16_t c : st.cbits) tot += c;
+    FFCZ_REQUIRE(tot == st.nbits, ERR_INVALID, "container chunk index does not add up to the stream");
+    st.payload = r.take(st.plbytes);
+    FFCZ_REQUIRE(st.zstd || st.plbytes == st.bsbytes, ERR_INVALID, "inconsistent container payload size");
+    return st;
+}
+
+template <typename S>
+std::vector<S> decode_stream(const Stream& st) {
+    if (st.nsym == 0) return {};
+    std::vector<uint8_t> raw;
+    const uint8_t* bits = st.payload;
+    if (st.zstd) {
+        raw = zstd_decompress(st.payload, st.plbytes, st.bsbytes);
+        FFCZ_REQUIRE(raw.size() == st.bsbytes, ERR_INVALID, "container payload does not decompress to its bitstream");
+        bits = raw.data();
+    }
+    std::vector<S> out = huffman_decode<S>(bits, st.bsbytes, st.t, st.nsym);
+    FFCZ_REQUIRE(out.size() == st.nsym, ERR_INVALID, "corrupt Huffman stream in container");
+    return out;
+}
+
+size_t popcount_first(const std::vector<uint8_t>& pack, size_t n) {
+    size_t c = 0;
+    const size_t full = n / 8;
+    for (size_t i = 0; i < full; ++i) c += (size_t)__builtin_popcount(pack[i]);
+    if (n % 8) c += (size_t)__builtin_popcount(pack[full] & ((1u << (n % 8)) - 1u));
+    return c;
+}
+
+// ---- reconstruction kernels -------------------------------------------------
+template <typename T>
+struct SymDesc {
+    int kind;
+    double step;
+    T mn, mx;
+};
+
+// Value of a coded edit: the reference quantiser's dequantisation
+// (projection_algorithm.cu:629-643, in T, no contraction) or q*step for the
+// zigzag step symbols.
+template <typename T, typename S>
+__device__ __forceinline__ T sym_value(S z, const SymDesc<T>& q) {
+#pragma clang fp contract(off)
+    if (q.kind == Q_REF16) {
+        const T qv = (T)z;
+        return qv / (T)65535 * (q.mx - q.mn) + q.mn;
+    }
+    const long long k = (long long)(z >> 1) ^ -(long long)(z & 1);
+    return (T)((double)k * q.step);
+}
+
+// word i of a flag pack with the bits at or beyond n cleared
+__device__ __forceinline__ uint32_t flag_word(const uint32_t* w, size_t i, size_t n) {
+    uint32_t v = w[i];
+    const size_t b0 = i * 32;
+    if (b0 + 32 > n) v &= n > b0 ? ((1u << (uint32_t)(n - b0)) - 1u) : 0u;
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_block_popc(const uint32_t* w, size_t nwords, size_t n, uint32_t* bsum) {
+    __shared__ uint32_t red[256];
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    red[threadIdx.x] = i < nwords ? (uint32_t)__popc(flag_word(w, i, n)) : 0u;
+    __syncthreads();
+    for (unsigned int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = red[0];
+}
+
+// exclusive prefix of the block counts, one workgroup
+__global__ __launch_bounds__(1024) void k_scan_blocks(const uint32_t* bsum, size_t nb, unsigned long long* boff) {
+    __shared__ unsigned long long s[1024];
+    const size_t per = (nb + 1023) / 1024;
+    const size_t a = (size_t)threadIdx.x * per, e = a + per < nb ? a + per : nb;
+    unsigned long long t = 0;
+    for (size_t i = a; i < e; ++i) t += bsum[i];
+    s[threadIdx.x] = t;
+    __syncthreads();
+    for (unsigned int off = 1; off < 1024; off <<= 1) {
+        const unsigned long long v = threadIdx.x >= off ? s[threadIdx.x - off] : 0ull;
+        __syncthreads();
+        s[threadIdx.x] += v;
+        __syncthreads();
+    }
+    unsigned long long run = threadIdx.x ? s[threadIdx.x - 1] : 0ull;
+    for (size_t i = a; i < e; ++i) {
+        boff[i] = run;
+        run += bsum[i];
+    }
+}
+
+// dense edits: out[i] = value of the next symbol where flag i is set, else 0
+// (one thread per 32-flag word; symbols are taken in index order)
+template <typename T, typename S>
+__global__ __launch_bounds__(256) void k_expand(const uint32_t* w, size_t nwords, size_t n,
+                                                const unsigned long long* boff, const S* sym, size_t m,
+                                                SymDesc<T> q, T* out) {
+    __shared__ uint32_t sc[256];
+    const size_t wi = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const uint32_t v = wi < nwords ? flag_word(w, wi, n) : 0u;
+    const uint32_t c = (uint32_t)__popc(v);
+    sc[threadIdx.x] = c;
+    __syncthreads();
+    for (unsigned int off = 1; off < 256; off <<= 1) {
+        const uint32_t t = threadIdx.x >= off ? sc[threadIdx.x - off] : 0u;
+        __syncthreads();
+        sc[threadIdx.x] += t;
+        __syncthreads();
+    }
+    if (wi >= nwords) return;
+    unsigned long long pos = boff[blockIdx.x] + sc[threadIdx.x] - c;
+    const size_t i0 = wi * 32;
+    for (int j = 0; j < 32; ++j) {
+        const size_t i = i0 + (size_t)j;
+        if (i >= n) break;
+        T val = T(0);
+        if ((v >> j) & 1u) {
+            if (pos < m) val = sym_value<T, S>(sym[pos], q);
+            ++pos;
+        }
+        out[i] = val;
+    }
+}
+
+// split [Re | Im] frequency edits into the padded half spectrum
+template <typename T>
+__global__ void k_split_to_spec(const T* dfd, size_t F, size_t rows, size_t nh, size_t mp, cplx<T>* spec) {
+    const size_t total = rows * mp;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
+        const size_t r = q / mp, k = q - r * mp;
+        cplx<T> v = mk(T(0), T(0));
+        if (k < nh) {
+            const size_t i = r * nh + k;
+            v = mk(dfd[i], dfd[F + i]);
+        }
+        spec[q] = v;
+    }
+}
+
+// x_hat = (x' + C2R(deq df)/N) + deq(ds)
+template <typename T>
+__global__ void k_sum_edits(const T* xp, const T* rf, const T* ds, T* xh, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        xh[i] = rf ? (xp[i] + rf[i]) + ds[i] : xp[i] + ds[i];
+}
+
+unsigned grid_for(size_t n) { return (unsigned)std::min<size_t>(std::max<size_t>(ceil_div(n, (size_t)256), 1), 65536); }
+
+// dense edits from a flag pack and its symbols
+template <typename T, typename S>
+void expand(const std::vector<uint8_t>& pack, size_t n, const std::vector<S>& sym, const SymDesc<T>& q, T* out,
+            hipStream_t s) {
+    const size_t nwords = ceil_div(n, (size_t)32), nblk = ceil_div(nwords, (size_t)256);
+    DBuf dw, ds, db, dboff;
+    const StreamWait wait{s};
+    uint32_t* w = dw.alloc<uint32_t>(nwords);
+    FFCZ_HIP(hipMemsetAsync(w, 0, nwords * 4, s));
+    FFCZ_HIP(hipMemcpyAsync(w, pack.data(), pack.size(), hipMemcpyHostToDevice, s));
+    S* dsym = ds.alloc<S>(sym.size());
+    if (!sym.empty()) FFCZ_HIP(hipMemcpyAsync(dsym, sym.data(), sym.size() * sizeof(S), hipMemcpyHostToDevice, s));
+    uint32_t* bsum = db.alloc<uint32_t>(nblk);
+    unsigned long long* boff = dboff.alloc<unsigned long long>(nblk);
+    hipLaunchKernelGGL(k_block_popc, dim3((unsigned)nblk), dim3(256), 0, s, (const uint32_t*)w, nwords, n, bsum);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, (const uint32_t*)bsum, nblk, boff);
+    hipLaunchKernelGGL((k_expand<T, S>), dim3((unsigned)nblk), dim3(256), 0, s, (const uint32_t*)w, nwords, n,
+                       (const unsigned long long*)boff, (const S*)dsym, sym.size(), q, out);
+    FFCZ_HIP(hipGetLastError());
+    FFCZ_HIP(hipStreamSynchronize(s));  // the staging buffers are freed on return
+}
+
+template <typename T>
+void decode_typed(const ContainerInfo& in, const std::vector<uint8_t>& fpack, const std::vector<uint8_t>& spack,
+                  const std::vector<uint16_t>& fsym, const std::vector<uint16_t>& ssym, const SymDesc<T>& qf,
+                  const SymDesc<T>& qs, const void* x_prime, void* x_hat, hipStream_t s) {
+    Fft3d<T> f;
+    f.create(in.nx, in.ny, in.nz);
+    struct FftGuard {
+        Fft3d<T>& f;
+        ~FftGuard() { f.destroy(); }
+    } fg{f};
+    const size_t N = f.N, F = f.F;
+    DBuf bxp, bxh, bds, bdf, bsp, brf;
+    const StreamWait wait{s};
+    const T* xp = static_cast<const T*>(x_prime);
+    if (!on_device(x_prime)) {
+        T* t = bxp.alloc<T>(N);
+        FFCZ_HIP(hipMemcpyAsync(t, x_prime, N * sizeof(T), hipMemcpyHostToDevice, s));
+        xp = t;
+    }
+    const bool dev_out = on_device(x_hat);
+    T* xh = dev_out ? static_cast<T*>(x_hat) : bxh.alloc<T>(N);
+    T* dsd = bds.alloc<T>(N);
+    expand<T, uint16_t>(spack, N, ssym, qs, dsd, s);
+    T* rf = nullptr;
+    if (!fsym.empty()) {
+        // C2R(deq df)/N: the split edits into the padded half spectrum, then
+        // the library's inverse transform scaled by 1/N (impl:242)
+        T* dfd = bdf.alloc<T>(2 * F);
+        expand<T, uint16_t>(fpack, 2 * F, fsym, qf, dfd, s);
+        cplx<T>* spec = bsp.alloc<cplx<T>>(f.spec_elems());
+        hipLaunchKernelGGL((k_split_to_spec<T>), dim3(grid_for(f.spec_elems())), dim3(256), 0, s, (const T*)dfd, F,
+                           f.rows, f.Nh, f.Mp, spec);
+        FFCZ_HIP(hipGetLastError());
+        rf = brf.alloc<T>(N);
+        fft_c2r<T>(f, spec, rf, T(1) / static_cast<T>(N), s);
+    }
+    hipLaunchKernelGGL((k_sum_edits<T>), dim3(grid_for(N)), dim3(256), 0, s, xp, (const T*)rf, (const T*)dsd, xh, N);
+    FFCZ_HIP(hipGetLastError());
+    if (!dev_out) FFCZ_HIP(hipMemcpyAsync(x_hat, xh, N * sizeof(T), hipMemcpyDeviceToHost, s));
+    FFCZ_HIP(hipStreamSynchronize(s));
+}
+
+struct Header {
+    ContainerInfo info;
+    double fstep = 0, sstep = 0, fmin = 0, fmax = 0, smin = 0, smax = 0;
+};
+
+Header read_header(Reader& r) {
+    Header h;
+    const uint8_t* m = r.take(4);
+    FFCZ_REQUIRE(std::memcmp(m, MAGIC, 4) == 0, ERR_INVALID, "not an FFCz container");
+    FFCZ_REQUIRE(r.get<uint32_t>() == VERSION, ERR_INVALID, "unsupported FFCz container version");
+    h.info.dtype = (int)r.get<uint32_t>();
+    h.info.mode = (int)r.get<uint32_t>();
+    h.info.nx = r.get<uint64_t>();
+    h.info.ny = r.get<uint64_t>();
+    h.info.nz = r.get<uint64_t>();
+    h.info.eps = r.get<double>();
+    h.info.delta = r.get<double>();
+    h.info.qkind = (int)r.get<uint32_t>();
+    const uint32_t ns = r.get<uint32_t>();
+    h.fstep = r.get<double>();
+    h.sstep = r.get<double>();
+    h.fmin = r.get<double>();
+    h.fmax = r.get<double>();
+    h.smin = r.get<double>();
+    h.smax = r.get<double>();
+    h.info.n_freq = r.get<uint64_t>();
+    h.info.n_spatial = r.get<uint64_t>();
+    const auto& in = h.info;
+    FFCZ_REQUIRE(in.dtype == 0 || in.dtype == 1, ERR_INVALID, "bad container dtype");
+    FFCZ_REQUIRE(in.qkind == Q_REF16 || in.qkind == Q_STEP, ERR_INVALID, "bad container quantiser");
+    FFCZ_REQUIRE(ns == 4, ERR_INVALID, "bad container stream count");
+    FFCZ_REQUIRE(in.nx >= 1 && in.ny >= 1 && in.nz >= 1 && in.nx <= (1u << 20) && in.ny <= (1u << 20) &&
+                     in.nz <= (1u << 20) && in.nx * in.ny <= ((size_t)1 << 40) / in.nz,
+                 ERR_INVALID, "bad container shape");
+    FFCZ_REQUIRE(in.qkind != Q_STEP || (h.fstep > 0 && h.sstep > 0), ERR_INVALID, "bad container quantiser steps");
+    return h;
+}
+
+}  // namespace
+
+template <typename T>
+std::vector<uint8_t> container_pack(Solver<T>& s, int zstd_level) {
+    FFCZ_REQUIRE(s.extracted(), ERR_STATE, "extract_compact_representation() has not run");
+    const size_t nf = s.n_active_freq(), ns = s.n_active_spatial();  // waits for the extraction
+    const QuantDesc qf = s.freq_quant(), qs = s.spatial_quant();
+    std::vector<uint8_t> buf;
+    buf.reserve(HDR_BYTES);
+    Writer w{buf};
+    w.bytes(MAGIC, 4);
+    w.put<uint32_t>(VERSION);
+    w.put<uint32_t>(sizeof(T) == 4 ? 0u : 1u);
+    w.put<uint32_t>((uint32_t)s.init_mode());
+    w.put<uint64_t>(s.nx());
+    w.put<uint64_t>(s.ny());
+    w.put<uint64_t>(s.nz());
+    w.put<double>((double)s.eps());
+    w.put<double>(s.init_mode() == MODE_DIRECT ? std::nan("") : (double)s.delta());
+    w.put<uint32_t>((uint32_t)qf.kind);
+    w.put<uint32_t>(4);
+    w.put<double>(qf.kind == Q_STEP ? qf.step : 0.0);
+    w.put<double>(qs.kind == Q_STEP ? qs.step : 0.0);
+    w.put<double>(s.freq_min());
+    w.put<double>(s.freq_max());
+    w.put<double>(s.spatial_min());
+    w.put<double>(s.spatial_max());
+    w.put<uint64_t>(nf);
+    w.put<uint64_t>(ns);
+    const hipStream_t st = s.stream();
+    pack_stream<uint8_t>(w, s.d_freq_pack(), ceil_div(2 * s.F(), (size_t)8), zstd_level, st);
+    pack_stream<uint8_t>(w, s.d_spatial_pack(), ceil_div(s.N(), (size_t)8), zstd_level, st);
+    pack_stream<uint16_t>(w, s.d_freq_quant(), nf, zstd_level, st);
+    pack_stream<uint16_t>(w, s.d_spatial_quant(), ns, zstd_level, st);
+    return buf;
+}
+
+ContainerInfo container_info(const uint8_t* buf, size_t len) {
+    FFCZ_REQUIRE(buf != nullptr, ERR_INVALID, "null container");
+    Reader r{buf, len};
+    return read_header(r).info;
+}
+
+void container_decode(const uint8_t* buf, size_t len, const void* x_prime, void* x_hat, hipStream_t s) {
+    FFCZ_REQUIRE(buf != nullptr && x_prime != nullptr && x_hat != nullptr, ERR_INVALID, "null argument");
+    Reader r{buf, len};
+    const Header h = read_header(r);
+    const ContainerInfo& in = h.info;
+    const size_t N = in.nx * in.ny * in.nz, F = in.nx * in.ny * (in.nz / 2 + 1);
+    const Stream sf = read_stream(r, 1), ss = read_stream(r, 1), qf = read_stream(r, 2), qs = read_stream(r, 2);
+    FFCZ_REQUIRE(sf.nsym == ceil_div(2 * F, (size_t)8) && ss.nsym == ceil_div(N, (size_t)8), ERR_INVALID,
+                 "container flag streams do not match the shape");
+    FFCZ_REQUIRE(qf.nsym == in.n_freq && qs.nsym == in.n_spatial, ERR_INVALID,
+                 "container symbol streams do not match the active counts");
+    // every stream decoded and checked on the host before any device work
+    const std::vector<uint8_t> fpack = decode_stream<uint8_t>(sf), spack = decode_stream<uint8_t>(ss);
+    const std::vector<uint16_t> fsym = decode_stream<uint16_t>(qf), ssym = decode_stream<uint16_t>(qs);
+    FFCZ_REQUIRE(popcount_first(fpack, 2 * F) == in.n_freq && popcount_first(spack, N) == in.n_spatial, ERR_INVALID,
+                 "container flags do not match the active counts");
+    if (in.dtype == 0) {
+        const SymDesc<float> df{in.qkind, h.fstep, (float)h.fmin, (float)h.fmax};
+        const SymDesc<float> ds{in.qkind, h.sstep, (float)h.smin, (float)h.smax};
+        decode_typed<float>(in, fpack, spack, fsym, ssym, df, ds, x_prime, x_hat, s);
+    } else {
+        const SymDesc<double> df{in.qkind, h.fstep, h.fmin, h.fmax};
+        const SymDesc<double> ds{in.qkind, h.sstep, h.smin, h.smax};
+        decode_typed<double>(in, fpack, spack, fsym, ssym, df, ds, x_prime, x_hat, s);
+    }
+}
+
+template std::vector<uint8_t> container_pack<float>(Solver<float>&, int);
+template std::vector<uint8_t> container_pack<double>(Solver<double>&, int);
+
+}  // namespace ffcz

@@ -101,6 +101,9 @@ class Solver {
     size_t ny() const { return f_.ny; }
     size_t nz() const { return f_.nz; }
     int iterations() const { return iters_; }
+    int init_mode() const { return init_mode_; }
+    T eps() const { return eps_; }
+    T delta() const { return init_mode_ == MODE_PTW ? delta_ptw_ : delta_; }
     double violation() const { return viol_; }
     bool converged() const { return converged_; }
     bool extracted() const { return extracted_; }

@@ -50,6 +50,7 @@ extern "C" {
 
 typedef struct ffcz_solver ffcz_solver;
 typedef struct ffcz_coded ffcz_coded;
+typedef struct ffcz_blob ffcz_blob;
 
 typedef struct ffcz_cert {
     unsigned long long spatial_violations; /* elements with |x_hat - x| > eps  */
@@ -240,6 +241,25 @@ int ffcz_huffman_zstd_decompress_u16(const uint8_t* comp, size_t comp_len, const
 int ffcz_huffman_zstd_decompress_u8(const uint8_t* comp, size_t comp_len, const uint32_t* symbols,
                                     const uint32_t* codes, const int32_t* lengths, size_t nsym, size_t num_elements,
                                     size_t bitstream_bytes, uint8_t* out, size_t* out_len);
+
+/* ---- container: the coded edits with everything a decoder needs ---------
+ * No counterpart in the reference, whose code tables live in memory only and
+ * which writes neither shape, type, bounds nor quantiser range
+ * (HuffmanZSTDCoder.cuh:240-246, projection_example.cu:202-246; the survey's
+ * "next" row F1).  The four streams are the ones the reference driver codes,
+ * each Huffman-coded bit-exactly as HuffmanZSTDCoder.cuh:48-357 and
+ * ZSTD-compressed (zstd_level <= 0: stored raw).  Layout: container.hip. */
+int ffcz_container_pack(ffcz_solver* h, int zstd_level, ffcz_blob** out); /* after ffcz_extract */
+int ffcz_blob_size(const ffcz_blob* b, size_t* len);
+int ffcz_blob_copy(const ffcz_blob* b, uint8_t* out, size_t cap);
+void ffcz_blob_free(ffcz_blob* b);
+/* header fields: dims[3], n_active[2] (frequency, spatial); delta is NaN for per-bin bounds */
+int ffcz_container_info(const uint8_t* buf, size_t len, int* dtype, size_t* dims, double* eps, double* delta,
+                        int* quantizer, size_t* n_active);
+/* x_hat = x' + C2R(deq df)/N + deq(ds) on the GPU (the reconstruction of the
+ * reference's commented get_results, projection_algorithm_impl.cuh:912-985);
+ * x_prime and x_hat: N elements of the container's dtype, host or device */
+int ffcz_container_decode(const uint8_t* buf, size_t len, const void* x_prime, void* x_hat, void* stream);
 
 /* ---- per-kernel entries on device pointers (stream may be NULL) ---------
  * 3D transforms with cuFFT's layout (compact half spectrum, interleaved

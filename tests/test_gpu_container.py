@@ -1,0 +1,95 @@
+"""The container on the GPU: the library's containers carry the reference
+encoder's bitstreams of the solver's own streams (bit-exact against the
+oracle coder), and decoding rebuilds x_hat = x' + C2R(deq df)/N + deq(ds) as
+the oracle does in FP64, for both quantisers and both types; decoding a
+container written by the oracle's coder gives the oracle's reconstruction."""
+import numpy as np
+import pytest
+import torch
+
+from ffcz_amd import ProjectionSolver, QUANT_REF16, QUANT_STEP, container
+from oracle import ffcz_oracle as O
+from tests import _container as K
+from tests._cases import binding_case
+from tests.test_container import N, SHAPE, tiny
+
+pytestmark = pytest.mark.gpu
+
+
+def unzigzag(z):
+    z = np.asarray(z, np.int64)
+    return (z >> 1) ^ -(z & 1)
+
+
+def dense(pack, n, values):
+    flags = np.unpackbits(np.asarray(pack, np.uint8), bitorder="little")[:n].astype(bool)
+    out = np.zeros(n, np.float64)
+    out[flags] = values
+    return out
+
+
+def test_decode_of_an_oracle_written_container(gpu):
+    buf = tiny(seed=3)
+    h, st = K.parse(buf)
+    F = SHAPE[0] * SHAPE[1] * (SHAPE[2] // 2 + 1)
+    sym = [O.huffman_decode(s["payload"], (s["sym"], s["codes"], s["lens"]), s["n"]) for s in st]
+    df = dense(sym[0], 2 * F, unzigzag(sym[2]) * 0.1)
+    ds = dense(sym[1], N, unzigzag(sym[3]) * 0.2)
+    xp = np.random.default_rng(4).random(SHAPE).astype(np.float32)
+    ref = xp.astype(np.float64) + O.reconstruct_edit(SHAPE, np.zeros(SHAPE), df, ds)
+    for x_in in (xp, torch.from_numpy(xp).to(gpu)):
+        out = container.decode(buf, x_in)
+        got = out.cpu().numpy() if isinstance(out, torch.Tensor) else out
+        assert np.abs(got.astype(np.float64) - ref).max() < 1e-5
+
+
+@pytest.mark.parametrize("level", [0, 3])
+@pytest.mark.parametrize("quant", ["ref16", "step"])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("shape", [(6, 10, 12), (16, 20, 30)])
+def test_container_round_trip(gpu, shape, dtype, quant, level):
+    x, e0, eps, delta = binding_case(shape, seed=12, dtype=dtype)
+    with ProjectionSolver(*shape, dtype=dtype) as s:
+        s.initialize_all_abs(x, eps, delta)
+        s.set_margins(0.125, 0.125)
+        if quant == "step":
+            s.set_quantizer(QUANT_STEP, delta / 8, eps / 8)
+        else:
+            s.set_quantizer(QUANT_REF16)
+        s.solve_all_abs(e0, 200, 1e-12)
+        cert = s.certify(True)
+        buf = container.pack(s, zstd_level=level)
+        fpack, spack = s.get_freq_flags_pack(), s.get_spatial_flags_pack()
+        fq, sq = s.get_active_freq_quant_edit_distances(), s.get_active_spatial_quant_edit_distances()
+        df, ds = s.get_freq_edits_full(), s.get_spatial_edits_full()
+        fmin, fmax, smin, smax = s.get_min_freq_edit(), s.get_max_freq_edit(), s.get_min_spatial_edit(), \
+            s.get_max_spatial_edit()
+    i = container.info(buf)
+    assert i["shape"] == shape and i["dtype"] == dtype and i["eps"] == float(dtype(eps))
+    assert i["n_active_freq"] == fq.size and i["n_active_spatial"] == sq.size
+    if level == 0:
+        # each stream is the reference encoder's bitstream of the solver's own symbols
+        _, st = K.parse(buf)
+        for s_, arr in zip(st, (fpack, spack, fq, sq)):
+            assert s_["n"] == arr.size
+            if arr.size == 0:
+                continue
+            bits, (sym, codes, lens) = O.huffman_encode(arr)
+            assert np.array_equal(s_["payload"], bits)
+            assert np.array_equal(s_["sym"], sym) and np.array_equal(s_["codes"], codes)
+            assert np.array_equal(s_["lens"], lens)
+    # the decoder's reconstruction against the oracle's, from the same edits
+    if quant == "step":
+        dfq = O.dequantize_step(O.quantize_step(df, delta / 8), delta / 8)
+        dsq = O.dequantize_step(O.quantize_step(ds, eps / 8), eps / 8)
+    else:
+        dfq = O.quantize_and_dequantize_ref(df, fmin, fmax).astype(np.float64)
+        dsq = O.quantize_and_dequantize_ref(ds, smin, smax).astype(np.float64)
+    xp = (x + e0).astype(dtype)
+    ref = xp.astype(np.float64) + O.reconstruct_edit(shape, np.zeros(shape), dfq, dsq)
+    got = container.decode(buf, torch.from_numpy(xp).to(gpu)).cpu().numpy()
+    tol = 2e-6 if dtype == np.float32 else 1e-13
+    assert np.abs(got.astype(np.float64) - ref).max() < tol
+    # the certified bounds hold for the decoded field (the margins cover its rounding)
+    if cert.ok:
+        assert np.abs(got.astype(np.float64) - x.astype(np.float64)).max() <= eps * (1 + 1e-6)
