@@ -219,27 +219,171 @@ Stream read_stream(Reader& r, uint32_t width) {
     return st;
 }
 
-template <typename S>
-std::vector<S> decode_stream(const Stream& st) {
-    if (st.nsym == 0) return {};
-    std::vector<uint8_t> raw;
-    const uint8_t* bits = st.payload;
+// A stream made ready for the device decoder on the host: its bitstream
+// (ZSTD undone), decode tables and chunk bit offsets.  Building the tables
+// checks that the code table is a prefix code.
+constexpr int LUT_BITS = 12;
+constexpr uint32_t LUT_BAD = 0xFFFFFFFFu;
+
+struct Prepared {
+    const Stream* st = nullptr;
+    std::vector<uint8_t> raw;  // decompressed bitstream (ZSTD payloads)
+    const uint8_t* bits = nullptr;
+    // lut[next 12 bits]: (length << 16) | symbol for codes of <= 12 bits, or
+    // bit 31 | the trie node reached after those 12 bits for longer codes
+    std::vector<uint32_t> lut;
+    std::vector<int32_t> child;  // trie: node*2 + bit -> node (-1: none)
+    std::vector<int32_t> leaf;   // node -> symbol (-1: inner node)
+    std::vector<unsigned long long> off;  // bit offset of each chunk, and the total
+};
+
+Prepared prepare(const Stream& st) {
+    Prepared pr;
+    pr.st = &st;
+    if (st.nsym == 0) return pr;
+    pr.bits = st.payload;
     if (st.zstd) {
-        raw = zstd_decompress(st.payload, st.plbytes, st.bsbytes);
-        FFCZ_REQUIRE(raw.size() == st.bsbytes, ERR_INVALID, "container payload does not decompress to its bitstream");
-        bits = raw.data();
+        pr.raw = zstd_decompress(st.payload, st.plbytes, st.bsbytes);
+        FFCZ_REQUIRE(pr.raw.size() == st.bsbytes, ERR_INVALID, "container payload does not decompress to its bitstream");
+        pr.bits = pr.raw.data();
     }
-    std::vector<S> out = huffman_decode<S>(bits, st.bsbytes, st.t, st.nsym);
-    FFCZ_REQUIRE(out.size() == st.nsym, ERR_INVALID, "corrupt Huffman stream in container");
-    return out;
+    pr.child.assign(2, -1);
+    pr.leaf.assign(1, -1);
+    for (size_t k = 0; k < st.t.sym.size(); ++k) {
+        size_t node = 0;
+        for (int b = st.t.len[k] - 1; b >= 0; --b) {
+            FFCZ_REQUIRE(pr.leaf[node] < 0, ERR_INVALID, "container code table is not a prefix code");
+            const int bit = (st.t.code[k] >> b) & 1;
+            if (pr.child[node * 2 + bit] < 0) {
+                pr.child[node * 2 + bit] = (int32_t)pr.leaf.size();
+                pr.leaf.push_back(-1);
+                pr.child.push_back(-1);
+                pr.child.push_back(-1);
+            }
+            node = (size_t)pr.child[node * 2 + bit];
+        }
+        FFCZ_REQUIRE(pr.leaf[node] < 0 && pr.child[node * 2] < 0 && pr.child[node * 2 + 1] < 0, ERR_INVALID,
+                     "container code table is not a prefix code");
+        pr.leaf[node] = (int32_t)st.t.sym[k];
+    }
+    pr.lut.assign((size_t)1 << LUT_BITS, LUT_BAD);
+    for (size_t k = 0; k < st.t.sym.size(); ++k) {
+        const int len = st.t.len[k];
+        const uint32_t code = st.t.code[k];
+        if (len <= LUT_BITS) {
+            const uint32_t base = code << (LUT_BITS - len), span = 1u << (LUT_BITS - len);
+            for (uint32_t i = 0; i < span; ++i) pr.lut[base + i] = ((uint32_t)len << 16) | st.t.sym[k];
+        } else {
+            const uint32_t pre = code >> (len - LUT_BITS);
+            size_t node = 0;
+            for (int b = LUT_BITS - 1; b >= 0; --b) node = (size_t)pr.child[node * 2 + ((pre >> b) & 1)];
+            pr.lut[pre] = 0x80000000u | (uint32_t)node;
+        }
+    }
+    pr.off.assign(st.cbits.size() + 1, 0);
+    for (size_t c = 0; c < st.cbits.size(); ++c) pr.off[c + 1] = pr.off[c] + st.cbits[c];
+    return pr;
 }
 
-size_t popcount_first(const std::vector<uint8_t>& pack, size_t n) {
-    size_t c = 0;
-    const size_t full = n / 8;
-    for (size_t i = 0; i < full; ++i) c += (size_t)__builtin_popcount(pack[i]);
-    if (n % 8) c += (size_t)__builtin_popcount(pack[full] & ((1u << (n % 8)) - 1u));
-    return c;
+// One thread per chunk of CHUNK symbols, from the chunk's bit offset: a
+// 64-bit MSB-first window over the big-endian bitstream words, codes of up to
+// 12 bits in one LDS lookup, longer ones down the trie.  A thread that meets
+// an invalid code, or whose codes do not span exactly its chunk's bits, sets
+// *err.
+template <typename S>
+__global__ __launch_bounds__(256) void k_huff_decode(const uint32_t* words, size_t nwords,
+                                                     const unsigned long long* off, size_t nchunks, size_t nsym,
+                                                     const uint32_t* glut, const int32_t* child, const int32_t* leaf,
+                                                     S* out, unsigned int* err) {
+    __shared__ uint32_t lut[1 << LUT_BITS];
+    for (unsigned int i = threadIdx.x; i < (1u << LUT_BITS); i += 256) lut[i] = glut[i];
+    __syncthreads();
+    const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (c >= nchunks) return;
+    const unsigned long long p0 = off[c], p1 = off[c + 1];
+    const size_t s0 = c * CHUNK;
+    const int cnt = (int)(nsym - s0 < (size_t)CHUNK ? nsym - s0 : (size_t)CHUNK);
+    // reads past the padded end repeat the last word (a damaged stream then
+    // fails the bit-count check below instead of reading out of bounds)
+    auto word = [&](size_t i) -> uint64_t { return __builtin_bswap32(words[i < nwords ? i : nwords - 1]); };
+    size_t wi = (size_t)(p0 >> 5);
+    uint64_t acc = (word(wi) << 32) | word(wi + 1);
+    wi += 2;
+    const int sk = (int)(p0 & 31);
+    acc <<= sk;
+    int nb = 64 - sk;
+    unsigned long long used = 0;
+    bool bad = false;
+    for (int i = 0; i < cnt; ++i) {
+        if (nb < 32) {  // at least 32 valid bits: a whole code
+            acc |= word(wi++) << (32 - nb);
+            nb += 32;
+        }
+        const uint32_t e = lut[acc >> (64 - LUT_BITS)];
+        uint32_t sym;
+        int len;
+        if (!(e & 0x80000000u)) {
+            len = (int)((e >> 16) & 31u);
+            sym = e & 0xFFFFu;
+        } else {
+            if (e == LUT_BAD) {
+                bad = true;
+                break;
+            }
+            int node = (int)(e & 0x7FFFFFFFu), lf = -1;
+            len = LUT_BITS;
+            while (len < 32) {
+                const int bit = (int)((acc >> (63 - len)) & 1u);
+                ++len;
+                node = child[node * 2 + bit];
+                if (node < 0) break;
+                lf = leaf[node];
+                if (lf >= 0) break;
+            }
+            if (lf < 0) {
+                bad = true;
+                break;
+            }
+            sym = (uint32_t)lf;
+        }
+        acc <<= len;
+        nb -= len;
+        used += (unsigned long long)len;
+        out[s0 + (size_t)i] = (S)sym;
+    }
+    if (bad || used != p1 - p0) atomicOr(err, 1u);
+}
+
+// the symbols of a prepared stream into dout (device, nsym elements)
+template <typename S>
+void gpu_decode(const Prepared& pr, S* dout, hipStream_t s) {
+    const Stream& st = *pr.st;
+    if (st.nsym == 0) return;
+    const size_t nwords = ceil_div((size_t)st.bsbytes, (size_t)4) + 2;
+    const size_t nch = pr.off.size() - 1;
+    DBuf bw, bo, bl, bc, bf, be;
+    const StreamWait wait{s};
+    uint32_t* w = bw.alloc<uint32_t>(nwords);
+    FFCZ_HIP(hipMemsetAsync(w, 0, nwords * 4, s));
+    FFCZ_HIP(hipMemcpyAsync(w, pr.bits, st.bsbytes, hipMemcpyHostToDevice, s));
+    unsigned long long* off = bo.alloc<unsigned long long>(nch + 1);
+    FFCZ_HIP(hipMemcpyAsync(off, pr.off.data(), (nch + 1) * 8, hipMemcpyHostToDevice, s));
+    uint32_t* lut = bl.alloc<uint32_t>(pr.lut.size());
+    FFCZ_HIP(hipMemcpyAsync(lut, pr.lut.data(), pr.lut.size() * 4, hipMemcpyHostToDevice, s));
+    int32_t* child = bc.alloc<int32_t>(pr.child.size());
+    FFCZ_HIP(hipMemcpyAsync(child, pr.child.data(), pr.child.size() * 4, hipMemcpyHostToDevice, s));
+    int32_t* leaf = bf.alloc<int32_t>(pr.leaf.size());
+    FFCZ_HIP(hipMemcpyAsync(leaf, pr.leaf.data(), pr.leaf.size() * 4, hipMemcpyHostToDevice, s));
+    unsigned int* err = be.alloc<unsigned int>(1);
+    FFCZ_HIP(hipMemsetAsync(err, 0, 4, s));
+    hipLaunchKernelGGL((k_huff_decode<S>), dim3((unsigned)ceil_div(nch, (size_t)256)), dim3(256), 0, s,
+                       (const uint32_t*)w, nwords, (const unsigned long long*)off, nch, (size_t)st.nsym,
+                       (const uint32_t*)lut, (const int32_t*)child, (const int32_t*)leaf, dout, err);
+    FFCZ_HIP(hipGetLastError());
+    unsigned int herr = 0;
+    FFCZ_HIP(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, s));
+    FFCZ_HIP(hipStreamSynchronize(s));
+    FFCZ_REQUIRE(herr == 0, ERR_INVALID, "corrupt Huffman stream in container");
 }
 
 // ---- reconstruction kernels -------------------------------------------------
@@ -284,7 +428,7 @@ __global__ __launch_bounds__(256) void k_block_popc(const uint32_t* w, size_t nw
     if (threadIdx.x == 0) bsum[blockIdx.x] = red[0];
 }
 
-// exclusive prefix of the block counts, one workgroup
+// exclusive prefix of the block counts (and their total in boff[nb]), one workgroup
 __global__ __launch_bounds__(1024) void k_scan_blocks(const uint32_t* bsum, size_t nb, unsigned long long* boff) {
     __shared__ unsigned long long s[1024];
     const size_t per = (nb + 1023) / 1024;
@@ -304,6 +448,7 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(const uint32_t* bsum, size
         boff[i] = run;
         run += bsum[i];
     }
+    if (threadIdx.x == 1023) boff[nb] = s[1023];  // the total
 }
 
 // dense edits: out[i] = value of the next symbol where flag i is set, else 0
@@ -363,32 +508,44 @@ __global__ void k_sum_edits(const T* xp, const T* rf, const T* ds, T* xh, size_t
 
 unsigned grid_for(size_t n) { return (unsigned)std::min<size_t>(std::max<size_t>(ceil_div(n, (size_t)256), 1), 65536); }
 
-// dense edits from a flag pack and its symbols
+// dense edits from a device flag pack (zero-padded to whole 32-bit words)
+// and its m device symbols; the flags must mark exactly m elements
 template <typename T, typename S>
-void expand(const std::vector<uint8_t>& pack, size_t n, const std::vector<S>& sym, const SymDesc<T>& q, T* out,
-            hipStream_t s) {
+void expand(const uint32_t* w, size_t n, const S* dsym, size_t m, const SymDesc<T>& q, T* out, hipStream_t s) {
     const size_t nwords = ceil_div(n, (size_t)32), nblk = ceil_div(nwords, (size_t)256);
-    DBuf dw, ds, db, dboff;
+    DBuf db, dboff;
     const StreamWait wait{s};
-    uint32_t* w = dw.alloc<uint32_t>(nwords);
-    FFCZ_HIP(hipMemsetAsync(w, 0, nwords * 4, s));
-    FFCZ_HIP(hipMemcpyAsync(w, pack.data(), pack.size(), hipMemcpyHostToDevice, s));
-    S* dsym = ds.alloc<S>(sym.size());
-    if (!sym.empty()) FFCZ_HIP(hipMemcpyAsync(dsym, sym.data(), sym.size() * sizeof(S), hipMemcpyHostToDevice, s));
     uint32_t* bsum = db.alloc<uint32_t>(nblk);
-    unsigned long long* boff = dboff.alloc<unsigned long long>(nblk);
-    hipLaunchKernelGGL(k_block_popc, dim3((unsigned)nblk), dim3(256), 0, s, (const uint32_t*)w, nwords, n, bsum);
+    unsigned long long* boff = dboff.alloc<unsigned long long>(nblk + 1);
+    hipLaunchKernelGGL(k_block_popc, dim3((unsigned)nblk), dim3(256), 0, s, w, nwords, n, bsum);
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, (const uint32_t*)bsum, nblk, boff);
-    hipLaunchKernelGGL((k_expand<T, S>), dim3((unsigned)nblk), dim3(256), 0, s, (const uint32_t*)w, nwords, n,
-                       (const unsigned long long*)boff, (const S*)dsym, sym.size(), q, out);
     FFCZ_HIP(hipGetLastError());
-    FFCZ_HIP(hipStreamSynchronize(s));  // the staging buffers are freed on return
+    unsigned long long total = 0;
+    FFCZ_HIP(hipMemcpyAsync(&total, boff + nblk, 8, hipMemcpyDeviceToHost, s));
+    FFCZ_HIP(hipStreamSynchronize(s));
+    FFCZ_REQUIRE(total == m, ERR_INVALID, "container flags do not match the active counts");
+    hipLaunchKernelGGL((k_expand<T, S>), dim3((unsigned)nblk), dim3(256), 0, s, w, nwords, n,
+                       (const unsigned long long*)boff, dsym, m, q, out);
+    FFCZ_HIP(hipGetLastError());
+    FFCZ_HIP(hipStreamSynchronize(s));
+}
+
+// a flag stream and its symbol stream, decoded on the device and expanded
+template <typename T>
+void decode_edits(const Prepared& flags, const Prepared& syms, size_t n, const SymDesc<T>& q, T* out, hipStream_t s) {
+    DBuf bp, bs;
+    const StreamWait wait{s};
+    uint32_t* w = bp.alloc<uint32_t>(ceil_div(n, (size_t)32));
+    FFCZ_HIP(hipMemsetAsync(w, 0, ceil_div(n, (size_t)32) * 4, s));
+    gpu_decode<uint8_t>(flags, reinterpret_cast<uint8_t*>(w), s);
+    uint16_t* sym = bs.alloc<uint16_t>(syms.st->nsym);
+    gpu_decode<uint16_t>(syms, sym, s);
+    expand<T, uint16_t>(w, n, sym, (size_t)syms.st->nsym, q, out, s);
 }
 
 template <typename T>
-void decode_typed(const ContainerInfo& in, const std::vector<uint8_t>& fpack, const std::vector<uint8_t>& spack,
-                  const std::vector<uint16_t>& fsym, const std::vector<uint16_t>& ssym, const SymDesc<T>& qf,
-                  const SymDesc<T>& qs, const void* x_prime, void* x_hat, hipStream_t s) {
+void decode_typed(const ContainerInfo& in, const Prepared* pr, const SymDesc<T>& qf, const SymDesc<T>& qs,
+                  const void* x_prime, void* x_hat, hipStream_t s) {
     Fft3d<T> f;
     f.create(in.nx, in.ny, in.nz);
     struct FftGuard {
@@ -407,13 +564,13 @@ void decode_typed(const ContainerInfo& in, const std::vector<uint8_t>& fpack, co
     const bool dev_out = on_device(x_hat);
     T* xh = dev_out ? static_cast<T*>(x_hat) : bxh.alloc<T>(N);
     T* dsd = bds.alloc<T>(N);
-    expand<T, uint16_t>(spack, N, ssym, qs, dsd, s);
+    decode_edits<T>(pr[1], pr[3], N, qs, dsd, s);
     T* rf = nullptr;
-    if (!fsym.empty()) {
+    if (pr[2].st->nsym > 0) {
         // C2R(deq df)/N: the split edits into the padded half spectrum, then
         // the library's inverse transform scaled by 1/N (impl:242)
         T* dfd = bdf.alloc<T>(2 * F);
-        expand<T, uint16_t>(fpack, 2 * F, fsym, qf, dfd, s);
+        decode_edits<T>(pr[0], pr[2], 2 * F, qf, dfd, s);
         cplx<T>* spec = bsp.alloc<cplx<T>>(f.spec_elems());
         hipLaunchKernelGGL((k_split_to_spec<T>), dim3(grid_for(f.spec_elems())), dim3(256), 0, s, (const T*)dfd, F,
                            f.rows, f.Nh, f.Mp, spec);
@@ -519,19 +676,16 @@ void container_decode(const uint8_t* buf, size_t len, const void* x_prime, void*
                  "container flag streams do not match the shape");
     FFCZ_REQUIRE(qf.nsym == in.n_freq && qs.nsym == in.n_spatial, ERR_INVALID,
                  "container symbol streams do not match the active counts");
-    // every stream decoded and checked on the host before any device work
-    const std::vector<uint8_t> fpack = decode_stream<uint8_t>(sf), spack = decode_stream<uint8_t>(ss);
-    const std::vector<uint16_t> fsym = decode_stream<uint16_t>(qf), ssym = decode_stream<uint16_t>(qs);
-    FFCZ_REQUIRE(popcount_first(fpack, 2 * F) == in.n_freq && popcount_first(spack, N) == in.n_spatial, ERR_INVALID,
-                 "container flags do not match the active counts");
+    // every stream checked and made ready on the host before any device work
+    const Prepared pr[4] = {prepare(sf), prepare(ss), prepare(qf), prepare(qs)};
     if (in.dtype == 0) {
         const SymDesc<float> df{in.qkind, h.fstep, (float)h.fmin, (float)h.fmax};
         const SymDesc<float> ds{in.qkind, h.sstep, (float)h.smin, (float)h.smax};
-        decode_typed<float>(in, fpack, spack, fsym, ssym, df, ds, x_prime, x_hat, s);
+        decode_typed<float>(in, pr, df, ds, x_prime, x_hat, s);
     } else {
         const SymDesc<double> df{in.qkind, h.fstep, h.fmin, h.fmax};
         const SymDesc<double> ds{in.qkind, h.sstep, h.smin, h.smax};
-        decode_typed<double>(in, fpack, spack, fsym, ssym, df, ds, x_prime, x_hat, s);
+        decode_typed<double>(in, pr, df, ds, x_prime, x_hat, s);
     }
 }
 

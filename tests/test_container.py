@@ -1,7 +1,7 @@
 """The container parser on the CPU: containers written by the oracle's coder
 (tests/_container.py) are read by the library; truncated and damaged ones are
-rejected by the format checks, before any device work (the messages name the
-check that failed)."""
+rejected by the format checks that run on the host before any device work
+(the messages name the check that failed)."""
 import struct
 
 import numpy as np
@@ -88,16 +88,15 @@ def test_damaged_code_table_and_index_are_rejected():
         decode_raw(_patch(buf, len_off + k, "<H", int(streams[0]["chunk_bits"][0]) + 1))
 
 
-def test_flags_that_disagree_with_the_counts_are_rejected():
-    # a well-formed container whose spatial flags mark one element more than
-    # the symbols it carries
-    rng = np.random.default_rng(1)
-    ff = rng.random(F2) < 0.3
-    sf = rng.random(N) < 0.5
-    sf[np.flatnonzero(~sf)[0]] = True
-    fsym = rng.integers(1, 9, int(ff.sum())).astype(np.uint16)
-    ssym = rng.integers(1, 40, int(sf.sum()) - 1).astype(np.uint16)
-    buf = K.build(SHAPE, np.float32, 1e-3, 0.5, 2, (0.1, 0.2), (0, 0, 0, 0), np.packbits(ff, bitorder="little"),
-                  np.packbits(sf, bitorder="little"), fsym, ssym)
-    with pytest.raises(FFCzError, match="flags do not match the active counts"):
-        decode_raw(buf)
+def test_a_code_table_that_is_not_a_prefix_code_is_rejected():
+    buf = tiny()
+    _, streams = K.parse(buf)
+    k = streams[0]["sym"].size
+    assert k >= 2
+    code_off = 128 + K.STREAM.size + 4 * k
+    len_off = code_off + 4 * k
+    # the second code made equal to the first (same length and bits)
+    b = _patch(buf, code_off + 4, "<I", int(streams[0]["codes"][0]))
+    b = _patch(b, len_off + 1, "<B", int(streams[0]["lens"][0]))
+    with pytest.raises(FFCzError, match="not a prefix code"):
+        decode_raw(b)

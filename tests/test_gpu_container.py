@@ -8,10 +8,11 @@ import pytest
 import torch
 
 from ffcz_amd import ProjectionSolver, QUANT_REF16, QUANT_STEP, container
+from ffcz_amd._lib import FFCzError
 from oracle import ffcz_oracle as O
 from tests import _container as K
 from tests._cases import binding_case
-from tests.test_container import N, SHAPE, tiny
+from tests.test_container import F2, N, SHAPE, _patch, tiny
 
 pytestmark = pytest.mark.gpu
 
@@ -93,3 +94,61 @@ def test_container_round_trip(gpu, shape, dtype, quant, level):
     # the certified bounds hold for the decoded field (the margins cover its rounding)
     if cert.ok:
         assert np.abs(got.astype(np.float64) - x.astype(np.float64)).max() <= eps * (1 + 1e-6)
+
+
+def test_flags_that_disagree_with_the_counts_are_rejected(gpu):
+    # well formed, but the spatial flags mark one element more than the
+    # symbols carried (checked on the device, after decoding the flags)
+    rng = np.random.default_rng(1)
+    ff = rng.random(F2) < 0.3
+    sf = rng.random(N) < 0.5
+    sf[np.flatnonzero(~sf)[0]] = True
+    fsym = rng.integers(1, 9, int(ff.sum())).astype(np.uint16)
+    ssym = rng.integers(1, 40, int(sf.sum()) - 1).astype(np.uint16)
+    buf = K.build(SHAPE, np.float32, 1e-3, 0.5, 2, (0.1, 0.2), (0, 0, 0, 0), np.packbits(ff, bitorder="little"),
+                  np.packbits(sf, bitorder="little"), fsym, ssym)
+    with pytest.raises(FFCzError, match="flags do not match the active counts"):
+        container.decode(buf, np.zeros(SHAPE, np.float32))
+
+
+def test_a_misplaced_chunk_boundary_is_caught_by_the_device_decoder(gpu):
+    # 3000 spatial symbols (three chunks); moving one bit from the first
+    # chunk's count to the second keeps the total, so only the device decoder
+    # (each chunk must end exactly at the next one's offset) can see it
+    shape = (5, 6, 100)
+    rng = np.random.default_rng(2)
+    sf = np.zeros(3000, bool)
+    sf[rng.choice(3000, 2500, replace=False)] = True
+    ssym = rng.integers(1, 300, int(sf.sum())).astype(np.uint16)
+    ff = np.zeros(2 * 5 * 6 * 51, bool)
+    buf = K.build(shape, np.float32, 1e-3, 0.5, 2, (0.1, 0.2), (0, 0, 0, 0), np.packbits(ff, bitorder="little"),
+                  np.packbits(sf, bitorder="little"), np.zeros(0, np.uint16), ssym)
+    _, st = K.parse(buf)
+    o = len(buf) - st[3]["payload"].size - 2 * st[3]["chunk_bits"].size
+    cb = st[3]["chunk_bits"]
+    b = _patch(_patch(buf, o, "<H", int(cb[0]) + 1), o + 2, "<H", int(cb[1]) - 1)
+    ok = container.decode(buf, np.zeros(shape, np.float32))
+    assert np.count_nonzero(ok) == int(sf.sum())
+    with pytest.raises(FFCzError, match="corrupt Huffman stream"):
+        container.decode(b, np.zeros(shape, np.float32))
+
+
+def test_codes_longer_than_the_lookup_table_decode(gpu):
+    # Fibonacci symbol counts: code lengths up to 19 bits, so the device
+    # decoder walks its trie past the 12-bit table
+    shape = (5, 36, 100)
+    n = int(np.prod(shape))
+    fib = [1, 1]
+    while len(fib) < 20:
+        fib.append(fib[-1] + fib[-2])
+    sym = np.concatenate([np.full(c, k + 1, np.uint16) for k, c in enumerate(fib)])
+    sym = np.concatenate([sym, np.full(n - sym.size, 20, np.uint16)])
+    np.random.default_rng(3).shuffle(sym)
+    _, (_, _, lens) = O.huffman_encode(sym)
+    assert lens.max() > 12
+    sf = np.ones(n, bool)
+    ff = np.zeros(2 * 5 * 36 * 51, bool)
+    buf = K.build(shape, np.float64, 1e-3, 0.5, 2, (0.1, 0.25), (0, 0, 0, 0), np.packbits(ff, bitorder="little"),
+                  np.packbits(sf, bitorder="little"), np.zeros(0, np.uint16), sym)
+    got = container.decode(buf, np.zeros(shape, np.float64)).ravel()
+    assert np.array_equal(got, unzigzag(sym) * 0.25)
