@@ -605,7 +605,8 @@ static void launch_strided(const Fft3d<R>& f, const SArgs<R, T>& a0, int mode, b
     const size_t lds = ((size_t)a.L + (size_t)a.L * a.W) * sizeof(cplx<R>);
     switch (mode) {
         case S_PLAIN: hipLaunchKernelGGL((k_strided<R, T, S_PLAIN>), dim3(grid), dim3(FFT_THREADS), lds, s, a); break;
-        case S_CLAMP: hipLaunchKernelGGL((k_strided<R, T, S_CLAMP>), dim3(grid), dim3(FFT_THREADS), lds, s, a); break;
+        case S_CLAMP:
+        case S_CLAMP_PASS: hipLaunchKernelGGL((k_strided<R, T, S_CLAMP>), dim3(grid), dim3(FFT_THREADS), lds, s, a); break;
         case S_CERT_CHECK: hipLaunchKernelGGL((k_strided<R, T, S_CERT_CHECK>), dim3(grid), dim3(FFT_THREADS), lds, s, a); break;
         case S_LOAD_DF: hipLaunchKernelGGL((k_strided<R, T, S_LOAD_DF>), dim3(grid), dim3(FFT_THREADS), lds, s, a); break;
     }
@@ -672,6 +673,15 @@ static size_t slab_planes(const Fft3d<R>& f) {
     return xs >= f.nx ? f.nx : xs;
 }
 
+// the fused x pass's mode: per-bin bounds clamp in a pass of their own over
+// the vector-ALU tile (S_CLAMP_PASS); abs and ptw bounds, and the MFMA
+// variants, clamp in the last stage
+template <typename R>
+static int x_clamp_mode(const Fft3d<R>& f, int fmode) {
+    if (fmode == FM_NONE) return S_PLAIN;
+    return fmode == FM_DENSE && f.prec == PREC_VALU ? S_CLAMP_PASS : S_CLAMP;
+}
+
 template <typename R>
 void iteration(const Fft3d<R>& f, const IterArgs<R>& ia, hipStream_t s, EventLog* log) {
     auto mark = [&](int kind) {
@@ -718,7 +728,7 @@ void iteration(const Fft3d<R>& f, const IterArgs<R>& ia, hipStream_t s, EventLog
     x.df = ia.df;
     x.orig = ia.orig;
     x.delta_ptw = ia.delta_ptw;
-    launch_strided(f, x, ia.fmode == FM_NONE ? S_PLAIN : S_CLAMP, true, s);
+    launch_strided(f, x, x_clamp_mode(f, ia.fmode), true, s);
     mark(2);
     for (size_t x0 = 0; x0 < f.nx; x0 += xs) {
         const size_t x1 = std::min(f.nx, x0 + xs);
@@ -764,7 +774,7 @@ void iteration_kernel(const Fft3d<R>& f, const IterArgs<R>& ia, hipStream_t s, i
         x.df = ia.df;
         x.orig = ia.orig;
         x.delta_ptw = ia.delta_ptw;
-        launch_strided(f, x, ia.fmode == FM_NONE ? S_PLAIN : S_CLAMP, true, s);
+        launch_strided(f, x, x_clamp_mode(f, ia.fmode), true, s);
     } else {
         launch_zinv<R, R, Z_ITER>(f, z, s);
     }

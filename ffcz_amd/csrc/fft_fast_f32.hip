@@ -18,7 +18,7 @@ bool fast_zinv<float, float>(int mode, const ZArgs<float, float>& a, hipStream_t
 template <>
 bool fast_strided<float, float>(int mode, const SArgs<float, float>& a, size_t outer, hipStream_t s) {
     if (a.prec != PREC_VALU && mfma_strided(mode, a, outer, s)) return true;
-    return strided_dispatch<float, float, (1u << S_PLAIN) | (1u << S_CLAMP)>(mode, a, outer, s);
+    return strided_dispatch<float, float, (1u << S_PLAIN) | (1u << S_CLAMP) | (1u << S_CLAMP_PASS)>(mode, a, outer, s);
 }
 
 bool fast_zfwd_batch(int Lz, const BatchTab<ZArgs<float, float>>& t, unsigned grid, hipStream_t s) {

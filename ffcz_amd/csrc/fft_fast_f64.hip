@@ -15,7 +15,8 @@ bool fast_zinv<double, double>(int mode, const ZArgs<double, double>& a, hipStre
 }
 template <>
 bool fast_strided<double, double>(int mode, const SArgs<double, double>& a, size_t outer, hipStream_t s) {
-    return strided_dispatch<double, double, (1u << S_PLAIN) | (1u << S_CLAMP) | (1u << S_CERT_CHECK) | (1u << S_LOAD_DF)>(mode, a, outer, s);
+    return strided_dispatch<double, double, (1u << S_PLAIN) | (1u << S_CLAMP) | (1u << S_CERT_CHECK) | (1u << S_LOAD_DF) |
+                                            (1u << S_CLAMP_PASS)>(mode, a, outer, s);
 }
 
 bool fast_cert_zfwd_pair(const ZArgs<double, double>& a, const ZArgs<double, double>& b, hipStream_t s) {

@@ -403,6 +403,60 @@ __device__ __forceinline__ void strided_body(const SArgs<R, T>& a, uint32_t blk)
         __syncthreads();
         if constexpr (PREC == PREC_VALU) pen::transform<R, L, TP, true, false>(lld, gst, lld, lst, tw, t, +1);
         else mx::transform<PREC, L, TP, W, W, NT, true, false, false, false>(gldp, gstp, buf, tw, tid, p, t, +1);
+    } else if constexpr (MODE == S_CLAMP_PASS) {
+        // Per-bin bounds: the forward transform ends in the tile, then the
+        // clamp runs as a pass of its own over it.  Each thread issues all of
+        // its bins' bound loads, then the accumulator loads of the bins that
+        // clamp, before any accumulator store: clamping in the last stage, each
+        // bin's loads waited for the previous bin's store to df, which the
+        // compiler cannot tell apart from the bounds.  The arithmetic is
+        // freq_clamp_bin's (the edit formed, then accumulated).
+        static_assert(PREC == PREC_VALU, "the clamp pass follows the vector-ALU transform");
+        pen::transform<R, L, TP, false, true>(gld, lst, lld, lst, tw, t, -1);
+        __syncthreads();
+        constexpr int PER = (L * W + NT - 1) / NT;
+        const size_t obase = o * a.cs_o + (size_t)ct * W;
+        R bound[PER], old_re[PER], old_im[PER];
+        cplx<R> v[PER], c[PER];
+        bool ok[PER], hre[PER], him[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int q = tid + k * NT;
+            const int pp = q % W;
+            ok[k] = q < L * W && ct * W + pp < a.ncols;
+            const size_t idx = (size_t)(q / W) * a.cs_n + obase + pp;
+            bound[k] = ok[k] ? a.delta_dense[idx] * a.dscale : R(0);
+        }
+        bool fired = false;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int q = tid + k * NT;
+            v[k] = ok[k] ? buf[q] : mk(R(0), R(0));
+            const R d = bound[k];
+            const bool xh = v[k].x > d, xl = v[k].x < -d, yh = v[k].y > d, yl = v[k].y < -d;
+            hre[k] = ok[k] && (xh || xl);
+            him[k] = ok[k] && (yh || yl);
+            c[k] = mk(xh ? d : (xl ? -d : v[k].x), yh ? d : (yl ? -d : v[k].y));
+            fired |= hre[k] || him[k];
+        }
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int q = tid + k * NT;
+            const size_t idx = (size_t)(q / W) * a.cs_n + obase + q % W;
+            if (hre[k]) old_re[k] = a.df[idx];
+            if (him[k]) old_im[k] = a.df[idx + a.F];
+        }
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int q = tid + k * NT;
+            const size_t idx = (size_t)(q / W) * a.cs_n + obase + q % W;
+            if (hre[k]) a.df[idx] = old_re[k] + (c[k].x - v[k].x);
+            if (him[k]) a.df[idx + a.F] = old_im[k] + (c[k].y - v[k].y);
+            if (ok[k]) buf[q] = c[k];
+        }
+        if (fired) a.st->freq_clamped = 1u;
+        __syncthreads();
+        pen::transform<R, L, TP, true, false>(lld, gst, lld, lst, tw, t, +1);
     } else {
         static_assert(PREC == PREC_VALU, "certification runs on the vector ALU");  // S_CERT_CHECK
         unsigned long long nv = 0;
@@ -587,6 +641,8 @@ bool strided_dispatch(int mode, const SArgs<R, T>& a, size_t outer, hipStream_t 
         if constexpr ((MASK & (1u << S_CERT_CHECK)) != 0) return strided_by_len<R, T, S_CERT_CHECK>(a, outer, s);
     } else if (mode == S_LOAD_DF) {
         if constexpr ((MASK & (1u << S_LOAD_DF)) != 0) return strided_by_len<R, T, S_LOAD_DF>(a, outer, s);
+    } else if (mode == S_CLAMP_PASS) {
+        if constexpr ((MASK & (1u << S_CLAMP_PASS)) != 0) return strided_by_len<R, T, S_CLAMP_PASS>(a, outer, s);
     }
     return false;
 }
