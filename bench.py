@@ -69,6 +69,9 @@ def parse():
                    help="skip the Huffman encoding and container measurements")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                    help="only run a plain device copy of BYTES (counter calibration) and exit")
+    p.add_argument("--ptw", type=float, default=0.0, metavar="DELTA",
+                   help="only time the iteration passes of real solves in the ptw mode (bound DELTA) beside the "
+                        "abs mode on the configuration's field, and exit")
     p.add_argument("--hbm-ceilings", type=int, default=0, metavar="BYTES",
                    help="only time plain write, copy and two-read-one-write kernels over arrays of BYTES "
                         "(the HBM rates simple streaming kernels reach) and exit")
@@ -233,6 +236,37 @@ def container_bench(solver, wl, torch, reps=3, level=3):
     return {"bytes": len(buf), "ratio": round(nbytes / len(buf), 2), "zstd_level": level,
             "pack_ms": round(t_pack * 1e3, 2), "decode_ms": round(t * 1e3, 2),
             "decode_GBps": round(nbytes / t / 1e9, 3), "max_abs_error": err}
+
+
+def bench_ptw(a, torch, dev):
+    """Per-pass times of real solves (HIP events between the launches, 8
+    iterations that never converge) in the ptw mode (projection_algorithm.cu:
+    190-259: F(x) read per bin in the fused x pass) and in the abs mode, on the
+    configuration's field."""
+    from ffcz_amd import ProjectionSolver, MODE_ABS, MODE_PTW
+    from ffcz_amd.workloads import make_config
+
+    wl = make_config(a.config, device=dev, seed=a.seed)
+    shape = tuple(wl.x.shape)
+    names = ["z_fwd", "y_fwd", "x_fused", "y_inv", "z_inv", "finalize"]
+    out = {"ptw_iteration": {"workload": f"{a.config}: {'x'.join(map(str, shape))} "
+                                         f"{'f32' if wl.x.dtype == torch.float32 else 'f64'}",
+                             "delta_ptw": a.ptw, "iterations": 8}}
+    for label, mode in (("ptw", MODE_PTW), ("abs", MODE_ABS)):
+        with ProjectionSolver(*shape, dtype=np.float32 if wl.x.dtype == torch.float32 else np.float64) as s:
+            if mode == MODE_PTW:
+                s.initialize_all_ptw(wl.x, wl.eps, a.ptw)
+            else:
+                s.initialize_all_abs(wl.x, wl.eps, wl.delta if wl.delta is not None else 1.0)
+            s.solve_loop(mode, wl.e0, 2, -1.0)  # warm-up
+            s.set_pass_timing(True)
+            s.reset_edit_accumulators()
+            s.solve_loop(mode, wl.e0, 8, -1.0)
+            ms6, n = s.pass_timing()
+            s.set_pass_timing(False)
+        out["ptw_iteration"][label] = {"ms": {k: round(v, 5) for k, v in zip(names, ms6)},
+                                       "iteration_ms": round(float(sum(ms6)), 4), "timed_iterations": n}
+    print(json.dumps(out))
 
 
 def bench_batch(a, torch, dev):
@@ -400,6 +434,8 @@ def main():
     dev = torch.device("cuda:0")
     if a.config == "c4":
         return bench_batch(a, torch, dev)
+    if a.ptw:
+        return bench_ptw(a, torch, dev)
     wl = make_config(a.config, device=dev, seed=a.seed)
     shape = tuple(wl.x.shape)
     N = wl.x.numel()

@@ -673,13 +673,13 @@ static size_t slab_planes(const Fft3d<R>& f) {
     return xs >= f.nx ? f.nx : xs;
 }
 
-// the fused x pass's mode: per-bin bounds clamp in a pass of their own over
-// the vector-ALU tile (S_CLAMP_PASS); abs and ptw bounds, and the MFMA
+// the fused x pass's mode: per-bin and ptw bounds clamp in a pass of their
+// own over the vector-ALU tile (S_CLAMP_PASS); abs bounds, and the MFMA
 // variants, clamp in the last stage
 template <typename R>
 static int x_clamp_mode(const Fft3d<R>& f, int fmode) {
     if (fmode == FM_NONE) return S_PLAIN;
-    return fmode == FM_DENSE && f.prec == PREC_VALU ? S_CLAMP_PASS : S_CLAMP;
+    return (fmode == FM_DENSE || fmode == FM_PTW) && f.prec == PREC_VALU ? S_CLAMP_PASS : S_CLAMP;
 }
 
 template <typename R>

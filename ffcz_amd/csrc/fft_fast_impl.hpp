@@ -404,20 +404,23 @@ __device__ __forceinline__ void strided_body(const SArgs<R, T>& a, uint32_t blk)
         if constexpr (PREC == PREC_VALU) pen::transform<R, L, TP, true, false>(lld, gst, lld, lst, tw, t, +1);
         else mx::transform<PREC, L, TP, W, W, NT, true, false, false, false>(gldp, gstp, buf, tw, tid, p, t, +1);
     } else if constexpr (MODE == S_CLAMP_PASS) {
-        // Per-bin bounds: the forward transform ends in the tile, then the
-        // clamp runs as a pass of its own over it.  Each thread issues all of
-        // its bins' bound loads, then the accumulator loads of the bins that
-        // clamp, before any accumulator store: clamping in the last stage, each
-        // bin's loads waited for the previous bin's store to df, which the
-        // compiler cannot tell apart from the bounds.  The arithmetic is
-        // freq_clamp_bin's (the edit formed, then accumulated).
+        // Per-bin and ptw bounds: the forward transform ends in the tile, then
+        // the clamp runs as a pass of its own over it.  Each thread issues all
+        // of its bins' bound loads (delta_k, or F(x) for ptw), then the
+        // accumulator loads of the bins that clamp, before any accumulator
+        // store: clamping in the last stage, each bin's loads waited for the
+        // previous bin's store to df, which the compiler cannot tell apart
+        // from the bounds.  The arithmetic is freq_clamp_bin's: the edit formed,
+        // then accumulated; ptw as ptw_clamp (the DC bin's real part zeroed).
         static_assert(PREC == PREC_VALU, "the clamp pass follows the vector-ALU transform");
         pen::transform<R, L, TP, false, true>(gld, lst, lld, lst, tw, t, -1);
         __syncthreads();
         constexpr int PER = (L * W + NT - 1) / NT;
+        const bool ptw = a.fmode == FM_PTW;
         const size_t obase = o * a.cs_o + (size_t)ct * W;
-        R bound[PER], old_re[PER], old_im[PER];
-        cplx<R> v[PER], c[PER];
+        cplx<R> bnd[PER];  // (delta_k, -) or F(x) of the bin
+        R old_re[PER], old_im[PER], ed_re[PER], ed_im[PER];
+        cplx<R> c[PER];
         bool ok[PER], hre[PER], him[PER];
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
@@ -425,18 +428,50 @@ __device__ __forceinline__ void strided_body(const SArgs<R, T>& a, uint32_t blk)
             const int pp = q % W;
             ok[k] = q < L * W && ct * W + pp < a.ncols;
             const size_t idx = (size_t)(q / W) * a.cs_n + obase + pp;
-            bound[k] = ok[k] ? a.delta_dense[idx] * a.dscale : R(0);
+            bnd[k] = mk(R(0), R(0));
+            if (ok[k]) bnd[k] = ptw ? a.orig[idx] : mk(a.delta_dense[idx] * a.dscale, R(0));
         }
+        // the ptw constants, as ptw_clamp forms them (projection_algorithm.cu:199-258)
+        const R sp = sqrt(R(1) + a.delta_ptw), sm = sqrt(R(1) - a.delta_ptw);
+        const R dptw = fmin(R(1) - sm, sp - R(1)) / (R)sqrtf(2.0f);
         bool fired = false;
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             const int q = tid + k * NT;
-            v[k] = ok[k] ? buf[q] : mk(R(0), R(0));
-            const R d = bound[k];
-            const bool xh = v[k].x > d, xl = v[k].x < -d, yh = v[k].y > d, yl = v[k].y < -d;
-            hre[k] = ok[k] && (xh || xl);
-            him[k] = ok[k] && (yh || yl);
-            c[k] = mk(xh ? d : (xl ? -d : v[k].x), yh ? d : (yl ? -d : v[k].y));
+            const cplx<R> v = ok[k] ? buf[q] : mk(R(0), R(0));
+            const size_t idx = (size_t)(q / W) * a.cs_n + obase + q % W;
+            hre[k] = him[k] = false;
+            c[k] = v;
+            ed_re[k] = ed_im[k] = R(0);
+            if (!ok[k]) continue;
+            R d = bnd[k].x;
+            bool active = true;
+            if (ptw) {
+                if (idx == 0) {
+                    const R lo = bnd[k].x * (R(1) / sp - R(1)), hi = bnd[k].x * (R(1) / sm - R(1));
+                    hre[k] = v.x > hi || v.x < lo;
+                    ed_re[k] = -v.x;
+                    c[k].x = hre[k] ? R(0) : v.x;
+                    fired |= hre[k];
+                    continue;
+                }
+                const R omag = bnd[k].x * bnd[k].x + bnd[k].y * bnd[k].y;
+                const R cr = bnd[k].x + v.x, ci = bnd[k].y + v.y;
+                const R ratio = (cr * cr + ci * ci) / omag;
+                active = !(ratio <= sp && ratio >= sm);
+                d = dptw;
+            }
+            const bool xh = v.x > d, xl = v.x < -d, yh = v.y > d, yl = v.y < -d;
+            hre[k] = active && (xh || xl);
+            him[k] = active && (yh || yl);
+            if (hre[k]) {
+                c[k].x = xh ? d : -d;
+                ed_re[k] = c[k].x - v.x;
+            }
+            if (him[k]) {
+                c[k].y = yh ? d : -d;
+                ed_im[k] = c[k].y - v.y;
+            }
             fired |= hre[k] || him[k];
         }
 #pragma unroll
@@ -450,8 +485,8 @@ __device__ __forceinline__ void strided_body(const SArgs<R, T>& a, uint32_t blk)
         for (int k = 0; k < PER; ++k) {
             const int q = tid + k * NT;
             const size_t idx = (size_t)(q / W) * a.cs_n + obase + q % W;
-            if (hre[k]) a.df[idx] = old_re[k] + (c[k].x - v[k].x);
-            if (him[k]) a.df[idx + a.F] = old_im[k] + (c[k].y - v[k].y);
+            if (hre[k]) a.df[idx] = old_re[k] + ed_re[k];
+            if (him[k]) a.df[idx + a.F] = old_im[k] + ed_im[k];
             if (ok[k]) buf[q] = c[k];
         }
         if (fired) a.st->freq_clamped = 1u;

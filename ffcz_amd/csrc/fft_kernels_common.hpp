@@ -132,9 +132,9 @@ struct BatchTab {
     int n;
 };
 
-// S_CLAMP_PASS: S_CLAMP with per-bin bounds, the clamp done as a pass of its
-// own over the transformed tile (vector-ALU kernels; the generic kernels run
-// it as S_CLAMP)
+// S_CLAMP_PASS: S_CLAMP with per-bin or ptw bounds, the clamp done as a pass
+// of its own over the transformed tile (vector-ALU kernels; the generic
+// kernels run it as S_CLAMP)
 enum SMode : int { S_PLAIN = 0, S_CLAMP = 1, S_CERT_CHECK = 2, S_LOAD_DF = 3, S_CLAMP_PASS = 4 };
 
 template <typename R, typename T>

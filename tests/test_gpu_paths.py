@@ -59,24 +59,29 @@ def test_fast_fft_matches_numpy(gpu, restore_tuning, shape):
     assert np.abs(X - ref).max() < 1e-13 * np.abs(ref).max()
 
 
-def run_direct(shape, dtype, force_generic):
+def run_direct(shape, dtype, force_generic, mode="direct"):
     from tests._cases import direct_case
 
     set_tuning(force_generic, 0)
     x, e0, eps, idx, dk = direct_case(shape, seed=7, dtype=dtype)
     with ProjectionSolver(*shape, dtype=dtype) as s:
-        s.initialize(x, idx, dk, eps)
-        s.solve(e0, 10, -1.0)  # a fixed count of iterations
+        if mode == "direct":
+            s.initialize(x, idx, dk, eps)
+            s.solve(e0, 10, -1.0)  # a fixed count of iterations
+        else:
+            s.initialize_all_ptw(x, eps, 0.05)
+            s.solve_all_ptw(e0, 10, -1.0)
         return s.iteration_count, s.get_results(), s.get_freq_edits_full(), s.get_spatial_edits_full()
 
 
+@pytest.mark.parametrize("mode", ["direct", "ptw"])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("shape", [(32, 24, 40), (60, 18, 20)])
-def test_direct_clamp_pass_matches_generic(gpu, restore_tuning, shape, dtype):
-    # per-bin bounds: the specialised x pass clamps in a pass of its own over
-    # the tile; the generic kernel clamps in its last stage
-    a = run_direct(shape, dtype, 0)
-    b = run_direct(shape, dtype, 1)
+def test_direct_clamp_pass_matches_generic(gpu, restore_tuning, shape, dtype, mode):
+    # per-bin and ptw bounds: the specialised x pass clamps in a pass of its
+    # own over the tile; the generic kernel clamps in its last stage
+    a = run_direct(shape, dtype, 0, mode)
+    b = run_direct(shape, dtype, 1, mode)
     assert a[0] == b[0] == 10
     tol = 1e-11 if dtype == np.float64 else 2e-5
     for u, v in zip(a[1:], b[1:]):

@@ -73,8 +73,8 @@ def test_abs_loop_f32_matches_fp32_oracle(gpu, shape):
         assert agree > 0.999
 
 
-def test_ptw_loop_f64_matches_oracle(gpu):
-    shape = (6, 10, 12)
+@pytest.mark.parametrize("shape", [(6, 10, 12), (16, 20, 30), (32, 12, 10)])
+def test_ptw_loop_f64_matches_oracle(gpu, shape):
     x, e0, eps, _ = binding_case(shape, seed=21)
     ref = O.solve(x, e0, eps, mode="ptw", delta_ptw=0.05, max_iter=20, tol=1e-12)
     with ProjectionSolver(*shape, dtype=np.float64) as s:
