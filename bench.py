@@ -214,7 +214,9 @@ def huffman_bench(solver, torch, dev, reps=3, cpu_sample=2_000_000):
 def container_bench(solver, wl, torch, reps=3, level=3):
     """The container of the last solve's coded edits (ZSTD level 3): its size
     against the field's N*s bytes, the time to pack it, and the time to decode
-    it back into x_hat = x' + C2R(deq df)/N + deq(ds) with x' and x_hat in HBM."""
+    it back into x_hat = x' + C2R(deq df)/N + deq(ds) with x' and x_hat in HBM;
+    and the GPU base compressor's stream for the same field and bound, with the
+    ratio of the field to both streams together."""
     from ffcz_amd import container
 
     t0 = time.perf_counter()
@@ -232,10 +234,26 @@ def container_bench(solver, wl, torch, reps=3, level=3):
     t = float(np.median(times))
     nbytes = xp.numel() * xp.element_size()
     err = float((xh.double() - wl.x.double()).abs().max())
+    # the GPU base compressor (dual-quantisation Lorenzo) on the same field and
+    # bound: its stream, compress and decompress times (x and x' in HBM)
+    tb, td = [], []
+    for r in range(reps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        base, _ = container.base_compress(wl.x, wl.eps, zstd_level=level, x_prime=xh)
+        t1 = time.perf_counter()
+        container.base_decompress(base, xh)
+        torch.cuda.synchronize()
+        if r:
+            tb.append(t1 - t0)
+            td.append(time.perf_counter() - t1)
     del xp, xh
     return {"bytes": len(buf), "ratio": round(nbytes / len(buf), 2), "zstd_level": level,
             "pack_ms": round(t_pack * 1e3, 2), "decode_ms": round(t * 1e3, 2),
-            "decode_GBps": round(nbytes / t / 1e9, 3), "max_abs_error": err}
+            "decode_GBps": round(nbytes / t / 1e9, 3), "max_abs_error": err,
+            "base": {"bytes": len(base), "compress_ms": round(float(np.median(tb)) * 1e3, 2),
+                     "decompress_ms": round(float(np.median(td)) * 1e3, 2),
+                     "ratio_with_edits": round(nbytes / (len(base) + len(buf)), 2)}}
 
 
 def bench_ptw(a, torch, dev):

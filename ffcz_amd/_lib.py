@@ -130,6 +130,9 @@ SIGNATURES = {
     "ffcz_blob_free": [_h],
     "ffcz_container_info": [_vp, _sz, C.POINTER(_i), _szp, C.POINTER(_d), C.POINTER(_d), C.POINTER(_i), _szp],
     "ffcz_container_decode": [_vp, _sz, _vp, _vp, _vp],
+    "ffcz_base_compress": [_i, _vp, _sz, _sz, _sz, _d, _i, _vp, C.POINTER(_h), _vp],
+    "ffcz_base_info": [_vp, _sz, C.POINTER(_i), _szp, C.POINTER(_d), _szp],
+    "ffcz_base_decompress": [_vp, _sz, _vp, _vp],
     "ffcz_fft_r2c": [_i, _sz, _sz, _sz, _vp, _vp, _vp],
     "ffcz_fft_c2r": [_i, _sz, _sz, _sz, _vp, _vp, _vp],
     "ffcz_fft_r2c_prec": [_i, _i, _sz, _sz, _sz, _vp, _vp, _vp],

@@ -681,6 +681,37 @@ int ffcz_container_info(const uint8_t* buf, size_t len, int* dtype, size_t* dims
 int ffcz_container_decode(const uint8_t* buf, size_t len, const void* x_prime, void* x_hat, void* stream) {
     return guard([&] { container_decode(buf, len, x_prime, x_hat, as_stream(stream)); });
 }
+int ffcz_base_compress(int dtype, const void* x, size_t nx, size_t ny, size_t nz, double eps, int zstd_level,
+                       void* x_prime, ffcz_blob** out, void* stream) {
+    return guard([&] {
+        FFCZ_REQUIRE(out != nullptr, ERR_INVALID, "null output");
+        auto b = std::make_unique<ffcz_blob>();
+        if (dtype == FFCZ_F32)
+            b->bytes = base_compress<float>((const float*)x, nx, ny, nz, eps, zstd_level, (float*)x_prime,
+                                            as_stream(stream));
+        else if (dtype == FFCZ_F64)
+            b->bytes = base_compress<double>((const double*)x, nx, ny, nz, eps, zstd_level, (double*)x_prime,
+                                             as_stream(stream));
+        else throw Error(ERR_INVALID, "bad dtype");
+        *out = b.release();
+    });
+}
+int ffcz_base_info(const uint8_t* buf, size_t len, int* dtype, size_t* dims, double* eps, size_t* n_outliers) {
+    return guard([&] {
+        const ContainerInfo in = base_info(buf, len);
+        if (dtype) *dtype = in.dtype;
+        if (dims) {
+            dims[0] = in.nx;
+            dims[1] = in.ny;
+            dims[2] = in.nz;
+        }
+        if (eps) *eps = in.eps;
+        if (n_outliers) *n_outliers = in.n_spatial;
+    });
+}
+int ffcz_base_decompress(const uint8_t* buf, size_t len, void* x_prime, void* stream) {
+    return guard([&] { base_decompress(buf, len, x_prime, as_stream(stream)); });
+}
 
 int ffcz_huffman_zstd_decompress_u16(const uint8_t* comp, size_t clen, const uint32_t* sym, const uint32_t* code,
                                      const int32_t* len, size_t nsym, size_t ne, size_t bb, uint16_t* out,

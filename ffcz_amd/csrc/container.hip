@@ -21,11 +21,13 @@
 //     payload
 // The bitstream is the reference encoder's (HuffmanZSTDCoder.cuh:307-357);
 // the chunk index is side information that lets a decoder start anywhere.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
 #include "container.hpp"
 #include "huffman.hpp"
+#include "quant.hpp"
 
 namespace ffcz {
 
@@ -622,6 +624,124 @@ Header read_header(Reader& r) {
     return h;
 }
 
+// ---- base compressor: dual-quantisation Lorenzo ---------------------------
+// q = round(x/(2 eps)) as int32 (x' = q * 2 eps, |x' - x| <= eps up to the
+// rounding of the product); the symbols are the zigzag of the 3D Lorenzo
+// residual of q (the mixed difference along x, y and z, zero outside the
+// field), exact in integers, so decoding is three prefix sums.  Residuals
+// whose zigzag does not fit 16 bits take the escape symbol and an outlier
+// entry (index, residual).
+constexpr char BMAGIC[4] = {'F', 'F', 'C', 'B'};
+constexpr size_t BHDR_BYTES = 56;
+constexpr uint16_t ESC = 65535;
+
+unsigned grid_rows(size_t n) { return (unsigned)std::min<size_t>(std::max<size_t>(n, 1), 65535); }
+
+template <typename T>
+__global__ void k_prequant(const T* x, size_t n, double inv2e, double two_eps, int32_t* q, T* xp, unsigned int* bad) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const double v = rint((double)x[i] * inv2e);
+        int32_t qi = 0;
+        if (fabs(v) < 1073741824.0) qi = (int32_t)v;
+        else atomicOr(bad, 1u);
+        q[i] = qi;
+        xp[i] = (T)((double)qi * two_eps);
+    }
+}
+
+// one workgroup per row (ix, iy) of nz elements
+__global__ __launch_bounds__(256) void k_lorenzo(const int32_t* q, size_t nx, size_t ny, size_t nz, uint16_t* sym,
+                                                 unsigned long long* ocount, unsigned long long* oidx,
+                                                 long long* oval, size_t ocap) {
+    for (size_t row = blockIdx.x; row < nx * ny; row += gridDim.x) {
+        const size_t ix = row / ny, iy = row - ix * ny;
+        const bool x0 = ix > 0, y0 = iy > 0;
+        const int32_t* c = q + row * nz;
+        const int32_t* cx = x0 ? c - ny * nz : nullptr;
+        const int32_t* cy = y0 ? c - nz : nullptr;
+        const int32_t* cxy = (x0 && y0) ? c - ny * nz - nz : nullptr;
+        for (size_t k = threadIdx.x; k < nz; k += 256) {
+            auto at = [&](const int32_t* p, size_t kk) -> long long { return p ? (long long)p[kk] : 0ll; };
+            long long r = (long long)c[k] - at(cx, k) - at(cy, k) + at(cxy, k);
+            if (k > 0) r -= (long long)c[k - 1] - at(cx, k - 1) - at(cy, k - 1) + at(cxy, k - 1);
+            const unsigned long long z = zigzag64(r);
+            const size_t i = row * nz + k;
+            if (z < ESC) {
+                sym[i] = (uint16_t)z;
+            } else {
+                sym[i] = ESC;
+                const unsigned long long slot = atomicAdd(ocount, 1ull);
+                if (slot < ocap) {
+                    oidx[slot] = i;
+                    oval[slot] = r;
+                }
+            }
+        }
+    }
+}
+
+// residuals from the symbols; counts the escapes
+__global__ void k_unsym(const uint16_t* sym, size_t n, long long* r, unsigned long long* nesc) {
+    unsigned long long c = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const unsigned int z = sym[i];
+        if (z == ESC) ++c;
+        r[i] = (long long)(z >> 1) ^ -(long long)(z & 1u);
+    }
+    if (c) atomicAdd(nesc, c);
+}
+
+// the outliers into place; an outlier whose symbol is not the escape is an error
+__global__ void k_outliers(const uint16_t* sym, const unsigned long long* oidx, const long long* oval, size_t m,
+                           long long* r, unsigned int* err) {
+    for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < m; s += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = oidx[s];
+        if (sym[i] != ESC) atomicOr(err, 1u);
+        r[i] = oval[s];
+    }
+}
+
+// inclusive prefix sums along z (a workgroup per row), y and x (a thread per column)
+__global__ __launch_bounds__(256) void k_scan_z(long long* v, size_t rows, size_t nz) {
+    __shared__ long long sh[256];
+    for (size_t row = blockIdx.x; row < rows; row += gridDim.x) {
+        long long carry = 0;
+        for (size_t c0 = 0; c0 < nz; c0 += 256) {
+            const size_t k = c0 + threadIdx.x;
+            sh[threadIdx.x] = k < nz ? v[row * nz + k] : 0ll;
+            __syncthreads();
+            for (unsigned int off = 1; off < 256; off <<= 1) {
+                const long long t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0ll;
+                __syncthreads();
+                sh[threadIdx.x] += t;
+                __syncthreads();
+            }
+            if (k < nz) v[row * nz + k] = sh[threadIdx.x] + carry;
+            carry += sh[255];
+            __syncthreads();
+        }
+    }
+}
+__global__ void k_scan_strided(long long* v, size_t outer, size_t len, size_t inner) {
+    // columns (o, i) of length len, element (o, l, i) at (o*len + l)*inner + i
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < outer * inner;
+         c += (size_t)gridDim.x * blockDim.x) {
+        const size_t o = c / inner, i = c - o * inner;
+        long long acc = 0;
+        for (size_t l = 0; l < len; ++l) {
+            const size_t at = (o * len + l) * inner + i;
+            acc += v[at];
+            v[at] = acc;
+        }
+    }
+}
+
+template <typename T>
+__global__ void k_dequant_base(const long long* q, size_t n, double two_eps, T* xp) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        xp[i] = (T)((double)(int32_t)q[i] * two_eps);
+}
+
 }  // namespace
 
 template <typename T>
@@ -689,6 +809,168 @@ void container_decode(const uint8_t* buf, size_t len, const void* x_prime, void*
     }
 }
 
+template <typename T>
+std::vector<uint8_t> base_compress(const T* x, size_t nx, size_t ny, size_t nz, double eps, int zstd_level,
+                                   T* x_prime, hipStream_t s) {
+    FFCZ_REQUIRE(x != nullptr && x_prime != nullptr, ERR_INVALID, "null argument");
+    FFCZ_REQUIRE(eps > 0 && std::isfinite(eps), ERR_INVALID, "the base bound must be positive");
+    FFCZ_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1 && nx * ny <= ((size_t)1 << 40) / nz, ERR_INVALID, "bad shape");
+    const size_t n = nx * ny * nz, ocap = n / 16 + 4096;
+    DBuf bx, bxp, bq, bs, bc, bi, bv, bb;
+    const StreamWait wait{s};
+    const T* dx = x;
+    if (!on_device(x)) {
+        T* t = bx.alloc<T>(n);
+        FFCZ_HIP(hipMemcpyAsync(t, x, n * sizeof(T), hipMemcpyHostToDevice, s));
+        dx = t;
+    }
+    const bool dev_out = on_device(x_prime);
+    T* dxp = dev_out ? x_prime : bxp.alloc<T>(n);
+    int32_t* q = bq.alloc<int32_t>(n);
+    uint16_t* sym = bs.alloc<uint16_t>(n);
+    unsigned long long* cnt = bc.alloc<unsigned long long>(1);
+    unsigned long long* oidx = bi.alloc<unsigned long long>(ocap);
+    long long* oval = bv.alloc<long long>(ocap);
+    unsigned int* bad = bb.alloc<unsigned int>(1);
+    FFCZ_HIP(hipMemsetAsync(cnt, 0, 8, s));
+    FFCZ_HIP(hipMemsetAsync(bad, 0, 4, s));
+    hipLaunchKernelGGL((k_prequant<T>), dim3(grid_for(n)), dim3(256), 0, s, dx, n, 1.0 / (2.0 * eps), 2.0 * eps, q,
+                       dxp, bad);
+    hipLaunchKernelGGL(k_lorenzo, dim3(grid_rows(nx * ny)), dim3(256), 0, s, (const int32_t*)q, nx, ny, nz, sym, cnt,
+                       oidx, oval, ocap);
+    FFCZ_HIP(hipGetLastError());
+    unsigned long long m = 0;
+    unsigned int hbad = 0;
+    FFCZ_HIP(hipMemcpyAsync(&m, cnt, 8, hipMemcpyDeviceToHost, s));
+    FFCZ_HIP(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, s));
+    FFCZ_HIP(hipStreamSynchronize(s));
+    FFCZ_REQUIRE(hbad == 0, ERR_RANGE, "the base bound is too small for the field's range (|x|/(2 eps) >= 2^30)");
+    FFCZ_REQUIRE(m <= ocap, ERR_RANGE, "too many base residuals outside 16 bits (the bound is too small)");
+    std::vector<unsigned long long> hi(m);
+    std::vector<long long> hv(m);
+    if (m) {
+        FFCZ_HIP(hipMemcpyAsync(hi.data(), oidx, m * 8, hipMemcpyDeviceToHost, s));
+        FFCZ_HIP(hipMemcpyAsync(hv.data(), oval, m * 8, hipMemcpyDeviceToHost, s));
+        FFCZ_HIP(hipStreamSynchronize(s));
+    }
+    // index order (the atomics filled the list in any order)
+    std::vector<size_t> ord(m);
+    for (size_t i = 0; i < m; ++i) ord[i] = i;
+    std::sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return hi[a] < hi[b]; });
+    std::vector<uint8_t> buf;
+    Writer w{buf};
+    w.bytes(BMAGIC, 4);
+    w.put<uint32_t>(VERSION);
+    w.put<uint32_t>(sizeof(T) == 4 ? 0u : 1u);
+    w.put<uint32_t>(0);
+    w.put<uint64_t>(nx);
+    w.put<uint64_t>(ny);
+    w.put<uint64_t>(nz);
+    w.put<double>(eps);
+    w.put<uint64_t>(m);
+    pack_stream<uint16_t>(w, sym, n, zstd_level, s);
+    for (size_t i : ord) w.put<uint64_t>(hi[i]);
+    for (size_t i : ord) w.put<int64_t>(hv[i]);
+    if (!dev_out) {
+        FFCZ_HIP(hipMemcpyAsync(x_prime, dxp, n * sizeof(T), hipMemcpyDeviceToHost, s));
+        FFCZ_HIP(hipStreamSynchronize(s));
+    }
+    return buf;
+}
+
+ContainerInfo base_info(const uint8_t* buf, size_t len) {
+    FFCZ_REQUIRE(buf != nullptr, ERR_INVALID, "null base stream");
+    Reader r{buf, len};
+    FFCZ_REQUIRE(std::memcmp(r.take(4), BMAGIC, 4) == 0, ERR_INVALID, "not an FFCz base stream");
+    FFCZ_REQUIRE(r.get<uint32_t>() == VERSION, ERR_INVALID, "unsupported FFCz base stream version");
+    ContainerInfo in;
+    in.dtype = (int)r.get<uint32_t>();
+    (void)r.get<uint32_t>();
+    in.nx = r.get<uint64_t>();
+    in.ny = r.get<uint64_t>();
+    in.nz = r.get<uint64_t>();
+    in.eps = r.get<double>();
+    in.n_spatial = r.get<uint64_t>();  // outliers
+    FFCZ_REQUIRE(in.dtype == 0 || in.dtype == 1, ERR_INVALID, "bad base stream dtype");
+    FFCZ_REQUIRE(in.nx >= 1 && in.ny >= 1 && in.nz >= 1 && in.nx <= (1u << 20) && in.ny <= (1u << 20) &&
+                     in.nz <= (1u << 20) && in.nx * in.ny <= ((size_t)1 << 40) / in.nz,
+                 ERR_INVALID, "bad base stream shape");
+    FFCZ_REQUIRE(in.eps > 0 && std::isfinite(in.eps), ERR_INVALID, "bad base stream bound");
+    return in;
+}
+
+void base_decompress(const uint8_t* buf, size_t len, void* x_prime, hipStream_t s) {
+    FFCZ_REQUIRE(x_prime != nullptr, ERR_INVALID, "null argument");
+    const ContainerInfo in = base_info(buf, len);
+    Reader r{buf, len};
+    (void)r.take(BHDR_BYTES);
+    const size_t n = in.nx * in.ny * in.nz, m = in.n_spatial;
+    const Stream st = read_stream(r, 2);
+    FFCZ_REQUIRE(st.nsym == n, ERR_INVALID, "base stream does not match the shape");
+    FFCZ_REQUIRE(m <= n, ERR_INVALID, "bad base outlier count");
+    const uint8_t* pi = r.take(m * 8);
+    const uint8_t* pv = r.take(m * 8);
+    std::vector<unsigned long long> hi(m);
+    std::vector<long long> hv(m);
+    if (m) {
+        std::memcpy(hi.data(), pi, m * 8);
+        std::memcpy(hv.data(), pv, m * 8);
+    }
+    for (size_t i = 0; i < m; ++i)
+        FFCZ_REQUIRE(hi[i] < n && (i == 0 || hi[i] > hi[i - 1]), ERR_INVALID, "bad base outlier index");
+    const Prepared pr = prepare(st);
+    DBuf bs, br, bi, bv, bc, be, bxp;
+    const StreamWait wait{s};
+    uint16_t* sym = bs.alloc<uint16_t>(n);
+    gpu_decode<uint16_t>(pr, sym, s);
+    long long* v = br.alloc<long long>(n);
+    unsigned long long* nesc = bc.alloc<unsigned long long>(1);
+    unsigned int* err = be.alloc<unsigned int>(1);
+    FFCZ_HIP(hipMemsetAsync(nesc, 0, 8, s));
+    FFCZ_HIP(hipMemsetAsync(err, 0, 4, s));
+    hipLaunchKernelGGL(k_unsym, dim3(grid_for(n)), dim3(256), 0, s, (const uint16_t*)sym, n, v, nesc);
+    if (m) {
+        unsigned long long* di = bi.alloc<unsigned long long>(m);
+        long long* dv = bv.alloc<long long>(m);
+        FFCZ_HIP(hipMemcpyAsync(di, hi.data(), m * 8, hipMemcpyHostToDevice, s));
+        FFCZ_HIP(hipMemcpyAsync(dv, hv.data(), m * 8, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_outliers, dim3(grid_for(m)), dim3(256), 0, s, (const uint16_t*)sym,
+                           (const unsigned long long*)di, (const long long*)dv, m, v, err);
+    }
+    FFCZ_HIP(hipGetLastError());
+    unsigned long long hesc = 0;
+    unsigned int herr = 0;
+    FFCZ_HIP(hipMemcpyAsync(&hesc, nesc, 8, hipMemcpyDeviceToHost, s));
+    FFCZ_HIP(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, s));
+    FFCZ_HIP(hipStreamSynchronize(s));
+    FFCZ_REQUIRE(herr == 0 && hesc == m, ERR_INVALID, "base outliers do not match the escaped symbols");
+    // the inverse of the mixed difference: prefix sums along z, y and x
+    hipLaunchKernelGGL(k_scan_z, dim3(grid_rows(in.nx * in.ny)), dim3(256), 0, s, v, in.nx * in.ny, in.nz);
+    hipLaunchKernelGGL(k_scan_strided, dim3(grid_for(in.nx * in.nz)), dim3(256), 0, s, v, in.nx, in.ny, in.nz);
+    hipLaunchKernelGGL(k_scan_strided, dim3(grid_for(in.ny * in.nz)), dim3(256), 0, s, v, (size_t)1, in.nx,
+                       in.ny * in.nz);
+    FFCZ_HIP(hipGetLastError());
+    const bool dev_out = on_device(x_prime);
+    const double two_eps = 2.0 * in.eps;
+    if (in.dtype == 0) {
+        float* xp = dev_out ? static_cast<float*>(x_prime) : bxp.alloc<float>(n);
+        hipLaunchKernelGGL((k_dequant_base<float>), dim3(grid_for(n)), dim3(256), 0, s, (const long long*)v, n, two_eps,
+                           xp);
+        if (!dev_out) FFCZ_HIP(hipMemcpyAsync(x_prime, xp, n * 4, hipMemcpyDeviceToHost, s));
+    } else {
+        double* xp = dev_out ? static_cast<double*>(x_prime) : bxp.alloc<double>(n);
+        hipLaunchKernelGGL((k_dequant_base<double>), dim3(grid_for(n)), dim3(256), 0, s, (const long long*)v, n,
+                           two_eps, xp);
+        if (!dev_out) FFCZ_HIP(hipMemcpyAsync(x_prime, xp, n * 8, hipMemcpyDeviceToHost, s));
+    }
+    FFCZ_HIP(hipGetLastError());
+    FFCZ_HIP(hipStreamSynchronize(s));
+}
+
+template std::vector<uint8_t> base_compress<float>(const float*, size_t, size_t, size_t, double, int, float*,
+                                                   hipStream_t);
+template std::vector<uint8_t> base_compress<double>(const double*, size_t, size_t, size_t, double, int, double*,
+                                                    hipStream_t);
 template std::vector<uint8_t> container_pack<float>(Solver<float>&, int);
 template std::vector<uint8_t> container_pack<double>(Solver<double>&, int);
 

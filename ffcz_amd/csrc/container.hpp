@@ -36,4 +36,17 @@ ContainerInfo container_info(const uint8_t* buf, size_t len);
 // before any device work.
 void container_decode(const uint8_t* buf, size_t len, const void* x_prime, void* x_hat, hipStream_t s);
 
+// The base compressor that produces x' for the solver (the survey's "next"
+// row F4): dual-quantisation Lorenzo.  x' = round(x/(2 eps)) * 2 eps is
+// written to x_prime (host or device, N elements), and the returned stream
+// codes the 3D Lorenzo residuals of the integers (Huffman + ZSTD, outliers
+// beyond 16 bits listed).  Layout: container.hip.
+template <typename T>
+std::vector<uint8_t> base_compress(const T* x, size_t nx, size_t ny, size_t nz, double eps, int zstd_level,
+                                   T* x_prime, hipStream_t s);
+// header fields of a base stream (n_spatial: its outlier count)
+ContainerInfo base_info(const uint8_t* buf, size_t len);
+// x' from a base stream (host or device, N elements of its dtype)
+void base_decompress(const uint8_t* buf, size_t len, void* x_prime, hipStream_t s);
+
 }  // namespace ffcz

@@ -261,6 +261,20 @@ int ffcz_container_info(const uint8_t* buf, size_t len, int* dtype, size_t* dims
  * x_prime and x_hat: N elements of the container's dtype, host or device */
 int ffcz_container_decode(const uint8_t* buf, size_t len, const void* x_prime, void* x_hat, void* stream);
 
+/* ---- base compressor producing x' on the GPU ----------------------------
+ * No counterpart in the reference, which takes x' from an external base
+ * compressor through a file (projection_example.cu:164-176; the survey's
+ * "next" row F4).  Dual-quantisation Lorenzo: x' = round(x/(2 eps)) * 2 eps
+ * goes to x_prime (host or device, N elements, |x' - x| <= eps up to the
+ * rounding of the product), and the blob codes the 3D Lorenzo residuals of
+ * the integers (Huffman + ZSTD).  x' can feed ffcz_init_* / ffcz_solve_*
+ * device to device. */
+int ffcz_base_compress(int dtype, const void* x, size_t nx, size_t ny, size_t nz, double eps, int zstd_level,
+                       void* x_prime, ffcz_blob** out, void* stream);
+/* header fields: dims[3], the outlier count (residuals beyond 16 bits) */
+int ffcz_base_info(const uint8_t* buf, size_t len, int* dtype, size_t* dims, double* eps, size_t* n_outliers);
+int ffcz_base_decompress(const uint8_t* buf, size_t len, void* x_prime, void* stream);
+
 /* ---- per-kernel entries on device pointers (stream may be NULL) ---------
  * 3D transforms with cuFFT's layout (compact half spectrum, interleaved
  * complex, unnormalised): cufftExecR2C/D2Z and C2R/Z2D at

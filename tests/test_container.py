@@ -100,3 +100,14 @@ def test_a_code_table_that_is_not_a_prefix_code_is_rejected():
     b = _patch(b, len_off + 1, "<B", int(streams[0]["lens"][0]))
     with pytest.raises(FFCzError, match="not a prefix code"):
         decode_raw(b)
+
+
+def test_base_stream_header_is_checked():
+    with pytest.raises(FFCzError, match="not an FFCz base stream"):
+        container.base_info(b"FFCZ" + bytes(60))
+    with pytest.raises(FFCzError, match="truncated container"):
+        container.base_info(b"FFCB" + struct.pack("<I", 1) + bytes(10))
+    hdr = struct.pack("<4sIII3QdQ", b"FFCB", 1, 0, 0, 4, 5, 6, 1e-3, 0)
+    assert container.base_info(hdr) == {"dtype": np.float32, "shape": (4, 5, 6), "eps": 1e-3, "outliers": 0}
+    with pytest.raises(FFCzError, match="bad base stream bound"):
+        container.base_info(struct.pack("<4sIII3QdQ", b"FFCB", 1, 0, 0, 4, 5, 6, -1.0, 0))
