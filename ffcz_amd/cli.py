@@ -10,6 +10,10 @@ Decompress (no counterpart in the reference, which cannot decode its output)::
 
     python -m ffcz_amd.cli -x -z edits.ffcz -e base.f32 -o decompressed.f32
 
+With ``--base`` the library's own GPU base compressor (dual-quantisation
+Lorenzo, bound -M) produces x' instead of the -e file, and its stream is
+written to ``<-z>.base``; ``-x --base -z edits.ffcz -o out`` decodes both.
+
 Differences from the reference driver:
   * ``-M REL xi`` and ``-F REL xi`` work (the reference parses and ignores
     them): the spatial bound becomes xi * (max x - min x), the frequency bound
@@ -49,6 +53,7 @@ USAGE = """Usage:
   --zstd <level>  : ZSTD level of the coded streams (default 20)
   --step-quant    : Quantise edits with step eps/8 and delta/8 (default: the reference's 16-bit quantiser)
   --margin <m>    : Iterate against bounds shrunk by m, m in [0, 1) (default 0)
+  --base          : Produce x' with the built-in GPU base compressor (no -e); its stream goes to <-z>.base
 """
 
 
@@ -74,6 +79,7 @@ class Args:
     zstd: int = 20
     step_quant: bool = False
     margin: float = 0.0
+    base_gpu: bool = False
     extra: dict = field(default_factory=dict)
 
 
@@ -128,14 +134,21 @@ def parse(argv) -> Args:
                     conv(argv[i]))
         elif arg == "--step-quant":
             a.step_quant = True
+        elif arg == "--base":
+            a.base_gpu = True
         else:
             raise UsageError("Unknown argument")
         i += 1
     if a.decompress:
-        if not (a.compressed and a.base and a.decompressed):
-            raise UsageError("Decompression needs the compressed file (-z), the base reconstruction (-e) and -o")
+        if not (a.compressed and (a.base or a.base_gpu) and a.decompressed):
+            raise UsageError("Decompression needs the compressed file (-z), the base reconstruction (-e or --base) "
+                             "and -o")
         return a
-    if not a.original or not a.base:
+    if a.base_gpu and a.base:
+        raise UsageError("--base replaces -e")
+    if a.base_gpu and not a.compressed:
+        raise UsageError("--base needs -z (its stream goes to <-z>.base)")
+    if not a.original or not (a.base or a.base_gpu):
         raise UsageError("Input files of original data (-i) and base decompressed data (-e) are mandatory")
     if not a.shape:
         raise UsageError("Missing size of the data (-1, -2 or -3)")
@@ -183,9 +196,17 @@ def compress(a: Args, out=print) -> dict:
     dev = torch.device("cuda:0")
     n = int(np.prod(a.shape))
     x = torch.from_numpy(read_raw(a.original, n, a.dtype).reshape(a.shape)).to(dev)
-    xb = torch.from_numpy(read_raw(a.base, n, a.dtype).reshape(a.shape)).to(dev)
-    e0 = xb - x  # projection_example.cu:173-176
     eps, delta = bounds(x, a, torch, dev)
+    if a.base_gpu:
+        tb = time.perf_counter()
+        base, xb = container.base_compress(x, eps, zstd_level=a.zstd)
+        torch.cuda.synchronize()
+        out(f"Base stream: {len(base)} bytes ({1e3 * (time.perf_counter() - tb):.1f} ms)")
+        with open(a.compressed + ".base", "wb") as f:
+            f.write(base)
+    else:
+        xb = torch.from_numpy(read_raw(a.base, n, a.dtype).reshape(a.shape)).to(dev)
+    e0 = xb - x  # projection_example.cu:173-176
     t0 = time.perf_counter()
     with ProjectionSolver(*a.shape, dtype=a.dtype) as s:
         s.initialize_all_abs(x, eps, delta)
@@ -234,7 +255,12 @@ def decompress(a: Args, out=print) -> None:
         buf = f.read()
     info = container.info(buf)
     n = int(np.prod(info["shape"]))
-    xb = torch.from_numpy(read_raw(a.base, n, info["dtype"])).to(torch.device("cuda:0"))
+    if a.base_gpu:
+        with open(a.compressed + ".base", "rb") as f:
+            xb = container.base_decompress(f.read(), torch.empty(info["shape"], dtype=getattr(
+                torch, np.dtype(info["dtype"]).name), device="cuda:0"))
+    else:
+        xb = torch.from_numpy(read_raw(a.base, n, info["dtype"])).to(torch.device("cuda:0"))
     t0 = time.perf_counter()
     xh = container.decode(buf, xb)
     torch.cuda.synchronize()

@@ -32,6 +32,8 @@ def test_reference_flags_and_dimension_conventions():
     ("-i", "Missing input file path"),
     ("-x -z c -o d", "Decompression needs"),
     ("-i o -e b -3 2 2 2 -M ABS 1 -F ABS 1 --margin 1.5", "margin"),
+    ("-i o -e b --base -z c -3 2 2 2 -M ABS 1 -F ABS 1", "replaces -e"),
+    ("-i o --base -3 2 2 2 -M ABS 1 -F ABS 1", "needs -z"),
 ])
 def test_bad_arguments_are_reported(argv, msg, capsys):
     with pytest.raises(cli.UsageError, match=msg):

@@ -37,3 +37,20 @@ def test_compress_then_decompress(gpu, tmp_path, shape, flag, dtype, tflag):
     assert info["shape"] == shape and info["n_active_spatial"] == r["n_active_spatial"]
     assert cli.main(["-x", "-z", fz, "-e", fb, "-o", fd2]) == 0
     assert open(fd, "rb").read() == open(fd2, "rb").read()
+
+
+def test_built_in_base_compressor(gpu, tmp_path):
+    shape = (16, 20, 30)
+    x = np.random.default_rng(6).random(shape).astype(np.float32)
+    eps = 1e-3
+    fo, fz, fd, fd2 = (str(tmp_path / n) for n in ("o.raw", "c.ffcz", "d.raw", "d2.raw"))
+    x.tofile(fo)
+    lines = []
+    argv = ["-i", fo, "--base", "-z", fz, "-o", fd, "-3", *map(str, shape), "-f", "-M", "ABS", str(eps), "-F", "REL",
+            "1e-3", "--margin", "0.125", "--step-quant", "--zstd", "3"]
+    r = cli.compress(cli.parse(argv), out=lines.append)
+    assert r["certified"] and any(l.startswith("Base stream:") for l in lines)
+    xh = np.fromfile(fd, dtype=np.float32).reshape(shape)
+    assert np.abs(xh.astype(np.float64) - x).max() <= eps * (1 + 1e-6)
+    assert cli.main(["-x", "--base", "-z", fz, "-o", fd2]) == 0
+    assert open(fd, "rb").read() == open(fd2, "rb").read()
