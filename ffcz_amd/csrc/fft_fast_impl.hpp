@@ -403,6 +403,29 @@ __device__ __forceinline__ void strided_body(const SArgs<R, T>& a, uint32_t blk)
         __syncthreads();
         if constexpr (PREC == PREC_VALU) pen::transform<R, L, TP, true, false>(lld, gst, lld, lst, tw, t, +1);
         else mx::transform<PREC, L, TP, W, W, NT, true, false, false, false>(gldp, gstp, buf, tw, tid, p, t, +1);
+    } else if constexpr (MODE == S_CLAMP_PASS && PREC != PREC_VALU) {
+        // The MFMA fused x pass (every bound kind): the forward transform ends
+        // in the tile and the clamp runs as a pass of its own over it, one bin
+        // at a time (freq_clamp_bin).  Clamping in the transform's last vector
+        // stage kept the clamp's state live there and pushed the kernel, held
+        // to 64 VGPRs, into spills.
+        auto bstp = [&](int n, int pp, cplx<R> v) { buf[n * W + pp] = v; };
+        mx::transform<PREC, L, TP, W, W, NT, false, false, true>(gldp, bstp, buf, tw, tid, p, t, -1);
+        __syncthreads();
+        constexpr int PER = (L * W + NT - 1) / NT;
+        const size_t obase = o * a.cs_o + (size_t)ct * W;
+        bool fired = false;
+        for (int k = 0; k < PER; ++k) {
+            const int q = tid + k * NT;
+            const int pp = q % W;
+            if (q >= L * W || ct * W + pp >= a.ncols) continue;
+            cplx<R> v = buf[q];
+            fired |= freq_clamp_bin(v, (size_t)(q / W) * a.cs_n + obase + pp, a);
+            buf[q] = v;
+        }
+        if (fired) a.st->freq_clamped = 1u;
+        __syncthreads();
+        mx::transform<PREC, L, TP, W, W, NT, true, false, false, false>(gldp, gstp, buf, tw, tid, p, t, +1);
     } else if constexpr (MODE == S_CLAMP_PASS) {
         // Per-bin and ptw bounds: the forward transform ends in the tile, then
         // the clamp runs as a pass of its own over it.  Each thread issues all
@@ -744,19 +767,22 @@ bool mfma_zinv_dispatch(int mode, const ZArgs<float, float>& a, hipStream_t s) {
 }
 template <int PREC>
 bool mfma_strided_dispatch(int mode, const SArgs<float, float>& a, size_t outer, hipStream_t s) {
-    if (mode != S_PLAIN && mode != S_CLAMP) return false;
+    // the MFMA fused x pass clamps in a pass of its own over the tile, for
+    // every bound kind
+    if (mode != S_PLAIN && mode != S_CLAMP && mode != S_CLAMP_PASS) return false;
+    const bool clamp = mode != S_PLAIN;
     switch (a.L) {
-#define FFCZ_MSL(Lv)                                                                                      \
-    case Lv:                                                                                              \
-        if constexpr (mfma_s_v1<Lv>()) {                                                                  \
-            if (a.var) {                                                                                  \
-                if (mode == S_CLAMP) launch_strided_len<float, float, S_CLAMP, Lv, PREC, 1>(a, outer, s); \
-                else launch_strided_len<float, float, S_PLAIN, Lv, PREC, 1>(a, outer, s);                 \
-                return true;                                                                              \
-            }                                                                                             \
-        }                                                                                                 \
-        if (mode == S_CLAMP) launch_strided_len<float, float, S_CLAMP, Lv, PREC>(a, outer, s);            \
-        else launch_strided_len<float, float, S_PLAIN, Lv, PREC>(a, outer, s);                            \
+#define FFCZ_MSL(Lv)                                                                                 \
+    case Lv:                                                                                         \
+        if constexpr (mfma_s_v1<Lv>()) {                                                             \
+            if (a.var) {                                                                             \
+                if (clamp) launch_strided_len<float, float, S_CLAMP_PASS, Lv, PREC, 1>(a, outer, s); \
+                else launch_strided_len<float, float, S_PLAIN, Lv, PREC, 1>(a, outer, s);            \
+                return true;                                                                         \
+            }                                                                                        \
+        }                                                                                            \
+        if (clamp) launch_strided_len<float, float, S_CLAMP_PASS, Lv, PREC>(a, outer, s);            \
+        else launch_strided_len<float, float, S_PLAIN, Lv, PREC>(a, outer, s);                       \
         return true;
         FFCZ_MFMA_S_LENGTHS(FFCZ_MSL)
 #undef FFCZ_MSL
