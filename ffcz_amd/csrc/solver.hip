@@ -436,7 +436,18 @@ void Solver<T>::extract() {
     materialize_ds();
     QuantDesc qf = freq_quant(), qs = spatial_quant();
     qf.dminmax = qs.dminmax = nullptr;  // the kernels compute the range themselves
-    extract_edits_async<T>(df_, 2 * f_.F, qf, fpack_, fcomp_, fquant_, xscratch_, &xglob_[0], &xhost_[0], s_);
+    if (df_touched_) {
+        extract_edits_async<T>(df_, 2 * f_.F, qf, fpack_, fcomp_, fquant_, xscratch_, &xglob_[0], &xhost_[0], s_);
+    } else {
+        // no frequency clamp fired since df was cleared, so df holds +0 only:
+        // what the full pass would find is an all-zero flag pack, no active
+        // bin and the range [0, 0], written here without reading df
+        const size_t words = ceil_div(2 * f_.F, (size_t)64);
+        hipLaunchKernelGGL((k_zero<unsigned long long>), dim3(gridn(words)), dim3(256), 0, s_,
+                           (unsigned long long*)fpack_, words);
+        FFCZ_HIP(hipGetLastError());
+        extract_edits_async<T>(df_, 0, qf, fpack_, fcomp_, fquant_, xscratch_, &xglob_[0], &xhost_[0], s_);
+    }
     extract_edits_async<T>(ds_, f_.N, qs, spack_, scomp_, squant_, xscratch_, &xglob_[1], &xhost_[1], s_);
     xpending_ = true;
     extracted_ = true;

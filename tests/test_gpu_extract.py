@@ -219,3 +219,53 @@ def test_certify_both_equals_the_two_certifications(gpu, shape, dtype, quant):
         again = (s.get_spatial_flags_pack(), s.get_freq_flags_pack(), s.get_active_spatial_quant_edit_distances(),
                  s.get_active_freq_quant_edit_distances(), s.get_active_spatial_edit_distances())
         assert all(np.array_equal(a, b) for a, b in zip(got, again))
+
+
+def _expected_extraction(edits, step):
+    """The oracle's extraction of one edit array: the reference quantiser's
+    (step None) or the step quantiser's, which flags the values whose symbol
+    is non-zero; the range is the full array's in both."""
+    if step is None:
+        return O.extract_one(edits)
+    q = O.quantize_step(edits, step)
+    flags = q != 0
+    return O.Extraction(flags=flags, pack=O.pack_flags(flags), compact=edits[flags],
+                        quant=O.zigzag(q[flags]).astype(np.uint16), vmin=edits.min(), vmax=edits.max())
+
+
+@pytest.mark.parametrize("quant", [False, True])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_extraction_of_untouched_frequency_edits(gpu, dtype, quant):
+    # a solve in which no frequency clamp fires leaves df at zero, and the
+    # extraction writes its results without reading df: they must be what the
+    # full extraction of an all-zero df gives, also after an extraction whose
+    # flag pack was not zero
+    shape = (6, 10, 12)
+    x, e0, eps, delta = binding_case(shape, seed=11, dtype=dtype)
+    with ProjectionSolver(*shape, dtype=dtype) as s:
+        # (bound, reset df first, df non-zero after the solve): the edits of a
+        # solve add to those kept from the previous one unless reset
+        for d, reset, fires in ((delta, False, True), (delta * 1e6, False, True), (delta * 1e6, True, False)):
+            s.initialize_all_abs(x, eps, d)
+            if reset:
+                s.reset_edit_accumulators()
+            if quant:
+                s.set_quantizer(QUANT_STEP, d / 8, eps / 8)
+            s.solve_all_abs(e0, 20, 1e-7)
+            df = s.get_freq_edits_full()
+            assert bool(np.any(df != 0)) == fires
+            for got, edits, step in (
+                ((s.get_freq_flags(), s.get_freq_flags_pack(), s.get_active_freq_edit_distances(),
+                  s.get_active_freq_quant_edit_distances(), s.get_min_freq_edit(), s.get_max_freq_edit()),
+                 df, d / 8 if quant else None),
+                ((s.get_spatial_flags(), s.get_spatial_flags_pack(), s.get_active_spatial_edit_distances(),
+                  s.get_active_spatial_quant_edit_distances(), s.get_min_spatial_edit(), s.get_max_spatial_edit()),
+                 s.get_spatial_edits_full(), eps / 8 if quant else None),
+            ):
+                ref = _expected_extraction(edits, step)
+                flags, pack, comp, qsym, lo, hi = got
+                assert np.array_equal(flags, ref.flags)
+                assert np.array_equal(pack, ref.pack)
+                assert np.array_equal(comp, ref.compact)
+                assert np.array_equal(qsym, ref.quant)
+                assert lo == ref.vmin and hi == ref.vmax
