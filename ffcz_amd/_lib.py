@@ -103,6 +103,7 @@ SIGNATURES = {
     "ffcz_autotune": [_h, _i, _i, C.POINTER(_i), C.POINTER(_f)],
     "ffcz_get_tile_vars": [_h, C.POINTER(_i)],
     "ffcz_set_tile_vars": [_h, C.POINTER(_i)],
+    "ffcz_set_cert_pair": [_h, _i],
     "ffcz_get_fft_precision": [_h, C.POINTER(_i)],
     "ffcz_certify": [_h, _i, C.POINTER(Cert)],
     "ffcz_certify_both": [_h, C.POINTER(Cert)],

@@ -520,7 +520,20 @@ int ffcz_set_tile_vars(ffcz_solver* h, const int* vars) {
     return guard([&] {
         FFCZ_REQUIRE(vars != nullptr, ERR_INVALID, "null input");
         with(h, [&](auto& s) {
-            s.set_tile_vars(TileVars{vars[0], vars[1], vars[2], vars[3], vars[4], vars[5], vars[6]});
+            TileVars v = s.tile_vars();  // keeps the paired certification's choice
+            int* p = &v.zf;
+            for (int i = 0; i < 7; ++i) p[i] = vars[i];
+            s.set_tile_vars(v);
+        });
+    });
+}
+int ffcz_set_cert_pair(ffcz_solver* h, int paired) {
+    return guard([&] {
+        FFCZ_REQUIRE(paired == 0 || paired == 1, ERR_INVALID, "paired is 0 or 1");
+        with(h, [&](auto& s) {
+            TileVars v = s.tile_vars();
+            v.cpair = paired;
+            s.set_tile_vars(v);
         });
     });
 }

@@ -66,6 +66,9 @@ struct Fft3d {
     // tile variant of each pass's length-specialised kernel (autotuned; the
     // MFMA variants and batched launches use variant 0)
     int zf_var = 0, zi_var = 0, y_var = 0, x_var = 0;
+    // paired certification: both fields' forward z transforms in one launch
+    // (tile variant 0), else one launch each (autotuned)
+    bool cz_pair = true;
     int zinv_var() const { return prec == 0 ? zi_var : 0; }
     cplx<R>* d_tz = nullptr;   // W_Lz^m
     cplx<R>* d_tz2 = nullptr;  // W_nz^k, k <= nz/2 (R2C post / C2R pre)

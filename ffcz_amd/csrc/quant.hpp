@@ -28,17 +28,17 @@ __device__ __forceinline__ long long quant_step(double v, double step) { return 
 // quant_step_f(v, step) with the quotient taken through inv = 1/step (a
 // multiply instead of an FP64 division) where that cannot change the result.
 // r = v*inv is within 1.5*2^-52 (relative) of the rounded quotient v/step, so
-// below 2^24 in magnitude the two differ by less than 6e-9; farther than 1e-6
-// from a half-integer both then round to the same integer (the addition of
-// 1/2 is exact to 2^-29 there).  Near half-integers, for large or non-finite
-// values, and without a reciprocal (inv <= 0) the division decides.
+// below 2^24 in magnitude the two differ by less than 6e-9; where r lies
+// farther than 1e-6 from a half-integer, both round to the same integer, which
+// rint(r) gives exactly (no tie to break there).  Near half-integers, for
+// large or non-finite values, and without a reciprocal (inv <= 0) the
+// division decides.  A zero result may carry either sign.
+__device__ __forceinline__ bool quant_step_fast(double r, double n, double inv) {
+    return inv > 0.0 && fabs(r) < 16777216.0 && fabs(r - n) < 0.499999;
+}
 __device__ __forceinline__ double quant_step_fi(double v, double step, double inv) {
-    const double r = v * inv;
-    const double a = fabs(r);
-    if (inv > 0.0 && a < 16777216.0 && fabs((a - floor(a)) - 0.5) > 1e-6) {
-        const double n = floor(a + 0.5);
-        return r < 0.0 ? -n : n;
-    }
+    const double r = v * inv, n = rint(r);
+    if (quant_step_fast(r, n, inv)) return n;
     return quant_step_f(v, step);
 }
 
@@ -70,9 +70,13 @@ __device__ __forceinline__ double dequant(T v, const QuantDesc& q) {
         const T r = qv / (T)65535 * (mx - mn) + mn;
         return (double)r;
     }
-    // edits below half a step dequantise to zero (with the quantiser's sign)
-    if (!quant_step_nonzero((double)v, q.step)) return v < (T)0 ? -0.0 : 0.0;
-    return quant_step_fi((double)v, q.step, q.inv_step) * q.step;  // no integer round trip
+    // edits below half a step dequantise to zero (with the quantiser's sign);
+    // the reciprocal decides wherever quant_step_fi's would, the division
+    // elsewhere, and the symbol stays an integer-valued double
+    const double dv = (double)v, r = dv * q.inv_step, n = rint(r);
+    if (quant_step_fast(r, n, q.inv_step)) return n != 0.0 ? n * q.step : (v < (T)0 ? -0.0 : 0.0);
+    if (!quant_step_nonzero(dv, q.step)) return v < (T)0 ? -0.0 : 0.0;
+    return quant_step_f(dv, q.step) * q.step;
 }
 
 }  // namespace ffcz

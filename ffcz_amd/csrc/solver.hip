@@ -719,6 +719,7 @@ TileVars Solver<T>::tile_vars() const {
         v.czf = f64_->zf_var;
         v.cy = f64_->y_var;
         v.cx = f64_->x_var;
+        v.cpair = f64_->cz_pair ? 1 : 0;
     }
     return v;
 }
@@ -735,6 +736,7 @@ void Solver<T>::set_tile_vars(const TileVars& v) {
     f64_->zf_var = v.czf;
     f64_->y_var = v.cy;
     f64_->x_var = v.cx;
+    f64_->cz_pair = v.cpair != 0;
 }
 
 template <typename T>
@@ -778,26 +780,37 @@ TileVars Solver<T>::autotune(int reps, bool use_cache, float* ms_out) {
     }
     ms[5] = best;
     reset_state<T>(st_, 0, 0.0, s_);
-    // certification passes: the whole FP64 check for each combination
+    // certification passes: the paired FP64 check of the unquantised and
+    // quantised fields, as certify_both runs it, for each combination of the
+    // z, y and x variants (combos 0-7, the two forward z transforms in one
+    // launch where the z variant is 0) and for z variant 0 with one z launch
+    // per field (combos 8-11)
     CertArgs<T> c = cert_args(false);
     c.have_df = false;
+    CertArgs<T> c2 = cert_args(true);
+    c2.have_df = false;
+    c2.work_spec = cert_spec_ + f64_->spec_elems();
+    auto set_combo = [&](int combo) {
+        const int k = combo < 8 ? combo : (combo - 8) << 1;
+        f64_->zf_var = k & 1;
+        f64_->y_var = (k >> 1) & 1;
+        f64_->x_var = (k >> 2) & 1;
+        f64_->cz_pair = combo < 8;
+    };
     float cbest = 0;
     int cb = 0;
     const bool cert_defined = init_mode_ == MODE_ABS || init_mode_ == MODE_DIRECT;
-    for (int combo = 0; combo < (cert_defined ? 8 : 0); ++combo) {
-        f64_->zf_var = combo & 1;
-        f64_->y_var = (combo >> 1) & 1;
-        f64_->x_var = (combo >> 2) & 1;
-        const float t = ev.time(s_, reps, [&] { ffcz::certify<T>(*f64_, c, s_); });
+    for (int combo = 0; combo < (cert_defined ? 12 : 0); ++combo) {
+        set_combo(combo);
+        const float t = ev.time(s_, reps, [&] { ffcz::certify_pair<T>(*f64_, c, c2, s_); });
         if (combo == 0) ms[6] = t;
         if (combo == 0 || t < 0.97f * cbest) {
             cbest = t;
             cb = combo;
         }
     }
-    f64_->zf_var = cb & 1;
-    f64_->y_var = (cb >> 1) & 1;
-    f64_->x_var = (cb >> 2) & 1;
+    set_combo(cb);
+    if (cert_defined) ms[8] = ev.time(s_, reps, [&] { ffcz::certify<T>(*f64_, c, s_); });
     accumulators_clobbered();
     reset_edit_accumulators();
     FFCZ_HIP(hipStreamSynchronize(s_));

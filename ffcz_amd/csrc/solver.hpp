@@ -16,6 +16,7 @@ enum SolveMode : int { MODE_DIRECT = 0, MODE_ABS = 1, MODE_PTW = 2 };
 // kernels and of the FP64 certification's z, y and x passes.
 struct TileVars {
     int zf, zi, y, x, czf, cy, cx;
+    int cpair = 1;  // paired certification's z launch (Fft3d::cz_pair)
 };
 
 struct CertResult {

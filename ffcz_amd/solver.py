@@ -270,6 +270,11 @@ class ProjectionSolver:
         v = (C.c_int * 7)(*[int(vars_.get(k, 0)) for k in self.TILE_VAR_NAMES])
         call("ffcz_set_tile_vars", self._h, v)
 
+    def set_cert_pair(self, paired: bool):
+        """certify_both's forward z transforms in one launch for both fields
+        (True) or one each (False); autotune chooses."""
+        call("ffcz_set_cert_pair", self._h, int(bool(paired)))
+
     def fft_precision(self) -> int:
         v = C.c_int()
         call("ffcz_get_fft_precision", self._h, C.byref(v))

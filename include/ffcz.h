@@ -131,7 +131,8 @@ int ffcz_set_margins(ffcz_solver* h, double spatial_margin, double freq_margin);
 int ffcz_set_quantizer(ffcz_solver* h, int kind, double freq_step, double spatial_step);
 int ffcz_certify(ffcz_solver* h, int quantized, ffcz_cert* out);
 /* both certifications after extraction, out[0] unquantised and out[1]
- * quantised, as one paired FP64 sequence (x'-x and the edits read once) */
+ * quantised, as one paired FP64 sequence (one y launch over both spectra;
+ * the two forward z transforms in one launch or two, as ffcz_autotune chose) */
 int ffcz_certify_both(ffcz_solver* h, ffcz_cert* out2);
 /* ---- batches: many fields of uneven shapes corrected together ----------
  * No counterpart in the reference (its driver, projection_example.cu:157-265,
@@ -188,13 +189,20 @@ int ffcz_set_fft_precision(ffcz_solver* h, int variant);
  * the same shape, type and FFT variant in this process.  vars[7] (optional)
  * receives the choice: iteration z fwd, z inv, y, x; certification z, y, x.
  * The iteration is tuned by coordinate descent over whole iterations (the
- * passes share caches); the certification over its 8 combinations.  ms[9]
+ * passes share caches); the certification over its 8 combinations, timed as
+ * the paired sequence of ffcz_certify_both, and, with z variant 0, also with
+ * one z launch per field instead of one for both (ffcz_set_cert_pair).  ms[9]
  * (optional; forces a fresh tuning): iteration ms with the defaults, with z
- * fwd, y, x, z inv switched in turn, with the choice; certification ms with
- * the defaults and with the choice; 0. */
+ * fwd, y, x, z inv switched in turn, with the choice; paired certification ms
+ * with the defaults and with the choice; single certification (ffcz_certify)
+ * ms with the choice. */
 int ffcz_autotune(ffcz_solver* h, int reps, int use_cache, int* vars, float* ms);
 int ffcz_get_tile_vars(ffcz_solver* h, int* vars);
 int ffcz_set_tile_vars(ffcz_solver* h, const int* vars);
+/* ffcz_certify_both's forward z transforms: paired=1 in one launch for both
+ * fields (when the certification z variant is 0), 0 in one launch each;
+ * ffcz_autotune sets it, ffcz_set_tile_vars keeps it. */
+int ffcz_set_cert_pair(ffcz_solver* h, int paired);
 int ffcz_get_fft_precision(ffcz_solver* h, int* variant);
 /* process-wide tuning: force_generic=1 runs the run-time-planned FFT kernels
  * instead of the length-specialised ones; slab_mb sets the slab size of the

@@ -192,13 +192,16 @@ def test_step_quantizer_overflow_is_reported(gpu):
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("shape", [(16, 20, 30), (32, 64, 128), (6, 10, 12), (9, 15, 25)])
 @pytest.mark.parametrize("quant", ["ref16", "step"])
-def test_certify_both_equals_the_two_certifications(gpu, shape, dtype, quant):
-    """The paired sequence (one z launch for both, one y launch over both
-    spectra) gives exactly the counts and maximal excesses of certify(False)
-    and certify(True); frequency edits present (the C2R of df runs)."""
+@pytest.mark.parametrize("paired", [True, False])
+def test_certify_both_equals_the_two_certifications(gpu, shape, dtype, quant, paired):
+    """The paired sequence (one y launch over both spectra; one z launch for
+    both or one each) gives exactly the counts and maximal excesses of
+    certify(False) and certify(True); frequency edits present (the C2R of df
+    runs)."""
     from ffcz_amd import QUANT_REF16
     x, e0, eps, delta = binding_case(shape, seed=11, dtype=dtype, spread=1.4, dfrac=0.8)
     with ProjectionSolver(*shape, dtype=dtype) as s:
+        s.set_cert_pair(paired)
         s.initialize_all_abs(x, eps, delta)
         s.set_margins(0.125, 0.125)
         if quant == "step":

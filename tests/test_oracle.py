@@ -346,11 +346,10 @@ def _quant_step_reciprocal(v, step):
     v = np.asarray(v, dtype=np.float64)
     inv = np.float64(1.0) / np.float64(step)
     r = v * inv
-    a = np.abs(r)
+    n = np.rint(r)  # round half to even, as the device's rint; no tie is taken where fast
     with np.errstate(invalid="ignore"):
-        fast = (a < 16777216.0) & (np.abs((a - np.floor(a)) - 0.5) > 1e-6)
-    n = np.floor(a + 0.5)
-    return fast, np.where(r < 0.0, -n, n)
+        fast = (np.abs(r) < 16777216.0) & (np.abs(r - n) < 0.499999)
+    return fast, n
 
 
 @pytest.mark.parametrize("step", [1e-5, 1.25e-5, 3.0e-3 / 7, 0.1, 6716.3 / 8])
