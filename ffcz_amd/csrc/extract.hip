@@ -477,7 +477,20 @@ void extract_edits_async(const T* v, size_t n, QuantDesc q, uint8_t* pack, T* co
     } else {
         extract_launch(v, n, q, pack, compact, quant, scratch, dglob, s);
     }
-    FFCZ_HIP(hipMemcpyAsync(hglob, dglob, sizeof(ExtractGlobals), hipMemcpyDeviceToHost, s));
+    if (hglob) FFCZ_HIP(hipMemcpyAsync(hglob, dglob, sizeof(ExtractGlobals), hipMemcpyDeviceToHost, s));
+}
+
+__global__ void k_extract_zeros(unsigned long long* pack, size_t words, ExtractGlobals* g) {
+    const size_t i0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i0 == 0) *g = ExtractGlobals{};
+    for (size_t i = i0; i < words; i += (size_t)gridDim.x * blockDim.x) pack[i] = 0ull;
+}
+
+void extract_zeros_async(uint8_t* pack, size_t n, ExtractGlobals* dglob, hipStream_t s) {
+    const size_t words = ceil_div(n, (size_t)64);
+    const unsigned g = (unsigned)std::max<size_t>(1, std::min<size_t>(ceil_div(words, (size_t)256), 4096));
+    hipLaunchKernelGGL(k_extract_zeros, dim3(g), dim3(256), 0, s, (unsigned long long*)pack, words, dglob);
+    FFCZ_HIP(hipGetLastError());
 }
 
 void ExtractBatchWork::release() {

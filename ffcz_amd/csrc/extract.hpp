@@ -39,11 +39,16 @@ template <typename T>
 ExtractResult extract_edits(const T* v, size_t n, QuantDesc q, uint8_t* pack, T* compact, uint16_t* quant,
                             void* scratch, hipStream_t s);
 // The same without a host synchronisation: results land in dglob (device,
-// kept until the next extraction into it) and are copied asynchronously to
-// hglob (pinned host memory) behind the kernels on stream s.
+// kept until the next extraction into it) and, unless hglob is null, are
+// copied asynchronously to hglob (pinned host memory) behind the kernels on
+// stream s.
 template <typename T>
 void extract_edits_async(const T* v, size_t n, QuantDesc q, uint8_t* pack, T* compact, uint16_t* quant,
                          void* scratch, ExtractGlobals* dglob, ExtractGlobals* hglob, hipStream_t s);
+// What extract_edits_async gives for n values that are all +0 (no flag set:
+// a zero flag pack of ceil(n/64) 64-bit words, count 0, range [0, 0]), in
+// one launch that reads no values.
+void extract_zeros_async(uint8_t* pack, size_t n, ExtractGlobals* dglob, hipStream_t s);
 
 // One edit array of a batched extraction, with its quantiser and outputs.
 template <typename T>

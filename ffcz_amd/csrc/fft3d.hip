@@ -495,6 +495,12 @@ __global__ void k_reset_stats(CertStats* s) {
         s->freq_max_bits = 0;
     }
 }
+__global__ void k_reset_stats2(CertStats* a, CertStats* b) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        *a = CertStats{};
+        *b = CertStats{};
+    }
+}
 
 // ===========================================================================
 // Host launchers
@@ -1118,8 +1124,7 @@ static void cert_xcheck(const Fft3d<double>& f, const CertArgs<T>& c, hipStream_
 
 template <typename T>
 void certify_pair(const Fft3d<double>& f, const CertArgs<T>& a, const CertArgs<T>& b, hipStream_t s) {
-    hipLaunchKernelGGL(k_reset_stats, dim3(1), dim3(64), 0, s, a.stats);
-    hipLaunchKernelGGL(k_reset_stats, dim3(1), dim3(64), 0, s, b.stats);
+    hipLaunchKernelGGL(k_reset_stats2, dim3(1), dim3(64), 0, s, a.stats, b.stats);
     FFCZ_HIP(hipGetLastError());
     const double* dfa = cert_df_field(f, a, s);
     const double* dfb = cert_df_field(f, b, s);

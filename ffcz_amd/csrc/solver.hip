@@ -437,18 +437,16 @@ void Solver<T>::extract() {
     QuantDesc qf = freq_quant(), qs = spatial_quant();
     qf.dminmax = qs.dminmax = nullptr;  // the kernels compute the range themselves
     if (df_touched_) {
-        extract_edits_async<T>(df_, 2 * f_.F, qf, fpack_, fcomp_, fquant_, xscratch_, &xglob_[0], &xhost_[0], s_);
+        extract_edits_async<T>(df_, 2 * f_.F, qf, fpack_, fcomp_, fquant_, xscratch_, &xglob_[0], nullptr, s_);
     } else {
         // no frequency clamp fired since df was cleared, so df holds +0 only:
-        // what the full pass would find is an all-zero flag pack, no active
-        // bin and the range [0, 0], written here without reading df
-        const size_t words = ceil_div(2 * f_.F, (size_t)64);
-        hipLaunchKernelGGL((k_zero<unsigned long long>), dim3(gridn(words)), dim3(256), 0, s_,
-                           (unsigned long long*)fpack_, words);
-        FFCZ_HIP(hipGetLastError());
-        extract_edits_async<T>(df_, 0, qf, fpack_, fcomp_, fquant_, xscratch_, &xglob_[0], &xhost_[0], s_);
+        // what the full pass would find (an all-zero flag pack, no active bin,
+        // the range [0, 0]) is written without reading df
+        extract_zeros_async(fpack_, 2 * f_.F, &xglob_[0], s_);
     }
-    extract_edits_async<T>(ds_, f_.N, qs, spack_, scomp_, squant_, xscratch_, &xglob_[1], &xhost_[1], s_);
+    extract_edits_async<T>(ds_, f_.N, qs, spack_, scomp_, squant_, xscratch_, &xglob_[1], nullptr, s_);
+    // both arrays' counts and ranges reach the host in one copy
+    FFCZ_HIP(hipMemcpyAsync(xhost_, xglob_, 2 * sizeof(ExtractGlobals), hipMemcpyDeviceToHost, s_));
     xpending_ = true;
     extracted_ = true;
 }
@@ -541,8 +539,8 @@ void Solver<T>::certify_both(CertResult* out) {
         b.work_real = cert_real2_;
     }
     ffcz::certify_pair<T>(*f64_, a, b, s_);
-    certify_copy_stats(false, s_);
-    certify_copy_stats(true, s_);
+    // both results in one copy (cstats_ and chost_ hold unquantised, quantised)
+    FFCZ_HIP(hipMemcpyAsync(chost_, cstats_, 2 * sizeof(CertStats), hipMemcpyDeviceToHost, s_));
     FFCZ_HIP(hipStreamSynchronize(s_));
     out[0] = certify_read(false);
     out[1] = certify_read(true);
