@@ -458,7 +458,12 @@ void Solver<T>::extract_segments(std::vector<ExtractSeg<T>>& out, hipStream_t s)
     materialize_ds(s);
     QuantDesc qf = freq_quant(), qs = spatial_quant();
     qf.dminmax = qs.dminmax = nullptr;
-    out.push_back(ExtractSeg<T>{df_, 2 * f_.F, qf, (unsigned int*)fpack_, fcomp_, fquant_, &xglob_[0], 0u});
+    // an untouched df (+0 only, as in extract()) goes in with no tiles: the
+    // batch's finalisation then reports count 0 and the range [0, 0], and
+    // its flag pack is cleared here on the batch stream
+    const bool dfz = !df_touched_;
+    if (dfz) extract_zeros_async(fpack_, 2 * f_.F, &xglob_[0], s);
+    out.push_back(ExtractSeg<T>{df_, dfz ? 0 : 2 * f_.F, qf, (unsigned int*)fpack_, fcomp_, fquant_, &xglob_[0], 0u});
     out.push_back(ExtractSeg<T>{ds_, f_.N, qs, (unsigned int*)spack_, scomp_, squant_, &xglob_[1], 0u});
     extracted_ = false;  // until the results are taken over
 }

@@ -236,3 +236,29 @@ def test_batched_extraction_reports_overflow_per_field(gpu):
         assert b.fields[0].certify(True).ok and b.fields[2].certify(True).ok
         with pytest.raises(FFCzError):
             b.fields[1].certify(True)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_batched_extraction_of_untouched_frequency_edits(gpu, dtype):
+    """Fields whose solve fired no frequency clamp keep df at zero, and the
+    batched extraction gives their df no tiles: flags, pack, compact values,
+    symbols and range equal the oracle's extraction of the downloaded edits,
+    for those fields and for the others of the same batch."""
+    shapes = SHAPES
+    cases = _cases(shapes, dtype, seed0=80)
+    with ProjectionBatch(shapes, dtype=dtype) as b:
+        for k, (f, (x, e0, eps, delta)) in enumerate(zip(b.fields, cases)):
+            # every other field with a bound no frequency bin reaches
+            f.initialize_all_abs(x, eps, delta * (1e6 if k % 2 else 1.0))
+        b.solve(MODE_ABS, [c[1] for c in cases], 60, 1e-7)
+        b.extract_all()
+        untouched = 0
+        for f in b.fields:
+            df, ds = f.get_freq_edits_full(), f.get_spatial_edits_full()
+            untouched += not np.any(df != 0)
+            got = _extraction(f)
+            rf, rs = O.extract_one(df), O.extract_one(ds)
+            for g, r in zip(got, (rf.pack, rs.pack, rf.compact, rs.compact, rf.quant, rs.quant, rf.vmin, rf.vmax,
+                                  rs.vmin, rs.vmax)):
+                assert np.array_equal(g, r)
+        assert 0 < untouched < len(shapes)
