@@ -76,10 +76,11 @@ def run_direct(shape, dtype, force_generic, mode="direct"):
 
 @pytest.mark.parametrize("mode", ["direct", "ptw"])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-@pytest.mark.parametrize("shape", [(32, 24, 40), (60, 18, 20)])
+@pytest.mark.parametrize("shape", [(32, 24, 40), (60, 18, 20), (600, 4, 12)])
 def test_direct_clamp_pass_matches_generic(gpu, restore_tuning, shape, dtype, mode):
     # per-bin and ptw bounds: the specialised x pass clamps in a pass of its
-    # own over the tile; the generic kernel clamps in its last stage
+    # own over the tile (the 600-point axis: one FP64 tile per CU); the
+    # generic kernel clamps in its last stage
     a = run_direct(shape, dtype, 0, mode)
     b = run_direct(shape, dtype, 1, mode)
     assert a[0] == b[0] == 10
