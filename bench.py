@@ -7,7 +7,8 @@ resident in HBM: reset the edit accumulators, the projection loop to
 convergence (iterating against bounds shrunk by the certification margins),
 extraction with the step quantiser (eps_s/8, delta/8), then the FP64
 certification of the unquantised and of the dequantised edits (one paired
-sequence that reads x'-x and the edits once).  ``value`` is
+sequence: both forward z transforms, in one launch or two as tuned, one y
+launch over both spectra, the two x checks).  ``value`` is
 certified GB/s = steps * N * sizeof(T) / elapsed (0 if any certification fails).
 
 The iteration runs on each FFT variant in turn (FP32 fields): the plain FP32
@@ -78,12 +79,14 @@ def parse():
     return p.parse_args()
 
 
-def kernel_bytes(shape, s, fresh_frac=0.0):
+def kernel_bytes(shape, s, fresh_frac=0.0, bound_bytes=0):
     """Compulsory HBM bytes per launch of the five iteration kernels (the
     accumulators are read/written only where a clamp fires).  The z forward
     pass of the second iteration applies the first spatial clamp since the
     accumulators were reset and stores the spatial edits whole (N*s bytes
-    more); fresh_frac is the share of the timed z forward launches that did."""
+    more); fresh_frac is the share of the timed z forward launches that did.
+    bound_bytes is what the fused x pass reads per bin for its bound (s for
+    per-bin delta_k, 0 for one bound for all bins)."""
     nx, ny, nz = shape
     N = nx * ny * nz
     F = nx * ny * (nz // 2 + 1)
@@ -91,7 +94,7 @@ def kernel_bytes(shape, s, fresh_frac=0.0):
     return {
         "z_fwd": N * s + F * cs + int(round(fresh_frac * N * s)),
         "y_fwd": 2 * F * cs,
-        "x_fused": 2 * F * cs,
+        "x_fused": 2 * F * cs + F * bound_bytes,
         "y_inv": 2 * F * cs,
         "z_inv": F * cs + N * s,
     }
@@ -565,7 +568,7 @@ def main():
     best = max(results, key=lambda r: (r["value"], r["variant"] == "valu-fp32"))
     ms, t_iter, dom = best["ms"], best["t_iter"], best["dom"]
     dname = names[dom]
-    kb = kernel_bytes(shape, s_in, best["fresh_frac"])
+    kb = kernel_bytes(shape, s_in, best["fresh_frac"], s_in if direct else 0)
     achieved = kb[dname] / (ms[dom] * 1e-3) / 1e9
     traffic = read_traffic(dname, a.config)
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
