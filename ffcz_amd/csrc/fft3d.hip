@@ -1145,8 +1145,8 @@ void certify_pair(const Fft3d<double>& f, const CertArgs<T>& a, const CertArgs<T
             launch_strided(f, y, S_PLAIN, false, s);
         }
     }
-    // the second spectrum first: the y launch wrote it last, so part of it is
-    // still in cache
+    // the checks are independent (each field's own statistics); the second
+    // spectrum goes first, being the last one the y launch wrote
     cert_xcheck(f, b, s);
     cert_xcheck(f, a, s);
 }
