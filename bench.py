@@ -115,11 +115,33 @@ def read_traffic(kernel: str, workload: str):
     return None
 
 
+def host_cores() -> int:
+    """The host cores this process may run on (its CPU affinity set)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        return os.cpu_count() or 1
+
+
+def median_of(fn, runs=3, warmup=1):
+    """Median wall time of ``runs`` calls of fn after ``warmup`` untimed ones."""
+    for _ in range(warmup):
+        fn()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
 def cpu_baseline(wl, gpu_iters, sample_iters, cores):
-    """The oracle restatement of the reference loop on the host cores
-    (scipy pocketfft with ``workers=cores``), on a bounded sample: a few
-    projection iterations and one FP64 certification of the same field,
-    scaled to the GPU run's iteration count and two certifications."""
+    """The oracle restatement of the reference loop on all the host cores
+    this process may use (scipy pocketfft with ``workers=cores``), on a
+    bounded sample: ``sample_iters`` projection iterations and one FP64
+    certification of the same field, each timed as the median of 3 runs after
+    a warm-up, then scaled to the GPU run's iteration count and two
+    certifications."""
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from oracle import ffcz_oracle as O
 
@@ -127,15 +149,18 @@ def cpu_baseline(wl, gpu_iters, sample_iters, cores):
     e0 = wl.e0.cpu().numpy()
     shape = x.shape
     dk = None if wl.delta_k is None else wl.delta_k.cpu().numpy()
-    kw = (dict(mode="abs", delta=wl.delta) if dk is None else
-          dict(mode="direct", idx=np.arange(dk.size), delta_k=dk))
-    t0 = time.perf_counter()
-    r = O.solve(x, e0, wl.eps, max_iter=sample_iters, tol=-1.0, dtype=x.dtype, workers=cores, **kw)
-    t_iter = (time.perf_counter() - t0) / sample_iters
-    t0 = time.perf_counter()
-    d = O.reconstruct_edit(shape, e0, r.df, r.ds, workers=cores)
-    O.certify(d, wl.eps, wl.delta, delta_dense=dk, workers=cores)
-    t_cert = time.perf_counter() - t0
+    kw = (dict(mode="abs", delta=wl.delta) if dk is None else dict(mode="direct", idx=None, delta_k=dk))
+    res = {}
+
+    def loop():
+        res["r"] = O.solve(x, e0, wl.eps, max_iter=sample_iters, tol=-1.0, dtype=x.dtype, workers=cores, **kw)
+
+    def cert():
+        d = O.reconstruct_edit(shape, e0, res["r"].df, res["r"].ds, workers=cores)
+        O.certify(d, wl.eps, wl.delta, delta_dense=dk, workers=cores)
+
+    t_iter = median_of(loop) / sample_iters
+    t_cert = median_of(cert)
     T = gpu_iters * t_iter + 2 * t_cert
     nbytes = x.size * x.dtype.itemsize
     return {
@@ -143,9 +168,10 @@ def cpu_baseline(wl, gpu_iters, sample_iters, cores):
         "unit": "GB/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"{sample_iters} {x.dtype} projection iteration(s) + 1 FP64 certification of the full "
-                  f"{'x'.join(map(str, shape))} field (numpy/scipy pocketfft, workers={cores}); "
-                  f"t_iter={t_iter:.3f}s t_cert={t_cert:.3f}s, scaled to {gpu_iters} iteration(s) + 2 certifications",
+        "sample": f"{sample_iters} {x.dtype} projection iteration(s) and 1 FP64 certification of the full "
+                  f"{'x'.join(map(str, shape))} field (numpy/scipy pocketfft, workers={cores}), each the median "
+                  f"of 3 runs after a warm-up; t_iter={t_iter:.3f}s t_cert={t_cert:.3f}s, scaled to "
+                  f"{gpu_iters} iteration(s) + 2 certifications",
     }
 
 
@@ -350,7 +376,7 @@ def bench_batch(a, torch, dev):
             "bytes_per_launch": B_iter, "ms_per_launch": round(t_iter * 1e3, 5)}
     cpu = None
     if not a.no_cpu_baseline:
-        cpu = cpu_baseline_batch(wls[:4], its[:4], min(16, os.cpu_count() or 1), N_tot)
+        cpu = cpu_baseline_batch(wls[:4], its[:4], host_cores(), N_tot)
     out = {
         "metric": "certified GB/s (both bounds met) on 1 GPU, MFMA split-prec vs plain FP32 FFT",
         "value": round(value, 3),
@@ -380,8 +406,9 @@ def bench_batch(a, torch, dev):
 
 
 def cpu_baseline_batch(wls, iters, cores, n_total):
-    """Oracle loop + certification on the first few fields of the batch, scaled
-    by element count to the whole batch."""
+    """Oracle loop + two certifications on the first few fields of the batch
+    (each field's work the median of 3 runs after a warm-up), scaled by
+    element count to the whole batch."""
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from oracle import ffcz_oracle as O
 
@@ -390,19 +417,21 @@ def cpu_baseline_batch(wls, iters, cores, n_total):
     for w, it in zip(wls, iters):
         x = w.x.cpu().numpy()
         e0 = w.e0.cpu().numpy()
-        t0 = time.perf_counter()
-        r = O.solve(x, e0, w.eps, mode="abs", delta=w.delta, max_iter=max(it, 1), tol=-1.0, dtype=np.float32,
-                    workers=cores)
-        d = O.reconstruct_edit(x.shape, e0, r.df, r.ds, workers=cores)
-        O.certify(d, w.eps, w.delta, workers=cores)
-        O.certify(d, w.eps, w.delta, workers=cores)
-        t += time.perf_counter() - t0
+
+        def work():
+            r = O.solve(x, e0, w.eps, mode="abs", delta=w.delta, max_iter=max(it, 1), tol=-1.0, dtype=np.float32,
+                        workers=cores)
+            d = O.reconstruct_edit(x.shape, e0, r.df, r.ds, workers=cores)
+            O.certify(d, w.eps, w.delta, workers=cores)
+            O.certify(d, w.eps, w.delta, workers=cores)
+
+        t += median_of(work)
         n += x.size
     T = t * n_total / n
     return {"value": n_total * 4 / T / 1e9, "unit": "GB/s", "cores": cores, "kind": "port",
             "sample": f"the first {len(wls)} fields ({n} elements) at their GPU iteration counts + 2 FP64 "
-                      f"certifications each (numpy/scipy pocketfft, workers={cores}), scaled by element count "
-                      f"to all {n_total} elements"}
+                      f"certifications each (numpy/scipy pocketfft, workers={cores}; each field the median of 3 "
+                      f"runs after a warm-up), scaled by element count to all {n_total} elements"}
 
 
 def main():
@@ -570,10 +599,20 @@ def main():
     dname = names[dom]
     kb = kernel_bytes(shape, s_in, best["fresh_frac"], s_in if direct else 0)
     achieved = kb[dname] / (ms[dom] * 1e-3) / 1e9
-    traffic = read_traffic(dname, a.config)
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic, "kernel": dname,
-            "fft": best["variant"], "bytes_per_launch": kb[dname], "ms_per_launch": round(ms[dom], 5)}
+    dominant = {"kernel": dname, "fft": best["variant"], "achieved": round(achieved, 1),
+                "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": read_traffic(dname, a.config),
+                "bytes_per_launch": kb[dname], "ms_per_launch": round(ms[dom], 5)}
+    # the roofline of the metric's unit of work: one projection iteration
+    # (the five passes and the violation reduction), B_iter algorithmic bytes
+    # over its measured time; traffic = the passes' counter bytes summed
+    pass_traffic = [read_traffic(n, a.config) for n in names]
+    it_achieved = B_iter / t_iter / 1e9
+    roof = {"bound": "hbm", "achieved": round(it_achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": round(it_achieved / HBM_PEAK_GBPS, 4),
+            "traffic": (sum(pass_traffic) if all(v is not None for v in pass_traffic) else None),
+            "kernel": "one projection iteration (z fwd, y fwd, x fused, y inv, z inv, violation reduction)",
+            "fft": best["variant"], "bytes_per_launch": B_iter, "ms_per_launch": round(t_iter * 1e3, 5),
+            "dominant_pass": dominant}
 
     def iteration_summary(r):
         # passes and the violation reduction, HIP events over the timed steps
@@ -584,8 +623,7 @@ def main():
 
     cpu = None
     if not a.no_cpu_baseline:
-        cores = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(wl, max(best["iterations"], 1), a.cpu_sample_iters, cores)
+        cpu = cpu_baseline(wl, max(best["iterations"], 1), a.cpu_sample_iters, host_cores())
 
     if direct:
         bounds = (f"eps={wl.eps:.3e}, per-bin delta_k in "

@@ -91,14 +91,31 @@ def project_frequency_all_abs(E: np.ndarray, delta, df: np.ndarray) -> None:
     Ef.imag = im
 
 
-def project_frequency_direct(E: np.ndarray, idx: np.ndarray, delta_k: np.ndarray, df: np.ndarray) -> None:
+def dense_bounds(F: int, idx, delta_k, dtype) -> np.ndarray:
+    """Per-bin bound array of the direct mode: ``delta_k[j]`` on bin
+    ``idx[j]``, +inf on bins no constraint lists, the smallest bound where an
+    index repeats.  ``idx=None`` means delta_k already lists every bin in
+    order (bin j has bound delta_k[j])."""
+    if idx is None:
+        dense = np.asarray(delta_k, dtype=dtype).reshape(-1)
+        if dense.size != F:
+            raise ValueError("a dense bound array needs one bound per bin")
+        return dense
+    dense = np.full(F, np.inf, dtype=dtype)
+    np.minimum.at(dense, np.asarray(idx, dtype=np.int64), np.asarray(delta_k, dtype=dtype))
+    return dense
+
+
+def project_frequency_direct(E: np.ndarray, idx: np.ndarray, delta_k: np.ndarray, df: np.ndarray,
+                             dense: np.ndarray | None = None) -> None:
     """projection_algorithm.cu:93-145: per-constraint bound ``delta_k`` on the
     bins listed in ``idx``; other bins are unconstrained.  Duplicate indices are
-    resolved to the smallest bound (the fixed point of repeated application)."""
+    resolved to the smallest bound (the fixed point of repeated application).
+    ``dense`` (optional) is the per-bin bound array of ``dense_bounds``."""
     T = df.dtype.type
     F = E.size
-    dense = np.full(F, np.inf, dtype=df.dtype)
-    np.minimum.at(dense, np.asarray(idx, dtype=np.int64), np.asarray(delta_k, dtype=df.dtype))
+    if dense is None:
+        dense = dense_bounds(F, idx, delta_k, df.dtype)
     Ef = E.reshape(-1)
     re = Ef.real.copy()
     im = Ef.imag.copy()
@@ -210,6 +227,7 @@ def solve(x: np.ndarray, e0: np.ndarray, eps, *, mode: str = "abs", delta=None,
     ds = np.zeros(N, dtype=dtype) if ds is None else ds
     norm = T(1) / T(N)  # impl:242
     orig = rfft3(x.astype(dtype), workers) if mode == "ptw" else None
+    dense = dense_bounds(F, idx, delta_k, dtype) if mode == "direct" else None
     it = 0
     v = T(0)
     hist = []
@@ -218,7 +236,7 @@ def solve(x: np.ndarray, e0: np.ndarray, eps, *, mode: str = "abs", delta=None,
         if mode == "abs":
             project_frequency_all_abs(E, delta, df)
         elif mode == "direct":
-            project_frequency_direct(E, idx, delta_k, df)
+            project_frequency_direct(E, idx, delta_k, df, dense=dense)
         elif mode == "ptw":
             project_frequency_all_ptw(E, orig, delta_ptw, df)
         else:
