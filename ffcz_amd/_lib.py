@@ -143,6 +143,7 @@ SIGNATURES = {
     "ffcz_k_convergence": [_i, _vp, _d, _sz, C.POINTER(_d)],
     "ffcz_k_quantize_dequantize": [_i, _vp, _vp, _d, _d, _sz, _d, _vp],
     "ffcz_k_extract": [_i, _vp, _sz, _i, _d, _vp, _vp, _vp, _szp, C.POINTER(_d), C.POINTER(_d), _vp],
+    "ffcz_k_extract_noalloc": [_i, _vp, _sz, _i, _d, _vp, _vp, _vp, _szp, C.POINTER(_d), C.POINTER(_d), _vp],
 }
 _RESTYPES = {"ffcz_last_error": C.c_char_p, "ffcz_coded_free": None, "ffcz_blob_free": None}
 

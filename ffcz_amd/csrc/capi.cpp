@@ -830,4 +830,27 @@ int ffcz_k_extract(int dtype, const void* v, size_t n, int kind, double step, ui
     });
 }
 
+int ffcz_k_extract_noalloc(int dtype, const void* v, size_t n, int kind, double step, uint8_t* pack, void* compact,
+                           uint16_t* quant, size_t* count, double* vmin, double* vmax, void* stream) {
+    return guard([&] {
+        FFCZ_REQUIRE(n <= EXTRACT_NOALLOC_MAX, ERR_CAPACITY,
+                     "array larger than the allocation-free extraction holds (2^30 elements)");
+        FFCZ_REQUIRE(kind == Q_REF16 || kind == Q_STEP, ERR_INVALID, "unknown quantizer kind");
+        FFCZ_REQUIRE(dtype == FFCZ_F32 || dtype == FFCZ_F64, ERR_INVALID, "unknown dtype");
+        QuantDesc q{};
+        q.kind = kind;
+        q.step = step;
+        q.inv_step = step > 0 ? 1.0 / step : 0.0;
+        q.threshold = 1e-7;
+        const ExtractResult r =
+            dtype == FFCZ_F32
+                ? extract_edits_noalloc<float>((const float*)v, n, q, pack, (float*)compact, quant, as_stream(stream))
+                : extract_edits_noalloc<double>((const double*)v, n, q, pack, (double*)compact, quant,
+                                                as_stream(stream));
+        if (count) *count = r.count;
+        if (vmin) *vmin = r.vmin;
+        if (vmax) *vmax = r.vmax;
+    });
+}
+
 }  // extern "C"

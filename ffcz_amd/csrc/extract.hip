@@ -11,6 +11,8 @@
 #include <algorithm>
 #include <cfloat>
 #include <cstring>
+#include <mutex>
+#include <string>
 
 #include "extract.hpp"
 #include "quant.hpp"
@@ -464,6 +466,50 @@ ExtractResult extract_edits(const T* v, size_t n, QuantDesc q, uint8_t* pack, T*
     return res;
 }
 
+// Scratch of the allocation-free extraction: results and ticket (256 bytes),
+// then the status word and tile minimum and maximum of every tile of an array
+// of EXTRACT_NOALLOC_MAX elements (extract_scratch_bytes).  In host code this
+// name is not a device address: noalloc_scratch_ptr() gives the current
+// device's copy.
+constexpr size_t NOALLOC_SCRATCH_BYTES =
+    256 + (EXTRACT_NOALLOC_MAX / X_TILE) * (sizeof(unsigned long long) + 2 * sizeof(double));
+static __device__ __align__(256) unsigned char noalloc_scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::once_flag g_noalloc_once[NOALLOC_MAX_DEVICES];
+void* g_noalloc_ptr[NOALLOC_MAX_DEVICES];
+hipError_t g_noalloc_err[NOALLOC_MAX_DEVICES];
+
+// The current device's address of noalloc_scratch, looked up once per device
+// (with that device current, as the calling thread has it).
+void* noalloc_scratch_ptr() {
+    int dev = 0;
+    FFCZ_HIP(hipGetDevice(&dev));
+    FFCZ_REQUIRE(dev >= 0 && dev < NOALLOC_MAX_DEVICES, ERR_INVALID, "device index beyond the scratch table");
+    std::call_once(g_noalloc_once[dev], [dev] {
+        void* p = nullptr;
+        g_noalloc_err[dev] = hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch));
+        g_noalloc_ptr[dev] = g_noalloc_err[dev] == hipSuccess ? p : nullptr;
+    });
+    FFCZ_REQUIRE(g_noalloc_err[dev] == hipSuccess && g_noalloc_ptr[dev], ERR_HIP,
+                 std::string("hipGetSymbolAddress of the extraction scratch failed: ") +
+                     hipGetErrorName(g_noalloc_err[dev]));
+    return g_noalloc_ptr[dev];
+}
+}  // namespace
+
+template <typename T>
+ExtractResult extract_edits_noalloc(const T* v, size_t n, QuantDesc q, uint8_t* pack, T* compact, uint16_t* quant,
+                                    hipStream_t s) {
+    FFCZ_REQUIRE(n <= EXTRACT_NOALLOC_MAX, ERR_CAPACITY,
+                 "array larger than the allocation-free extraction holds (2^30 elements)");
+    static_assert(NOALLOC_SCRATCH_BYTES >= 256 + (EXTRACT_NOALLOC_MAX / X_TILE) * 24, "scratch size");
+    if (n == 0) return ExtractResult{};
+    // extract_launch clears the ticket, status words and results first
+    return extract_edits(v, n, q, pack, compact, quant, noalloc_scratch_ptr(), s);
+}
+
 __global__ void k_clear_globals(ExtractGlobals* g) {
     if (threadIdx.x == 0) *g = ExtractGlobals{};
 }
@@ -585,6 +631,10 @@ void dequantize_dense(const T* v, size_t n, QuantDesc q, T* out, hipStream_t s) 
 
 template ExtractResult extract_edits<float>(const float*, size_t, QuantDesc, uint8_t*, float*, uint16_t*, void*, hipStream_t);
 template ExtractResult extract_edits<double>(const double*, size_t, QuantDesc, uint8_t*, double*, uint16_t*, void*, hipStream_t);
+template ExtractResult extract_edits_noalloc<float>(const float*, size_t, QuantDesc, uint8_t*, float*, uint16_t*,
+                                                   hipStream_t);
+template ExtractResult extract_edits_noalloc<double>(const double*, size_t, QuantDesc, uint8_t*, double*, uint16_t*,
+                                                    hipStream_t);
 template void extract_edits_async<float>(const float*, size_t, QuantDesc, uint8_t*, float*, uint16_t*, void*,
                                          ExtractGlobals*, ExtractGlobals*, hipStream_t);
 template void extract_edits_async<double>(const double*, size_t, QuantDesc, uint8_t*, double*, uint16_t*, void*,

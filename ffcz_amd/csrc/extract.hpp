@@ -81,6 +81,16 @@ struct ExtractBatchWork {
 template <typename T>
 const ExtractGlobals* extract_edits_batch(std::vector<ExtractSeg<T>>& segs, ExtractBatchWork& w, hipStream_t s);
 
+// extract_edits without any device allocation: the look-back status words,
+// tile ranges, ticket and results live in a fixed device array of this
+// library (one per GPU, taken at the first call on each device), which holds
+// arrays of up to EXTRACT_NOALLOC_MAX elements (ERR_CAPACITY beyond).  Every
+// call on a GPU shares that GPU's array: calls that overlap on one GPU race.
+constexpr size_t EXTRACT_NOALLOC_MAX = size_t(1) << 30;
+template <typename T>
+ExtractResult extract_edits_noalloc(const T* v, size_t n, QuantDesc q, uint8_t* pack, T* compact, uint16_t* quant,
+                                    hipStream_t s);
+
 // Dense dequantised copy (projection_algorithm.cu:615-644 for Q_REF16).
 template <typename T>
 void dequantize_dense(const T* v, size_t n, QuantDesc q, T* out, hipStream_t s);

@@ -30,7 +30,8 @@ enum Status : int {
     ERR_RANGE = 4,        // value does not fit (Huffman code > 32 bits, quantised symbol > 16 bits)
     ERR_ALLOC = 5,        // device allocation failed
     ERR_ZSTD = 6,         // libzstd failure
-    ERR_NOT_CONVERGED = 7 // informational: loop hit max_iterations
+    ERR_NOT_CONVERGED = 7, // informational: loop hit max_iterations
+    ERR_CAPACITY = 8      // input larger than a fixed-size buffer holds
 };
 
 #define FFCZ_HIP(call)                                                                          \

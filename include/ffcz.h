@@ -308,6 +308,20 @@ int ffcz_k_quantize_dequantize(int dtype, const void* v, void* out, double vmin,
 int ffcz_k_extract(int dtype, const void* v, size_t n, int kind, double step, uint8_t* pack, void* compact,
                    uint16_t* quant, size_t* count, double* vmin, double* vmax, void* stream);
 
+/* ffcz_k_extract without device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The same kernels and results as ffcz_k_extract; the look-back
+ * status words, tile ranges, ticket and results live in a fixed device array
+ * of the library (about 3 MB per GPU, taken at the first call on each device
+ * and held until the process ends), so n is at most FFCZ_EXTRACT_NOALLOC_MAX
+ * (FFCZ_ERR_CAPACITY beyond).  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (other threads, other streams) race.  The
+ * caller passes every input and output; the call waits for its stream. */
+#define FFCZ_EXTRACT_NOALLOC_MAX ((size_t)1 << 30)
+int ffcz_k_extract_noalloc(int dtype, const void* v, size_t n, int kind, double step, uint8_t* pack, void* compact,
+                           uint16_t* quant, size_t* count, double* vmin, double* vmax, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
