@@ -38,8 +38,13 @@ void Batch<T>::solve(int mode, const T* const* e0, int max_iter, T tol, bool use
     // the fields' own streams may still run their initialisation
     for (auto* f : f_) FFCZ_HIP(hipStreamSynchronize(f->stream()));
     for (size_t i = 0; i < f_.size(); ++i) f_[i]->prepare_solve(mode, e0[i], max_iter, tol, s_, false);
+    // the batched fields run the fused iteration form (its batched launches
+    // use tile variant 0); the others the five-pass form
+    for (auto* f : f_)
+        if (batch_fusable(f->fft())) f->set_fused_run(0);
     build(use_graph && max_iter > 0);
     batch_reset_states(plan_, max_iter, (double)tol, s_);
+    if (max_iter > 0) batch_prologue(plan_, s_);
     int launched = 0, chunk = 2;
     while (launched < max_iter) {
         const int k = std::min(chunk, max_iter - launched);

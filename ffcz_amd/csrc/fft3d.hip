@@ -751,6 +751,156 @@ void iteration(const Fft3d<R>& f, const IterArgs<R>& ia, hipStream_t s, EventLog
 }
 
 template <typename R>
+bool fused_iteration_available(const Fft3d<R>& f) {
+    return fast_fft_enabled() && tuning().slab_mb.load() <= 0 && fast_z_rows(f.Lz, f.half, f.zi_var) > 0;
+}
+template <typename R>
+size_t zz_blocks(const Fft3d<R>& f) {
+    const int zr = fast_z_rows(f.Lz, f.half, f.zi_var);
+    return zr ? ceil_div(f.rows, (size_t)zr) : 0;
+}
+template <typename R>
+size_t zz_log_entries(const Fft3d<R>& f) {
+    // a segment of P rows' elements per workgroup, for either tile variant
+    size_t n = 0;
+    for (int v = 0; v < 2; ++v) {
+        const int zr = fast_z_rows(f.Lz, f.half, v);
+        if (zr) n = std::max(n, ceil_div(f.rows, (size_t)zr) * (size_t)zr * f.nz);
+    }
+    return n;
+}
+template <typename R>
+size_t zz_blocks_max(const Fft3d<R>& f) {
+    size_t n = 0;
+    for (int v = 0; v < 2; ++v) {
+        const int zr = fast_z_rows(f.Lz, f.half, v);
+        if (zr) n = std::max(n, ceil_div(f.rows, (size_t)zr));
+    }
+    return n;
+}
+
+template <typename R>
+void iteration_fused(const Fft3d<R>& f, const IterArgs<R>& ia, bool first, hipStream_t s, EventLog* log) {
+    auto mark = [&](int kind) {
+        if (log) log->mark(kind, s);
+    };
+    mark(-1);
+    auto z = zargs<R, R>(f);
+    z.e = ia.e;
+    z.e_init = ia.e_init;
+    z.e_copy = ia.e_copy;
+    z.ds_fresh = ia.ds_fresh;
+    z.ds = ia.ds;
+    z.eps = ia.eps;
+    z.st = ia.st;
+    z.spec = ia.spec;
+    z.scale = ia.norm;
+    z.vpart = ia.vpart;
+    z.row_off = 0;
+    z.log_off = ia.log_off;
+    z.log_v = ia.log_v;
+    z.log_cnt = ia.log_cnt;
+    if (first) {
+        // the first iteration's forward z transform of the initial error
+        launch_zfwd<R, R, Z_ITER>(f, z, s);
+        mark(0);
+    }
+    auto y = sargs<R, R>(f, ia.spec, false);
+    y.st = ia.st;
+    y.sign = -1;
+    launch_strided(f, y, S_PLAIN, false, s);
+    mark(1);
+    auto x = sargs<R, R>(f, ia.spec, true);
+    x.st = ia.st;
+    x.fmode = ia.fmode;
+    x.delta = ia.delta;
+    x.delta_dense = ia.delta_dense;
+    x.dscale = ia.dscale;
+    x.df = ia.df;
+    x.orig = ia.orig;
+    x.delta_ptw = ia.delta_ptw;
+    launch_strided(f, x, x_clamp_mode(f, ia.fmode), true, s);
+    mark(2);
+    y.sign = +1;
+    launch_strided(f, y, S_PLAIN, false, s);
+    mark(3);
+    z.var = f.zi_var;
+    const bool launched = fast_zinv<R, R>(Z_FUSED, z, s);
+    FFCZ_REQUIRE(launched, ERR_INVALID, "no fused z kernel for this z length");
+    FFCZ_HIP(hipGetLastError());
+    mark(4);
+    hipLaunchKernelGGL((k_finalize_iteration<R>), dim3(1), dim3(1024), 0, s, (const R*)ia.vpart, zz_blocks(f), ia.st);
+    FFCZ_HIP(hipGetLastError());
+    mark(5);
+}
+
+// Adds the last z fused pass's deferred clamps to ds after a loop that
+// stopped at max_iter without converging (a converged loop drops them).
+// Counts and offsets are held to each workgroup's segment and to the field.
+template <typename R>
+__global__ void k_zz_apply(R* ds, R eps, const uint32_t* off, const R* v, const uint32_t* cnt, size_t nblk,
+                           size_t seg, size_t n_elem, const DevState* st) {
+    if (st->converged || st->iter == 0) return;
+    for (size_t b = blockIdx.x; b < nblk; b += gridDim.x) {
+        const uint32_t n = (uint32_t)min((size_t)cnt[b], seg);
+        const size_t l0 = b * seg;
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+            const size_t j = l0 + off[l0 + i];
+            if (off[l0 + i] >= seg || j >= n_elem) continue;
+            R x = v[l0 + i];
+            const R ed = clamp_edit(x, eps);
+            ds[j] = ds[j] + ed;
+        }
+    }
+}
+template <typename R>
+__global__ void k_zz_patch_e(R* e, const uint32_t* off, const R* v, const uint32_t* cnt, size_t nblk, size_t seg,
+                             size_t n_elem, const DevState* st) {
+    if (!st->converged) return;
+    for (size_t b = blockIdx.x; b < nblk; b += gridDim.x) {
+        const uint32_t n = (uint32_t)min((size_t)cnt[b], seg);
+        const size_t l0 = b * seg;
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+            const size_t j = l0 + off[l0 + i];
+            if (off[l0 + i] < seg && j < n_elem) e[j] = v[l0 + i];
+        }
+    }
+}
+template <typename R>
+static size_t zz_seg(const Fft3d<R>& f, int var) {
+    return (size_t)fast_z_rows(f.Lz, f.half, var) * f.nz;
+}
+template <typename R>
+static size_t zz_nblk(const Fft3d<R>& f, int var) {
+    return ceil_div(f.rows, (size_t)fast_z_rows(f.Lz, f.half, var));
+}
+template <typename R>
+void zz_apply(const Fft3d<R>& f, const IterArgs<R>& a, int var, hipStream_t s) {
+    const size_t nb = zz_nblk(f, var);
+    const unsigned g = (unsigned)std::min<size_t>(nb, 4096);
+    hipLaunchKernelGGL((k_zz_apply<R>), dim3(g), dim3(256), 0, s, a.ds, a.eps, (const uint32_t*)a.log_off,
+                       (const R*)a.log_v, (const uint32_t*)a.log_cnt, nb, zz_seg(f, var), f.N,
+                       (const DevState*)a.st);
+    FFCZ_HIP(hipGetLastError());
+}
+template <typename R>
+void zz_recover_e(const Fft3d<R>& f, const IterArgs<R>& a, int var, hipStream_t s) {
+    // the rows hold the z transform of the clamped iterate: C2R along z
+    // returns it times nz
+    auto z = zargs<R, R>(f);
+    z.spec = a.spec;
+    z.out = a.e;
+    z.scale = R(1) / static_cast<R>(f.nz);
+    launch_zinv<R, R, Z_PLAIN>(f, z, s);
+    const size_t nb = zz_nblk(f, var);
+    const unsigned g = (unsigned)std::min<size_t>(nb, 4096);
+    hipLaunchKernelGGL((k_zz_patch_e<R>), dim3(g), dim3(256), 0, s, a.e, (const uint32_t*)a.log_off,
+                       (const R*)a.log_v, (const uint32_t*)a.log_cnt, nb, zz_seg(f, var), f.N,
+                       (const DevState*)a.st);
+    FFCZ_HIP(hipGetLastError());
+}
+
+template <typename R>
 void iteration_kernel(const Fft3d<R>& f, const IterArgs<R>& ia, hipStream_t s, int kind) {
     auto z = zargs<R, R>(f);
     z.e = ia.e;
@@ -949,6 +1099,9 @@ void batch_plan_build(BatchPlan<R>& bp, const std::vector<const Fft3d<R>*>& fiel
                 z.scale = ia.norm;
                 z.vpart = ia.vpart;
                 z.row_off = 0;
+                z.log_off = ia.log_off;
+                z.log_v = ia.log_v;
+                z.log_cnt = ia.log_cnt;
                 tab.push_back(z);
                 start.push_back((uint32_t)tiles);
                 tiles += ceil_div(f.rows, (size_t)fast_z_rows(f.Lz, f.half));
@@ -1005,6 +1158,9 @@ void batch_plan_build(BatchPlan<R>& bp, const std::vector<const Fft3d<R>*>& fiel
             p.start_off = put(start.data(), start.size() * sizeof(uint32_t));
             pend.push_back(p);
         };
+        // the fused iteration form: the forward z pass of the initial error
+        // once (the prologue), then [y fwd, x fused, y inv, z fused] per
+        // iteration
         for (auto& g : zgroups) add_z(0, g);
         for (auto& g : ygroups) add_s(1, false, g);
         for (auto& g : xgroups) add_s(2, true, g);
@@ -1033,7 +1189,7 @@ void batch_plan_build(BatchPlan<R>& bp, const std::vector<const Fft3d<R>*>& fiel
     for (auto& p : pend) {
         p.g.args = bp.dmem + p.args_off;
         p.g.start = reinterpret_cast<const uint32_t*>(bp.dmem + p.start_off);
-        bp.groups.push_back(p.g);
+        (p.g.pass == 0 ? bp.pro : bp.groups).push_back(p.g);
     }
     bp.nfin = (int)fin.size();
     bp.fin = fin.empty() ? nullptr : bp.dmem + fin_off;
@@ -1042,13 +1198,29 @@ void batch_plan_build(BatchPlan<R>& bp, const std::vector<const Fft3d<R>*>& fiel
 }
 
 template <typename R>
+bool batch_fusable(const Fft3d<R>& f) {
+    return batchable(f);
+}
+
+template <typename R>
+void batch_prologue(const BatchPlan<R>& bp, hipStream_t s) {
+    if constexpr (sizeof(R) == 4) {
+        for (const auto& g : bp.pro) {
+            BatchTab<ZArgs<R, R>> t{static_cast<const ZArgs<R, R>*>(g.args), g.start, g.n};
+            FFCZ_REQUIRE(fast_zfwd_batch(g.L, t, g.grid, s), ERR_STATE, "batched pass without a kernel for its length");
+            FFCZ_HIP(hipGetLastError());
+        }
+    }
+}
+
+template <typename R>
 void batch_iteration(const BatchPlan<R>& bp, hipStream_t s) {
     if constexpr (sizeof(R) == 4) {
         for (const auto& g : bp.groups) {
             bool ok;
-            if (g.pass == 0 || g.pass == 4) {
+            if (g.pass == 4) {
                 BatchTab<ZArgs<R, R>> t{static_cast<const ZArgs<R, R>*>(g.args), g.start, g.n};
-                ok = g.pass == 0 ? fast_zfwd_batch(g.L, t, g.grid, s) : fast_zinv_batch(g.L, t, g.grid, s);
+                ok = fast_zz_batch(g.L, t, g.grid, s);
             } else {
                 BatchTab<SArgs<R, R>> t{static_cast<const SArgs<R, R>*>(g.args), g.start, g.n};
                 ok = fast_strided_batch(g.smode, g.L, t, g.grid, s);
@@ -1363,6 +1535,20 @@ template void iteration_kernel<float>(const Fft3d<float>&, const IterArgs<float>
 template void iteration_kernel<double>(const Fft3d<double>&, const IterArgs<double>&, hipStream_t, int);
 template void iteration<float>(const Fft3d<float>&, const IterArgs<float>&, hipStream_t, EventLog*);
 template void iteration<double>(const Fft3d<double>&, const IterArgs<double>&, hipStream_t, EventLog*);
+template bool fused_iteration_available<float>(const Fft3d<float>&);
+template bool fused_iteration_available<double>(const Fft3d<double>&);
+template size_t zz_blocks<float>(const Fft3d<float>&);
+template size_t zz_blocks<double>(const Fft3d<double>&);
+template size_t zz_log_entries<float>(const Fft3d<float>&);
+template size_t zz_log_entries<double>(const Fft3d<double>&);
+template size_t zz_blocks_max<float>(const Fft3d<float>&);
+template size_t zz_blocks_max<double>(const Fft3d<double>&);
+template void iteration_fused<float>(const Fft3d<float>&, const IterArgs<float>&, bool, hipStream_t, EventLog*);
+template void iteration_fused<double>(const Fft3d<double>&, const IterArgs<double>&, bool, hipStream_t, EventLog*);
+template void zz_apply<float>(const Fft3d<float>&, const IterArgs<float>&, int, hipStream_t);
+template void zz_apply<double>(const Fft3d<double>&, const IterArgs<double>&, int, hipStream_t);
+template void zz_recover_e<float>(const Fft3d<float>&, const IterArgs<float>&, int, hipStream_t);
+template void zz_recover_e<double>(const Fft3d<double>&, const IterArgs<double>&, int, hipStream_t);
 template void epilogue_clamp<float>(const Fft3d<float>&, const IterArgs<float>&, hipStream_t);
 template void epilogue_clamp<double>(const Fft3d<double>&, const IterArgs<double>&, hipStream_t);
 template void certify<float>(const Fft3d<double>&, const CertArgs<float>&, hipStream_t);
@@ -1380,6 +1566,10 @@ template void batch_plan_build<double>(BatchPlan<double>&, const std::vector<con
                                        const std::vector<IterArgs<double>>&);
 template void batch_plan_free<float>(BatchPlan<float>&);
 template void batch_plan_free<double>(BatchPlan<double>&);
+template bool batch_fusable<float>(const Fft3d<float>&);
+template bool batch_fusable<double>(const Fft3d<double>&);
+template void batch_prologue<float>(const BatchPlan<float>&, hipStream_t);
+template void batch_prologue<double>(const BatchPlan<double>&, hipStream_t);
 template void batch_iteration<float>(const BatchPlan<float>&, hipStream_t);
 template void batch_iteration<double>(const BatchPlan<double>&, hipStream_t);
 template void batch_gather_states<float>(const BatchPlan<float>&, DevState*, hipStream_t);

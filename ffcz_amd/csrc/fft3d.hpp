@@ -118,6 +118,11 @@ struct IterArgs {
     R norm;               // T(1)/T(N)
     DevState* st;
     R* vpart;             // [rows] per-workgroup violation partials (zero-initialised)
+    // log of the fused z pass's deferred clamps (iteration_fused):
+    // zz_log_entries() entries of each kind, zz_blocks_max() counts
+    uint32_t* log_off;
+    R* log_v;
+    uint32_t* log_cnt;
 };
 
 // forward z (with the deferred spatial clamp of the previous iteration fused
@@ -151,6 +156,38 @@ void iteration(const Fft3d<R>& f, const IterArgs<R>& a, hipStream_t s, EventLog*
 // for timing the kernels one at a time.
 template <typename R>
 void iteration_kernel(const Fft3d<R>& f, const IterArgs<R>& a, hipStream_t s, int kind);
+// The fused iteration form: [y fwd, x fused, y inv, z fused, finalise], the
+// z fused pass running the inverse z transform of one iteration, its spatial
+// violation, the (speculative) spatial clamp and the forward z transform of
+// the next iteration in place on the half spectrum (k2_zz: clamps a
+// converged pass could make are deferred through a log); the iterate e is
+// not written.  The first iteration of a solve starts with the forward z
+// pass of the initial error.  Available where the z axis has a
+// length-specialised half-length kernel and no slab interleave is set.
+template <typename R>
+bool fused_iteration_available(const Fft3d<R>& f);
+// workgroups of the fused z pass (its tile variant; the larger of the two
+// variants' counts), and the log entries (of each kind) either variant needs
+template <typename R>
+size_t zz_blocks(const Fft3d<R>& f);
+template <typename R>
+size_t zz_blocks_max(const Fft3d<R>& f);
+template <typename R>
+size_t zz_log_entries(const Fft3d<R>& f);
+template <typename R>
+void iteration_fused(const Fft3d<R>& f, const IterArgs<R>& a, bool first, hipStream_t s, EventLog* log = nullptr);
+// After a fused loop: the last pass's deferred clamps (those with an excess
+// below tol, which a converged loop drops) go into ds when the loop stopped
+// at max_iter without converging; otherwise nothing is done.
+// var: the z tile variant the loop ran with (the log's layout)
+template <typename R>
+void zz_apply(const Fft3d<R>& f, const IterArgs<R>& a, int var, hipStream_t s);
+// The iterate after a fused loop: the inverse z transform of the rows the
+// last z fused pass wrote (times 1/nz: the clamped iterate) into a.e, and,
+// after a converged loop, the logged elements' unclamped values.
+template <typename R>
+void zz_recover_e(const Fft3d<R>& f, const IterArgs<R>& a, int var, hipStream_t s);
+
 // Final spatial clamp when the loop stopped at max_iter without convergence.
 template <typename R>
 void epilogue_clamp(const Fft3d<R>& f, const IterArgs<R>& a, hipStream_t s);
@@ -158,13 +195,16 @@ template <typename R>
 void reset_state(DevState* st, int max_iter, double tol, hipStream_t s);
 
 // ---- one iteration over a batch of fields ----------------------------------
-// Fields whose passes all have length-specialised kernels run each pass as
-// one launch per axis length over every such field (tile -> field by a
-// prefix table in device memory); their iterations are finalised by one
-// launch with a workgroup per field.  The other fields (FP64, MFMA variants,
-// lengths without a specialised kernel, slab interleave) run iteration()
-// one by one inside the same sequence.  A field whose state is done skips
-// its tiles, so a batch keeps iterating until every field has converged.
+// Fields whose passes all have length-specialised kernels (batch_fusable) run
+// the fused iteration form, each pass as one launch per axis length over
+// every such field (tile -> field by a prefix table in device memory): the
+// forward z pass of the initial error once (batch_prologue), then per
+// iteration [y fwd, x fused, y inv, z fused] and one finalising launch with a
+// workgroup per field.  The other fields (FP64, MFMA variants, lengths
+// without a specialised kernel, slab interleave) run the five-pass
+// iteration() one by one inside the same sequence.  A field whose state is
+// done skips its tiles, so a batch keeps iterating until every field has
+// converged.
 template <typename R>
 struct BatchPlan {
     struct Group {
@@ -176,7 +216,8 @@ struct BatchPlan {
         const uint32_t* start;
         int n;
     };
-    std::vector<Group> groups;
+    std::vector<Group> groups;   // the iteration's passes
+    std::vector<Group> pro;      // the prologue: forward z of the initial errors
     std::vector<std::pair<const Fft3d<R>*, IterArgs<R>>> fallback;
     const void* fin = nullptr;   // finalize table of the batched fields
     int nfin = 0;
@@ -193,6 +234,10 @@ void batch_plan_build(BatchPlan<R>& bp, const std::vector<const Fft3d<R>*>& fiel
                       const std::vector<IterArgs<R>>& args);
 template <typename R>
 void batch_plan_free(BatchPlan<R>& bp);
+template <typename R>
+bool batch_fusable(const Fft3d<R>& f);
+template <typename R>
+void batch_prologue(const BatchPlan<R>& bp, hipStream_t s);
 template <typename R>
 void batch_iteration(const BatchPlan<R>& bp, hipStream_t s);
 // copies every field's state into host memory `out` (asynchronous on s)

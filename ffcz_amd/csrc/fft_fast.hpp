@@ -41,6 +41,8 @@ bool mfma_strided(int mode, const SArgs<float, float>& a, size_t outer, hipStrea
 bool fast_zfwd_batch(int Lz, const BatchTab<ZArgs<float, float>>& t, unsigned grid, hipStream_t s);
 bool fast_zinv_batch(int Lz, const BatchTab<ZArgs<float, float>>& t, unsigned grid, hipStream_t s);
 bool fast_strided_batch(int mode, int L, const BatchTab<SArgs<float, float>>& t, unsigned grid, hipStream_t s);
+// the fused z pass of the fused iteration form (tile variant 0) over every field in the table
+bool fast_zz_batch(int Lz, const BatchTab<ZArgs<float, float>>& t, unsigned grid, hipStream_t s);
 // Batched FP64 certification passes over float fields (fft_fast_c32.hip):
 // z forward in Z_CERT mode, z inverse in Z_PLAIN mode, y/x in S_PLAIN,
 // S_CERT_CHECK or S_LOAD_DF mode.

@@ -13,7 +13,7 @@ bool fast_zfwd<float, float>(int mode, const ZArgs<float, float>& a, hipStream_t
 template <>
 bool fast_zinv<float, float>(int mode, const ZArgs<float, float>& a, hipStream_t s) {
     if (a.prec != PREC_VALU && mfma_zinv(mode, a, s)) return true;
-    return zinv_dispatch<float, float, (1u << Z_PLAIN) | (1u << Z_ITER)>(mode, a, s);
+    return zinv_dispatch<float, float, (1u << Z_PLAIN) | (1u << Z_ITER) | (1u << Z_FUSED)>(mode, a, s);
 }
 template <>
 bool fast_strided<float, float>(int mode, const SArgs<float, float>& a, size_t outer, hipStream_t s) {
@@ -26,6 +26,9 @@ bool fast_zfwd_batch(int Lz, const BatchTab<ZArgs<float, float>>& t, unsigned gr
 }
 bool fast_zinv_batch(int Lz, const BatchTab<ZArgs<float, float>>& t, unsigned grid, hipStream_t s) {
     return zinv_batch_dispatch<float, float, Z_ITER>(Lz, t, grid, s);
+}
+bool fast_zz_batch(int Lz, const BatchTab<ZArgs<float, float>>& t, unsigned grid, hipStream_t s) {
+    return zz_batch_dispatch<float>(Lz, t, grid, s);
 }
 bool fast_strided_batch(int mode, int L, const BatchTab<SArgs<float, float>>& t, unsigned grid, hipStream_t s) {
     if (mode == S_CLAMP) return strided_batch_dispatch<float, float, S_CLAMP>(L, t, grid, s);

@@ -11,7 +11,7 @@ bool fast_zfwd<double, double>(int mode, const ZArgs<double, double>& a, hipStre
 }
 template <>
 bool fast_zinv<double, double>(int mode, const ZArgs<double, double>& a, hipStream_t s) {
-    return zinv_dispatch<double, double, (1u << Z_PLAIN) | (1u << Z_ITER)>(mode, a, s);
+    return zinv_dispatch<double, double, (1u << Z_PLAIN) | (1u << Z_ITER) | (1u << Z_FUSED)>(mode, a, s);
 }
 template <>
 bool fast_strided<double, double>(int mode, const SArgs<double, double>& a, size_t outer, hipStream_t s) {

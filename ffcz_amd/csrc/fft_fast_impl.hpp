@@ -3,6 +3,8 @@
 // instantiate the dispatchers for their (R, T) pair and pass modes.
 #pragma once
 
+#include <type_traits>
+
 #include "fft_fast.hpp"
 #include "fft_mfma.hpp"
 
@@ -339,6 +341,246 @@ __device__ __forceinline__ void zinv_body(const ZArgs<R, T>& a, uint32_t blk) {
 }
 
 // ---------------------------------------------------------------------------
+// The iteration's z pass: inverse z of iteration k, its spatial violation,
+// the spatial clamp of iteration k and the forward z of iteration k + 1, in
+// place on the tile's half-spectrum rows (the iterate e never goes to HBM).
+//
+// The loop stops before the spatial projection of a converged iteration
+// (projection_algorithm_impl.cuh:845-851), which only the finalisation after
+// this pass can tell, so the clamp is speculative.  A converged pass has
+// every excess |e| - eps below tol (the finalisation's test), so only clamps
+// with an excess below tol can belong to one.  Those are deferred: the
+// spectrum gets the clamped value, but instead of adding the edit to ds the
+// pass logs the element (tile-relative offset, unclamped value) in its
+// workgroup's segment.  The next pass over the same tile, which runs only
+// if this one did not converge, first adds the logged edits to ds (each
+// element's edits still accumulate in iteration order); a converged loop
+// drops the log, and one stopped by max_iter applies it (k_zz_apply).
+// Clamps with a larger excess go into ds at once.
+// ---------------------------------------------------------------------------
+// First slot of this lane's n entries in the workgroup's log: a prefix sum
+// of the counts over the wave and one LDS atomic per wave (by its last lane;
+// the lanes of a workgroup's last wave may not all exist).
+__device__ __forceinline__ uint32_t wave_log_base(uint32_t n, uint32_t* counter) {
+    const int lane = threadIdx.x & 63;
+    uint32_t x = n;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    const int last = 63 - __clzll((long long)__ballot(1));
+    uint32_t base = 0u;
+    if (lane == last) base = atomicAdd(counter, x);
+    base = __shfl(base, last, 64);
+    return base + x - n;
+}
+
+template <typename R, int LZ, int V>
+__device__ __forceinline__ void zz_body(const ZArgs<R, R>& a, uint32_t blk) {
+    constexpr int TP = z_tp(LZ), P = z_rows(LZ, V), NT = TP * P;
+    constexpr int NZ = 2 * LZ, M = LZ, NH = M + 1;
+    constexpr int RS = zrows8_stride(LZ);
+    constexpr int NST = pen::nstages(LZ), SL = NST - 1;
+    // The inverse transform runs the radices in the usual order and the
+    // forward one in reversed order, so the forward's first stage reads
+    // exactly the positions (pairs j + r L/R) the inverse's last stage leaves
+    // in a thread's registers: the spatial clamp works there, with no LDS
+    // round trip between the transforms.
+    using DL = pen::StageDim<LZ, TP, SL, false>;
+    using DF = pen::StageDim<LZ, TP, 0, true>;
+    static_assert(DL::RAD == DF::RAD && DL::NB == DF::NB && DL::NS == DF::LR, "the transforms meet in registers");
+    __shared__ cplx<R> tw[LZ];
+    __shared__ cplx<R> tw2[M / 2 + 1];
+    __shared__ cplx<R> buf[P * RS];
+    __shared__ uint32_t nlog;
+    auto bidx = [&](int n, int pp) { return zrows8_idx(n, pp, RS); };
+    auto tw2v = [&](int k) { return half_twiddle<R, M>(tw2, k); };
+    if (a.st->done) return;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < LZ; i += NT) tw[i] = a.tz[i];
+    for (int i = tid; i <= M / 2; i += NT) tw2[i] = a.tz2[i];
+    if (tid == 0) nlog = 0u;
+    const int t = tid % TP, p = tid / TP;
+    const size_t row0 = (size_t)blk * P;
+    const bool rv = row0 + p < a.rows;
+    cplx<R>* spec_t = a.spec + row0 * a.Mp;
+    const uint32_t mp = (uint32_t)a.Mp;
+    const R eps = a.eps;
+    const size_t t0 = row0 * (size_t)NZ;
+    R* dse = a.ds + t0;
+    cplx<R>* dsp = reinterpret_cast<cplx<R>*>(dse);
+    const size_t gb = a.row_off / P + blk;  // the workgroup's log segment and violation slot
+    const size_t seg = gb * (size_t)(P * NZ);
+    const uint32_t nvalid = (uint32_t)(row0 + P <= a.rows ? P * LZ : (a.rows - row0) * LZ);  // pairs
+    if (a.st->iter > 0) {
+        // the previous pass's deferred clamps of this tile (it did not
+        // converge); counts and offsets are held to the tile, whatever wrote
+        // the log
+        const uint32_t npend = min(a.log_cnt[gb], 2u * nvalid);
+        for (uint32_t i = tid; i < npend; i += NT) {
+            const uint32_t o = a.log_off[seg + i];
+            if (o >= 2u * nvalid) continue;
+            R v = a.log_v[seg + i];
+            const R ed = clamp_edit(v, eps);
+            dse[o] = dse[o] + ed;
+        }
+    }
+    __syncthreads();
+    // Inverse C2R along z.  The first stage's inputs Z[n] = (X[n] + conj
+    // X[M-n]) + i W^-n (X[n] - conj X[M-n]) (imaginary parts of the DC and
+    // Nyquist bins dropped) come straight from the row: every X[n] and
+    // X[M-n] of the thread is loaded before any is used.
+    constexpr int NB0 = pen::first_nb<LZ, TP>(), R0 = pen::radix(LZ, 0);
+    cplx<R> xa[NB0][R0], xb[NB0][R0];
+#pragma unroll
+    for (int b = 0; b < NB0; ++b) {
+        const bool has = rv && pen::first_has<LZ, TP>(t, b);
+#pragma unroll
+        for (int r = 0; r < R0; ++r) {
+            const int n = pen::first_elem<LZ>(t, b, r, TP);
+            xa[b][r] = has ? tile_at(spec_t, (uint32_t)p * mp + (uint32_t)n) : mk(R(0), R(0));
+            xb[b][r] = has ? tile_at(spec_t, (uint32_t)p * mp + (uint32_t)(M - n)) : mk(R(0), R(0));
+        }
+    }
+    auto zload = [&](int n, int b, int r) {
+        cplx<R> xn = xa[b][r], xm = xb[b][r];
+        if (n == 0) {
+            xn.y = R(0);
+            xm.y = R(0);
+        }
+        const cplx<R> sm = xn + conjg(xm);
+        const cplx<R> d = cmul(xn - conjg(xm), conjg(tw2v(n)));
+        return mk(sm.x - d.y, sm.y + d.x);
+    };
+    auto lld = [&](int n) { return buf[bidx(n, p)]; };
+    auto lst = [&](int n, cplx<R> v) { buf[bidx(n, p)] = v; };
+    cplx<R> v[DL::NB][DL::RAD];
+    if constexpr (NST == 1) {
+        pen::stage_compute<R, LZ, TP, 0, false, true>(zload, v, tw, t, +1);
+    } else {
+        // stages 0 .. SL-1 through the tile, then the last one into registers
+        pen::transform<R, LZ, TP, false, true, true, 0, false, SL>(zload, lst, lld, lst, tw, t, +1);
+        pen::stage_compute<R, LZ, TP, SL, false>(lld, v, tw, t, +1);
+    }
+    // The spatial clamp on the thread's pairs n = j + r L/R of its row (pair
+    // n is elements 2n, 2n+1: consecutive threads, consecutive pairs, so ds
+    // moves coalesced).  The first clamp since the accumulators were cleared
+    // stores ds whole; later ones read and write it only where they fire,
+    // every load issued before any store.
+    const bool fresh = a.ds_fresh && a.st->iter == 0;
+    R tolr;
+    if constexpr (sizeof(R) == 4) tolr = a.st->tol_f;
+    else tolr = (R)a.st->tol;
+    R vm = R(0);
+    cplx<R> old[DL::NB][DL::RAD];
+    bool f0[DL::NB][DL::RAD], f1[DL::NB][DL::RAD], l0[DL::NB][DL::RAD], l1[DL::NB][DL::RAD];
+#pragma unroll
+    for (int b = 0; b < DL::NB; ++b) {
+        const int j = t + b * TP;
+        const bool ok = rv && ((DL::LR % TP == 0) || j < DL::LR);
+#pragma unroll
+        for (int r = 0; r < DL::RAD; ++r) {
+            v[b][r].x *= a.scale;
+            v[b][r].y *= a.scale;
+            const R x0 = fabs(v[b][r].x) - eps, x1 = fabs(v[b][r].y) - eps;
+            if (ok) vm = fmax(vm, fmax(x0, x1));
+            const bool c0 = ok && (v[b][r].x > eps || v[b][r].x < -eps);
+            const bool c1 = ok && (v[b][r].y > eps || v[b][r].y < -eps);
+            // deferred (excess below tol) or applied now
+            l0[b][r] = c0 && x0 < tolr;
+            l1[b][r] = c1 && x1 < tolr;
+            f0[b][r] = c0 && !l0[b][r];
+            f1[b][r] = c1 && !l1[b][r];
+            old[b][r] = mk(R(0), R(0));
+            const uint32_t q = (uint32_t)(p * LZ + j + r * DL::NS);
+            if (!fresh && (f0[b][r] || f1[b][r])) old[b][r] = tile_at(dsp, q);
+        }
+    }
+    // deferred clamps: counted per thread, one wave-uniform test, and only
+    // then slots (one reservation per wave) and log entries
+    uint32_t nl = 0u;
+#pragma unroll
+    for (int b = 0; b < DL::NB; ++b)
+#pragma unroll
+        for (int r = 0; r < DL::RAD; ++r) nl += (l0[b][r] ? 1u : 0u) + (l1[b][r] ? 1u : 0u);
+    if (__ballot(nl != 0u)) {
+        uint32_t slot = wave_log_base(nl, &nlog);
+#pragma unroll
+        for (int b = 0; b < DL::NB; ++b) {
+            const int j = t + b * TP;
+#pragma unroll
+            for (int r = 0; r < DL::RAD; ++r) {
+                const uint32_t q = (uint32_t)(p * LZ + j + r * DL::NS);
+                if (l0[b][r]) {
+                    a.log_off[seg + slot] = 2u * q;
+                    a.log_v[seg + slot] = v[b][r].x;
+                    ++slot;
+                }
+                if (l1[b][r]) {
+                    a.log_off[seg + slot] = 2u * q + 1u;
+                    a.log_v[seg + slot] = v[b][r].y;
+                    ++slot;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < DL::NB; ++b) {
+        const int j = t + b * TP;
+        const bool ok = rv && ((DL::LR % TP == 0) || j < DL::LR);
+#pragma unroll
+        for (int r = 0; r < DL::RAD; ++r) {
+            const uint32_t q = (uint32_t)(p * LZ + j + r * DL::NS);
+            const R e0 = clamp_edit(v[b][r].x, eps), e1 = clamp_edit(v[b][r].y, eps);
+            // the accumulation of clamp_acc: edit formed, then added
+            cplx<R> nd = old[b][r];
+            if (f0[b][r]) nd.x = old[b][r].x + e0;
+            if (f1[b][r]) nd.y = old[b][r].y + e1;
+            if (fresh ? ok : (f0[b][r] || f1[b][r])) tile_at(dsp, q) = nd;
+        }
+    }
+    block_max_store<R, NT>(vm, &a.vpart[gb]);
+    // every thread is past its reads of the tile (block_max_store's barrier)
+    if (tid == 0) a.log_cnt[gb] = nlog;
+    // Forward transform, radices reversed: its first stage butterflies the
+    // clamped pairs in registers (no twiddles: one sub-transform), and the
+    // rest runs through the tile.  Its twiddles and tile addresses are reached
+    // through an offset the compiler cannot see is zero, so they are formed
+    // again rather than kept live from the inverse transform (which would
+    // double the kernel's registers).
+    int zoff;
+    asm volatile("s_mov_b32 %0, 0" : "=s"(zoff));
+    cplx<R>* bf = buf + zoff;
+    auto lldf = [&](int n) { return bf[bidx(n, p)]; };
+    auto lstf = [&](int n, cplx<R> u) { bf[bidx(n, p)] = u; };
+#pragma unroll
+    for (int b = 0; b < DF::NB; ++b) {
+        const int j = t + b * TP;
+        if ((DF::LR % TP == 0) || j < DF::LR) dftR<R, DF::RAD>(v[b], -1);
+    }
+    pen::stage_store<R, LZ, TP, 0, true>(lstf, v, t);
+    __syncthreads();
+    if constexpr (NST > 1) {
+        pen::transform<R, LZ, TP, true, true, false, 1, true>(lldf, lstf, lldf, lstf, tw + zoff, t, -1);
+        __syncthreads();
+    }
+    // R2C split X[k] = A[k] + W_nz^k B[k] into the rows, as zfwd_body
+    for (int q = tid; q < P * NH; q += NT) {
+        const int pp = q / NH;
+        const int k = q - pp * NH;
+        if (row0 + pp >= a.rows) continue;
+        const int ik = (k == M) ? 0 : k;
+        const int im = (k == 0) ? 0 : M - k;
+        const cplx<R> zk = bf[bidx(ik, pp)];
+        const cplx<R> zm = bf[bidx(im, pp)];
+        const cplx<R> A = mk(R(0.5) * (zk.x + zm.x), R(0.5) * (zk.y - zm.y));
+        const cplx<R> B = mk(R(0.5) * (zk.y + zm.y), R(0.5) * (zm.x - zk.x));
+        tile_at(spec_t, (uint32_t)pp * mp + (uint32_t)k) = A + cmul(tw2v(k), B);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // y / x passes over 128-byte column tiles.
 // ---------------------------------------------------------------------------
 template <typename R, typename T, int L, int MODE, int PREC, int V>
@@ -559,6 +801,15 @@ template <typename R, typename T, int LZ, int MODE, int PREC = PREC_VALU, int V 
 __global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ, V), PREC != PREC_VALU ? 8 : 1) void k2_zinv(ZArgs<R, T> a) {
     zinv_body<R, T, LZ, MODE, PREC, V>(a, blockIdx.x);
 }
+// FP32 power-of-two lengths at eight waves per SIMD (at most 64 VGPRs, no
+// spills), as the separate z kernels run; other lengths and FP64 spill under
+// that bound and take the registers they need
+template <typename R, int LZ>
+constexpr int zz_min_waves() { return (sizeof(R) == 4 && (LZ & (LZ - 1)) == 0) ? 8 : 1; }
+template <typename R, int LZ, int V = 0>
+__global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ, V), (zz_min_waves<R, LZ>())) void k2_zz(ZArgs<R, R> a) {
+    zz_body<R, LZ, V>(a, blockIdx.x);
+}
 // The MFMA variants ask for 8 waves per SIMD (at most 64 VGPRs): two
 // 1024-thread tiles per CU, like the vector-ALU kernels.  The fused clamp
 // kernel spills a few registers under that bound and still runs faster than
@@ -589,6 +840,12 @@ __global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ)) void k2_zinv_b(BatchTab<ZArg
     const uint32_t f = batch_field(t, blockIdx.x);
     const ZArgs<R, T> a = t.args[f];
     zinv_body<R, T, LZ, MODE, PREC_VALU, 0>(a, blockIdx.x - t.start[f]);
+}
+template <typename R, int LZ>
+__global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ), (zz_min_waves<R, LZ>())) void k2_zz_b(BatchTab<ZArgs<R, R>> t) {
+    const uint32_t f = batch_field(t, blockIdx.x);
+    const ZArgs<R, R> a = t.args[f];
+    zz_body<R, LZ, 0>(a, blockIdx.x - t.start[f]);
 }
 template <typename R, typename T, int L, int MODE>
 __global__ __launch_bounds__(s_tp<R>(L) * s_width<R>()) void k2_strided_b(BatchTab<SArgs<R, T>> t) {
@@ -631,6 +888,26 @@ bool zfwd_by_len(const ZArgs<R, T>& a, hipStream_t s) {
         return true;
         FFCZ_FAST_Z_LENGTHS(FFCZ_ZF)
 #undef FFCZ_ZF
+        default:
+            return false;
+    }
+}
+template <typename R, int LZ, int V = 0>
+inline void launch_zz_len(const ZArgs<R, R>& a, hipStream_t s) {
+    const unsigned grid = (unsigned)ceil_div(a.rows, (size_t)z_rows(LZ, V));
+    hipLaunchKernelGGL((k2_zz<R, LZ, V>), dim3(grid), dim3(z_tp(LZ) * z_rows(LZ, V)), 0, s, a);
+}
+template <typename R>
+bool zz_by_len(const ZArgs<R, R>& a, hipStream_t s) {
+    if (!a.half) return false;
+    switch (a.Lz) {
+#define FFCZ_ZZ(Lv)                                        \
+    case Lv:                                               \
+        if (a.var) launch_zz_len<R, Lv, 1>(a, s);          \
+        else launch_zz_len<R, Lv, 0>(a, s);                \
+        return true;
+        FFCZ_FAST_Z_LENGTHS(FFCZ_ZZ)
+#undef FFCZ_ZZ
         default:
             return false;
     }
@@ -686,6 +963,8 @@ bool zinv_dispatch(int mode, const ZArgs<R, T>& a, hipStream_t s) {
         if constexpr ((MASK & (1u << Z_PLAIN)) != 0) return zinv_by_len<R, T, Z_PLAIN>(a, s);
     } else if (mode == Z_ITER) {
         if constexpr ((MASK & (1u << Z_ITER)) != 0) return zinv_by_len<R, T, Z_ITER>(a, s);
+    } else if (mode == Z_FUSED) {
+        if constexpr ((MASK & (1u << Z_FUSED)) != 0 && std::is_same<R, T>::value) return zz_by_len<R>(a, s);
     }
     return false;
 }
@@ -832,6 +1111,19 @@ bool zinv_batch_dispatch(int Lz, const BatchTab<ZArgs<R, T>>& t, unsigned grid, 
         return true;
         FFCZ_FAST_Z_LENGTHS(FFCZ_BZI)
 #undef FFCZ_BZI
+        default:
+            return false;
+    }
+}
+template <typename R>
+bool zz_batch_dispatch(int Lz, const BatchTab<ZArgs<R, R>>& t, unsigned grid, hipStream_t s) {
+    switch (Lz) {
+#define FFCZ_BZZ(Lv)                                                                                   \
+    case Lv:                                                                                           \
+        hipLaunchKernelGGL((k2_zz_b<R, Lv>), dim3(grid), dim3(z_tp(Lv) * z_rows(Lv)), 0, s, t);        \
+        return true;
+        FFCZ_FAST_Z_LENGTHS(FFCZ_BZZ)
+#undef FFCZ_BZZ
         default:
             return false;
     }

@@ -88,7 +88,7 @@ __device__ __forceinline__ void block_count_atomic(unsigned long long c, unsigne
     if ((threadIdx.x & 63) == 0 && c) atomicAdd(target, c);
 }
 
-enum ZMode : int { Z_PLAIN = 0, Z_ITER = 1, Z_CERT = 2 };
+enum ZMode : int { Z_PLAIN = 0, Z_ITER = 1, Z_CERT = 2, Z_FUSED = 3 };
 
 template <typename R, typename T>
 struct ZArgs {
@@ -114,6 +114,11 @@ struct ZArgs {
     DevState* st;
     R* vpart;  // per-workgroup max violation of the inverse z pass
     size_t row_off;  // first row of this launch within the field (slab launches)
+    // Z_FUSED log of deferred clamps: per workgroup a segment of P*nz entries
+    // (tile-relative element offset, the unclamped value) and a count
+    uint32_t* log_off;
+    R* log_v;
+    uint32_t* log_cnt;
     // Z_CERT
     const T* e0;
     const T* cds;

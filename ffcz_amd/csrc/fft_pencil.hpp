@@ -47,6 +47,16 @@ constexpr int nsub(int L, int s) {
     }
     return ns;
 }
+// The radices in forward or reversed order.  A transform with the reversed
+// order starts with the radix the normal order ends with, so its first stage
+// reads exactly the positions (j + r L/R) the normal order's last stage
+// writes: one transform can feed the other from registers.
+constexpr int radix_o(int L, int s, bool rev) { return rev ? radix(L, nstages(L) - 1 - s) : radix(L, s); }
+constexpr int nsub_o(int L, int s, bool rev) {
+    int ns = 1;
+    for (int i = 0; i < s; ++i) ns *= radix_o(L, i, rev);
+    return ns;
+}
 // threads per pencil: ~8 elements per thread up to L = 512, ~16 beyond
 constexpr int threads_per_pencil(int L) { return L <= 1 ? 1 : (L <= 512 ? (L + 7) / 8 : (L + 15) / 16); }
 
@@ -95,10 +105,61 @@ __device__ __forceinline__ void stage_rn(const LD& ld, const ST& st, const cplx<
     }
 }
 
-template <typename R, int L, int TP, int S, bool SYNC_MID, bool BR = false, class LD, class ST>
+template <typename R, int L, int TP, int S, bool SYNC_MID, bool BR = false, bool REV = false, class LD, class ST>
 __device__ __forceinline__ void run_stage(const LD& ld, const ST& st, const cplx<R>* __restrict__ tw, int t,
                                           int sign) {
-    stage_rn<R, L, TP, radix(L, S), nsub(L, S), SYNC_MID, BR>(ld, st, tw, t, sign);
+    stage_rn<R, L, TP, radix_o(L, S, REV), nsub_o(L, S, REV), SYNC_MID, BR>(ld, st, tw, t, sign);
+}
+
+// Stage S (radix order REV) split in two: its loads, twiddles and
+// butterflies into registers, and its stores from them.
+template <int L, int TP, int S, bool REV>
+struct StageDim {
+    static constexpr int RAD = radix_o(L, S, REV), NS = nsub_o(L, S, REV), LR = L / RAD;
+    static constexpr int NB = (LR + TP - 1) / TP, TWS = L / (NS * RAD);
+};
+template <typename R, int L, int TP, int S, bool REV, bool BR = false, class LD>
+__device__ __forceinline__ void stage_compute(const LD& ld,
+                                              cplx<R> (&v)[StageDim<L, TP, S, REV>::NB][StageDim<L, TP, S, REV>::RAD],
+                                              const cplx<R>* __restrict__ tw, int t, int sign) {
+    using D = StageDim<L, TP, S, REV>;
+#pragma unroll
+    for (int b = 0; b < D::NB; ++b) {
+        const int j = t + b * TP;
+        if ((D::LR % TP == 0) || j < D::LR) {
+#pragma unroll
+            for (int r = 0; r < D::RAD; ++r) {
+                if constexpr (BR) v[b][r] = ld(j + r * D::LR, b, r);
+                else v[b][r] = ld(j + r * D::LR);
+            }
+            if constexpr (D::NS > 1) {
+                const int k = j % D::NS;
+#pragma unroll
+                for (int r = 1; r < D::RAD; ++r) {
+                    cplx<R> w = tw[k * r * D::TWS];
+                    w.y = sign > 0 ? -w.y : w.y;
+                    v[b][r] = cmul(v[b][r], w);
+                }
+            }
+            dftR<R, D::RAD>(v[b], sign);
+        }
+    }
+}
+template <typename R, int L, int TP, int S, bool REV, class ST>
+__device__ __forceinline__ void stage_store(const ST& st,
+                                            const cplx<R> (&v)[StageDim<L, TP, S, REV>::NB][StageDim<L, TP, S, REV>::RAD],
+                                            int t) {
+    using D = StageDim<L, TP, S, REV>;
+#pragma unroll
+    for (int b = 0; b < D::NB; ++b) {
+        const int j = t + b * TP;
+        if ((D::LR % TP == 0) || j < D::LR) {
+            const int k = j % D::NS;
+            const int base = (j - k) * D::RAD + k;
+#pragma unroll
+            for (int r = 0; r < D::RAD; ++r) st(base + r * D::NS, v[b][r]);
+        }
+    }
 }
 
 // The first stage's loads of thread t: position (b, r) is element
@@ -122,11 +183,15 @@ __device__ __forceinline__ int first_elem(int t, int b, int r, int TP) {
 // between its reads and its writes).  Returns after the last stage's stores
 // without a trailing barrier.
 // BR0: the first stage's loader takes (n, b, r) (stage_rn's BR).
-template <typename R, int L, int TP, bool IN_LDS, bool OUT_LDS, bool BR0 = false, int S = 0, class GLD, class GST,
-          class LLD, class LST>
+// REV: the radices in reversed order.  SE >= 0 stops before stage SE (its
+// last stage run then stores through lst and ends with a barrier, like a
+// middle stage), for a caller that runs the rest itself.
+template <typename R, int L, int TP, bool IN_LDS, bool OUT_LDS, bool BR0 = false, int S = 0, bool REV = false,
+          int SE = -1, class GLD, class GST, class LLD, class LST>
 __device__ __forceinline__ void transform(const GLD& gld, const GST& gst, const LLD& lld, const LST& lst,
                                           const cplx<R>* __restrict__ tw, int t, int sign) {
-    constexpr int NST = nstages(L);
+    constexpr int NST = SE < 0 ? nstages(L) : SE;
+    constexpr bool STOPS = SE >= 0 && SE < nstages(L);
     static_assert(NST >= 0, "length must be 2/3/5-smooth");
     if constexpr (NST == 0) {
         if (t == 0) {
@@ -139,19 +204,19 @@ __device__ __forceinline__ void transform(const GLD& gld, const GST& gst, const 
             __syncthreads();
         }
     } else if constexpr (S < NST) {
-        constexpr bool FIRST = (S == 0), LAST = (S == NST - 1);
+        constexpr bool FIRST = (S == 0), LAST = (S == NST - 1) && !STOPS;
         if constexpr (FIRST && LAST) {
-            run_stage<R, L, TP, S, IN_LDS && OUT_LDS, BR0>(gld, gst, tw, t, sign);
+            run_stage<R, L, TP, S, IN_LDS && OUT_LDS, BR0, REV>(gld, gst, tw, t, sign);
         } else if constexpr (FIRST) {
-            run_stage<R, L, TP, S, IN_LDS, BR0>(gld, lst, tw, t, sign);
+            run_stage<R, L, TP, S, IN_LDS, BR0, REV>(gld, lst, tw, t, sign);
             __syncthreads();
         } else if constexpr (LAST) {
-            run_stage<R, L, TP, S, OUT_LDS>(lld, gst, tw, t, sign);
+            run_stage<R, L, TP, S, OUT_LDS, false, REV>(lld, gst, tw, t, sign);
         } else {
-            run_stage<R, L, TP, S, true>(lld, lst, tw, t, sign);
+            run_stage<R, L, TP, S, true, false, REV>(lld, lst, tw, t, sign);
             __syncthreads();
         }
-        transform<R, L, TP, IN_LDS, OUT_LDS, BR0, S + 1>(gld, gst, lld, lst, tw, t, sign);
+        transform<R, L, TP, IN_LDS, OUT_LDS, BR0, S + 1, REV, SE>(gld, gst, lld, lst, tw, t, sign);
     }
 }
 

@@ -81,7 +81,7 @@ void Solver<T>::release() {
     if (s_) (void)hipStreamSynchronize(s_);
     void* ptrs[] = {x_, e_, e0_, ds_, df_, spec_, st_, vpart_, delta_dense_, orig_, partial_idx_, fpack_, spack_,
                     fcomp_, scomp_, fquant_, squant_, xscratch_, cert_real_, cert_spec_, cstats_, xglob_,
-                    cert_real2_};
+                    cert_real2_, zlog_off_, zlog_v_, zlog_cnt_};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (xhost_) (void)hipHostFree(xhost_);
@@ -104,6 +104,8 @@ void Solver<T>::release() {
     cert_real2_ = nullptr;
     cert_spec_ = nullptr;
     cstats_ = nullptr;
+    zlog_off_ = zlog_cnt_ = nullptr;
+    zlog_v_ = nullptr;
     if (f64_) f64_->destroy();
     f64_.reset();
     f_.destroy();
@@ -114,6 +116,9 @@ void Solver<T>::release() {
 template <typename T>
 void Solver<T>::upload_x(const T* x) {
     FFCZ_REQUIRE(x != nullptr, ERR_INVALID, "null original data");
+    // initialize* leaves the previous solve's iterate in place (the ptw
+    // modes reuse the spectrum rows it may still be held in)
+    materialize_e();
     FFCZ_HIP(hipMemcpyAsync(x_, x, f_.N * sizeof(T), hipMemcpyDefault, s_));
 }
 
@@ -247,7 +252,33 @@ IterArgs<T> Solver<T>::iter_args() const {
     a.norm = T(1) / static_cast<T>(f_.N);  // projection_algorithm_impl.cuh:242
     a.st = st_;
     a.vpart = vpart_;
+    a.log_off = zlog_off_;
+    a.log_v = zlog_v_;
+    a.log_cnt = zlog_cnt_;
     return a;
+}
+
+template <typename T>
+void Solver<T>::ensure_zz_log() {
+    if (zlog_off_) return;
+    const size_t n = zz_log_entries(f_), nb = zz_blocks_max(f_);
+    FFCZ_REQUIRE(n > 0 && nb > 0, ERR_INVALID, "no fused z pass for this shape");
+    dalloc(&zlog_off_, n);
+    dalloc(&zlog_v_, n);
+    dalloc(&zlog_cnt_, nb);
+}
+
+template <typename T>
+void Solver<T>::run_iteration(const IterArgs<T>& a, bool fused, bool first, EventLog* log) {
+    if (fused) iteration_fused(f_, a, first, s_, log);
+    else iteration(f_, a, s_, log);
+}
+
+template <typename T>
+void Solver<T>::materialize_e() {
+    if (!e_lazy_) return;
+    zz_recover_e(f_, iter_args(), fused_var_, s_);
+    e_lazy_ = false;
 }
 
 template <typename T>
@@ -261,6 +292,8 @@ void Solver<T>::prepare_solve(int mode, const T* e0, int max_iter, T tol, hipStr
     // reads the initial error directly (its inverse pass writes the iterate
     // e_); a device-resident e0 is copied into e0_ by that same pass.
     e_src_ = nullptr;
+    fused_ = false;
+    e_lazy_ = false;
     hipPointerAttribute_t attr{};
     if (max_iter > 0 && hipPointerGetAttributes(&attr, e0) == hipSuccess &&
         (attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged))
@@ -274,7 +307,11 @@ void Solver<T>::prepare_solve(int mode, const T* e0, int max_iter, T tol, hipStr
 
 template <typename T>
 void Solver<T>::solve_loop(int mode, const T* e0, int max_iter, T tol) {
+    const bool fused = use_fused();
+    if (fused) ensure_zz_log();
     prepare_solve(mode, e0, max_iter, tol, s_);
+    fused_ = fused;
+    fused_var_ = f_.zi_var;
     const IterArgs<T> a = iter_args();
     // iterations go out in chunks of 2, 4, 8, 16 between host checks; the
     // kernels of an iteration past convergence return at once
@@ -288,7 +325,7 @@ void Solver<T>::solve_loop(int mode, const T* e0, int max_iter, T tol) {
     while (launched < max_iter) {
         const int k = std::min(chunk, max_iter - launched);
         if (pass_timing_) chunk_first_.push_back(launched + 1);
-        for (int i = 0; i < k; ++i) iteration(f_, a, s_, pass_timing_ ? plog_.get() : nullptr);
+        for (int i = 0; i < k; ++i) run_iteration(a, fused, launched + i == 0, pass_timing_ ? plog_.get() : nullptr);
         launched += k;
         FFCZ_HIP(hipMemcpyAsync(&h, st_, sizeof(h), hipMemcpyDeviceToHost, s_));
         FFCZ_HIP(hipStreamSynchronize(s_));
@@ -347,9 +384,17 @@ long Solver<T>::pass_timing(float* ms6) const {
 template <typename T>
 void Solver<T>::finish_solve(hipStream_t s, const DevState* known) {
     const IterArgs<T> a = iter_args();
-    // the clamp applies after an unconverged loop only: a known final state
-    // that says otherwise needs no launch
-    if (!known || (known->done && !known->converged && known->iter > 0)) epilogue_clamp(f_, a, s);
+    if (fused_) {
+        // the last z fused pass clamped: kept after an unconverged loop (the
+        // reference's final spatial projection, impl:851), its deferred
+        // clamps then added to ds; a converged loop drops them (it stops
+        // before that projection, impl:845)
+        if (!known || (!known->converged && known->iter > 0)) zz_apply(f_, a, fused_var_, s);
+    } else if (!known || (known->done && !known->converged && known->iter > 0)) {
+        // the clamp applies after an unconverged loop only: a known final
+        // state that says otherwise needs no launch
+        epilogue_clamp(f_, a, s);
+    }
     DevState h{};
     if (known) {
         h = *known;  // the epilogue leaves the state as it is
@@ -361,13 +406,15 @@ void Solver<T>::finish_solve(hipStream_t s, const DevState* known) {
     if (ds_fresh_) {
         // the first spatial clamp stored ds whole: the forward z pass of
         // iteration 1, or the epilogue after one unconverged iteration
-        const bool written = h.iter >= 2 || (h.iter == 1 && !h.converged);
+        // (the fused form: the z fused pass of iteration 1, deferred clamps as 0)
+        const bool written = fused_ ? h.iter >= 1 : (h.iter >= 2 || (h.iter == 1 && !h.converged));
         ds_fresh_ = false;
         if (!written) {
             hipLaunchKernelGGL((k_zero<T>), dim3(gridn(f_.N)), dim3(256), 0, s, ds_, f_.N);
             FFCZ_HIP(hipGetLastError());
         }
     }
+    e_lazy_ = fused_ && h.iter > 0;
     iters_ = (int)h.iter;
     viol_ = h.last_viol;
     converged_ = h.converged != 0;
@@ -615,8 +662,9 @@ CertResult Solver<T>::certify_read(bool quantized) {
 }
 
 template <typename T>
-void Solver<T>::get_results(T* e) const {
+void Solver<T>::get_results(T* e) {
     // projection_algorithm_impl.cuh:904-910
+    materialize_e();
     FFCZ_HIP(hipMemcpyAsync(e, e_, f_.N * sizeof(T), hipMemcpyDefault, s_));
     FFCZ_HIP(hipStreamSynchronize(s_));
 }
@@ -626,17 +674,21 @@ void Solver<T>::time_iteration(int reps, float* ms5) {
     // ms5[0..4]: the five passes; ms5[5]: a whole iteration
     FFCZ_REQUIRE(solved_, ERR_STATE, "time_iteration requires a completed solve");
     FFCZ_REQUIRE(reps >= 1, ERR_INVALID, "reps must be >= 1");
+    materialize_e();
     accumulators_clobbered();
+    const bool fused = use_fused();
+    if (fused) ensure_zz_log();
     reset_state<T>(st_, 1 << 30, -1.0, s_);  // tol < 0: never converges
     const IterArgs<T> a = iter_args();
     // passes: events between the launches of real iterations (an interval
-    // is attributed to the launch that ends it)
+    // is attributed to the launch that ends it); the fused form's iterations
+    // after the first (no forward z pass of the initial error)
     float acc[6] = {0, 0, 0, 0, 0, 0};  // [5]: violation reduction
     EventLog log;
-    iteration(f_, a, s_);  // warm-up
+    run_iteration(a, fused, false, nullptr);  // warm-up
     for (int r = 0; r < reps; ++r) {
         log.used = 0;
-        iteration(f_, a, s_, &log);
+        run_iteration(a, fused, false, &log);
         FFCZ_REQUIRE(log.used >= 2, ERR_HIP, "event creation failed");
         FFCZ_HIP(hipEventSynchronize(log.pool[log.used - 1]));
         for (size_t i = 1; i < log.used; ++i) {
@@ -650,7 +702,7 @@ void Solver<T>::time_iteration(int reps, float* ms5) {
     // between the launches
     hipEvent_t e0 = log.pool[0], e1 = log.pool[1];
     FFCZ_HIP(hipEventRecord(e0, s_));
-    for (int r = 0; r < reps; ++r) iteration(f_, a, s_);
+    for (int r = 0; r < reps; ++r) run_iteration(a, fused, false, nullptr);
     FFCZ_HIP(hipEventRecord(e1, s_));
     FFCZ_HIP(hipEventSynchronize(e1));
     float t = 0;
@@ -760,15 +812,24 @@ TileVars Solver<T>::autotune(int reps, bool use_cache, float* ms_out) {
     // share caches, so each is timed in its real sequence): a zero initial
     // error (no violations, as in a converged field), zeroed accumulators and
     // a state that never stops the kernels
+    materialize_e();
+    e_lazy_ = false;
     FFCZ_HIP(hipMemsetAsync(e0_, 0, f_.N * sizeof(T), s_));
     reset_edit_accumulators();
     materialize_ds();
     reset_state<T>(st_, 1 << 30, -1.0, s_);
+    f_.zf_var = f_.y_var = f_.x_var = f_.zi_var = 0;
+    const bool fused = use_fused();
+    if (fused) ensure_zz_log();
     const IterArgs<T> a = iter_args();
     EventPair ev;
     float ms[9] = {};
-    f_.zf_var = f_.y_var = f_.x_var = f_.zi_var = 0;
-    auto t_iter = [&] { return ev.time(s_, reps, [&] { iteration(f_, a, s_); }); };
+    // each timing starts from iteration 0: no fused pass replays deferred
+    // clamps logged under another tile variant
+    auto t_iter = [&] {
+        reset_state<T>(st_, 1 << 30, -1.0, s_);
+        return ev.time(s_, reps, [&] { run_iteration(a, fused, false, nullptr); });
+    };
     float best = t_iter();
     ms[0] = best;
     // coordinate descent over the four iteration pass kernels; an alternative

@@ -51,6 +51,13 @@ class Solver {
     void prepare_solve(int mode, const T* e0, int max_iter, T tol, hipStream_t s, bool reset = true);
     // known: the final state when the caller has already read it (no extra wait)
     void finish_solve(hipStream_t s, const DevState* known = nullptr);
+    // after prepare_solve, for a caller running the fused iteration form
+    // itself (batches): this solve's passes use tile variant var
+    void set_fused_run(int var) {
+        ensure_zz_log();
+        fused_ = true;
+        fused_var_ = var;
+    }
     // iterations run outside a solve (timing) leave the accumulators undefined
     void accumulators_clobbered();
     // solve / solve_all_abs / solve_all_ptw: loop + extraction (impl:772-902).
@@ -92,8 +99,9 @@ class Solver {
     void set_quantizer(int kind, double freq_step, double spatial_step);
 
     // getters (copies into caller memory; host or device)
-    void get_results(T* e) const;
-    const T* d_e() const { return e_; }
+    void get_results(T* e);
+    // the iterate (recovered first when the last solve ran the fused form)
+    const T* d_e() { return materialize_e(), e_; }
     const T* d_df() const { return df_; }
     const T* d_ds() const { return materialize_ds(), ds_; }
     size_t N() const { return f_.N; }
@@ -222,6 +230,18 @@ class Solver {
     mutable bool ds_fresh_ = false;
     const T* e_src_ = nullptr;  // device initial error read by the running solve's first iteration
     void materialize_ds(hipStream_t s = nullptr) const;
+    // Fused iteration form (iteration_fused): its log of deferred clamps, whether the last
+    // solve ran it, and the iterate it leaves in the spectrum rows until asked for
+    bool use_fused() const { return fused_iteration_available(f_); }
+    void ensure_zz_log();
+    void materialize_e();
+    void run_iteration(const IterArgs<T>& a, bool fused, bool first, EventLog* log);
+    uint32_t* zlog_off_ = nullptr;
+    T* zlog_v_ = nullptr;
+    uint32_t* zlog_cnt_ = nullptr;
+    bool fused_ = false;
+    int fused_var_ = 0;  // the z tile variant of the last fused loop (its log layout)
+    bool e_lazy_ = false;
     void take_extract_results(const ExtractGlobals& hf, const ExtractGlobals& hs) const;
 };
 

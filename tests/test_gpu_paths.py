@@ -1,7 +1,7 @@
-"""The length-specialised FFT kernels and the slab-interleaved iteration give
-the same results as the generic run-time-planned kernels and the whole-field
-iteration (FP64: to rounding; the two paths use different radix schedules
-only where noted)."""
+"""The length-specialised FFT kernels, the fused iteration form and the
+slab-interleaved five-pass iteration give the same results as the generic
+run-time-planned kernels and each other (FP64: to rounding; the two paths
+use different radix schedules only where noted)."""
 import numpy as np
 import pytest
 import torch
@@ -39,14 +39,26 @@ def test_fast_matches_generic_f64(gpu, restore_tuning, shape):
         assert np.linalg.norm(u - v) <= 1e-11 * max(np.linalg.norm(v), 1e-300)
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("shape", [(32, 64, 128), (16, 32, 256)])
-def test_slabs_match_whole_field(gpu, restore_tuning, shape):
-    # 1 MiB slabs force several slabs on these sizes; results are bit-identical
-    a = run(shape, np.float32, 0, 0)
-    b = run(shape, np.float32, 0, 1)
+def test_fused_form_matches_five_pass_form(gpu, restore_tuning, shape, dtype):
+    """The whole-field loop runs the fused form (inverse z, spatial clamp and
+    forward z in one pass, speculative clamps undone after convergence); a
+    slab setting (1 MiB slabs: several slabs at these sizes) runs the five
+    passes slab by slab.  The same operations in the same order, compiled in
+    different kernels (the compiler's contraction of multiply-adds may
+    differ): iteration count equal, edits and iterate to rounding (the fused
+    form's iterate is recovered from the spectrum rows after the loop, a z
+    transform round trip).  Measured: FP64 1.5e-15, 7e-16, 4.4e-15 (e, df,
+    ds); FP32 5.7e-7, 2.5e-7, 1.6e-6, the spatial edits amplifying the
+    rounding as against the oracle (test_gpu_workloads)."""
+    a = run(shape, dtype, 0, 0)
+    b = run(shape, dtype, 0, 1)
     assert a[0] == b[0]
-    for u, v in zip(a[1:], b[1:]):
-        assert np.array_equal(u, v)
+    tol = [1e-6, 1e-6, 5e-6] if dtype == np.float32 else [1e-13] * 3
+    errs = [float(np.linalg.norm(u - v) / np.linalg.norm(v)) for u, v in zip(a[1:], b[1:])]
+    print("fused vs five-pass (e, df, ds):", errs)
+    assert all(e <= t for e, t in zip(errs, tol))
 
 
 @pytest.mark.parametrize("shape", [(8, 60, 72), (4, 512, 16), (2, 600, 8)])

@@ -121,8 +121,10 @@ def test_repeated_solve_accumulates_and_reset(gpu):
         assert not np.any(s.get_freq_edits_full())
 
 
-def test_device_inputs_and_max_iterations_epilogue(gpu):
-    shape = (16, 20, 30)
+@pytest.mark.parametrize("shape", [(16, 20, 30), (16, 20, 32)])
+def test_device_inputs_and_max_iterations_epilogue(gpu, shape):
+    # (16, 20, 30): the five-pass form and its epilogue clamp; (16, 20, 32):
+    # the fused form, whose last z pass made the final clamp
     x, e0, eps, delta = binding_case(shape, seed=4, dfrac=0.2)
     ref = O.solve(x, e0, eps, mode="abs", delta=delta, max_iter=3, tol=0.0)
     with ProjectionSolver(*shape, dtype=np.float64) as s:
@@ -133,3 +135,33 @@ def test_device_inputs_and_max_iterations_epilogue(gpu):
         # the last iteration's spatial clamp is applied (impl:851)
         assert rel(s.get_results(), ref.e.ravel()) < 1e-9
         assert rel(s.get_spatial_edits_full(), ref.ds) < 1e-9
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("shape", [(16, 20, 32), (8, 64, 64)])
+def test_converged_iteration_clamps_are_dropped(gpu, shape, dtype):
+    """tol = 2 % of eps: the converged iteration still has elements beyond
+    eps by less than tol.  The reference stops before clamping them
+    (projection_algorithm_impl.cuh:845); the fused form clamped them
+    speculatively and must leave them out of ds, and report the unclamped
+    iterate."""
+    x, e0, eps, delta = binding_case(shape, seed=17, dtype=dtype, dfrac=0.3)
+    tol = 0.02 * eps
+    ref = O.solve(x, e0, eps, mode="abs", delta=delta, max_iter=40, tol=tol, dtype=dtype)
+    # the final iterate has elements in (eps, eps + tol): clamps to undo
+    over = np.abs(ref.e) - eps
+    assert np.count_nonzero((over > 0) & (over < tol)) > 0
+    with ProjectionSolver(*shape, dtype=dtype) as s:
+        s.initialize_all_abs(x, eps, delta)
+        s.solve_all_abs(e0, 40, tol)
+        it, v, conv = s.status()
+        assert conv and it == ref.iterations
+        lim = 1e-9 if dtype == np.float64 else 1e-5
+        assert rel(s.get_spatial_edits_full(), ref.ds) < lim
+        assert rel(s.get_freq_edits_full(), ref.df) < lim
+        e = s.get_results()
+        assert rel(e, ref.e.ravel()) < lim
+        # elements the reference left beyond eps stay beyond it (FP64: no
+        # element sits within rounding of eps)
+        if dtype == np.float64:
+            assert np.count_nonzero(np.abs(e) > eps) == np.count_nonzero(np.abs(ref.e) > eps)
