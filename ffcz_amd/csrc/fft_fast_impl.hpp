@@ -565,18 +565,21 @@ __device__ __forceinline__ void zz_body(const ZArgs<R, R>& a, uint32_t blk) {
         pen::transform<R, LZ, TP, true, true, false, 1, true>(lldf, lstf, lldf, lstf, tw + zoff, t, -1);
         __syncthreads();
     }
-    // R2C split X[k] = A[k] + W_nz^k B[k] into the rows, as zfwd_body
-    for (int q = tid; q < P * NH; q += NT) {
-        const int pp = q / NH;
-        const int k = q - pp * NH;
-        if (row0 + pp >= a.rows) continue;
-        const int ik = (k == M) ? 0 : k;
-        const int im = (k == 0) ? 0 : M - k;
-        const cplx<R> zk = bf[bidx(ik, pp)];
-        const cplx<R> zm = bf[bidx(im, pp)];
-        const cplx<R> A = mk(R(0.5) * (zk.x + zm.x), R(0.5) * (zk.y - zm.y));
-        const cplx<R> B = mk(R(0.5) * (zk.y + zm.y), R(0.5) * (zm.x - zk.x));
-        tile_at(spec_t, (uint32_t)pp * mp + (uint32_t)k) = A + cmul(tw2v(k), B);
+    // R2C split X[k] = A[k] + W_nz^k B[k] into the thread's own row, bins k
+    // and M-k together: with A, B from Z[k] and Z[M-k], X[M-k] is
+    // conj(A - W^k B) (W^(M-k) = -conj W^k), so one product serves both; k = 0
+    // gives the DC and Nyquist bins
+    if (rv) {
+        cplx<R>* rowp = spec_t + (size_t)p * mp;
+        for (int k = t; k <= M / 2; k += TP) {
+            const cplx<R> zk = bf[bidx(k, p)];
+            const cplx<R> zm = bf[bidx(k == 0 ? 0 : M - k, p)];
+            const cplx<R> A = mk(R(0.5) * (zk.x + zm.x), R(0.5) * (zk.y - zm.y));
+            const cplx<R> B = mk(R(0.5) * (zk.y + zm.y), R(0.5) * (zm.x - zk.x));
+            const cplx<R> wb = cmul(tw2[k], B);
+            rowp[k] = A + wb;
+            if (k != M - k) rowp[M - k] = conjg(A - wb);
+        }
     }
 }
 
