@@ -360,7 +360,8 @@ __global__ __launch_bounds__(FFT_THREADS) void k_strided(SArgs<R, T> a) {
         if (c0 + c < a.ncols) {
             if constexpr (MODE == S_LOAD_DF) {
                 const size_t idx = (size_t)n * a.cs_n + o * a.cs_o + c0 + c;
-                v = mk((R)dequant(a.dfsrc[idx], a.qf), (R)dequant(a.dfsrc[idx + a.F], a.qf));
+                const cplx<T> d = a.dfsrc[idx];
+                v = mk((R)dequant(d.x, a.qf), (R)dequant(d.y, a.qf));
             } else {
                 v = a.data[base + (size_t)n * a.es + c];
             }

@@ -106,7 +106,7 @@ struct IterArgs {
                           // spatial clamp (forward z pass of iteration 1, or the epilogue)
                           // stores every element instead of adding to it
     R* ds;                // spatial edit accumulator [N]
-    R* df;                // frequency edit accumulator [2F] split Re|Im, reference layout
+    cplx<R>* df;          // frequency edit accumulator [F], one (Re, Im) pair per bin (compact order)
     cplx<R>* spec;        // padded half spectrum work buffer
     R eps;                // spatial bound used by the iteration
     int fmode;            // FreqMode
@@ -253,7 +253,7 @@ struct CertArgs {
     size_t nx, ny, nz;
     const T* e0;          // initial error x' - x [N]
     const T* ds;          // spatial edits [N]
-    const T* df;          // frequency edits [2F] (reference layout)
+    const cplx<T>* df;    // frequency edits [F], one (Re, Im) pair per bin (compact order)
     QuantDesc qs, qf;     // dequantisation applied to ds / df
     bool have_df;         // any frequency edit present
     double eps;

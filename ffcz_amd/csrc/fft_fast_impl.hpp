@@ -606,7 +606,8 @@ __device__ __forceinline__ void strided_body(const SArgs<R, T>& a, uint32_t blk)
         if (!valid) return mk(R(0), R(0));
         if constexpr (MODE == S_LOAD_DF) {
             const size_t idx = (size_t)n * a.cs_n + o * a.cs_o + col;
-            return mk((R)dequant(a.dfsrc[idx], a.qf), (R)dequant(a.dfsrc[idx + a.F], a.qf));
+            const cplx<T> d = a.dfsrc[idx];
+            return mk((R)dequant(d.x, a.qf), (R)dequant(d.y, a.qf));
         } else {
             return base[(size_t)n * a.es];
         }
@@ -686,76 +687,95 @@ __device__ __forceinline__ void strided_body(const SArgs<R, T>& a, uint32_t blk)
         constexpr int PER = (L * W + NT - 1) / NT;
         const bool ptw = a.fmode == FM_PTW;
         const size_t obase = o * a.cs_o + (size_t)ct * W;
-        cplx<R> bnd[PER];  // (delta_k, -) or F(x) of the bin
-        R old_re[PER], old_im[PER], ed_re[PER], ed_im[PER];
-        cplx<R> c[PER];
-        bool ok[PER], hre[PER], him[PER];
-#pragma unroll
-        for (int k = 0; k < PER; ++k) {
+        // the bounds (delta_k of the bin, or F(x) for ptw), every bin's first
+        constexpr bool BND_FIRST = true;
+        auto bound = [&](int k) {
             const int q = tid + k * NT;
             const int pp = q % W;
-            ok[k] = q < L * W && ct * W + pp < a.ncols;
             const size_t idx = (size_t)(q / W) * a.cs_n + obase + pp;
-            bnd[k] = mk(R(0), R(0));
-            if (ok[k]) bnd[k] = ptw ? a.orig[idx] : mk(a.delta_dense[idx] * a.dscale, R(0));
+            if (k < PER && q < L * W && ct * W + pp < a.ncols)
+                return ptw ? a.orig[idx] : mk(a.delta_dense[idx] * a.dscale, R(0));
+            return mk(R(0), R(0));
+        };
+        cplx<R> bnd[BND_FIRST ? PER : 1];
+        if constexpr (BND_FIRST) {
+#pragma unroll
+            for (int k = 0; k < PER; ++k) bnd[k] = bound(k);
         }
         // the ptw constants, as ptw_clamp forms them (projection_algorithm.cu:199-258)
         const R sp = sqrt(R(1) + a.delta_ptw), sm = sqrt(R(1) - a.delta_ptw);
         const R dptw = fmin(R(1) - sm, sp - R(1)) / (R)sqrtf(2.0f);
         bool fired = false;
+        // then rounds of CB bins: the round's accumulator loads go out
+        // together, ahead of its stores, and few values stay live
+        constexpr int CB = sizeof(R) == 8 ? 2 : 4;
 #pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            const int q = tid + k * NT;
-            const cplx<R> v = ok[k] ? buf[q] : mk(R(0), R(0));
-            const size_t idx = (size_t)(q / W) * a.cs_n + obase + q % W;
-            hre[k] = him[k] = false;
-            c[k] = v;
-            ed_re[k] = ed_im[k] = R(0);
-            if (!ok[k]) continue;
-            R d = bnd[k].x;
-            bool active = true;
-            if (ptw) {
-                if (idx == 0) {
-                    const R lo = bnd[k].x * (R(1) / sp - R(1)), hi = bnd[k].x * (R(1) / sm - R(1));
-                    hre[k] = v.x > hi || v.x < lo;
-                    ed_re[k] = -v.x;
-                    c[k].x = hre[k] ? R(0) : v.x;
-                    fired |= hre[k];
-                    continue;
+        for (int k0 = 0; k0 < PER; k0 += CB) {
+            cplx<R> c[CB], ed[CB], old[CB], rb[CB];
+            bool hre[CB], him[CB];
+            if constexpr (!BND_FIRST) {
+#pragma unroll
+                for (int j = 0; j < CB; ++j) rb[j] = bound(k0 + j);
+            }
+#pragma unroll
+            for (int j = 0; j < CB; ++j) {
+                const int k = k0 + j;
+                const int q = tid + k * NT;
+                const bool ok = k < PER && q < L * W && ct * W + q % W < a.ncols;
+                const cplx<R> v = ok ? buf[q] : mk(R(0), R(0));
+                const size_t idx = (size_t)(q / W) * a.cs_n + obase + q % W;
+                hre[j] = him[j] = false;
+                c[j] = v;
+                ed[j] = mk(R(0), R(0));
+                old[j] = mk(R(0), R(0));
+                if (!ok) continue;
+                cplx<R> b;
+                if constexpr (BND_FIRST) b = bnd[k < PER ? k : 0];
+                else b = rb[j];
+                R d = b.x;
+                bool active = true;
+                if (ptw) {
+                    if (idx == 0) {
+                        const R lo = b.x * (R(1) / sp - R(1)), hi = b.x * (R(1) / sm - R(1));
+                        hre[j] = v.x > hi || v.x < lo;
+                        ed[j].x = -v.x;
+                        c[j].x = hre[j] ? R(0) : v.x;
+                        fired |= hre[j];
+                        if (hre[j]) old[j] = a.df[idx];
+                        continue;
+                    }
+                    const R omag = b.x * b.x + b.y * b.y;
+                    const R cr = b.x + v.x, ci = b.y + v.y;
+                    const R ratio = (cr * cr + ci * ci) / omag;
+                    active = !(ratio <= sp && ratio >= sm);
+                    d = dptw;
                 }
-                const R omag = bnd[k].x * bnd[k].x + bnd[k].y * bnd[k].y;
-                const R cr = bnd[k].x + v.x, ci = bnd[k].y + v.y;
-                const R ratio = (cr * cr + ci * ci) / omag;
-                active = !(ratio <= sp && ratio >= sm);
-                d = dptw;
+                const bool xh = v.x > d, xl = v.x < -d, yh = v.y > d, yl = v.y < -d;
+                hre[j] = active && (xh || xl);
+                him[j] = active && (yh || yl);
+                if (hre[j]) {
+                    c[j].x = xh ? d : -d;
+                    ed[j].x = c[j].x - v.x;
+                }
+                if (him[j]) {
+                    c[j].y = yh ? d : -d;
+                    ed[j].y = c[j].y - v.y;
+                }
+                fired |= hre[j] || him[j];
+                // the bin's (Re, Im) edit pair in one access: a tile row's
+                // bins are one contiguous segment of the accumulator
+                if (hre[j] || him[j]) old[j] = a.df[idx];
             }
-            const bool xh = v.x > d, xl = v.x < -d, yh = v.y > d, yl = v.y < -d;
-            hre[k] = active && (xh || xl);
-            him[k] = active && (yh || yl);
-            if (hre[k]) {
-                c[k].x = xh ? d : -d;
-                ed_re[k] = c[k].x - v.x;
-            }
-            if (him[k]) {
-                c[k].y = yh ? d : -d;
-                ed_im[k] = c[k].y - v.y;
-            }
-            fired |= hre[k] || him[k];
-        }
 #pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            const int q = tid + k * NT;
-            const size_t idx = (size_t)(q / W) * a.cs_n + obase + q % W;
-            if (hre[k]) old_re[k] = a.df[idx];
-            if (him[k]) old_im[k] = a.df[idx + a.F];
-        }
-#pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            const int q = tid + k * NT;
-            const size_t idx = (size_t)(q / W) * a.cs_n + obase + q % W;
-            if (hre[k]) a.df[idx] = old_re[k] + ed_re[k];
-            if (him[k]) a.df[idx + a.F] = old_im[k] + ed_im[k];
-            if (ok[k]) buf[q] = c[k];
+            for (int j = 0; j < CB; ++j) {
+                const int k = k0 + j;
+                const int q = tid + k * NT;
+                const bool ok = k < PER && q < L * W && ct * W + q % W < a.ncols;
+                const size_t idx = (size_t)(q / W) * a.cs_n + obase + q % W;
+                if (hre[j] || him[j])
+                    a.df[idx] = mk(hre[j] ? old[j].x + ed[j].x : old[j].x, him[j] ? old[j].y + ed[j].y : old[j].y);
+                if (ok) buf[q] = c[j];
+            }
         }
         if (fired) a.st->freq_clamped = 1u;
         __syncthreads();

@@ -160,25 +160,25 @@ struct SArgs {
     R delta;
     const R* delta_dense;
     R dscale;
-    R* df;
+    cplx<R>* df;          // frequency edits, one (Re, Im) pair per bin (the solver's working layout)
     const cplx<R>* orig;
     R delta_ptw;
     // certification
     double cdelta;
     const T* cdelta_dense;
     CertStats* stats;
-    const T* dfsrc;
+    const cplx<T>* dfsrc;  // frequency edits, one (Re, Im) pair per bin
     QuantDesc qf;
 };
 
 template <typename R>
-__device__ __forceinline__ void ptw_clamp(cplx<R>& v, size_t idx, size_t F, const cplx<R>* orig, R dptw, R* df) {
+__device__ __forceinline__ void ptw_clamp(cplx<R>& v, size_t idx, const cplx<R>* orig, R dptw, cplx<R>* df) {
     // projection_algorithm.cu:199-258
     if (idx == 0) {
         const R lo = orig[0].x * (R(1) / sqrt(R(1) + dptw) - R(1));
         const R hi = orig[0].x * (R(1) / sqrt(R(1) - dptw) - R(1));
         if (v.x > hi || v.x < lo) {
-            df[0] += -v.x;
+            df[0].x += -v.x;
             v.x = R(0);
         }
         return;
@@ -190,8 +190,8 @@ __device__ __forceinline__ void ptw_clamp(cplx<R>& v, size_t idx, size_t F, cons
     const R ratio = cmag / omag;
     if (ratio <= sqrt(R(1) + dptw) && ratio >= sqrt(R(1) - dptw)) return;
     const R d = fmin(R(1) - sqrt(R(1) - dptw), sqrt(R(1) + dptw) - R(1)) / (R)sqrtf(2.0f);
-    clamp_acc(v.x, d, &df[idx]);
-    clamp_acc(v.y, d, &df[idx + F]);
+    clamp_acc(v.x, d, &df[idx].x);
+    clamp_acc(v.y, d, &df[idx].y);
 }
 
 // The frequency projection of one half-spectrum bin for the pass's mode
@@ -201,14 +201,14 @@ template <typename R, typename T>
 __device__ __forceinline__ bool freq_clamp_bin(cplx<R>& v, size_t idx, const SArgs<R, T>& a) {
     const cplx<R> v0 = v;
     if (a.fmode == FM_ABS) {
-        clamp_acc(v.x, a.delta, &a.df[idx]);
-        clamp_acc(v.y, a.delta, &a.df[idx + a.F]);
+        clamp_acc(v.x, a.delta, &a.df[idx].x);
+        clamp_acc(v.y, a.delta, &a.df[idx].y);
     } else if (a.fmode == FM_DENSE) {
         const R d = a.delta_dense[idx] * a.dscale;
-        clamp_acc(v.x, d, &a.df[idx]);
-        clamp_acc(v.y, d, &a.df[idx + a.F]);
+        clamp_acc(v.x, d, &a.df[idx].x);
+        clamp_acc(v.y, d, &a.df[idx].y);
     } else if (a.fmode == FM_PTW) {
-        ptw_clamp(v, idx, a.F, a.orig, a.delta_ptw, a.df);
+        ptw_clamp(v, idx, a.orig, a.delta_ptw, a.df);
     }
     return v.x != v0.x || v.y != v0.y;
 }

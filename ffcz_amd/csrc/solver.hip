@@ -9,6 +9,17 @@
 
 namespace ffcz {
 
+// the reference's split layout [Re(F) | Im(F)] (cu:164,179) of the solver's
+// (Re, Im)-per-bin frequency edits
+template <typename T>
+__global__ void k_df_split(const cplx<T>* dfc, T* df, size_t F) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < F; i += (size_t)gridDim.x * blockDim.x) {
+        const cplx<T> v = dfc[i];
+        df[i] = v.x;
+        df[i + F] = v.y;
+    }
+}
+
 template <typename T>
 __global__ void k_zero(T* p, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -58,6 +69,7 @@ Solver<T>::Solver(size_t nx, size_t ny, size_t nz) {
         dalloc(&e0_, f_.N);
         dalloc(&ds_, f_.N);
         dalloc(&df_, 2 * f_.F);
+        dalloc(&dfc_, f_.F);
         dalloc(&spec_, f_.spec_elems());
         dalloc(&st_, 1);
         dalloc(&vpart_, f_.rows);
@@ -81,7 +93,7 @@ void Solver<T>::release() {
     if (s_) (void)hipStreamSynchronize(s_);
     void* ptrs[] = {x_, e_, e0_, ds_, df_, spec_, st_, vpart_, delta_dense_, orig_, partial_idx_, fpack_, spack_,
                     fcomp_, scomp_, fquant_, squant_, xscratch_, cert_real_, cert_spec_, cstats_, xglob_,
-                    cert_real2_, zlog_off_, zlog_v_, zlog_cnt_};
+                    cert_real2_, zlog_off_, zlog_v_, zlog_cnt_, dfc_};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (xhost_) (void)hipHostFree(xhost_);
@@ -105,6 +117,7 @@ void Solver<T>::release() {
     cert_spec_ = nullptr;
     cstats_ = nullptr;
     zlog_off_ = zlog_cnt_ = nullptr;
+    dfc_ = nullptr;
     zlog_v_ = nullptr;
     if (f64_) f64_->destroy();
     f64_.reset();
@@ -127,7 +140,9 @@ void Solver<T>::reset_edit_accumulators() {
     // df: only when it may hold edits; ds: deferred (materialize_ds, or the
     // next solve's first spatial clamp stores it whole)
     if (df_touched_) {
-        hipLaunchKernelGGL((k_zero<T>), dim3(gridn(2 * f_.F)), dim3(256), 0, s_, df_, 2 * f_.F);
+        hipLaunchKernelGGL((k_zero<T>), dim3(gridn(2 * f_.F)), dim3(256), 0, s_, reinterpret_cast<T*>(dfc_),
+                           2 * f_.F);
+        df_split_stale_ = true;
         FFCZ_HIP(hipGetLastError());
         df_touched_ = false;
     }
@@ -138,6 +153,15 @@ template <typename T>
 void Solver<T>::accumulators_clobbered() {
     materialize_ds();
     df_touched_ = true;
+    df_split_stale_ = true;
+}
+
+template <typename T>
+void Solver<T>::materialize_df_split(hipStream_t s) const {
+    if (!df_split_stale_) return;
+    hipLaunchKernelGGL((k_df_split<T>), dim3(gridn(f_.F)), dim3(256), 0, s ? s : s_, (const cplx<T>*)dfc_, df_, f_.F);
+    FFCZ_HIP(hipGetLastError());
+    df_split_stale_ = false;
 }
 
 template <typename T>
@@ -239,7 +263,7 @@ IterArgs<T> Solver<T>::iter_args() const {
     a.e_copy = e_src_ ? e0_ : nullptr;
     a.ds_fresh = ds_fresh_ ? 1 : 0;
     a.ds = ds_;
-    a.df = df_;
+    a.df = dfc_;
     a.spec = spec_;
     a.eps = (T)((double)eps_ * (1.0 - margin_s_));
     if (margin_s_ == 0.0) a.eps = eps_;
@@ -419,6 +443,7 @@ void Solver<T>::finish_solve(hipStream_t s, const DevState* known) {
     viol_ = h.last_viol;
     converged_ = h.converged != 0;
     df_touched_ = df_touched_ || h.freq_clamped != 0;
+    if (h.freq_clamped) df_split_stale_ = true;
     solved_ = true;
     extracted_ = false;
 }
@@ -484,6 +509,7 @@ void Solver<T>::extract() {
     QuantDesc qf = freq_quant(), qs = spatial_quant();
     qf.dminmax = qs.dminmax = nullptr;  // the kernels compute the range themselves
     if (df_touched_) {
+        materialize_df_split();
         extract_edits_async<T>(df_, 2 * f_.F, qf, fpack_, fcomp_, fquant_, xscratch_, &xglob_[0], nullptr, s_);
     } else {
         // no frequency clamp fired since df was cleared, so df holds +0 only:
@@ -510,6 +536,7 @@ void Solver<T>::extract_segments(std::vector<ExtractSeg<T>>& out, hipStream_t s)
     // its flag pack is cleared here on the batch stream
     const bool dfz = !df_touched_;
     if (dfz) extract_zeros_async(fpack_, 2 * f_.F, &xglob_[0], s);
+    else materialize_df_split(s);
     out.push_back(ExtractSeg<T>{df_, dfz ? 0 : 2 * f_.F, qf, (unsigned int*)fpack_, fcomp_, fquant_, &xglob_[0], 0u});
     out.push_back(ExtractSeg<T>{ds_, f_.N, qs, (unsigned int*)spack_, scomp_, squant_, &xglob_[1], 0u});
     extracted_ = false;  // until the results are taken over
@@ -613,7 +640,7 @@ CertArgs<T> Solver<T>::cert_args(bool quantized) const {
     c.nz = f_.nz;
     c.e0 = e0_;
     c.ds = ds_;
-    c.df = df_;
+    c.df = dfc_;
     if (quantized) {
         // the dequantisers read the extraction's range on the device
         c.qs = spatial_quant();

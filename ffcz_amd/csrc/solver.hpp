@@ -102,7 +102,8 @@ class Solver {
     void get_results(T* e);
     // the iterate (recovered first when the last solve ran the fused form)
     const T* d_e() { return materialize_e(), e_; }
-    const T* d_df() const { return df_; }
+    // the frequency edits in the reference's split layout [Re | Im]
+    const T* d_df() const { return materialize_df_split(), df_; }
     const T* d_ds() const { return materialize_ds(), ds_; }
     size_t N() const { return f_.N; }
     size_t F() const { return f_.F; }
@@ -184,7 +185,10 @@ class Solver {
     T* x_ = nullptr;
     T* e_ = nullptr;
     T* e0_ = nullptr;
-    T* df_ = nullptr;
+    T* df_ = nullptr;           // frequency edits, reference layout [Re | Im] (derived from dfc_)
+    cplx<T>* dfc_ = nullptr;    // frequency edits, one (Re, Im) pair per bin: what the passes update
+    mutable bool df_split_stale_ = false;  // df_ behind dfc_
+    void materialize_df_split(hipStream_t s = nullptr) const;
     T* ds_ = nullptr;
     cplx<T>* spec_ = nullptr;
     DevState* st_ = nullptr;
