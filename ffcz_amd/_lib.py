@@ -22,7 +22,7 @@ FFCZ_ERR_ZSTD = 6
 
 FFCZ_F32 = 0
 FFCZ_F64 = 1
-MODE_DIRECT, MODE_ABS, MODE_PTW = 0, 1, 2
+MODE_DIRECT, MODE_ABS, MODE_PTW, MODE_PARTIAL_PTW = 0, 1, 2, 3
 QUANT_REF16, QUANT_STEP = 1, 2
 FFT_VALU, FFT_BF16X3, FFT_FP16X3 = 0, 1, 2
 FFT_VARIANTS = {"valu-fp32": FFT_VALU, "mfma-bf16x3": FFT_BF16X3, "mfma-fp16x3": FFT_FP16X3}
@@ -72,6 +72,9 @@ SIGNATURES = {
     "ffcz_solve_all_abs_f64": [_h, _vp, _i, _d],
     "ffcz_solve_all_ptw_f32": [_h, _vp, _i, _f],
     "ffcz_solve_all_ptw_f64": [_h, _vp, _i, _d],
+    "ffcz_solve_partial_ptw_f32": [_h, _vp, _i, _f],
+    "ffcz_solve_partial_ptw_f64": [_h, _vp, _i, _d],
+    "ffcz_set_ptw_semantics": [_h, _i],
     "ffcz_solve_loop": [_h, _i, _vp, _i, _d],
     "ffcz_get_status": [_h, C.POINTER(_i), C.POINTER(_d), C.POINTER(_i)],
     "ffcz_extract": [_h],

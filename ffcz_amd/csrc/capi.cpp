@@ -333,12 +333,20 @@ INIT_PAIR(init_partial_ptw, init_partial_ptw(x, idx, delta_ptw, eps, K))
 INIT_PAIR(solve, solve(MODE_DIRECT, e0, max_iterations, tol))
 INIT_PAIR(solve_all_abs, solve(MODE_ABS, e0, max_iterations, tol))
 INIT_PAIR(solve_all_ptw, solve(MODE_PTW, e0, max_iterations, tol))
+INIT_PAIR(solve_partial_ptw, solve(MODE_PARTIAL_PTW, e0, max_iterations, tol))
 #undef INIT_ARGS
+
+int ffcz_set_ptw_semantics(ffcz_solver* h, int corrected) {
+    return guard([&] {
+        FFCZ_REQUIRE(corrected == 0 || corrected == 1, ERR_INVALID, "ptw semantics: 0 reference, 1 corrected");
+        with(h, [&](auto& s) { s.set_ptw_semantics(corrected == 1); });
+    });
+}
 
 int ffcz_solve_loop(ffcz_solver* h, int mode, const void* e0, int max_iterations, double tol) {
     return guard([&] {
         NEED(h);
-        FFCZ_REQUIRE(mode >= 0 && mode <= 2, ERR_INVALID, "unknown solve mode");
+        FFCZ_REQUIRE(mode >= 0 && mode <= 3, ERR_INVALID, "unknown solve mode");
         if (h->dtype == FFCZ_F32) h->f->solve_loop(mode, (const float*)e0, max_iterations, (float)tol);
         else h->d->solve_loop(mode, (const double*)e0, max_iterations, tol);
     });

@@ -755,12 +755,12 @@ std::vector<uint8_t> container_pack(Solver<T>& s, int zstd_level) {
     w.bytes(MAGIC, 4);
     w.put<uint32_t>(VERSION);
     w.put<uint32_t>(sizeof(T) == 4 ? 0u : 1u);
-    w.put<uint32_t>((uint32_t)s.init_mode());
+    w.put<uint32_t>((uint32_t)s.solve_mode());
     w.put<uint64_t>(s.nx());
     w.put<uint64_t>(s.ny());
     w.put<uint64_t>(s.nz());
     w.put<double>((double)s.eps());
-    w.put<double>(s.init_mode() == MODE_DIRECT ? std::nan("") : (double)s.delta());
+    w.put<double>(s.solve_mode() == MODE_DIRECT ? std::nan("") : (double)s.delta());
     w.put<uint32_t>((uint32_t)qf.kind);
     w.put<uint32_t>(4);
     w.put<double>(qf.kind == Q_STEP ? qf.step : 0.0);

@@ -735,6 +735,8 @@ void iteration(const Fft3d<R>& f, const IterArgs<R>& ia, hipStream_t s, EventLog
     x.df = ia.df;
     x.orig = ia.orig;
     x.delta_ptw = ia.delta_ptw;
+    x.ptw_mask = ia.ptw_mask;
+    x.ptw_corr = ia.ptw_corr;
     launch_strided(f, x, x_clamp_mode(f, ia.fmode), true, s);
     mark(2);
     for (size_t x0 = 0; x0 < f.nx; x0 += xs) {
@@ -820,6 +822,8 @@ void iteration_fused(const Fft3d<R>& f, const IterArgs<R>& ia, bool first, hipSt
     x.df = ia.df;
     x.orig = ia.orig;
     x.delta_ptw = ia.delta_ptw;
+    x.ptw_mask = ia.ptw_mask;
+    x.ptw_corr = ia.ptw_corr;
     launch_strided(f, x, x_clamp_mode(f, ia.fmode), true, s);
     mark(2);
     y.sign = +1;
@@ -931,6 +935,8 @@ void iteration_kernel(const Fft3d<R>& f, const IterArgs<R>& ia, hipStream_t s, i
         x.df = ia.df;
         x.orig = ia.orig;
         x.delta_ptw = ia.delta_ptw;
+        x.ptw_mask = ia.ptw_mask;
+        x.ptw_corr = ia.ptw_corr;
         launch_strided(f, x, x_clamp_mode(f, ia.fmode), true, s);
     } else {
         launch_zinv<R, R, Z_ITER>(f, z, s);
@@ -1137,6 +1143,8 @@ void batch_plan_build(BatchPlan<R>& bp, const std::vector<const Fft3d<R>*>& fiel
                     a.df = ia.df;
                     a.orig = ia.orig;
                     a.delta_ptw = ia.delta_ptw;
+                    a.ptw_mask = ia.ptw_mask;
+                    a.ptw_corr = ia.ptw_corr;
                     // one mode per launch: a field without frequency bounds still
                     // runs the clamp kernel, whose FM_NONE projection is a no-op
                     smode = S_CLAMP;

@@ -96,6 +96,9 @@ void compact_to_spec(const Fft3d<R>& f, const cplx<R>* compact, cplx<R>* spec, h
 
 // ---- iteration passes ------------------------------------------------------
 enum FreqMode : int { FM_ABS = 0, FM_DENSE = 1, FM_PTW = 2, FM_NONE = 3 };
+// Per-bin rules of the pointwise power-spectrum bound (FM_PTW): unconstrained,
+// the power-ratio rule, the DC rule.
+enum PtwRule : uint8_t { PTW_NONE = 0, PTW_RATIO = 1, PTW_DC = 2 };
 
 template <typename R>
 struct IterArgs {
@@ -115,6 +118,8 @@ struct IterArgs {
     R dscale;             // FM_DENSE bound multiplier (1 = the given bounds)
     const cplx<R>* orig;  // FM_PTW: F(x) compact layout [F]
     R delta_ptw;
+    const uint8_t* ptw_mask;  // FM_PTW: per-bin rule [F] (PtwRule), null = every bin (DC by the DC rule)
+    int ptw_corr;         // FM_PTW: corrected power-spectrum bound (not the reference's)
     R norm;               // T(1)/T(N)
     DevState* st;
     R* vpart;             // [rows] per-workgroup violation partials (zero-initialised)

@@ -705,6 +705,7 @@ __device__ __forceinline__ void strided_body(const SArgs<R, T>& a, uint32_t blk)
         // the ptw constants, as ptw_clamp forms them (projection_algorithm.cu:199-258)
         const R sp = sqrt(R(1) + a.delta_ptw), sm = sqrt(R(1) - a.delta_ptw);
         const R dptw = fmin(R(1) - sm, sp - R(1)) / (R)sqrtf(2.0f);
+        const R dcorr = fmin(R(1) - sm, sp - R(1)) / sqrt(R(2));
         bool fired = false;
         // then rounds of CB bins: the round's accumulator loads go out
         // together, ahead of its stores, and few values stay live
@@ -735,7 +736,10 @@ __device__ __forceinline__ void strided_body(const SArgs<R, T>& a, uint32_t blk)
                 R d = b.x;
                 bool active = true;
                 if (ptw) {
-                    if (idx == 0) {
+                    // as ptw_clamp (reference or corrected rule per bin)
+                    const int rule = ptw_rule<R>(idx, a.ptw_mask, a.ptw_corr);
+                    if (rule == PTW_NONE) continue;
+                    if (rule == PTW_DC) {
                         const R lo = b.x * (R(1) / sp - R(1)), hi = b.x * (R(1) / sm - R(1));
                         hre[j] = v.x > hi || v.x < lo;
                         ed[j].x = -v.x;
@@ -747,8 +751,13 @@ __device__ __forceinline__ void strided_body(const SArgs<R, T>& a, uint32_t blk)
                     const R omag = b.x * b.x + b.y * b.y;
                     const R cr = b.x + v.x, ci = b.y + v.y;
                     const R ratio = (cr * cr + ci * ci) / omag;
-                    active = !(ratio <= sp && ratio >= sm);
-                    d = dptw;
+                    if (a.ptw_corr) {
+                        active = !(ratio <= R(1) + a.delta_ptw && ratio >= R(1) - a.delta_ptw);
+                        d = dcorr * sqrt(omag);
+                    } else {
+                        active = !(ratio <= sp && ratio >= sm);
+                        d = dptw;
+                    }
                 }
                 const bool xh = v.x > d, xl = v.x < -d, yh = v.y > d, yl = v.y < -d;
                 hre[j] = active && (xh || xl);

@@ -163,6 +163,8 @@ struct SArgs {
     cplx<R>* df;          // frequency edits, one (Re, Im) pair per bin (the solver's working layout)
     const cplx<R>* orig;
     R delta_ptw;
+    const uint8_t* ptw_mask;  // FM_PTW: per-bin rule (PTW_*; null: every bin, DC by the DC rule)
+    int ptw_corr;             // FM_PTW: the corrected power-spectrum bound instead of the reference's
     // certification
     double cdelta;
     const T* cdelta_dense;
@@ -172,9 +174,24 @@ struct SArgs {
 };
 
 template <typename R>
-__device__ __forceinline__ void ptw_clamp(cplx<R>& v, size_t idx, const cplx<R>* orig, R dptw, cplx<R>* df) {
-    // projection_algorithm.cu:199-258
-    if (idx == 0) {
+__device__ __forceinline__ int ptw_rule(size_t idx, const uint8_t* mask, int corr) {
+    if (mask) return mask[idx];
+    return (idx == 0 && !corr) ? PTW_DC : PTW_RATIO;
+}
+
+// The pointwise bound on one bin.  Reference semantics (projection_algorithm.
+// cu:199-258): the DC rule zeroes the DC error outside [X0(1/sqrt(1+d)-1),
+// X0(1/sqrt(1-d)-1)]; the ratio rule tests |X+E|^2/|X|^2 against sqrt(1+-d)
+// and clamps Re, Im to the absolute d' = min(1-sqrt(1-d), sqrt(1+d)-1) /
+// sqrtf(2).  Corrected semantics (corr; D11, not the reference's): the power
+// ratio is tested against [1-d, 1+d] and Re, Im are clamped to d'|X| (sqrt(2)
+// in R), for every constrained bin, DC included.
+template <typename R>
+__device__ __forceinline__ void ptw_clamp(cplx<R>& v, size_t idx, const cplx<R>* orig, R dptw, cplx<R>* df,
+                                          const uint8_t* mask, int corr) {
+    const int rule = ptw_rule<R>(idx, mask, corr);
+    if (rule == PTW_NONE) return;
+    if (rule == PTW_DC) {
         const R lo = orig[0].x * (R(1) / sqrt(R(1) + dptw) - R(1));
         const R hi = orig[0].x * (R(1) / sqrt(R(1) - dptw) - R(1));
         if (v.x > hi || v.x < lo) {
@@ -188,8 +205,14 @@ __device__ __forceinline__ void ptw_clamp(cplx<R>& v, size_t idx, const cplx<R>*
     const R cr = o.x + v.x, ci = o.y + v.y;
     const R cmag = cr * cr + ci * ci;
     const R ratio = cmag / omag;
-    if (ratio <= sqrt(R(1) + dptw) && ratio >= sqrt(R(1) - dptw)) return;
-    const R d = fmin(R(1) - sqrt(R(1) - dptw), sqrt(R(1) + dptw) - R(1)) / (R)sqrtf(2.0f);
+    R d;
+    if (corr) {
+        if (ratio <= R(1) + dptw && ratio >= R(1) - dptw) return;
+        d = fmin(R(1) - sqrt(R(1) - dptw), sqrt(R(1) + dptw) - R(1)) / sqrt(R(2)) * sqrt(omag);
+    } else {
+        if (ratio <= sqrt(R(1) + dptw) && ratio >= sqrt(R(1) - dptw)) return;
+        d = fmin(R(1) - sqrt(R(1) - dptw), sqrt(R(1) + dptw) - R(1)) / (R)sqrtf(2.0f);
+    }
     clamp_acc(v.x, d, &df[idx].x);
     clamp_acc(v.y, d, &df[idx].y);
 }
@@ -208,7 +231,7 @@ __device__ __forceinline__ bool freq_clamp_bin(cplx<R>& v, size_t idx, const SAr
         clamp_acc(v.x, d, &a.df[idx].x);
         clamp_acc(v.y, d, &a.df[idx].y);
     } else if (a.fmode == FM_PTW) {
-        ptw_clamp(v, idx, a.orig, a.delta_ptw, a.df);
+        ptw_clamp(v, idx, a.orig, a.delta_ptw, a.df, a.ptw_mask, a.ptw_corr);
     }
     return v.x != v0.x || v.y != v0.y;
 }

@@ -93,7 +93,7 @@ void Solver<T>::release() {
     if (s_) (void)hipStreamSynchronize(s_);
     void* ptrs[] = {x_, e_, e0_, ds_, df_, spec_, st_, vpart_, delta_dense_, orig_, partial_idx_, fpack_, spack_,
                     fcomp_, scomp_, fquant_, squant_, xscratch_, cert_real_, cert_spec_, cstats_, xglob_,
-                    cert_real2_, zlog_off_, zlog_v_, zlog_cnt_, dfc_};
+                    cert_real2_, zlog_off_, zlog_v_, zlog_cnt_, dfc_, ptw_mask_};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (xhost_) (void)hipHostFree(xhost_);
@@ -108,6 +108,8 @@ void Solver<T>::release() {
     delta_dense_ = nullptr;
     orig_ = nullptr;
     partial_idx_ = nullptr;
+    ptw_mask_ = nullptr;
+    ptw_mask_corr_ = -1;
     fpack_ = spack_ = nullptr;
     fcomp_ = scomp_ = nullptr;
     fquant_ = squant_ = nullptr;
@@ -179,6 +181,7 @@ void Solver<T>::init_all_abs(const T* x, T eps, T delta) {
     eps_ = eps;
     delta_ = delta;
     init_mode_ = MODE_ABS;
+    solve_mode_ = -1;
     initialized_ = true;
     FFCZ_HIP(hipStreamSynchronize(s_));
 }
@@ -221,6 +224,7 @@ void Solver<T>::init_direct(const T* x, const size_t* idx, const T* dk, T eps, s
         FFCZ_REQUIRE(hbad == 0, ERR_INVALID, "frequency constraint index >= F or negative/NaN bound");
     }
     init_mode_ = MODE_DIRECT;
+    solve_mode_ = -1;
     initialized_ = true;
     FFCZ_HIP(hipStreamSynchronize(s_));
 }
@@ -235,24 +239,55 @@ void Solver<T>::init_all_ptw(const T* x, T eps, T delta_ptw) {
     fft_r2c(f_, x_, spec_, s_);
     spec_to_compact(f_, spec_, orig_, s_);
     init_mode_ = MODE_PTW;
+    solve_mode_ = -1;
     initialized_ = true;
     FFCZ_HIP(hipStreamSynchronize(s_));
 }
 
 template <typename T>
 void Solver<T>::init_partial_ptw(const T* x, const size_t* idx, T delta_ptw, T eps, size_t K) {
-    // projection_algorithm_impl.cuh:701-738: stores F(x) and the index list;
-    // the reference has no solve that consumes them.
+    // projection_algorithm_impl.cuh:701-738: stores F(x) and the index list,
+    // consumed by the partial ptw projection (projection_algorithm.cu:261-333).
+    std::vector<size_t> h(K);
+    if (K) {
+        FFCZ_REQUIRE(idx != nullptr, ERR_INVALID, "null index array");
+        FFCZ_HIP(hipMemcpy(h.data(), idx, K * sizeof(size_t), hipMemcpyDefault));
+        for (size_t j = 0; j < K; ++j)
+            FFCZ_REQUIRE(h[j] < f_.F, ERR_INVALID, "frequency index out of range (>= nx*ny*(nz/2+1))");
+    }
     init_all_ptw(x, eps, delta_ptw);
+    init_mode_ = MODE_PARTIAL_PTW;
+    solve_mode_ = -1;
     if (partial_idx_) (void)hipFree(partial_idx_);
     partial_idx_ = nullptr;
     partial_K_ = K;
+    ptw_mask_corr_ = -1;
     if (K) {
-        FFCZ_REQUIRE(idx != nullptr, ERR_INVALID, "null index array");
         dalloc(&partial_idx_, K);
-        FFCZ_HIP(hipMemcpyAsync(partial_idx_, idx, K * sizeof(size_t), hipMemcpyDefault, s_));
+        FFCZ_HIP(hipMemcpyAsync(partial_idx_, h.data(), K * sizeof(size_t), hipMemcpyHostToDevice, s_));
         FFCZ_HIP(hipStreamSynchronize(s_));
     }
+}
+
+// The per-bin rule of the partial ptw projection.  Reference semantics
+// (projection_algorithm.cu:261-333, one thread per constraint): constraint 0
+// takes the DC rule on bin 0 whatever idx[0] holds, constraint j >= 1 the
+// ratio rule on bin idx[j] (a later entry naming bin 0 is left to the DC rule:
+// in the reference both would race on bin 0).  Corrected semantics: the
+// listed bins take the ratio rule.
+template <typename T>
+void Solver<T>::build_ptw_mask() {
+    if (ptw_mask_corr_ == (ptw_corr_ ? 1 : 0)) return;
+    std::vector<size_t> h(partial_K_);
+    if (partial_K_)
+        FFCZ_HIP(hipMemcpy(h.data(), partial_idx_, partial_K_ * sizeof(size_t), hipMemcpyDeviceToHost));
+    std::vector<uint8_t> m(f_.F, PTW_NONE);
+    for (size_t j = 0; j < partial_K_; ++j) m[h[j]] = PTW_RATIO;
+    if (!ptw_corr_ && partial_K_) m[0] = PTW_DC;
+    if (!ptw_mask_) dalloc(&ptw_mask_, f_.F);
+    FFCZ_HIP(hipMemcpyAsync(ptw_mask_, m.data(), f_.F, hipMemcpyHostToDevice, s_));
+    FFCZ_HIP(hipStreamSynchronize(s_));
+    ptw_mask_corr_ = ptw_corr_ ? 1 : 0;
 }
 
 template <typename T>
@@ -267,7 +302,10 @@ IterArgs<T> Solver<T>::iter_args() const {
     a.spec = spec_;
     a.eps = (T)((double)eps_ * (1.0 - margin_s_));
     if (margin_s_ == 0.0) a.eps = eps_;
-    a.fmode = init_mode_ == MODE_ABS ? FM_ABS : (init_mode_ == MODE_DIRECT ? FM_DENSE : FM_PTW);
+    const int mode = solve_mode();
+    a.fmode = mode == MODE_ABS ? FM_ABS : (mode == MODE_DIRECT ? FM_DENSE : FM_PTW);
+    a.ptw_mask = mode == MODE_PARTIAL_PTW ? ptw_mask_ : nullptr;
+    a.ptw_corr = ptw_corr_ ? 1 : 0;
     a.delta = margin_f_ == 0.0 ? delta_ : (T)((double)delta_ * (1.0 - margin_f_));
     a.dscale = margin_f_ == 0.0 ? T(1) : (T)(1.0 - margin_f_);
     a.delta_dense = delta_dense_;
@@ -308,10 +346,14 @@ void Solver<T>::materialize_e() {
 template <typename T>
 void Solver<T>::prepare_solve(int mode, const T* e0, int max_iter, T tol, hipStream_t s, bool reset) {
     FFCZ_REQUIRE(initialized_, ERR_STATE, "Solver not initialized");  // impl:778-781
-    FFCZ_REQUIRE(mode == init_mode_, ERR_STATE,
+    // solve_all_ptw also runs after initialize_partial_ptw, which computes
+    // everything it needs (projection_algorithm_impl.cuh:701-738, 860-902)
+    FFCZ_REQUIRE(mode == init_mode_ || (mode == MODE_PTW && init_mode_ == MODE_PARTIAL_PTW), ERR_STATE,
                  "solve mode does not match the initialize* call (direct<->initialize, abs<->initialize_all_abs, "
-                 "ptw<->initialize_all_ptw)");
+                 "ptw<->initialize_all_ptw or initialize_partial_ptw, partial ptw<->initialize_partial_ptw)");
     FFCZ_REQUIRE(e0 != nullptr, ERR_INVALID, "null spatial error");
+    solve_mode_ = mode;
+    if (mode == MODE_PARTIAL_PTW) build_ptw_mask();
     // e0_ keeps the initial error for certification.  The first iteration
     // reads the initial error directly (its inverse pass writes the iterate
     // e_); a device-resident e0 is copied into e0_ by that same pass.

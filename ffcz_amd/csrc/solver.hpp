@@ -10,7 +10,9 @@
 
 namespace ffcz {
 
-enum SolveMode : int { MODE_DIRECT = 0, MODE_ABS = 1, MODE_PTW = 2 };
+// MODE_PARTIAL_PTW: the pointwise power-spectrum bound on the bins listed by
+// initialize_partial_ptw only (the reference's partial ptw kernel).
+enum SolveMode : int { MODE_DIRECT = 0, MODE_ABS = 1, MODE_PTW = 2, MODE_PARTIAL_PTW = 3 };
 
 // Tile variants (0/1) of the iteration's z forward, z inverse, y and x pass
 // kernels and of the FP64 certification's z, y and x passes.
@@ -41,6 +43,11 @@ class Solver {
     void init_all_abs(const T* x, T eps, T delta);
     void init_all_ptw(const T* x, T eps, T delta_ptw);
     void init_partial_ptw(const T* x, const size_t* idx, T delta_ptw, T eps, size_t K);
+    // Which pointwise bound the ptw modes apply: the reference's arithmetic
+    // (default) or the corrected one (power ratio within [1-d, 1+d], Re and Im
+    // clamped relative to |X|, DC held to the same rule).
+    void set_ptw_semantics(bool corrected) { ptw_corr_ = corrected; }
+    bool ptw_corrected() const { return ptw_corr_; }
 
     // Projection loop only (impl:791-808 / 835-852 / 879-896).
     void solve_loop(int mode, const T* e0, int max_iter, T tol);
@@ -112,8 +119,10 @@ class Solver {
     size_t nz() const { return f_.nz; }
     int iterations() const { return iters_; }
     int init_mode() const { return init_mode_; }
+    // the mode of the last solve (the initialize* call's before any)
+    int solve_mode() const { return solve_mode_ >= 0 ? solve_mode_ : init_mode_; }
     T eps() const { return eps_; }
-    T delta() const { return init_mode_ == MODE_PTW ? delta_ptw_ : delta_; }
+    T delta() const { return init_mode_ == MODE_PTW || init_mode_ == MODE_PARTIAL_PTW ? delta_ptw_ : delta_; }
     double violation() const { return viol_; }
     bool converged() const { return converged_; }
     bool extracted() const { return extracted_; }
@@ -197,6 +206,12 @@ class Solver {
     cplx<T>* orig_ = nullptr;    // F(x), compact layout (ptw modes)
     size_t* partial_idx_ = nullptr;
     size_t partial_K_ = 0;
+    // per-bin PtwRule of the partial ptw mode for the semantics ptw_mask_corr_
+    // (built at the first solve that needs it)
+    uint8_t* ptw_mask_ = nullptr;
+    int ptw_mask_corr_ = -1;
+    bool ptw_corr_ = false;
+    void build_ptw_mask();
     // extraction outputs
     uint8_t* fpack_ = nullptr;
     uint8_t* spack_ = nullptr;
@@ -220,6 +235,7 @@ class Solver {
     // parameters
     bool initialized_ = false;
     int init_mode_ = -1;
+    int solve_mode_ = -1;  // the running (or last) solve's mode; -1 before the first solve after an initialize*
     T eps_ = 0, delta_ = 0, delta_ptw_ = 0;
     double margin_s_ = 0, margin_f_ = 0;
     int qkind_ = Q_REF16;

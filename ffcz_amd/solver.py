@@ -18,7 +18,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import (FFCZ_F32, FFCZ_F64, FFT_BF16X3, FFT_FP16X3, FFT_VALU, FFT_VARIANTS, MODE_ABS, MODE_DIRECT,  # noqa: F401
-                   MODE_PTW, QUANT_REF16, QUANT_STEP, call)
+                   MODE_PARTIAL_PTW, MODE_PTW, QUANT_REF16, QUANT_STEP, call)
 
 
 def _np_dtype(dtype) -> np.dtype:
@@ -152,6 +152,15 @@ class ProjectionSolver:
         e, ke = self._field(spatial_error, "spatial_error")
         call(f"ffcz_solve_all_ptw_{self._sfx}", self._h, e, int(max_iterations), self._scalar(convergence_tolerance))
 
+    def solve_partial_ptw(self, spatial_error, max_iterations, convergence_tolerance):
+        """The solve loop with the partial ptw projection over the bins given to
+        initialize_partial_ptw (projection_algorithm_impl.cuh:303-333,
+        projection_algorithm.cu:261-333; the reference stores the list but has
+        no solve that launches it)."""
+        e, ke = self._field(spatial_error, "spatial_error")
+        call(f"ffcz_solve_partial_ptw_{self._sfx}", self._h, e, int(max_iterations),
+             self._scalar(convergence_tolerance))
+
     def solve_loop(self, mode: int, spatial_error, max_iterations, convergence_tolerance):
         """The projection loop without the trailing extraction."""
         e, ke = self._field(spatial_error, "spatial_error")
@@ -240,6 +249,13 @@ class ProjectionSolver:
     # -- additions: certified output ----------------------------------------
     def set_margins(self, spatial: float, freq: float):
         call("ffcz_set_margins", self._h, float(spatial), float(freq))
+
+    def set_ptw_semantics(self, corrected: bool):
+        """The bound of the ptw solves: the reference's arithmetic (False,
+        projection_algorithm.cu:199-258) or the corrected one (True): the power
+        ratio |X+E|^2/|X|^2 held within [1-d, 1+d] on every constrained bin, DC
+        included, with Re and Im clamped relative to |X|."""
+        call("ffcz_set_ptw_semantics", self._h, 1 if corrected else 0)
 
     def set_fft_precision(self, variant):
         """FFT variant of the iteration: FFT_VALU, FFT_BF16X3, FFT_FP16X3 or
@@ -336,5 +352,5 @@ def _is_tensor(a) -> bool:
         return False
 
 
-__all__ = ["ProjectionSolver", "CertResult", "as_pointer", "MODE_ABS", "MODE_DIRECT", "MODE_PTW",
+__all__ = ["ProjectionSolver", "CertResult", "as_pointer", "MODE_ABS", "MODE_DIRECT", "MODE_PTW", "MODE_PARTIAL_PTW",
            "QUANT_REF16", "QUANT_STEP"]

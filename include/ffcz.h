@@ -43,6 +43,7 @@ extern "C" {
 #define FFCZ_MODE_DIRECT 0 /* solve()         projection_algorithm_impl.cuh:772-814 */
 #define FFCZ_MODE_ABS 1    /* solve_all_abs() projection_algorithm_impl.cuh:816-858 */
 #define FFCZ_MODE_PTW 2    /* solve_all_ptw() projection_algorithm_impl.cuh:860-902 */
+#define FFCZ_MODE_PARTIAL_PTW 3 /* project_onto_frequency_polytope_partial_ptw() impl:303-333 in the loop */
 
 /* edit quantisers */
 #define FFCZ_QUANT_REF16 1 /* projection_algorithm.cu:447-455 (default) */
@@ -92,6 +93,13 @@ int ffcz_solve_all_abs_f32(ffcz_solver* h, const float* e0, int max_iterations, 
 int ffcz_solve_all_abs_f64(ffcz_solver* h, const double* e0, int max_iterations, double tol);
 int ffcz_solve_all_ptw_f32(ffcz_solver* h, const float* e0, int max_iterations, float tol);
 int ffcz_solve_all_ptw_f64(ffcz_solver* h, const double* e0, int max_iterations, double tol);
+/* the same loop with the partial ptw projection over initialize_partial_ptw's
+ * index list (projection_algorithm_impl.cuh:303-333, projection_algorithm.cu:
+ * 261-333); the reference stores the list (impl:701-738) but has no solve
+ * that launches it.  Constraint 0 is the DC rule on bin 0, constraint j >= 1
+ * the ratio rule on bin idx[j]. */
+int ffcz_solve_partial_ptw_f32(ffcz_solver* h, const float* e0, int max_iterations, float tol);
+int ffcz_solve_partial_ptw_f64(ffcz_solver* h, const double* e0, int max_iterations, double tol);
 /* the loop alone (impl:791-808 / 835-852 / 879-896); e0 has the solver dtype */
 int ffcz_solve_loop(ffcz_solver* h, int mode, const void* e0, int max_iterations, double tol);
 /* iteration count and last violation: private results_ in the reference
@@ -128,6 +136,11 @@ int ffcz_get_edit_range(ffcz_solver* h, double* fmin, double* fmax, double* smin
  * loop bounds shrunk by (1-margin) for certified output; quantiser choice;
  * FP64 certification of x_hat = x' + edits (quantized=1: dequantised edits). */
 int ffcz_set_margins(ffcz_solver* h, double spatial_margin, double freq_margin);
+/* the bound the ptw modes apply: 0 the reference's arithmetic (default;
+ * projection_algorithm.cu:199-258), 1 the corrected bound: power ratio
+ * |X+E|^2/|X|^2 within [1-d, 1+d] on every constrained bin, DC included,
+ * Re and Im clamped to +-d'|X| with d' = min(1-sqrt(1-d), sqrt(1+d)-1)/sqrt(2) */
+int ffcz_set_ptw_semantics(ffcz_solver* h, int corrected);
 int ffcz_set_quantizer(ffcz_solver* h, int kind, double freq_step, double spatial_step);
 int ffcz_certify(ffcz_solver* h, int quantized, ffcz_cert* out);
 /* both certifications after extraction, out[0] unquantised and out[1]

@@ -175,6 +175,74 @@ def project_frequency_all_ptw(E: np.ndarray, orig: np.ndarray, delta_ptw, df: np
     Ef.imag = im
 
 
+def project_frequency_partial_ptw(E: np.ndarray, orig: np.ndarray, idx, delta_ptw, df: np.ndarray) -> None:
+    """project_frequency_constraints_partial_ptw, projection_algorithm.cu:261-333,
+    which the reference's projection_algorithm_impl.cuh:303-333 launches over
+    initialize_partial_ptw's index list (impl:701-738).  Constraint 0 is the
+    DC rule of cu:204-216 on bin 0 whatever idx[0] holds (cu:276-289); every
+    later constraint j applies the ratio rule of cu:219-258 to bin idx[j].  A
+    bin listed twice is projected once; a later entry naming bin 0 is left to
+    the DC rule (in the reference both would race on bin 0)."""
+    T = df.dtype.type
+    F = E.size
+    Ef = E.reshape(-1)
+    idx = np.asarray(idx, dtype=np.int64).reshape(-1)
+    if idx.size == 0:
+        return
+    sel = np.unique(idx[1:])
+    sel = sel[sel != 0]
+    # the all-ptw rule on the selected bins, and the DC rule on bin 0
+    sub = np.concatenate([[0], sel])
+    Es = Ef[sub].copy()
+    dfs = np.concatenate([df[:F][sub], df[F:][sub]])
+    project_frequency_all_ptw(Es, np.asarray(orig).reshape(-1)[sub], delta_ptw, dfs)
+    n = sub.size
+    Ef[sub] = Es
+    df[:F][sub] = dfs[:n]
+    df[F:][sub] = dfs[n:]
+    del T
+
+
+def project_frequency_ptw_corrected(E: np.ndarray, orig: np.ndarray, delta_ptw, df: np.ndarray, mask=None) -> None:
+    """The pointwise power-spectrum bound with the defects of cu:199-258 (D11)
+    corrected; not the reference's arithmetic.  Every constrained bin,
+    DC included, must keep its power ratio |X+E|^2/|X|^2 within [1-d, 1+d]
+    (the reference tests the squared ratio against sqrt(1 +- d)).  Outside
+    it, Re and Im of E are clamped to +-d'|X| with
+    d' = min(1 - sqrt(1-d), sqrt(1+d) - 1) / sqrt(2) (the reference clamps to
+    an absolute d'), which puts |X+E'| within [sqrt(1-d), sqrt(1+d)]|X|.
+    mask: the constrained bins (None: all); edits accumulate new - old."""
+    T = df.dtype.type
+    F = E.size
+    Ef = E.reshape(-1)
+    Of = np.asarray(orig).reshape(-1)
+    dp = T(delta_ptw)
+    one = T(1)
+    re = Ef.real.copy()
+    im = Ef.imag.copy()
+    omag = (Of.real * Of.real + Of.imag * Of.imag).astype(df.dtype)
+    cr = Of.real + re
+    ci = Of.imag + im
+    cmag = cr * cr + ci * ci
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ratio = cmag / omag
+    inside = (ratio <= one + dp) & (ratio >= one - dp)
+    dl = T(min(one - np.sqrt(one - dp), np.sqrt(one + dp) - one) / np.sqrt(T(2)))
+    lvl = (dl * np.sqrt(omag)).astype(df.dtype)
+    act = ~inside
+    if mask is not None:
+        act &= np.asarray(mask, bool).reshape(-1)
+    sub_re, sub_im = re[act], im[act]
+    acc_re, acc_im = df[:F][act], df[F:][act]
+    lv = lvl[act]
+    _clamp_component(sub_re, acc_re, -lv, lv)
+    _clamp_component(sub_im, acc_im, -lv, lv)
+    re[act], im[act] = sub_re, sub_im
+    df[:F][act], df[F:][act] = acc_re, acc_im
+    Ef.real = re
+    Ef.imag = im
+
+
 def project_spatial(e: np.ndarray, eps, ds: np.ndarray) -> None:
     """projection_algorithm.cu:335-361 (clamp e to [-eps, eps], ds += delta)."""
     T = ds.dtype.type
@@ -205,7 +273,7 @@ class SolveResult:
 
 def solve(x: np.ndarray, e0: np.ndarray, eps, *, mode: str = "abs", delta=None,
           idx=None, delta_k=None, delta_ptw=None, max_iter: int = 100, tol=1e-7,
-          dtype=np.float64, df=None, ds=None, workers: int = 1) -> SolveResult:
+          dtype=np.float64, df=None, ds=None, workers: int = 1, ptw_corrected: bool = False) -> SolveResult:
     """Loop of ``solve_all_abs`` / ``solve`` / ``solve_all_ptw``
     (projection_algorithm_impl.cuh:835-852, 791-808, 879-896):
 
@@ -226,7 +294,11 @@ def solve(x: np.ndarray, e0: np.ndarray, eps, *, mode: str = "abs", delta=None,
     df = np.zeros(2 * F, dtype=dtype) if df is None else df
     ds = np.zeros(N, dtype=dtype) if ds is None else ds
     norm = T(1) / T(N)  # impl:242
-    orig = rfft3(x.astype(dtype), workers) if mode == "ptw" else None
+    orig = rfft3(x.astype(dtype), workers) if mode in ("ptw", "partial_ptw") else None
+    pmask = None
+    if mode == "partial_ptw" and ptw_corrected:
+        pmask = np.zeros(F, bool)
+        pmask[np.asarray(idx, dtype=np.int64)] = True
     dense = dense_bounds(F, idx, delta_k, dtype) if mode == "direct" else None
     it = 0
     v = T(0)
@@ -237,8 +309,14 @@ def solve(x: np.ndarray, e0: np.ndarray, eps, *, mode: str = "abs", delta=None,
             project_frequency_all_abs(E, delta, df)
         elif mode == "direct":
             project_frequency_direct(E, idx, delta_k, df, dense=dense)
-        elif mode == "ptw":
+        elif mode == "ptw" and not ptw_corrected:
             project_frequency_all_ptw(E, orig, delta_ptw, df)
+        elif mode == "ptw":
+            project_frequency_ptw_corrected(E, orig, delta_ptw, df)
+        elif mode == "partial_ptw" and not ptw_corrected:
+            project_frequency_partial_ptw(E, orig, idx, delta_ptw, df)
+        elif mode == "partial_ptw":
+            project_frequency_ptw_corrected(E, orig, delta_ptw, df, mask=pmask)
         else:
             raise ValueError(mode)
         e = (irfft3_unnorm(E, shape, workers) * norm).astype(dtype, copy=False)

@@ -157,3 +157,97 @@ def test_ptw_dc_window(dtype, d):
         else:
             assert E[0] == e0T and df[0] == 0
         assert df[3] == 0  # the imaginary part of DC is never edited
+
+
+# ---------------------------------------------------------------------------
+# corrected semantics (D11, opt-in) and the partial projection
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("d", [0.02, 0.1, 0.3])
+def test_ptw_corrected_keeps_every_power_ratio_in_the_band(dtype, d):
+    """The corrected rule is a projection onto a box inscribed in the annulus
+    sqrt(1-d)|X| <= |X+E| <= sqrt(1+d)|X|: after one application every bin's
+    power ratio |X+E|^2/|X|^2 lies in [1-d, 1+d] (checked in exact arithmetic
+    of the stored values), bins already inside are untouched, a clamped
+    component sits at +-d'|X| with d' = min(1-sqrt(1-d), sqrt(1+d)-1)/sqrt(2)
+    (40-digit decimal), and a bin of zero amplitude ends with zero edit."""
+    T = np.dtype(dtype).type
+    n = 20000
+    X, E = random_bins(n, d, dtype, seed=int(d * 1000) + 3)
+    X[5] = 0
+    E[5] = 0.3 + 0.1j  # zero amplitude: only E = 0 keeps |X+E| = |X|
+    E_in = E.copy()
+    df = np.zeros(2 * n, dtype)
+    O.project_frequency_ptw_corrected(E, X, T(d), df)
+    xr, xi = X.real.astype(np.float64), X.imag.astype(np.float64)
+    p0 = xr * xr + xi * xi
+    r0 = ((xr + E_in.real) ** 2 + (xi + E_in.imag) ** 2) / np.where(p0 > 0, p0, 1)
+    r1 = ((xr + E.real.astype(np.float64)) ** 2 + (xi + E.imag.astype(np.float64)) ** 2) / np.where(p0 > 0, p0, 1)
+    tol = 16 * np.finfo(dtype).eps
+    nz = p0 > 0
+    assert np.all(r1[nz] <= 1 + d + tol) and np.all(r1[nz] >= 1 - d - tol)
+    assert E[5] == 0 and df[5] == -np.float64(E_in[5].real) and df[n + 5] == -np.float64(E_in[5].imag)
+    inside = nz & (r0 < (1 + d) * (1 - 1e-4)) & (r0 > (1 - d) * (1 + 1e-4))
+    assert inside.sum() > 2000 and np.array_equal(E[inside], E_in[inside])
+    dl = float(clamp_level(d, SQRT2))
+    amp = np.sqrt(p0)
+    outside = nz & ((r0 > (1 + d) * (1 + 1e-4)) | (r0 < (1 - d) * (1 - 1e-4)))
+    assert outside.sum() > 2000
+    for new, old, acc in ((E.real, E_in.real, df[:n]), (E.imag, E_in.imag, df[n:])):
+        lv = dl * amp
+        fired = outside & (np.abs(old) > lv * (1 + 1e-4))
+        kept = outside & (np.abs(old) < lv * (1 - 1e-4))
+        assert fired.sum() > 100 and kept.sum() > 100
+        # d' formed in T loses bits to the cancellation in 1 - sqrt(1-d): a few
+        # ulps of 1 absolute, so relative to d' itself
+        rtol = 8 * np.finfo(dtype).eps * (1 + 1 / dl)
+        assert np.allclose(np.abs(new[fired]), lv[fired], rtol=rtol, atol=0)
+        assert np.array_equal(np.sign(new[fired]), np.sign(old[fired]))
+        assert np.array_equal(new[kept], old[kept])
+        assert np.array_equal(acc[fired], (new[fired] - old[fired]).astype(dtype))
+
+
+def test_partial_ptw_reference_dc_is_the_first_constraint():
+    """cu:276-289: constraint 0 runs the DC rule on bin 0 whatever idx[0]
+    names; later constraints run the ratio rule on their bins; unlisted bins
+    are untouched.  The listed bins end as the all-bins projection leaves
+    them, and bin idx[0] (listed nowhere else) is untouched."""
+    rng = np.random.default_rng(11)
+    n, d = 4000, 0.1
+    X, E = random_bins(n, d, np.float64, seed=17)
+    X[0] = 10.0 + 0j
+    E[0] = 0.9  # outside the DC window [-0.465, 0.541]: zeroed by the DC rule
+    idx = np.concatenate([[1234], rng.choice(np.arange(1, n), 1500, replace=False)])
+    idx = idx[idx != 1234]
+    idx = np.concatenate([[1234], idx])
+    E_in = E.copy()
+    df = np.zeros(2 * n)
+    O.project_frequency_partial_ptw(E, X, idx, d, df)
+    Ea = E_in.copy()
+    dfa = np.zeros(2 * n)
+    O.project_frequency_all_ptw(Ea, X, d, dfa)
+    listed = np.zeros(n, bool)
+    listed[idx[1:]] = True
+    assert E[0] == 0 and df[0] == -0.9  # the DC rule on bin 0
+    assert E[1234] == E_in[1234] and df[1234] == 0 and df[n + 1234] == 0
+    assert np.array_equal(E[listed], Ea[listed])
+    assert np.array_equal(df[:n][listed], dfa[:n][listed]) and np.array_equal(df[n:][listed], dfa[n:][listed])
+    rest = ~listed
+    rest[0] = False
+    assert np.array_equal(E[rest], E_in[rest]) and not np.any(df[:n][rest]) and not np.any(df[n:][rest])
+    assert np.any(E[listed] != E_in[listed])  # the ratio rule fired somewhere
+
+
+def test_partial_ptw_corrected_constrains_exactly_the_listed_bins():
+    d, n = 0.1, 3000
+    X, E = random_bins(n, d, np.float64, seed=29)
+    idx = np.random.default_rng(2).choice(n, 1000, replace=False)
+    E_in = E.copy()
+    df = np.zeros(2 * n)
+    mask = np.zeros(n, bool)
+    mask[idx] = True
+    O.project_frequency_ptw_corrected(E, X, d, df, mask=mask)
+    assert np.array_equal(E[~mask], E_in[~mask]) and not np.any(df[:n][~mask])
+    p0 = np.abs(X) ** 2
+    r1 = np.abs(X + E) ** 2 / p0
+    assert np.all(r1[mask] <= 1 + d + 1e-12) and np.all(r1[mask] >= 1 - d - 1e-12)
