@@ -80,24 +80,39 @@ def parse():
 
 
 def kernel_bytes(shape, s, fresh_frac=0.0, bound_bytes=0):
-    """Compulsory HBM bytes per launch of the five iteration kernels (the
-    accumulators are read/written only where a clamp fires).  The z forward
-    pass of the second iteration applies the first spatial clamp since the
-    accumulators were reset and stores the spatial edits whole (N*s bytes
-    more); fresh_frac is the share of the timed z forward launches that did.
-    bound_bytes is what the fused x pass reads per bin for its bound (s for
-    per-bin delta_k, 0 for one bound for all bins)."""
+    """Compulsory HBM bytes per launch of the iteration kernels (the
+    accumulators are read/written only where a clamp fires).
+
+    Five-pass form: the z forward pass of the second iteration applies the
+    first spatial clamp since the accumulators were reset and stores the
+    spatial edits whole (N*s bytes more).  Fused form (zz: inverse z, spatial
+    clamp and forward z in one pass over the spectrum rows, in place): the
+    first iteration's zz stores the spatial edits whole.  fresh_frac is the
+    share of the timed launches of that kernel that did.  bound_bytes is what
+    the fused x pass reads per bin for its bound (s for per-bin delta_k, 0 for
+    one bound for all bins)."""
     nx, ny, nz = shape
     N = nx * ny * nz
     F = nx * ny * (nz // 2 + 1)
     cs = 2 * s
+    fresh = int(round(fresh_frac * N * s))
     return {
-        "z_fwd": N * s + F * cs + int(round(fresh_frac * N * s)),
+        "z_fwd": N * s + F * cs + fresh,
         "y_fwd": 2 * F * cs,
         "x_fused": 2 * F * cs + F * bound_bytes,
         "y_inv": 2 * F * cs,
         "z_inv": F * cs + N * s,
+        "zz": 2 * F * cs + fresh,
     }
+
+
+def pass_names(counts):
+    """The iteration's launch kinds (0 z forward, 1 y forward, 2 x fused, 3 y
+    inverse, 4 z inverse or, in the fused form, zz) by the launches counted:
+    the fused form launches its z forward transform once per solve, before
+    the first iteration, and never inside the timed iterations."""
+    fused = counts[0] == 0 and counts[4] > 0
+    return ["z_fwd", "y_fwd", "x_fused", "y_inv", "zz" if fused else "z_inv"], fused
 
 
 def read_traffic(kernel: str, workload: str):
@@ -295,7 +310,6 @@ def bench_ptw(a, torch, dev):
 
     wl = make_config(a.config, device=dev, seed=a.seed)
     shape = tuple(wl.x.shape)
-    names = ["z_fwd", "y_fwd", "x_fused", "y_inv", "z_inv", "finalize"]
     out = {"ptw_iteration": {"workload": f"{a.config}: {'x'.join(map(str, shape))} "
                                          f"{'f32' if wl.x.dtype == torch.float32 else 'f64'}",
                              "delta_ptw": a.ptw, "iterations": 8}}
@@ -310,8 +324,9 @@ def bench_ptw(a, torch, dev):
             s.reset_edit_accumulators()
             s.solve_loop(mode, wl.e0, 8, -1.0)
             ms6, n = s.pass_timing()
+            names = pass_names(s.pass_counts())[0] + ["finalize"]
             s.set_pass_timing(False)
-        out["ptw_iteration"][label] = {"ms": {k: round(v, 5) for k, v in zip(names, ms6)},
+        out["ptw_iteration"][label] = {"ms": {k: round(v, 5) for k, v in zip(names, ms6) if v > 0},
                                        "iteration_ms": round(float(sum(ms6)), 4), "timed_iterations": n}
     print(json.dumps(out))
 
@@ -536,9 +551,10 @@ def main():
         del X, d
         return {k: float(f"{v:.3e}") for k, v in out.items()}
 
-    names = ["z_fwd", "y_fwd", "x_fused", "y_inv", "z_inv"]
     # per iteration: the method's count, four real arrays and four half spectra
-    # (independent of how many passes the FFT takes)
+    # (independent of how many passes the FFT takes; the fused form's own
+    # count, four passes over the half spectrum in place, is 8*F*s*2 bytes,
+    # within 0.5 % of it at the benchmark shapes)
     C = 2 * (shape[0] * shape[1] * (shape[2] // 2 + 1)) * s_in
     B_iter = 4 * N * s_in + 4 * C
 
@@ -568,7 +584,8 @@ def main():
             torch.cuda.synchronize()
             elapsed = time.perf_counter() - t0
         ms6, n_timed = solver.pass_timing()
-        n_zfwd = solver.pass_counts()[0]
+        counts = solver.pass_counts()
+        names, fused = pass_names(counts)
         solver.set_pass_timing(False)
         iters, viol, conv = solver.status()
         ok = all(c1.ok and c2.ok for c1, c2 in certs)
@@ -577,12 +594,15 @@ def main():
               f"certified={ok}; passes timed over {n_timed} iterations", file=sys.stderr, flush=True)
         ms, t_iter = ms6[:5], sum(ms6) * 1e-3
         dom = int(np.argmax(ms))
-        # every solve's second iteration is timed (only the first launch after
-        # each host check is left out) and its z forward pass stores ds whole
-        fresh = min(1.0, a.steps / n_zfwd) if (iters >= 2 and n_zfwd > 0) else 0.0
+        # the launch that stores ds whole is timed once per solve (only the
+        # first launch after each host check is left out): the z forward pass
+        # of the second iteration, or the fused form's first zz
+        n_st = counts[4] if fused else counts[0]
+        fresh = min(1.0, a.steps / n_st) if ((fused or iters >= 2) and n_st > 0) else 0.0
         return {
             "variant": variant, "value": value, "elapsed": elapsed, "iterations": iters, "converged": conv,
             "certified": ok, "ms": ms, "t_iter": t_iter, "dom": dom, "fft_error": fft_error(variant),
+            "names": names, "fused": fused,
             "fresh_frac": fresh,
             "tile_vars": tuned.get(variant),
             "cert_excess": {"spatial": max(max(c1.spatial_max_excess, c2.spatial_max_excess) for c1, c2 in certs),
@@ -595,7 +615,7 @@ def main():
     huff = None if a.no_huffman else huffman_bench(solver, torch, dev)
     cont = None if a.no_huffman else container_bench(solver, wl, torch)
     best = max(results, key=lambda r: (r["value"], r["variant"] == "valu-fp32"))
-    ms, t_iter, dom = best["ms"], best["t_iter"], best["dom"]
+    ms, t_iter, dom, names = best["ms"], best["t_iter"], best["dom"], best["names"]
     dname = names[dom]
     kb = kernel_bytes(shape, s_in, best["fresh_frac"], s_in if direct else 0)
     achieved = kb[dname] / (ms[dom] * 1e-3) / 1e9
@@ -603,14 +623,15 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": read_traffic(dname, a.config),
                 "bytes_per_launch": kb[dname], "ms_per_launch": round(ms[dom], 5)}
     # the roofline of the metric's unit of work: one projection iteration
-    # (the five passes and the violation reduction), B_iter algorithmic bytes
-    # over its measured time; traffic = the passes' counter bytes summed
-    pass_traffic = [read_traffic(n, a.config) for n in names]
+    # (its passes and the violation reduction), B_iter algorithmic bytes over
+    # its measured time; traffic = the counter bytes of the passes it runs
+    run = [n for n, m in zip(names, ms) if m > 0]
+    pass_traffic = [read_traffic(n, a.config) for n in run]
     it_achieved = B_iter / t_iter / 1e9
     roof = {"bound": "hbm", "achieved": round(it_achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": round(it_achieved / HBM_PEAK_GBPS, 4),
             "traffic": (sum(pass_traffic) if all(v is not None for v in pass_traffic) else None),
-            "kernel": "one projection iteration (z fwd, y fwd, x fused, y inv, z inv, violation reduction)",
+            "kernel": "one projection iteration (" + ", ".join(run) + ", violation reduction)",
             "fft": best["variant"], "bytes_per_launch": B_iter, "ms_per_launch": round(t_iter * 1e3, 5),
             "dominant_pass": dominant}
 
@@ -619,7 +640,7 @@ def main():
         return {"ms": round(r["t_iter"] * 1e3, 4), "B_iter": B_iter,
                 "achieved_GBps": round(B_iter / r["t_iter"] / 1e9, 1),
                 "frac": round(B_iter / r["t_iter"] / 1e9 / HBM_PEAK_GBPS, 4),
-                "kernels_ms": {n: round(m, 5) for n, m in zip(names, r["ms"])}}
+                "kernels_ms": {n: round(m, 5) for n, m in zip(r["names"], r["ms"]) if m > 0}}
 
     cpu = None
     if not a.no_cpu_baseline:

@@ -466,15 +466,20 @@ __device__ __forceinline__ void zz_body(const ZArgs<R, R>& a, uint32_t blk) {
     // The spatial clamp on the thread's pairs n = j + r L/R of its row (pair
     // n is elements 2n, 2n+1: consecutive threads, consecutive pairs, so ds
     // moves coalesced).  The first clamp since the accumulators were cleared
-    // stores ds whole; later ones read and write it only where they fire,
-    // every load issued before any store.
+    // stores ds whole; later ones add their edits where they fire, as atomic
+    // adds that return nothing (each element has one writer per pass, so the
+    // sum is the plain old + edit, and no register waits for the old value).
     const bool fresh = a.ds_fresh && a.st->iter == 0;
     R tolr;
     if constexpr (sizeof(R) == 4) tolr = a.st->tol_f;
     else tolr = (R)a.st->tol;
     R vm = R(0);
-    cplx<R> old[DL::NB][DL::RAD];
-    bool f0[DL::NB][DL::RAD], f1[DL::NB][DL::RAD], l0[DL::NB][DL::RAD], l1[DL::NB][DL::RAD];
+    // deferred clamps (excess below tol) of this thread, one bit per component
+    // (bit 2(b RAD + r) + c), so the log below writes exactly what it counts
+    using DMask = std::conditional_t<(2 * DL::NB * DL::RAD <= 32), uint32_t, uint64_t>;
+    static_assert(2 * DL::NB * DL::RAD <= 64, "deferred-clamp mask too narrow");
+    DMask dmask = 0;
+    auto fires = [&](bool ok, R u) { return ok && (u > eps || u < -eps); };
 #pragma unroll
     for (int b = 0; b < DL::NB; ++b) {
         const int j = t + b * TP;
@@ -485,26 +490,16 @@ __device__ __forceinline__ void zz_body(const ZArgs<R, R>& a, uint32_t blk) {
             v[b][r].y *= a.scale;
             const R x0 = fabs(v[b][r].x) - eps, x1 = fabs(v[b][r].y) - eps;
             if (ok) vm = fmax(vm, fmax(x0, x1));
-            const bool c0 = ok && (v[b][r].x > eps || v[b][r].x < -eps);
-            const bool c1 = ok && (v[b][r].y > eps || v[b][r].y < -eps);
-            // deferred (excess below tol) or applied now
-            l0[b][r] = c0 && x0 < tolr;
-            l1[b][r] = c1 && x1 < tolr;
-            f0[b][r] = c0 && !l0[b][r];
-            f1[b][r] = c1 && !l1[b][r];
-            old[b][r] = mk(R(0), R(0));
-            const uint32_t q = (uint32_t)(p * LZ + j + r * DL::NS);
-            if (!fresh && (f0[b][r] || f1[b][r])) old[b][r] = tile_at(dsp, q);
+            const bool l0 = fires(ok, v[b][r].x) && x0 < tolr, l1 = fires(ok, v[b][r].y) && x1 < tolr;
+            const int bit = 2 * (b * DL::RAD + r);
+            dmask |= (l0 ? DMask(1) << bit : DMask(0)) | (l1 ? DMask(2) << bit : DMask(0));
         }
     }
     // deferred clamps: counted per thread, one wave-uniform test, and only
-    // then slots (one reservation per wave) and log entries
-    uint32_t nl = 0u;
-#pragma unroll
-    for (int b = 0; b < DL::NB; ++b)
-#pragma unroll
-        for (int r = 0; r < DL::RAD; ++r) nl += (l0[b][r] ? 1u : 0u) + (l1[b][r] ? 1u : 0u);
-    if (__ballot(nl != 0u)) {
+    // then slots (one reservation per wave) and log entries of the unclamped
+    // values
+    if (__ballot(dmask != 0)) {
+        const uint32_t nl = (uint32_t)__popcll((unsigned long long)dmask);
         uint32_t slot = wave_log_base(nl, &nlog);
 #pragma unroll
         for (int b = 0; b < DL::NB; ++b) {
@@ -512,12 +507,13 @@ __device__ __forceinline__ void zz_body(const ZArgs<R, R>& a, uint32_t blk) {
 #pragma unroll
             for (int r = 0; r < DL::RAD; ++r) {
                 const uint32_t q = (uint32_t)(p * LZ + j + r * DL::NS);
-                if (l0[b][r]) {
+                const int bit = 2 * (b * DL::RAD + r);
+                if ((dmask >> bit) & 1u) {
                     a.log_off[seg + slot] = 2u * q;
                     a.log_v[seg + slot] = v[b][r].x;
                     ++slot;
                 }
-                if (l1[b][r]) {
+                if ((dmask >> (bit + 1)) & 1u) {
                     a.log_off[seg + slot] = 2u * q + 1u;
                     a.log_v[seg + slot] = v[b][r].y;
                     ++slot;
@@ -525,19 +521,25 @@ __device__ __forceinline__ void zz_body(const ZArgs<R, R>& a, uint32_t blk) {
             }
         }
     }
+    // the clamp itself (v clamped in place for the forward transform), with
+    // the edits of the clamps not deferred
 #pragma unroll
     for (int b = 0; b < DL::NB; ++b) {
         const int j = t + b * TP;
         const bool ok = rv && ((DL::LR % TP == 0) || j < DL::LR);
 #pragma unroll
         for (int r = 0; r < DL::RAD; ++r) {
+            const int bit = 2 * (b * DL::RAD + r);
+            const bool f0 = fires(ok, v[b][r].x) && !((dmask >> bit) & 1u);
+            const bool f1 = fires(ok, v[b][r].y) && !((dmask >> (bit + 1)) & 1u);
             const uint32_t q = (uint32_t)(p * LZ + j + r * DL::NS);
             const R e0 = clamp_edit(v[b][r].x, eps), e1 = clamp_edit(v[b][r].y, eps);
-            // the accumulation of clamp_acc: edit formed, then added
-            cplx<R> nd = old[b][r];
-            if (f0[b][r]) nd.x = old[b][r].x + e0;
-            if (f1[b][r]) nd.y = old[b][r].y + e1;
-            if (fresh ? ok : (f0[b][r] || f1[b][r])) tile_at(dsp, q) = nd;
+            if (fresh) {
+                if (ok) tile_at(dsp, q) = mk(f0 ? e0 : R(0), f1 ? e1 : R(0));
+            } else {
+                if (f0) atomicAdd(&tile_at(dse, 2u * q), e0);
+                if (f1) atomicAdd(&tile_at(dse, 2u * q + 1u), e1);
+            }
         }
     }
     block_max_store<R, NT>(vm, &a.vpart[gb]);
