@@ -677,116 +677,86 @@ __device__ __forceinline__ void strided_body(const SArgs<R, T>& a, uint32_t blk)
     } else if constexpr (MODE == S_CLAMP_PASS) {
         // Per-bin and ptw bounds: the forward transform ends in the tile, then
         // the clamp runs as a pass of its own over it.  Each thread issues all
-        // of its bins' bound loads (delta_k, or F(x) for ptw), then the
-        // accumulator loads of the bins that clamp, before any accumulator
-        // store: clamping in the last stage, each bin's loads waited for the
-        // previous bin's store to df, which the compiler cannot tell apart
-        // from the bounds.  The arithmetic is freq_clamp_bin's: the edit formed,
-        // then accumulated; ptw as ptw_clamp (the DC bin's real part zeroed).
+        // of its bins' bound loads (delta_k, or F(x) for ptw) first; the edits
+        // of the bins that clamp go to the accumulator as atomic adds that
+        // return nothing (one writer per bin and pass: the sum is the plain
+        // old + edit), so no load of the accumulator holds the pass up.  The
+        // arithmetic is freq_clamp_bin's: the edit formed, then accumulated;
+        // ptw as ptw_clamp (the DC bin's real part zeroed).
         static_assert(PREC == PREC_VALU, "the clamp pass follows the vector-ALU transform");
         pen::transform<R, L, TP, false, true>(gld, lst, lld, lst, tw, t, -1);
         __syncthreads();
         constexpr int PER = (L * W + NT - 1) / NT;
         const bool ptw = a.fmode == FM_PTW;
         const size_t obase = o * a.cs_o + (size_t)ct * W;
-        // the bounds (delta_k of the bin, or F(x) for ptw), every bin's first
-        constexpr bool BND_FIRST = true;
-        auto bound = [&](int k) {
+        auto bin_ok = [&](int k) {
             const int q = tid + k * NT;
-            const int pp = q % W;
-            const size_t idx = (size_t)(q / W) * a.cs_n + obase + pp;
-            if (k < PER && q < L * W && ct * W + pp < a.ncols)
-                return ptw ? a.orig[idx] : mk(a.delta_dense[idx] * a.dscale, R(0));
-            return mk(R(0), R(0));
+            return k < PER && q < L * W && ct * W + q % W < a.ncols;
         };
-        cplx<R> bnd[BND_FIRST ? PER : 1];
-        if constexpr (BND_FIRST) {
+        auto bin_idx = [&](int k) {
+            const int q = tid + k * NT;
+            return (size_t)(q / W) * a.cs_n + obase + q % W;
+        };
+        cplx<R> bnd[PER];
 #pragma unroll
-            for (int k = 0; k < PER; ++k) bnd[k] = bound(k);
+        for (int k = 0; k < PER; ++k) {
+            const size_t idx = bin_idx(k);
+            bnd[k] = !bin_ok(k) ? mk(R(0), R(0))
+                                : (ptw ? a.orig[idx] : mk(a.delta_dense[idx] * a.dscale, R(0)));
         }
         // the ptw constants, as ptw_clamp forms them (projection_algorithm.cu:199-258)
         const R sp = sqrt(R(1) + a.delta_ptw), sm = sqrt(R(1) - a.delta_ptw);
         const R dptw = fmin(R(1) - sm, sp - R(1)) / (R)sqrtf(2.0f);
         const R dcorr = fmin(R(1) - sm, sp - R(1)) / sqrt(R(2));
         bool fired = false;
-        // then rounds of CB bins: the round's accumulator loads go out
-        // together, ahead of its stores, and few values stay live
-        constexpr int CB = sizeof(R) == 8 ? 2 : 4;
+        R* dfr = &a.df[0].x;
 #pragma unroll
-        for (int k0 = 0; k0 < PER; k0 += CB) {
-            cplx<R> c[CB], ed[CB], old[CB], rb[CB];
-            bool hre[CB], him[CB];
-            if constexpr (!BND_FIRST) {
-#pragma unroll
-                for (int j = 0; j < CB; ++j) rb[j] = bound(k0 + j);
-            }
-#pragma unroll
-            for (int j = 0; j < CB; ++j) {
-                const int k = k0 + j;
-                const int q = tid + k * NT;
-                const bool ok = k < PER && q < L * W && ct * W + q % W < a.ncols;
-                const cplx<R> v = ok ? buf[q] : mk(R(0), R(0));
-                const size_t idx = (size_t)(q / W) * a.cs_n + obase + q % W;
-                hre[j] = him[j] = false;
-                c[j] = v;
-                ed[j] = mk(R(0), R(0));
-                old[j] = mk(R(0), R(0));
-                if (!ok) continue;
-                cplx<R> b;
-                if constexpr (BND_FIRST) b = bnd[k < PER ? k : 0];
-                else b = rb[j];
-                R d = b.x;
-                bool active = true;
-                if (ptw) {
-                    // as ptw_clamp (reference or corrected rule per bin)
-                    const int rule = ptw_rule<R>(idx, a.ptw_mask, a.ptw_corr);
-                    if (rule == PTW_NONE) continue;
-                    if (rule == PTW_DC) {
-                        const R lo = b.x * (R(1) / sp - R(1)), hi = b.x * (R(1) / sm - R(1));
-                        hre[j] = v.x > hi || v.x < lo;
-                        ed[j].x = -v.x;
-                        c[j].x = hre[j] ? R(0) : v.x;
-                        fired |= hre[j];
-                        if (hre[j]) old[j] = a.df[idx];
-                        continue;
+        for (int k = 0; k < PER; ++k) {
+            if (!bin_ok(k)) continue;
+            const int q = tid + k * NT;
+            const size_t idx = bin_idx(k);
+            const cplx<R> v = buf[q];
+            const cplx<R> b = bnd[k];
+            cplx<R> c = v;
+            R d = b.x;
+            bool active = true;
+            if (ptw) {
+                // as ptw_clamp (reference or corrected rule per bin)
+                const int rule = ptw_rule<R>(idx, a.ptw_mask, a.ptw_corr);
+                if (rule == PTW_NONE) continue;
+                if (rule == PTW_DC) {
+                    const R lo = b.x * (R(1) / sp - R(1)), hi = b.x * (R(1) / sm - R(1));
+                    if (v.x > hi || v.x < lo) {
+                        atomicAdd(dfr + 2 * idx, -v.x);
+                        c.x = R(0);
+                        fired = true;
                     }
-                    const R omag = b.x * b.x + b.y * b.y;
-                    const R cr = b.x + v.x, ci = b.y + v.y;
-                    const R ratio = (cr * cr + ci * ci) / omag;
-                    if (a.ptw_corr) {
-                        active = !(ratio <= R(1) + a.delta_ptw && ratio >= R(1) - a.delta_ptw);
-                        d = dcorr * sqrt(omag);
-                    } else {
-                        active = !(ratio <= sp && ratio >= sm);
-                        d = dptw;
-                    }
+                    buf[q] = c;
+                    continue;
                 }
-                const bool xh = v.x > d, xl = v.x < -d, yh = v.y > d, yl = v.y < -d;
-                hre[j] = active && (xh || xl);
-                him[j] = active && (yh || yl);
-                if (hre[j]) {
-                    c[j].x = xh ? d : -d;
-                    ed[j].x = c[j].x - v.x;
+                const R omag = b.x * b.x + b.y * b.y;
+                const R cr = b.x + v.x, ci = b.y + v.y;
+                const R ratio = (cr * cr + ci * ci) / omag;
+                if (a.ptw_corr) {
+                    active = !(ratio <= R(1) + a.delta_ptw && ratio >= R(1) - a.delta_ptw);
+                    d = dcorr * sqrt(omag);
+                } else {
+                    active = !(ratio <= sp && ratio >= sm);
+                    d = dptw;
                 }
-                if (him[j]) {
-                    c[j].y = yh ? d : -d;
-                    ed[j].y = c[j].y - v.y;
-                }
-                fired |= hre[j] || him[j];
-                // the bin's (Re, Im) edit pair in one access: a tile row's
-                // bins are one contiguous segment of the accumulator
-                if (hre[j] || him[j]) old[j] = a.df[idx];
             }
-#pragma unroll
-            for (int j = 0; j < CB; ++j) {
-                const int k = k0 + j;
-                const int q = tid + k * NT;
-                const bool ok = k < PER && q < L * W && ct * W + q % W < a.ncols;
-                const size_t idx = (size_t)(q / W) * a.cs_n + obase + q % W;
-                if (hre[j] || him[j])
-                    a.df[idx] = mk(hre[j] ? old[j].x + ed[j].x : old[j].x, him[j] ? old[j].y + ed[j].y : old[j].y);
-                if (ok) buf[q] = c[j];
+            const bool xh = v.x > d, xl = v.x < -d, yh = v.y > d, yl = v.y < -d;
+            if (active && (xh || xl)) {
+                c.x = xh ? d : -d;
+                atomicAdd(dfr + 2 * idx, c.x - v.x);
+                fired = true;
             }
+            if (active && (yh || yl)) {
+                c.y = yh ? d : -d;
+                atomicAdd(dfr + 2 * idx + 1, c.y - v.y);
+                fired = true;
+            }
+            buf[q] = c;
         }
         if (fired) a.st->freq_clamped = 1u;
         __syncthreads();
@@ -850,10 +820,23 @@ __global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ, V), (zz_min_waves<R, LZ>())) 
 // at one tile per CU (0.41 against 0.53 ms at 512^3).  The half-width tiles
 // (V=1, 512 threads) ask for 6 waves per SIMD (80 VGPRs, no spills): three
 // tiles per CU.
+// Workgroup i of a launch runs on XCD i mod 8 (each XCD has its own L2).
+// Neighbouring column tiles share cache lines: of a tile narrower than a
+// 128-byte line, and of the per-bin arrays in compact order (bounds, edit
+// accumulators), whose rows are not line-aligned.  So each XCD takes a
+// contiguous run of tiles, in order: workgroup i gets tile (i mod 8) n/8 +
+// i/8, and a tile's neighbours run on the same XCD at about the same time and
+// find the shared lines in its L2.  A bijection on [0, n) (the last n mod 8
+// workgroups keep their tiles).
+__device__ __forceinline__ uint32_t xcd_chunk_tile(uint32_t i, uint32_t n) {
+    const uint32_t per = n >> 3;
+    if (i >= (per << 3)) return i;
+    return (i & 7u) * per + (i >> 3);
+}
 template <typename R, typename T, int L, int MODE, int PREC = PREC_VALU, int V = 0>
 __global__ __launch_bounds__(s_tp<R>(L) * s_width<R>(V),
                              (PREC != PREC_VALU) ? (V == 0 ? 8 : 6) : 1) void k2_strided(SArgs<R, T> a) {
-    strided_body<R, T, L, MODE, PREC, V>(a, blockIdx.x);
+    strided_body<R, T, L, MODE, PREC, V>(a, xcd_chunk_tile(blockIdx.x, gridDim.x));
 }
 // Two forward z passes over the same rows in one workgroup (the paired
 // certification): the second re-reads the rows the first just read, from cache.
