@@ -105,6 +105,9 @@ SIGNATURES = {
     "ffcz_set_fft_precision": [_h, _i],
     "ffcz_autotune": [_h, _i, _i, C.POINTER(_i), C.POINTER(_f)],
     "ffcz_get_tile_vars": [_h, C.POINTER(_i)],
+    "ffcz_get_tune_source": [_h, C.POINTER(_i)],
+    "ffcz_set_plan_cache": [C.c_char_p],
+    "ffcz_get_plan_cache": [C.c_char_p, _sz, _szp],
     "ffcz_set_tile_vars": [_h, C.POINTER(_i)],
     "ffcz_set_cert_pair": [_h, _i],
     "ffcz_get_fft_precision": [_h, C.POINTER(_i)],
@@ -115,6 +118,7 @@ SIGNATURES = {
     "ffcz_get_stream": [_h, C.POINTER(_vp)],
     "ffcz_time_iteration": [_h, _i, C.POINTER(_f)],
     "ffcz_set_pass_timing": [_h, _i],
+    "ffcz_set_graph": [_h, _i],
     "ffcz_get_pass_timing": [_h, C.POINTER(_f), C.POINTER(C.c_long)],
     "ffcz_get_pass_counts": [_h, C.POINTER(C.c_long)],
     "ffcz_huffman_zstd_compress_u16": [_vp, _sz, _i, C.POINTER(_h)],

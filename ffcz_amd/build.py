@@ -30,7 +30,7 @@ FLAGS = [
 ]
 SOURCES = ["fft3d.hip", "fft_fast_f32.hip", "fft_fast_f64.hip", "fft_fast_c32.hip", "fft_fast_mfma.hip", "solver.hip",
            "batch.hip", "extract.hip", "container.hip",
-           "huffman.hip", "kernels.hip", "capi.cpp"]
+           "huffman.hip", "kernels.hip", "plan_cache.cpp", "capi.cpp"]
 
 
 def _compile(src: Path) -> Path:

@@ -1,6 +1,7 @@
 // C ABI over the HIP solver (include/ffcz.h).  Exceptions never cross the
 // boundary: every entry point returns a status and records a thread-local
 // message for ffcz_last_error().
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -13,6 +14,7 @@
 #include "fft_fast.hpp"
 #include "huffman.hpp"
 #include "kernels.hpp"
+#include "plan_cache.hpp"
 #include "solver.hpp"
 
 using namespace ffcz;
@@ -518,6 +520,27 @@ static void vars_out(const TileVars& v, int* out) {
 int ffcz_autotune(ffcz_solver* h, int reps, int use_cache, int* vars, float* ms) {
     return guard([&] { with(h, [&](auto& s) { vars_out(s.autotune(reps, use_cache != 0, ms), vars); }); });
 }
+int ffcz_get_tune_source(ffcz_solver* h, int* source) {
+    return guard([&] {
+        FFCZ_REQUIRE(source != nullptr, ERR_INVALID, "null output");
+        with(h, [&](auto& s) { *source = s.tune_source(); });
+    });
+}
+int ffcz_set_plan_cache(const char* path) {
+    return guard([&] { set_plan_cache_path(path); });
+}
+int ffcz_get_plan_cache(char* buf, size_t cap, size_t* len) {
+    return guard([&] {
+        FFCZ_REQUIRE(len != nullptr, ERR_INVALID, "null length");
+        const std::string p = plan_cache_path();
+        *len = p.size();
+        if (buf && cap > 0) {
+            const size_t n = std::min(cap - 1, p.size());
+            std::memcpy(buf, p.data(), n);
+            buf[n] = '\0';
+        }
+    });
+}
 int ffcz_get_tile_vars(ffcz_solver* h, int* vars) {
     return guard([&] {
         FFCZ_REQUIRE(vars != nullptr, ERR_INVALID, "null output");
@@ -578,6 +601,9 @@ int ffcz_get_stream(ffcz_solver* h, void** stream) {
         FFCZ_REQUIRE(stream != nullptr, ERR_INVALID, "null output");
         with(h, [&](auto& s) { *stream = (void*)s.stream(); });
     });
+}
+int ffcz_set_graph(ffcz_solver* h, int enable) {
+    return guard([&] { with(h, [&](auto& s) { s.set_graph(enable != 0); }); });
 }
 int ffcz_set_pass_timing(ffcz_solver* h, int enable) {
     return guard([&] { with(h, [&](auto& s) { s.set_pass_timing(enable != 0); }); });

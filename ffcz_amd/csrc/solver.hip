@@ -5,6 +5,8 @@
 #include <limits>
 #include <mutex>
 
+#include "fft_fast.hpp"
+#include "plan_cache.hpp"
 #include "solver.hpp"
 
 namespace ffcz {
@@ -91,6 +93,7 @@ Solver<T>::~Solver() {
 template <typename T>
 void Solver<T>::release() {
     if (s_) (void)hipStreamSynchronize(s_);
+    release_graph();
     void* ptrs[] = {x_, e_, e0_, ds_, df_, spec_, st_, vpart_, delta_dense_, orig_, partial_idx_, fpack_, spack_,
                     fcomp_, scomp_, fquant_, squant_, xscratch_, cert_real_, cert_spec_, cstats_, xglob_,
                     cert_real2_, zlog_off_, zlog_v_, zlog_cnt_, dfc_, ptw_mask_};
@@ -371,6 +374,62 @@ void Solver<T>::prepare_solve(int mode, const T* e0, int max_iter, T tol, hipStr
     if (reset) reset_state<T>(st_, max_iter, (double)tol, s);
 }
 
+// Every field of two iteration argument sets (the graph of one holds its
+// pointers and values as kernel arguments).
+template <typename R>
+static bool same_iter_args(const IterArgs<R>& a, const IterArgs<R>& b) {
+    return a.e == b.e && a.e_init == b.e_init && a.e_copy == b.e_copy && a.ds_fresh == b.ds_fresh && a.ds == b.ds &&
+           a.df == b.df && a.spec == b.spec && a.eps == b.eps && a.fmode == b.fmode && a.delta == b.delta &&
+           a.delta_dense == b.delta_dense && a.dscale == b.dscale && a.orig == b.orig &&
+           a.delta_ptw == b.delta_ptw && a.ptw_mask == b.ptw_mask && a.ptw_corr == b.ptw_corr && a.norm == b.norm &&
+           a.st == b.st && a.vpart == b.vpart && a.log_off == b.log_off && a.log_v == b.log_v &&
+           a.log_cnt == b.log_cnt;
+}
+
+template <typename T>
+void Solver<T>::release_graph() {
+    if (gexec_) (void)hipGraphExecDestroy(gexec_);
+    gexec_ = nullptr;
+}
+
+template <typename T>
+void Solver<T>::set_graph(bool on) {
+    graph_on_ = on;
+    if (!on) {
+        FFCZ_HIP(hipStreamSynchronize(s_));
+        release_graph();
+    }
+}
+
+// The graph of one iteration after the first (run_iteration with first =
+// false): kernels only, no memset or copy nodes.
+template <typename T>
+bool Solver<T>::ensure_graph(const IterArgs<T>& a, bool fused) {
+    const int key[8] = {fused ? 1 : 0, f_.prec, f_.zf_var, f_.zi_var, f_.y_var, f_.x_var, tuning().force_generic.load(),
+                        (int)tuning().slab_mb.load()};
+    if (gexec_ && same_iter_args(a, gargs_) && std::equal(key, key + 8, gkey_)) return true;
+    // work queued earlier may still run the graph replaced here
+    FFCZ_HIP(hipStreamSynchronize(s_));
+    release_graph();
+    hipGraph_t g = nullptr;
+    FFCZ_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
+    try {
+        run_iteration(a, fused, false, nullptr);
+    } catch (...) {
+        (void)hipStreamEndCapture(s_, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    FFCZ_HIP(hipStreamEndCapture(s_, &g));
+    const hipError_t e = hipGraphInstantiate(&gexec_, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) gexec_ = nullptr;
+    FFCZ_HIP(e);
+    gargs_ = a;
+    std::copy(key, key + 8, gkey_);
+    return true;
+}
+
 template <typename T>
 void Solver<T>::solve_loop(int mode, const T* e0, int max_iter, T tol) {
     const bool fused = use_fused();
@@ -379,6 +438,7 @@ void Solver<T>::solve_loop(int mode, const T* e0, int max_iter, T tol) {
     fused_ = fused;
     fused_var_ = f_.zi_var;
     const IterArgs<T> a = iter_args();
+    const bool graph = graph_on_ && !pass_timing_ && max_iter > 1 && ensure_graph(a, fused);
     // iterations go out in chunks of 2, 4, 8, 16 between host checks; the
     // kernels of an iteration past convergence return at once
     int launched = 0, chunk = 2;
@@ -391,7 +451,11 @@ void Solver<T>::solve_loop(int mode, const T* e0, int max_iter, T tol) {
     while (launched < max_iter) {
         const int k = std::min(chunk, max_iter - launched);
         if (pass_timing_) chunk_first_.push_back(launched + 1);
-        for (int i = 0; i < k; ++i) run_iteration(a, fused, launched + i == 0, pass_timing_ ? plog_.get() : nullptr);
+        for (int i = 0; i < k; ++i) {
+            const bool first = launched + i == 0;
+            if (graph && !first) FFCZ_HIP(hipGraphLaunch(gexec_, s_));
+            else run_iteration(a, fused, first, pass_timing_ ? plog_.get() : nullptr);
+        }
         launched += k;
         FFCZ_HIP(hipMemcpyAsync(&h, st_, sizeof(h), hipMemcpyDeviceToHost, s_));
         FFCZ_HIP(hipStreamSynchronize(s_));
@@ -791,20 +855,11 @@ void Solver<T>::set_fft_precision(int prec) {
 
 // ---------------------------------------------------------------------------
 // autotuner: tile variant of every pass kernel, chosen by timing each
-// candidate on this field's own buffers; results are cached per shape, FFT
-// variant and type for the life of the process
+// candidate on this field's own buffers; plans are kept per device, type,
+// shape, FFT variant and bound mode, for the process and on disk
+// (plan_cache.hpp)
 // ---------------------------------------------------------------------------
 namespace {
-struct TuneKey {
-    size_t bytes, nx, ny, nz;
-    int prec;
-    bool operator==(const TuneKey& o) const {
-        return bytes == o.bytes && nx == o.nx && ny == o.ny && nz == o.nz && prec == o.prec;
-    }
-};
-std::mutex g_tune_mu;
-std::vector<std::pair<TuneKey, TileVars>> g_tune_cache;
-
 struct EventPair {
     hipEvent_t a = nullptr, b = nullptr;
     EventPair() {
@@ -867,14 +922,15 @@ template <typename T>
 TileVars Solver<T>::autotune(int reps, bool use_cache, float* ms_out) {
     FFCZ_REQUIRE(initialized_, ERR_STATE, "autotune requires an initialized solver");
     FFCZ_REQUIRE(reps >= 1, ERR_INVALID, "reps must be >= 1");
-    const TuneKey key{sizeof(T), f_.nx, f_.ny, f_.nz, f_.prec};
+    const PlanKey key{sizeof(T), f_.nx, f_.ny, f_.nz, f_.prec, init_mode_};
     if (use_cache && !ms_out) {
-        std::lock_guard<std::mutex> lk(g_tune_mu);
-        for (auto& kv : g_tune_cache)
-            if (kv.first == key) {
-                set_tile_vars(kv.second);
-                return kv.second;
-            }
+        TileVars cached{};
+        const int src = plan_lookup(key, cached);
+        if (src != TUNE_NONE) {
+            set_tile_vars(cached);
+            tune_source_ = src;
+            return cached;
+        }
     }
     ensure_cert_buffers();
     // the tuner runs whole iterations on this solver's buffers (the passes
@@ -954,14 +1010,8 @@ TileVars Solver<T>::autotune(int reps, bool use_cache, float* ms_out) {
         for (int i = 0; i < 9; ++i) ms_out[i] = ms[i];
     }
     const TileVars v = tile_vars();
-    std::lock_guard<std::mutex> lk(g_tune_mu);
-    bool found = false;
-    for (auto& kv : g_tune_cache)
-        if (kv.first == key) {
-            kv.second = v;
-            found = true;
-        }
-    if (!found) g_tune_cache.push_back({key, v});
+    plan_store(key, v);
+    tune_source_ = TUNE_TIMED;
     return v;
 }
 

@@ -157,6 +157,9 @@ class Solver {
     // (coordinate descent), the chosen iteration; then the certification with
     // the default and the chosen variants; the last slot is 0.
     TileVars autotune(int reps, bool use_cache, float* ms = nullptr);
+    // where the last autotune's plan came from (TuneSource: 0 none yet,
+    // 1 timed, 2 this process's plans, 3 the plan file)
+    int tune_source() const { return tune_source_; }
     TileVars tile_vars() const;
     void set_tile_vars(const TileVars& v);
     const uint8_t* d_freq_pack() const { return fpack_; }
@@ -175,9 +178,22 @@ class Solver {
     QuantDesc freq_quant() const;
     QuantDesc spatial_quant() const;
 
+    // Graph replay of the iteration in solve_loop (default on): the loop's
+    // iterations after the first are one HIP graph launch each, captured once
+    // and captured again whenever the iteration's arguments, tile variants or
+    // FFT variant change.  Per-pass timing runs the launches eagerly.
+    void set_graph(bool on);
+    bool graph() const { return graph_on_; }
+
    private:
     CertArgs<T> cert_args(bool quantized) const;
     void account_pass_times(int iterations);
+    bool ensure_graph(const IterArgs<T>& a, bool fused);
+    void release_graph();
+    hipGraphExec_t gexec_ = nullptr;
+    IterArgs<T> gargs_{};
+    int gkey_[8] = {};
+    bool graph_on_ = true;
     bool pass_timing_ = false;
     std::unique_ptr<EventLog> plog_;
     std::vector<int> chunk_first_;  // iterations (1-based) that start a chunk
@@ -261,6 +277,7 @@ class Solver {
     uint32_t* zlog_cnt_ = nullptr;
     bool fused_ = false;
     int fused_var_ = 0;  // the z tile variant of the last fused loop (its log layout)
+    int tune_source_ = 0;  // TuneSource of the last autotune
     bool e_lazy_ = false;
     void take_extract_results(const ExtractGlobals& hf, const ExtractGlobals& hs) const;
 };

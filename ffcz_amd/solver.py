@@ -277,6 +277,16 @@ class ProjectionSolver:
             out["ms"] = [round(float(t), 5) for t in ms]
         return out
 
+    TUNE_SOURCES = ("none", "timed", "process", "disk")
+
+    def tune_source(self) -> str:
+        """Where the last autotune's plan came from: "none" (no autotune yet),
+        "timed" (tuned by that call), "process" (an earlier tuning in this
+        process) or "disk" (the plan file, see set_plan_cache)."""
+        v = C.c_int()
+        call("ffcz_get_tune_source", self._h, C.byref(v))
+        return self.TUNE_SOURCES[v.value]
+
     def tile_vars(self) -> dict:
         v = (C.c_int * 7)()
         call("ffcz_get_tile_vars", self._h, v)
@@ -318,6 +328,11 @@ class ProjectionSolver:
         call("ffcz_get_stream", self._h, C.byref(s))
         return s.value or 0
 
+    def set_graph(self, enable: bool = True):
+        """Replay each iteration after a solve's first as one HIP graph launch
+        (the default), or launch every kernel eagerly."""
+        call("ffcz_set_graph", self._h, int(bool(enable)))
+
     def set_pass_timing(self, enable: bool = True):
         """Record HIP events between the launches of every later solve's
         iterations (resets the averages)."""
@@ -354,3 +369,19 @@ def _is_tensor(a) -> bool:
 
 __all__ = ["ProjectionSolver", "CertResult", "as_pointer", "MODE_ABS", "MODE_DIRECT", "MODE_PTW", "MODE_PARTIAL_PTW",
            "QUANT_REF16", "QUANT_STEP"]
+
+
+def set_plan_cache(path):
+    """Path of the autotuner's plan file, shared by processes (None or "":
+    plans stay in this process).  The default is FFCZ_PLAN_CACHE, else
+    $XDG_CACHE_HOME/ffcz/plans.txt, else ~/.cache/ffcz/plans.txt."""
+    call("ffcz_set_plan_cache", None if path is None else str(path).encode())
+
+
+def plan_cache() -> str:
+    """The plan file's path ("" when plans stay in the process)."""
+    n = C.c_size_t()
+    call("ffcz_get_plan_cache", None, 0, C.byref(n))
+    buf = C.create_string_buffer(n.value + 1)
+    call("ffcz_get_plan_cache", buf, n.value + 1, C.byref(n))
+    return buf.value.decode()

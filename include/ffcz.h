@@ -198,8 +198,9 @@ int ffcz_set_fft_precision(ffcz_solver* h, int variant);
  * 128- or 64-byte column tiles; z: up to 256 or 128 threads per workgroup);
  * ffcz_autotune times each on this field's buffers and keeps the fastest.
  * Call it after initialize* and before solving: it overwrites the iterate
- * and resets the edit accumulators.  use_cache=1 reuses an earlier choice for
- * the same shape, type and FFT variant in this process.  vars[7] (optional)
+ * and resets the edit accumulators.  use_cache=1 reuses an earlier plan for
+ * the same device, shape, type, FFT variant and bound mode (this process's,
+ * or the plan file's; see ffcz_set_plan_cache) without timing.  vars[7] (optional)
  * receives the choice: iteration z fwd, z inv, y, x; certification z, y, x.
  * The iteration is tuned by coordinate descent over whole iterations (the
  * passes share caches); the certification over its 8 combinations, timed as
@@ -210,6 +211,20 @@ int ffcz_set_fft_precision(ffcz_solver* h, int variant);
  * with the defaults and with the choice; single certification (ffcz_certify)
  * ms with the choice. */
 int ffcz_autotune(ffcz_solver* h, int reps, int use_cache, int* vars, float* ms);
+/* Plans persist across processes (no counterpart in the reference, which
+ * re-plans with cuFFT in every constructor, projection_algorithm_impl.cuh:
+ * 109-131).  A plan is keyed by device model (name, architecture, compute
+ * units), library build, type, shape, FFT variant and bound mode; the file
+ * is FFCZ_PLAN_CACHE, else $XDG_CACHE_HOME/ffcz/plans.txt, else
+ * ~/.cache/ffcz/plans.txt, read once per process and merged under a file lock
+ * after each tuning.  ffcz_set_plan_cache(path) replaces the path (NULL or ""
+ * keeps plans in the process only); ffcz_get_plan_cache returns it ("" when
+ * off), with the two-call size pattern.  ffcz_get_tune_source reports where
+ * the solver's last ffcz_autotune plan came from: 0 none yet, 1 timed now,
+ * 2 this process's plans, 3 the plan file. */
+int ffcz_set_plan_cache(const char* path);
+int ffcz_get_plan_cache(char* buf, size_t cap, size_t* len);
+int ffcz_get_tune_source(ffcz_solver* h, int* source);
 int ffcz_get_tile_vars(ffcz_solver* h, int* vars);
 int ffcz_set_tile_vars(ffcz_solver* h, const int* vars);
 /* ffcz_certify_both's forward z transforms: paired=1 in one launch for both
@@ -237,6 +252,13 @@ int ffcz_time_iteration(ffcz_solver* h, int reps, float* ms6);
  * first launch after each host check is left out (its interval includes the
  * device waiting for the host).  Enabling resets the averages. */
 int ffcz_set_pass_timing(ffcz_solver* h, int enable);
+/* Graph replay of the projection loop (an addition; the reference launches
+ * every kernel of every iteration, projection_algorithm_impl.cuh:835-852).
+ * enable=1 (the default): each iteration after a solve's first is one HIP
+ * graph launch on the solver's stream, captured on first use and again when
+ * the iteration's buffers, bounds, mode, tile variants or FFT variant change.
+ * Per-pass timing (above) launches eagerly.  Results are identical either way. */
+int ffcz_set_graph(ffcz_solver* h, int enable);
 int ffcz_get_pass_timing(ffcz_solver* h, float* ms6, long* iterations);
 /* n6[6]: the number of timed launches behind each of those six means */
 int ffcz_get_pass_counts(ffcz_solver* h, long* n6);

@@ -8,6 +8,12 @@ ROOT = Path(__file__).resolve().parents[1]
 if str(ROOT) not in sys.path:
     sys.path.insert(0, str(ROOT))
 
+# the autotuner's plan file: the test session's own, not the user's cache
+if "FFCZ_PLAN_CACHE" not in os.environ:
+    import tempfile
+
+    os.environ["FFCZ_PLAN_CACHE"] = str(Path(tempfile.mkdtemp(prefix="ffcz-plans-")) / "plans.txt")
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) and the built libffcz.so")

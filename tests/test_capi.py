@@ -45,6 +45,20 @@ def test_null_handle_is_an_error_not_a_crash():
     assert b"null solver handle" in lib.ffcz_last_error()
 
 
+def test_plan_cache_path_is_settable_without_a_device(tmp_path):
+    """The autotuner's plan file path (host-side state, no device call)."""
+    import ffcz_amd
+
+    old = ffcz_amd.plan_cache()
+    try:
+        ffcz_amd.set_plan_cache(tmp_path / "plans.txt")
+        assert ffcz_amd.plan_cache() == str(tmp_path / "plans.txt")
+        ffcz_amd.set_plan_cache(None)
+        assert ffcz_amd.plan_cache() == ""
+    finally:
+        ffcz_amd.set_plan_cache(old)
+
+
 def _compile_dropin(out: Path):
     gxx = shutil.which("g++")
     if gxx is None:

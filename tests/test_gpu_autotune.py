@@ -7,6 +7,12 @@ bit-identical iterates, edits and certification counts.  That is the parity
 check here, over all 2^7 combinations on small fields and on the tuner's own
 choice."""
 import itertools
+import json
+import os
+import subprocess
+import sys
+import textwrap
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -87,6 +93,7 @@ def test_autotune_choice_cache_and_errors(gpu):
         s.initialize_all_abs(case[0], case[2], case[3])
         again = s.autotune(reps=1, use_cache=True)  # from the cache: no timing
         assert again == {k: chosen[k] for k in NAMES}
+        assert s.tune_source() == "process"
         with pytest.raises(FFCzError):
             s.set_tile_vars({"z_fwd": 2})
     with ProjectionSolver(*shape, dtype=dtype) as s:
@@ -120,3 +127,45 @@ def test_pass_timing_inside_solves(gpu):
         assert np.array_equal(s.get_results(), ref[1])
         s.set_pass_timing(False)
         assert s.pass_timing()[1] == 0
+
+
+_PLAN_SCRIPT = textwrap.dedent("""
+    import json, sys
+    import numpy as np
+    sys.path.insert(0, sys.argv[1])
+    from ffcz_amd import ProjectionSolver
+    from tests._cases import binding_case
+    shape = tuple(int(v) for v in sys.argv[2].split(","))
+    x, e0, eps, delta = binding_case(shape, seed=4, dtype=np.float32)
+    with ProjectionSolver(*shape, dtype=np.float32) as s:
+        s.initialize_all_abs(x, eps, delta)
+        v = s.autotune(reps=1, use_cache=True)
+        print(json.dumps({"vars": v, "source": s.tune_source()}))
+""")
+
+
+def _plan_process(path, shape):
+    env = dict(os.environ, FFCZ_PLAN_CACHE=str(path))
+    root = str(Path(__file__).resolve().parents[1])
+    out = subprocess.run([sys.executable, "-c", _PLAN_SCRIPT, root, ",".join(map(str, shape))], env=env,
+                         capture_output=True, text=True, timeout=240)
+    assert out.returncode == 0, out.stderr[-2000:]
+    return json.loads(out.stdout.strip().splitlines()[-1])
+
+
+def test_plan_file_reused_by_a_second_process(gpu, tmp_path):
+    """A plan tuned in one process is written to the plan file; the next
+    process reads it from there and gets the same variants without timing.
+    Another shape in the same file is tuned and kept beside it."""
+    path = tmp_path / "cache" / "plans.txt"
+    first = _plan_process(path, (32, 64, 128))
+    assert first["source"] == "timed"
+    second = _plan_process(path, (32, 64, 128))
+    assert second == {"vars": first["vars"], "source": "disk"}
+    third = _plan_process(path, (16, 60, 120))
+    assert third["source"] == "timed"
+    plans = [line.split() for line in path.read_text().splitlines() if line and not line.startswith("#")]
+    assert len(plans) == 2
+    assert sorted(tuple(p[3:6]) for p in plans) == [("16", "60", "120"), ("32", "64", "128")]
+    again = _plan_process(path, (16, 60, 120))
+    assert again == {"vars": third["vars"], "source": "disk"}
