@@ -198,8 +198,10 @@ def huffman_bench(solver, torch, dev, reps=3, cpu_sample=2_000_000):
     streams the reference driver codes (projection_example.cu:202-246): the
     quantised frequency and spatial edits (u16) and the packed flags (u8) of
     the last solve, symbols resident in HBM.  Rate = input symbol bytes per
-    second of the whole encode call (histogram, host codebook, bit emission,
-    bitstream back on the host; ZSTD is host libzstd and left out).  The
+    second of the whole encode call (histogram, codebook, bit emission,
+    bitstream back on the host; ZSTD is host libzstd and left out), and of its
+    device work alone (HIP events inside the call: histogram to the bitstream
+    in HBM).  The
     device bitstream of the stream with the most coded bits is checked bit
     for bit against the oracle encoder, which is also timed on a bounded
     sample of it."""
@@ -225,7 +227,7 @@ def huffman_bench(solver, torch, dev, reps=3, cpu_sample=2_000_000):
         k = "u16" if arr.dtype == np.uint16 else "u8"
         dev_arr = torch.from_numpy(arr.view(np.int16) if k == "u16" else arr).to(dev)
         torch.cuda.synchronize()
-        times, res = [], None
+        times, dtimes, res = [], [], None
         for r in range(reps + 1):
             h = C.c_void_p()
             t0 = time.perf_counter()
@@ -234,10 +236,12 @@ def huffman_bench(solver, torch, dev, reps=3, cpu_sample=2_000_000):
             res = _collect(h)
             if r:  # the first call is a warm-up
                 times.append(dt)
-        t = float(np.median(times))
+                dtimes.append(res.device_ms * 1e-3)
+        t, td = float(np.median(times)), float(np.median(dtimes))
         nbytes = arr.size * arr.dtype.itemsize
         per[name] = {"symbols": int(arr.size), "alphabet": int(res.symbols.size), "bits": int(res.num_bits),
-                     "ms": round(t * 1e3, 3), "GBps": round(nbytes / t / 1e9, 3)}
+                     "ms": round(t * 1e3, 3), "GBps": round(nbytes / t / 1e9, 3),
+                     "device_ms": round(td * 1e3, 3), "device_GBps": round(nbytes / td / 1e9, 3)}
         tot_bytes += nbytes
         tot_s += t
         if big is None or res.num_bits > big_res.num_bits:

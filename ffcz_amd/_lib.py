@@ -127,6 +127,7 @@ SIGNATURES = {
     "ffcz_huffman_encode_u8": [_vp, _sz, C.POINTER(_h)],
     "ffcz_coded_info": [_h, _szp, _szp, _szp, _szp, C.POINTER(C.c_ulonglong)],
     "ffcz_coded_data": [_h, _vp, _sz],
+    "ffcz_coded_device_ms": [_h, C.POINTER(_f)],
     "ffcz_coded_bitstream": [_h, _vp, _sz],
     "ffcz_coded_table": [_h, _vp, _vp, _vp, _sz],
     "ffcz_coded_free": [_h],

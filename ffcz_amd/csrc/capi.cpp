@@ -39,6 +39,7 @@ struct ffcz_coded {
     HuffmanCoded h;
     std::vector<uint8_t> zdata;
     bool zstd = false;
+    ~ffcz_coded() { recycle_bitstream(std::move(h.bitstream)); }
 };
 
 struct ffcz_blob {
@@ -652,6 +653,12 @@ int ffcz_coded_info(const ffcz_coded* c, size_t* cb, size_t* bb, size_t* ne, siz
         if (ne) *ne = c->h.nelem;
         if (ns) *ns = c->h.table.sym.size();
         if (nbits) *nbits = c->h.nbits;
+    });
+}
+int ffcz_coded_device_ms(const ffcz_coded* c, float* ms) {
+    return guard([&] {
+        FFCZ_REQUIRE(c != nullptr && ms != nullptr, ERR_INVALID, "null argument");
+        *ms = c->h.device_ms;
     });
 }
 int ffcz_coded_data(const ffcz_coded* c, uint8_t* out, size_t cap) {

@@ -19,7 +19,13 @@ struct HuffmanCoded {
     HuffmanTable table;
     unsigned long long nbits = 0;
     size_t nelem = 0;
+    float device_ms = 0.f;  // the encoder's device work (histogram to emitted words), HIP events
 };
+
+// Host bitstream buffers are reused between calls (a fresh buffer's pages
+// would be faulted in by the copy into it): a coded result's owner hands its
+// buffer back here when it is done with it.
+void recycle_bitstream(std::vector<uint8_t>&& v);
 
 // Codebook of HuffmanZSTDCoder.cuh:86-178 from (ascending symbols, counts).
 // Throws ERR_RANGE if a code would exceed 32 bits.

@@ -28,6 +28,7 @@ class CompressionResult:
     symbols: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))
     codes: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))
     lengths: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    device_ms: float = 0.0  # the encoder's device work (histogram to emitted bitstream)
 
 
 def _kind(data) -> str:
@@ -53,11 +54,13 @@ def _collect(h) -> CompressionResult:
         code = np.zeros(ns.value, np.uint32)
         ln = np.zeros(ns.value, np.int32)
         call("ffcz_coded_table", h, sym.ctypes.data, code.ctypes.data, ln.ctypes.data, ns.value)
+        dms = C.c_float()
+        call("ffcz_coded_device_ms", h, C.byref(dms))
     finally:
         lib.ffcz_coded_free(h)
     table = {int(s): (int(c), int(l)) for s, c, l in zip(sym, code, ln)}
     return CompressionResult(bytes(data)[: cb.value], table, bb.value, ne.value, bytes(bits)[: bb.value],
-                             nbits.value, sym, code, ln)
+                             nbits.value, sym, code, ln, float(dms.value))
 
 
 def cuda_huffman_zstd_compress(data, zstd_level: int = 3) -> CompressionResult:

@@ -277,6 +277,10 @@ int ffcz_coded_data(const ffcz_coded* c, uint8_t* out, size_t cap);      /* ZSTD
 int ffcz_coded_bitstream(const ffcz_coded* c, uint8_t* out, size_t cap); /* pre-ZSTD bytes */
 /* code table sorted by symbol: CompressionResult::code_table, HuffmanZSTDCoder.cuh:243 */
 int ffcz_coded_table(const ffcz_coded* c, uint32_t* symbols, uint32_t* codes, int32_t* lengths, size_t cap);
+/* the encoder's device time for this result (histogram, codebook, lengths,
+ * scan and emission, HIP events; the host transfers and, for alphabets of
+ * more than 2048 symbols, the host codebook are not counted) */
+int ffcz_coded_device_ms(const ffcz_coded* c, float* ms);
 void ffcz_coded_free(ffcz_coded* c);
 int ffcz_huffman_zstd_decompress_u16(const uint8_t* comp, size_t comp_len, const uint32_t* symbols,
                                      const uint32_t* codes, const int32_t* lengths, size_t nsym,

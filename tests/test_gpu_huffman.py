@@ -32,6 +32,12 @@ def _cases():
     yield "u8skew", np.minimum(rng.geometric(0.5, 50_000), 255).astype(np.uint8)
     yield "fib", np.repeat(np.arange(20), [1, 1, 2, 3, 5, 8, 13, 21, 34, 55, 89, 144, 233, 377, 610, 987,
                                            1597, 2584, 4181, 6765]).astype(np.uint16)
+    # the device codebook holds alphabets of up to 2048 symbols; one more
+    # goes to the host codebook: both sides of the boundary, with many ties
+    for k in (2047, 2048, 2049):
+        counts = 1 + rng.integers(0, 4, k)
+        yield f"alphabet{k}", rng.permutation(np.repeat(rng.choice(65536, k, replace=False), counts)).astype(np.uint16)
+    yield "ties", np.repeat(np.arange(0, 4000, 3), 5).astype(np.uint16)
 
 
 @pytest.mark.parametrize("name,data", list(_cases()), ids=[c[0] for c in _cases()])
@@ -69,3 +75,18 @@ def test_long_codes_raise(gpu):
 def test_empty_raises(gpu):
     with pytest.raises(ffcz_amd.FFCzError, match="Empty data"):
         huffman_encode(np.zeros(0, np.uint16))
+
+
+def test_calls_of_growing_and_shrinking_size_reuse_the_encoder_buffers(gpu):
+    """The encoder keeps its device buffers between calls (growing them when a
+    call needs more): results stay exact for calls of any order of sizes."""
+    rng = np.random.default_rng(9)
+    sizes = [1000, 3_000_000, 10, 700_000, 5, 4_500_000, 64]
+    for i, n in enumerate(sizes):
+        hi = 300 if i % 2 else 3000
+        data = rng.integers(0, hi, n).astype(np.uint16 if i % 3 else np.uint8 if hi <= 256 else np.uint16)
+        ref_bytes, (sym, codes, lens) = O.huffman_encode(data)
+        r = huffman_encode(data)
+        assert np.array_equal(r.symbols, sym.astype(np.uint32)), n
+        assert np.array_equal(r.codes, codes.astype(np.uint32)), n
+        assert r.bitstream == ref_bytes.tobytes(), n
