@@ -684,8 +684,6 @@ __device__ __forceinline__ void strided_body(const SArgs<R, T>& a, uint32_t blk)
         // arithmetic is freq_clamp_bin's: the edit formed, then accumulated;
         // ptw as ptw_clamp (the DC bin's real part zeroed).
         static_assert(PREC == PREC_VALU, "the clamp pass follows the vector-ALU transform");
-        pen::transform<R, L, TP, false, true>(gld, lst, lld, lst, tw, t, -1);
-        __syncthreads();
         constexpr int PER = (L * W + NT - 1) / NT;
         const bool ptw = a.fmode == FM_PTW;
         const size_t obase = o * a.cs_o + (size_t)ct * W;
@@ -697,6 +695,9 @@ __device__ __forceinline__ void strided_body(const SArgs<R, T>& a, uint32_t blk)
             const int q = tid + k * NT;
             return (size_t)(q / W) * a.cs_n + obase + q % W;
         };
+        // the bins' bounds are loaded before the forward transform, so their
+        // latency overlaps it (the tiles of the long FP64 axes leave room for
+        // one workgroup per CU, too few to hide it otherwise)
         cplx<R> bnd[PER];
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
@@ -704,6 +705,8 @@ __device__ __forceinline__ void strided_body(const SArgs<R, T>& a, uint32_t blk)
             bnd[k] = !bin_ok(k) ? mk(R(0), R(0))
                                 : (ptw ? a.orig[idx] : mk(a.delta_dense[idx] * a.dscale, R(0)));
         }
+        pen::transform<R, L, TP, false, true>(gld, lst, lld, lst, tw, t, -1);
+        __syncthreads();
         // the ptw constants, as ptw_clamp forms them (projection_algorithm.cu:199-258)
         const R sp = sqrt(R(1) + a.delta_ptw), sm = sqrt(R(1) - a.delta_ptw);
         const R dptw = fmin(R(1) - sm, sp - R(1)) / (R)sqrtf(2.0f);
