@@ -839,15 +839,26 @@ __device__ __forceinline__ uint32_t xcd_chunk_tile(uint32_t i, uint32_t n) {
     if (i >= (per << 3)) return i;
     return (i & 7u) * per + (i >> 3);
 }
-template <int MODE, int V>
-constexpr bool s_chunked() {
-    return V != 0 || MODE == S_CLAMP_PASS || MODE == S_CERT_CHECK || MODE == S_LOAD_DF;
+template <typename R, typename T, int MODE, int V>
+__device__ __forceinline__ bool s_chunked(const SArgs<R, T>& a) {
+    if (V != 0) return true;
+    switch (MODE) {
+        case S_CLAMP_PASS:
+        case S_LOAD_DF:
+            return true;
+        case S_CLAMP:
+            return a.fmode != FM_ABS;
+        case S_CERT_CHECK:
+            return a.cdelta_dense != nullptr;
+        default:
+            return false;
+    }
 }
 template <typename R, typename T, int L, int MODE, int PREC = PREC_VALU, int V = 0>
 __global__ __launch_bounds__(s_tp<R>(L) * s_width<R>(V),
                              (PREC != PREC_VALU) ? (V == 0 ? 8 : 6) : 1) void k2_strided(SArgs<R, T> a) {
-    strided_body<R, T, L, MODE, PREC, V>(a, s_chunked<MODE, V>() ? xcd_chunk_tile(blockIdx.x, gridDim.x)
-                                                                   : blockIdx.x);
+    strided_body<R, T, L, MODE, PREC, V>(a, s_chunked<R, T, MODE, V>(a) ? xcd_chunk_tile(blockIdx.x, gridDim.x)
+                                                                         : blockIdx.x);
 }
 // Two forward z passes over the same rows in one workgroup (the paired
 // certification): the second re-reads the rows the first just read, from cache.
