@@ -820,9 +820,7 @@ __global__ __launch_bounds__(z_tp(LZ) * z_rows(LZ, V), (zz_min_waves<R, LZ>())) 
 // The MFMA variants ask for 8 waves per SIMD (at most 64 VGPRs): two
 // 1024-thread tiles per CU, like the vector-ALU kernels.  The fused clamp
 // kernel spills a few registers under that bound and still runs faster than
-// at one tile per CU (0.41 against 0.53 ms at 512^3).  The half-width tiles
-// (V=1, 512 threads) ask for 6 waves per SIMD (80 VGPRs, no spills): three
-// tiles per CU.
+// at one tile per CU (0.41 against 0.53 ms at 512^3).
 // Workgroup i of a launch runs on XCD i mod 8 (each XCD has its own L2).
 // Neighbouring column tiles share cache lines: of a tile narrower than a
 // 128-byte line, and of the per-bin arrays in compact order (bounds, edit
@@ -1031,11 +1029,9 @@ bool strided_dispatch(int mode, const SArgs<R, T>& a, size_t outer, hipStream_t 
 #define FFCZ_MFMA_S_LENGTHS(X) X(32) X(64) X(96) X(128) X(256) X(512)
 #define FFCZ_MFMA_Z_LENGTHS(X) X(16) X(32) X(48) X(64) X(128) X(256)
 
-// the half-size tile variant (V=1) where its workgroup is whole waves
-template <int LZ>
-constexpr bool mfma_z_v1() { return (z_tp(LZ) * z_rows(LZ, 1)) % 64 == 0; }
-template <int L>
-constexpr bool mfma_s_v1() { return (s_tp<float>(L) * s_width<float>(1)) % 64 == 0; }
+// The MFMA passes run the full-width tiles only: the tile variants apply to
+// the vector-ALU passes (the fused z pass included, which is a vector-ALU
+// kernel in every FFT variant).
 
 template <int PREC>
 bool mfma_zfwd_dispatch(int mode, const ZArgs<float, float>& a, hipStream_t s) {
@@ -1044,13 +1040,6 @@ bool mfma_zfwd_dispatch(int mode, const ZArgs<float, float>& a, hipStream_t s) {
     switch (a.Lz) {
 #define FFCZ_MZF(Lv)                                                                                      \
     case Lv:                                                                                              \
-        if constexpr (mfma_z_v1<Lv>()) {                                                                  \
-            if (a.var) {                                                                                  \
-                if (mode == Z_ITER) launch_zfwd_len<float, float, Z_ITER, Lv, PREC, 1>(a, s);             \
-                else launch_zfwd_len<float, float, Z_PLAIN, Lv, PREC, 1>(a, s);                           \
-                return true;                                                                              \
-            }                                                                                             \
-        }                                                                                                 \
         if (mode == Z_ITER) launch_zfwd_len<float, float, Z_ITER, Lv, PREC>(a, s);                        \
         else launch_zfwd_len<float, float, Z_PLAIN, Lv, PREC>(a, s);                                      \
         return true;
@@ -1067,13 +1056,6 @@ bool mfma_zinv_dispatch(int mode, const ZArgs<float, float>& a, hipStream_t s) {
     switch (a.Lz) {
 #define FFCZ_MZI(Lv)                                                                                      \
     case Lv:                                                                                              \
-        if constexpr (mfma_z_v1<Lv>()) {                                                                  \
-            if (a.var) {                                                                                  \
-                if (mode == Z_ITER) launch_zinv_len<float, float, Z_ITER, Lv, PREC, 1>(a, s);             \
-                else launch_zinv_len<float, float, Z_PLAIN, Lv, PREC, 1>(a, s);                           \
-                return true;                                                                              \
-            }                                                                                             \
-        }                                                                                                 \
         if (mode == Z_ITER) launch_zinv_len<float, float, Z_ITER, Lv, PREC>(a, s);                        \
         else launch_zinv_len<float, float, Z_PLAIN, Lv, PREC>(a, s);                                      \
         return true;
@@ -1092,13 +1074,6 @@ bool mfma_strided_dispatch(int mode, const SArgs<float, float>& a, size_t outer,
     switch (a.L) {
 #define FFCZ_MSL(Lv)                                                                                 \
     case Lv:                                                                                         \
-        if constexpr (mfma_s_v1<Lv>()) {                                                             \
-            if (a.var) {                                                                             \
-                if (clamp) launch_strided_len<float, float, S_CLAMP_PASS, Lv, PREC, 1>(a, outer, s); \
-                else launch_strided_len<float, float, S_PLAIN, Lv, PREC, 1>(a, outer, s);            \
-                return true;                                                                         \
-            }                                                                                        \
-        }                                                                                            \
         if (clamp) launch_strided_len<float, float, S_CLAMP_PASS, Lv, PREC>(a, outer, s);            \
         else launch_strided_len<float, float, S_PLAIN, Lv, PREC>(a, outer, s);                       \
         return true;

@@ -958,9 +958,15 @@ TileVars Solver<T>::autotune(int reps, bool use_cache, float* ms_out) {
     float best = t_iter();
     ms[0] = best;
     // coordinate descent over the four iteration pass kernels; an alternative
-    // must win clearly, so timing noise keeps the default
+    // must win clearly, so timing noise keeps the default.  The MFMA passes
+    // run the full-width tiles only; in their iteration just the fused z pass
+    // (a vector-ALU kernel) has tile variants.
     int* fv[4] = {&f_.zf_var, &f_.y_var, &f_.x_var, &f_.zi_var};
     for (int p = 0; p < 4; ++p) {
+        if (f_.prec != PREC_VALU && !(p == 3 && fused)) {
+            ms[1 + p] = best;
+            continue;
+        }
         *fv[p] = 1;
         const float t = t_iter();
         ms[1 + p] = t;

@@ -207,7 +207,8 @@ int ffcz_set_fft_precision(ffcz_solver* h, int variant);
  * the paired sequence of ffcz_certify_both, and, with z variant 0, also with
  * one z launch per field instead of one for both (ffcz_set_cert_pair).  ms[9]
  * (optional; forces a fresh tuning): iteration ms with the defaults, with z
- * fwd, y, x, z inv switched in turn, with the choice; paired certification ms
+ * fwd, y, x, z inv switched in turn (a pass the FFT variant runs without tile
+ * variants, as the MFMA passes, repeats the time before it), with the choice; paired certification ms
  * with the defaults and with the choice; single certification (ffcz_certify)
  * ms with the choice. */
 int ffcz_autotune(ffcz_solver* h, int reps, int use_cache, int* vars, float* ms);

@@ -63,12 +63,14 @@ def test_every_tile_variant_gives_identical_results(gpu, shape, dtype):
 
 @pytest.mark.parametrize("fft", [FFT_BF16X3, FFT_FP16X3])
 @pytest.mark.parametrize("shape", [(32, 64, 512), (16, 128, 256), (64, 32, 96)])
-def test_mfma_tile_variants_give_identical_results(gpu, shape, fft):
-    """The MFMA kernels' tile variants regroup the radix-16 batches (16
-    butterflies of adjacent columns or row positions per instruction) without
-    changing any butterfly's operands: bit-identical to the default tiles.
-    The shapes cover the 16 x 16 row transform of the z passes (nz = 512),
-    radix-16 stages with vector-ALU remainders and the strided passes."""
+def test_mfma_iteration_under_every_tile_variant(gpu, shape, fft):
+    """The MFMA passes run their full-width tiles whatever the tile variants
+    say; the variants reach the vector-ALU kernels of the MFMA iteration (the
+    fused z pass) and the FP64 certification, which regroup pencils without
+    changing any operation: the results stay bit-identical.  The tuner times
+    only those variants under an MFMA precision.  The shapes cover the 16 x 16
+    row transform of the z passes (nz = 512), radix-16 stages with vector-ALU
+    remainders and the strided passes."""
     dtype = np.float32
     case = binding_case(shape, seed=6, dtype=dtype, spread=1.4, dfrac=0.9)
     ref = _run(shape, dtype, case, fft=fft)
@@ -77,6 +79,11 @@ def test_mfma_tile_variants_give_identical_results(gpu, shape, fft):
         assert got[0] == ref[0], combo
         for a, b in zip(got[1:4], ref[1:4]):
             assert np.array_equal(a, b), combo
+    tuned = _run(shape, dtype, case, tune=True, fft=fft)
+    ms = tuned[5]["ms"]
+    # z forward, y and x are not timed: their slots repeat the default time
+    assert ms[1] == ms[2] == ms[3] == ms[0]
+    assert tuned[5]["z_fwd"] == tuned[5]["y"] == tuned[5]["x"] == 0
 
 
 def test_autotune_choice_cache_and_errors(gpu):
