@@ -58,6 +58,9 @@ bool fast_cert_zfwd_pair(const ZArgs<double, double>& a, const ZArgs<double, dou
 // tile of those kernels for a real type of `bytes` bytes
 bool fast_s_supported(int L);
 int fast_s_width(size_t bytes, int var = 0);
+// a y/x length with a length-specialised kernel whose reversed radix order is
+// a different plan (the tile variant 2 of the iteration's vector-ALU passes)
+bool fast_s_rev_supported(int L);
 
 // Process-wide tuning knobs (C ABI: ffcz_set_tuning).
 struct Tuning {

@@ -45,6 +45,7 @@ bool fast_s_supported(int L) {
             return false;
     }
 }
+bool fast_s_rev_supported(int L) { return fast_s_supported(L) && pen::rev_differs(L); }
 int fast_s_width(size_t bytes, int var) { return bytes == 4 ? s_width<float>(var) : s_width<double>(var); }
 
 bool fast_fft_enabled() { return tuning().force_generic.load() == 0; }

@@ -19,10 +19,14 @@ namespace ffcz {
     X(4) X(8) X(16) X(30) X(32) X(36) X(48) X(60) X(64) X(75) X(90) X(120) X(125) X(128) X(150) X(256) X(300)
 
 // Tile variants (chosen per pass by the autotuner, fft3d.hpp TileVars):
-//   y/x passes: V=0 columns of 128 bytes per tile, V=1 64 bytes (half the LDS)
+//   y/x passes: V=0 columns of 128 bytes per tile, V=1 64 bytes (half the LDS),
+//               V=2 128-byte tiles with the length's radices in reversed order
+//               (vector-ALU iteration passes of lengths whose order is not a
+//               palindrome; a different rounding, not bit-identical to V=0)
 //   z passes:   V=0 up to 256 threads per workgroup, V=1 up to 128
 template <typename R>
-constexpr int s_width(int V = 0) { return (sizeof(R) == 4 ? 16 : 8) >> V; }
+constexpr int s_width(int V = 0) { return (sizeof(R) == 4 ? 16 : 8) >> (V & 1); }
+constexpr bool s_rev(int V) { return (V & 2) != 0; }
 constexpr int z_tp(int LZ) { return pen::threads_per_pencil(LZ); }
 // threads per pencil of the y/x kernels: FP64 pencils keep ~8 elements per
 // thread at every length (more threads in flight for the long FP64 axes)
@@ -591,6 +595,8 @@ __device__ __forceinline__ void zz_body(const ZArgs<R, R>& a, uint32_t blk) {
 template <typename R, typename T, int L, int MODE, int PREC, int V>
 __device__ __forceinline__ void strided_body(const SArgs<R, T>& a, uint32_t blk) {
     constexpr int TP = s_tp<R>(L), W = s_width<R>(V), NT = TP * W;
+    constexpr bool RV = s_rev(V);
+    static_assert(!RV || PREC == PREC_VALU, "radix order variants are vector-ALU transforms");
     __shared__ cplx<R> tw[L];
     __shared__ cplx<R> buf[L * W];
     if (a.st && a.st->done) return;
@@ -632,7 +638,7 @@ __device__ __forceinline__ void strided_body(const SArgs<R, T>& a, uint32_t blk)
     };
     if constexpr (MODE == S_PLAIN || MODE == S_LOAD_DF) {
         const int sg = MODE == S_LOAD_DF ? +1 : a.sign;
-        if constexpr (PREC == PREC_VALU) pen::transform<R, L, TP, false, false>(gld, gst, lld, lst, tw, t, sg);
+        if constexpr (PREC == PREC_VALU) pen::transform<R, L, TP, false, false, false, 0, RV>(gld, gst, lld, lst, tw, t, sg);
         else mx::transform<PREC, L, TP, W, W, NT, false, false, false>(gldp, gstp, buf, tw, tid, p, t, sg);
     } else if constexpr (MODE == S_CLAMP) {
         bool fired = false;
@@ -645,11 +651,11 @@ __device__ __forceinline__ void strided_body(const SArgs<R, T>& a, uint32_t blk)
             if (c < a.ncols) fired |= freq_clamp_bin(v, (size_t)n * a.cs_n + o * a.cs_o + c, a);
             buf[n * W + pp] = v;
         };
-        if constexpr (PREC == PREC_VALU) pen::transform<R, L, TP, false, true>(gld, cst, lld, lst, tw, t, -1);
+        if constexpr (PREC == PREC_VALU) pen::transform<R, L, TP, false, true, false, 0, RV>(gld, cst, lld, lst, tw, t, -1);
         else mx::transform<PREC, L, TP, W, W, NT, false, false, true>(gldp, cstp, buf, tw, tid, p, t, -1);
         if (fired) a.st->freq_clamped = 1u;
         __syncthreads();
-        if constexpr (PREC == PREC_VALU) pen::transform<R, L, TP, true, false>(lld, gst, lld, lst, tw, t, +1);
+        if constexpr (PREC == PREC_VALU) pen::transform<R, L, TP, true, false, false, 0, RV>(lld, gst, lld, lst, tw, t, +1);
         else mx::transform<PREC, L, TP, W, W, NT, true, false, false, false>(gldp, gstp, buf, tw, tid, p, t, +1);
     } else if constexpr (MODE == S_CLAMP_PASS && PREC != PREC_VALU) {
         // The MFMA fused x pass (every bound kind): the forward transform ends
@@ -705,7 +711,7 @@ __device__ __forceinline__ void strided_body(const SArgs<R, T>& a, uint32_t blk)
             bnd[k] = !bin_ok(k) ? mk(R(0), R(0))
                                 : (ptw ? a.orig[idx] : mk(a.delta_dense[idx] * a.dscale, R(0)));
         }
-        pen::transform<R, L, TP, false, true>(gld, lst, lld, lst, tw, t, -1);
+        pen::transform<R, L, TP, false, true, false, 0, RV>(gld, lst, lld, lst, tw, t, -1);
         __syncthreads();
         // the ptw constants, as ptw_clamp forms them (projection_algorithm.cu:199-258)
         const R sp = sqrt(R(1) + a.delta_ptw), sm = sqrt(R(1) - a.delta_ptw);
@@ -763,7 +769,7 @@ __device__ __forceinline__ void strided_body(const SArgs<R, T>& a, uint32_t blk)
         }
         if (fired) a.st->freq_clamped = 1u;
         __syncthreads();
-        pen::transform<R, L, TP, true, false>(lld, gst, lld, lst, tw, t, +1);
+        pen::transform<R, L, TP, true, false, false, 0, RV>(lld, gst, lld, lst, tw, t, +1);
     } else {
         static_assert(PREC == PREC_VALU, "certification runs on the vector ALU");  // S_CERT_CHECK
         unsigned long long nv = 0;
@@ -778,7 +784,7 @@ __device__ __forceinline__ void strided_body(const SArgs<R, T>& a, uint32_t blk)
                 vmax = ex > vmax ? ex : vmax;
             }
         };
-        pen::transform<R, L, TP, false, false>(gld, chk, lld, lst, tw, t, -1);
+        pen::transform<R, L, TP, false, false, false, 0, RV>(gld, chk, lld, lst, tw, t, -1);
         block_count_atomic(nv, &a.stats->freq_viol);
         block_max_atomic<double, NT>(vmax, &a.stats->freq_max_bits);
     }
@@ -839,7 +845,7 @@ __device__ __forceinline__ uint32_t xcd_chunk_tile(uint32_t i, uint32_t n) {
 }
 template <typename R, typename T, int MODE, int V>
 __device__ __forceinline__ bool s_chunked(const SArgs<R, T>& a) {
-    if (V != 0) return true;
+    if ((V & 1) != 0) return true;
     switch (MODE) {
         case S_CLAMP_PASS:
         case S_LOAD_DF:
@@ -964,12 +970,23 @@ bool zinv_by_len(const ZArgs<R, T>& a, hipStream_t s) {
             return false;
     }
 }
+// the pass modes of the iteration take the radix order variant (V=2)
+template <int MODE>
+constexpr bool s_rev_variant() {
+    return MODE == S_PLAIN || MODE == S_CLAMP || MODE == S_CLAMP_PASS;
+}
 template <typename R, typename T, int MODE>
 bool strided_by_len(const SArgs<R, T>& a, size_t outer, hipStream_t s) {
     switch (a.L) {
 #define FFCZ_SL(Lv)                                                                 \
     case Lv:                                                                        \
-        if (a.var) launch_strided_len<R, T, MODE, Lv, PREC_VALU, 1>(a, outer, s);   \
+        if constexpr (s_rev_variant<MODE>() && pen::rev_differs(Lv)) {               \
+            if (a.var == 2) {                                                       \
+                launch_strided_len<R, T, MODE, Lv, PREC_VALU, 2>(a, outer, s);      \
+                return true;                                                        \
+            }                                                                       \
+        }                                                                           \
+        if (a.var & 1) launch_strided_len<R, T, MODE, Lv, PREC_VALU, 1>(a, outer, s); \
         else launch_strided_len<R, T, MODE, Lv, PREC_VALU, 0>(a, outer, s);         \
         return true;
         FFCZ_FAST_S_LENGTHS(FFCZ_SL)

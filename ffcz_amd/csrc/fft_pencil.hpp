@@ -57,6 +57,13 @@ constexpr int nsub_o(int L, int s, bool rev) {
     for (int i = 0; i < s; ++i) ns *= radix_o(L, i, rev);
     return ns;
 }
+// whether the reversed radix order is a different plan (not a palindrome)
+constexpr bool rev_differs(int L) {
+    const int n = nstages(L);
+    for (int s = 0; s < n; ++s)
+        if (radix(L, s) != radix(L, n - 1 - s)) return true;
+    return false;
+}
 // threads per pencil: ~8 elements per thread up to L = 512, ~16 beyond
 constexpr int threads_per_pencil(int L) { return L <= 1 ? 1 : (L <= 512 ? (L + 7) / 8 : (L + 15) / 16); }
 

@@ -97,7 +97,7 @@ bool parse(const std::string& line, Entry& e) {
     if (build != build_id()) return false;
     const int* p = &e.v.zf;
     for (int i = 0; i < 7; ++i)
-        if (p[i] != 0 && p[i] != 1) return false;
+        if (p[i] != 0 && p[i] != 1 && !((i == 2 || i == 3) && p[i] == 2)) return false;
     return e.v.cpair == 0 || e.v.cpair == 1;
 }
 

@@ -906,7 +906,9 @@ TileVars Solver<T>::tile_vars() const {
 template <typename T>
 void Solver<T>::set_tile_vars(const TileVars& v) {
     const int* p = &v.zf;
-    for (int i = 0; i < 7; ++i) FFCZ_REQUIRE(p[i] == 0 || p[i] == 1, ERR_INVALID, "tile variants are 0 or 1");
+    for (int i = 0; i < 7; ++i)
+        FFCZ_REQUIRE(p[i] == 0 || p[i] == 1 || ((i == 2 || i == 3) && p[i] == 2), ERR_INVALID,
+                     "tile variants are 0 or 1 (the iteration's y and x passes also 2: radices in reversed order)");
     f_.zf_var = v.zf;
     f_.zi_var = v.zi;
     f_.y_var = v.y;
@@ -961,17 +963,26 @@ TileVars Solver<T>::autotune(int reps, bool use_cache, float* ms_out) {
     // must win clearly, so timing noise keeps the default.  The MFMA passes
     // run the full-width tiles only; in their iteration just the fused z pass
     // (a vector-ALU kernel) has tile variants.
+    // The y and x passes of a vector-ALU iteration also try their length's
+    // radices in reversed order (variant 2) where that is a different plan.
     int* fv[4] = {&f_.zf_var, &f_.y_var, &f_.x_var, &f_.zi_var};
+    const size_t plen[4] = {0, f_.ny, f_.nx, 0};
     for (int p = 0; p < 4; ++p) {
         if (f_.prec != PREC_VALU && !(p == 3 && fused)) {
             ms[1 + p] = best;
             continue;
         }
-        *fv[p] = 1;
-        const float t = t_iter();
-        ms[1 + p] = t;
-        if (t < 0.97f * best) best = t;
-        else *fv[p] = 0;
+        float tmin = 0.f;
+        for (int cand = 1; cand <= 2; ++cand) {
+            if (cand == 2 && !((p == 1 || p == 2) && fast_s_rev_supported((int)plen[p]))) continue;
+            const int keep = *fv[p];
+            *fv[p] = cand;
+            const float t = t_iter();
+            tmin = (cand == 1 || t < tmin) ? t : tmin;
+            if (t < 0.97f * best) best = t;
+            else *fv[p] = keep;
+        }
+        ms[1 + p] = tmin;
     }
     ms[5] = best;
     reset_state<T>(st_, 0, 0.0, s_);

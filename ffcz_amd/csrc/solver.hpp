@@ -14,7 +14,8 @@ namespace ffcz {
 // initialize_partial_ptw only (the reference's partial ptw kernel).
 enum SolveMode : int { MODE_DIRECT = 0, MODE_ABS = 1, MODE_PTW = 2, MODE_PARTIAL_PTW = 3 };
 
-// Tile variants (0/1) of the iteration's z forward, z inverse, y and x pass
+// Tile variants (0/1; the y and x passes also 2, their radices reversed) of
+// the iteration's z forward, z inverse, y and x pass
 // kernels and of the FP64 certification's z, y and x passes.
 struct TileVars {
     int zf, zi, y, x, czf, cy, cx;

@@ -195,8 +195,12 @@ int ffcz_set_fft_precision(ffcz_solver* h, int variant);
 /* FFT plan autotuner (an addition; the reference takes cuFFT's plans,
  * projection_algorithm_impl.cuh:109-131).  Every pass kernel of the
  * iteration and of the FP64 certification has two tile variants (y/x:
- * 128- or 64-byte column tiles; z: up to 256 or 128 threads per workgroup);
- * ffcz_autotune times each on this field's buffers and keeps the fastest.
+ * 128- or 64-byte column tiles; z: up to 256 or 128 threads per workgroup),
+ * and the iteration's vector-ALU y and x passes a third, 2: 128-byte tiles
+ * with the length's radices in reversed order, where that is a different
+ * plan (another rounding: results agree within the precision, not bit for
+ * bit); ffcz_autotune times each on this field's buffers and keeps the
+ * fastest.
  * Call it after initialize* and before solving: it overwrites the iterate
  * and resets the edit accumulators.  use_cache=1 reuses an earlier plan for
  * the same device, shape, type, FFT variant and bound mode (this process's,

@@ -1,11 +1,14 @@
-"""FFT plan autotuner: tile variants of every pass kernel.
+"""FFT plan autotuner: tile variants of every pass kernel, and the radix order
+of the iteration's vector-ALU y and x passes.
 
-The variants change only how pencils are grouped into workgroups (64- or
+The tile variants change only how pencils are grouped into workgroups (64- or
 128-byte column tiles for y/x, 128 or 256 threads for z); each pencil's
-stages, twiddles and operation order are the same, so every variant gives
+stages, twiddles and operation order are the same, so every tile variant gives
 bit-identical iterates, edits and certification counts.  That is the parity
 check here, over all 2^7 combinations on small fields and on the tuner's own
-choice."""
+choice.  The radix-order variant (2) runs a length's radices reversed: a
+different rounding, checked against the default within the FP32 / FP64
+tolerances."""
 import itertools
 import json
 import os
@@ -176,3 +179,31 @@ def test_plan_file_reused_by_a_second_process(gpu, tmp_path):
     assert sorted(tuple(p[3:6]) for p in plans) == [("16", "60", "120"), ("32", "64", "128")]
     again = _plan_process(path, (16, 60, 120))
     assert again == {"vars": third["vars"], "source": "disk"}
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.mark.parametrize("shape,dtype,tol", [((60, 120, 96), np.float32, 2e-5), ((150, 60, 48), np.float64, 1e-11)])
+def test_radix_order_variant_agrees_with_the_default(gpu, shape, dtype, tol):
+    """Variant 2 of the y and x passes runs the length's radices reversed
+    (60 = 4*5*3 as 3*5*4, 120 = 8*5*3 as 3*5*8, 150 = 2*5*5*3 as 3*5*5*2): the
+    same transform with another rounding, so the loop takes as many
+    iterations, its edits agree within the precision's tolerance, the
+    certification passes, and the bits differ (the reversed kernels ran)."""
+    case = binding_case(shape, seed=8, dtype=dtype, spread=1.4, dfrac=0.9)
+    ref = _run(shape, dtype, case)
+    got = _run(shape, dtype, case, dict(zip(NAMES, (0, 0, 2, 2, 0, 0, 0))))
+    assert got[0][0] == ref[0][0] and got[0][2] == ref[0][2]
+    for a, b in zip(got[1:4], ref[1:4]):
+        assert _rel(a, b) < tol
+    assert any(not np.array_equal(a, b) for a, b in zip(got[1:4], ref[1:4]))
+    assert got[4].ok and ref[4].ok
+    with ProjectionSolver(*shape, dtype=dtype) as s:
+        with pytest.raises(FFCzError):
+            s.set_tile_vars(dict(zip(NAMES, (2, 0, 0, 0, 0, 0, 0))))
+        with pytest.raises(FFCzError):
+            s.set_tile_vars(dict(zip(NAMES, (0, 0, 0, 0, 0, 2, 0))))
