@@ -68,6 +68,8 @@ def parse():
     p.add_argument("--no-fft-error", action="store_true")
     p.add_argument("--no-graph", action="store_true", help="c4: launch the batched iteration eagerly")
     p.add_argument("--no-autotune", action="store_true", help="keep the default tile variants")
+    p.add_argument("--tile-vars", default="", metavar="JSON",
+                   help='pin the tile variants instead of tuning them, e.g. \'{"x": 1}\' (profiling runs)')
     p.add_argument("--cpu-sample-iters", type=int, default=1)
     p.add_argument("--no-huffman", action="store_true",
                    help="skip the Huffman encoding and container measurements")
@@ -567,7 +569,12 @@ def main():
 
     def run_variant(variant):
         solver.set_fft_precision(FFT_VARIANTS[variant])
-        if not a.no_autotune:
+        if a.tile_vars:
+            pinned = {**solver.tile_vars(), **json.loads(a.tile_vars)}
+            solver.set_tile_vars(pinned)
+            tuned[variant] = solver.tile_vars()
+            print(f"[bench] {variant}: pinned tile variants {tuned[variant]}", file=sys.stderr, flush=True)
+        elif not a.no_autotune:
             tuned[variant] = solver.autotune(reps=3, timings=True)
             print(f"[bench] {variant}: autotuned tile variants {tuned[variant]}", file=sys.stderr, flush=True)
 
