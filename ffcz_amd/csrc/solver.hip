@@ -374,11 +374,13 @@ void Solver<T>::prepare_solve(int mode, const T* e0, int max_iter, T tol, hipStr
     if (reset) reset_state<T>(st_, max_iter, (double)tol, s);
 }
 
-// Every field of two iteration argument sets (the graph of one holds its
-// pointers and values as kernel arguments).
+// The fields of two iteration argument sets that an iteration after a
+// solve's first reads (the graph holds them as kernel arguments).  The
+// initial error and its copy are read by the first iteration only, which
+// runs eagerly, so a caller's new initial-error buffer needs no new capture.
 template <typename R>
 static bool same_iter_args(const IterArgs<R>& a, const IterArgs<R>& b) {
-    return a.e == b.e && a.e_init == b.e_init && a.e_copy == b.e_copy && a.ds_fresh == b.ds_fresh && a.ds == b.ds &&
+    return a.e == b.e && a.ds_fresh == b.ds_fresh && a.ds == b.ds &&
            a.df == b.df && a.spec == b.spec && a.eps == b.eps && a.fmode == b.fmode && a.delta == b.delta &&
            a.delta_dense == b.delta_dense && a.dscale == b.dscale && a.orig == b.orig &&
            a.delta_ptw == b.delta_ptw && a.ptw_mask == b.ptw_mask && a.ptw_corr == b.ptw_corr && a.norm == b.norm &&

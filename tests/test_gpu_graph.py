@@ -65,6 +65,11 @@ def test_graph_replay_per_bin_bounds_and_device_input(gpu):
         s.set_graph(True)
         for _ in range(2):
             _same(_solve(s, MODE_DIRECT, e0d), ref)
+        # another device buffer with the same initial error: only the first,
+        # eager iteration reads it, so the replayed iterations are unchanged
+        e0d2 = e0d.clone()
+        _same(_solve(s, MODE_DIRECT, e0d2), ref)
+        del e0d2
 
 
 def test_pass_timing_runs_eagerly_with_the_same_results(gpu):
