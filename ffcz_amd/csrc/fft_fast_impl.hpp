@@ -483,7 +483,11 @@ __device__ __forceinline__ void zz_body(const ZArgs<R, R>& a, uint32_t blk) {
     using DMask = std::conditional_t<(2 * DL::NB * DL::RAD <= 32), uint32_t, uint64_t>;
     static_assert(2 * DL::NB * DL::RAD <= 64, "deferred-clamp mask too narrow");
     DMask dmask = 0;
-    auto fires = [&](bool ok, R u) { return ok && (u > eps || u < -eps); };
+    // Branch-free tests (bitwise, not short-circuit, so no per-element exec
+    // mask branches): a component fires where |u| > eps (u > eps or
+    // u < -eps) at a valid position.  Absent rows were loaded as zeros and
+    // transform to zeros, so their excess |0| - eps < 0 leaves vm alone; only
+    // butterflies beyond the stage's count (j >= LR) hold no values.
 #pragma unroll
     for (int b = 0; b < DL::NB; ++b) {
         const int j = t + b * TP;
@@ -493,8 +497,10 @@ __device__ __forceinline__ void zz_body(const ZArgs<R, R>& a, uint32_t blk) {
             v[b][r].x *= a.scale;
             v[b][r].y *= a.scale;
             const R x0 = fabs(v[b][r].x) - eps, x1 = fabs(v[b][r].y) - eps;
-            if (ok) vm = fmax(vm, fmax(x0, x1));
-            const bool l0 = fires(ok, v[b][r].x) && x0 < tolr, l1 = fires(ok, v[b][r].y) && x1 < tolr;
+            if constexpr (DL::LR % TP == 0) vm = fmax(vm, fmax(x0, x1));
+            else vm = (j < DL::LR) ? fmax(vm, fmax(x0, x1)) : vm;
+            const bool l0 = ok & (fabs(v[b][r].x) > eps) & (x0 < tolr);
+            const bool l1 = ok & (fabs(v[b][r].y) > eps) & (x1 < tolr);
             const int bit = 2 * (b * DL::RAD + r);
             dmask |= (l0 ? DMask(1) << bit : DMask(0)) | (l1 ? DMask(2) << bit : DMask(0));
         }
@@ -526,21 +532,39 @@ __device__ __forceinline__ void zz_body(const ZArgs<R, R>& a, uint32_t blk) {
         }
     }
     // the clamp itself (v clamped in place for the forward transform), with
-    // the edits of the clamps not deferred
+    // the edits of the clamps not deferred: the store of a fresh pass and the
+    // atomic adds of a later one in loops of their own
+    auto clamp_pair = [&](int b, int r, bool ok, R& e0, R& e1, bool& f0, bool& f1) {
+        const int bit = 2 * (b * DL::RAD + r);
+        f0 = ok & !((dmask >> bit) & 1u);
+        f1 = ok & !((dmask >> (bit + 1)) & 1u);
+        e0 = clamp_edit_if(v[b][r].x, eps, f0);
+        e1 = clamp_edit_if(v[b][r].y, eps, f1);
+    };
+    if (fresh) {
 #pragma unroll
-    for (int b = 0; b < DL::NB; ++b) {
-        const int j = t + b * TP;
-        const bool ok = rv && ((DL::LR % TP == 0) || j < DL::LR);
+        for (int b = 0; b < DL::NB; ++b) {
+            const int j = t + b * TP;
+            const bool ok = rv && ((DL::LR % TP == 0) || j < DL::LR);
 #pragma unroll
-        for (int r = 0; r < DL::RAD; ++r) {
-            const int bit = 2 * (b * DL::RAD + r);
-            const bool f0 = fires(ok, v[b][r].x) && !((dmask >> bit) & 1u);
-            const bool f1 = fires(ok, v[b][r].y) && !((dmask >> (bit + 1)) & 1u);
-            const uint32_t q = (uint32_t)(p * LZ + j + r * DL::NS);
-            const R e0 = clamp_edit(v[b][r].x, eps), e1 = clamp_edit(v[b][r].y, eps);
-            if (fresh) {
-                if (ok) tile_at(dsp, q) = mk(f0 ? e0 : R(0), f1 ? e1 : R(0));
-            } else {
+            for (int r = 0; r < DL::RAD; ++r) {
+                R e0, e1;
+                bool f0, f1;
+                clamp_pair(b, r, ok, e0, e1, f0, f1);
+                if (ok) tile_at(dsp, (uint32_t)(p * LZ + j + r * DL::NS)) = mk(e0, e1);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int b = 0; b < DL::NB; ++b) {
+            const int j = t + b * TP;
+            const bool ok = rv && ((DL::LR % TP == 0) || j < DL::LR);
+#pragma unroll
+            for (int r = 0; r < DL::RAD; ++r) {
+                R e0, e1;
+                bool f0, f1;
+                clamp_pair(b, r, ok, e0, e1, f0, f1);
+                const uint32_t q = (uint32_t)(p * LZ + j + r * DL::NS);
                 if (f0) atomicAdd(&tile_at(dse, 2u * q), e0);
                 if (f1) atomicAdd(&tile_at(dse, 2u * q + 1u), e1);
             }

@@ -30,6 +30,21 @@ __device__ __forceinline__ R clamp_edit(R& v, R d) {
     return e;
 }
 
+// clamp_edit without branches, for an edit that is kept only where `keep`
+// holds: v is clamped in any case; the edit (c - v) is returned, and `keep`
+// left true, only where v was clamped and keep held (0 and false otherwise).
+// |v| > d is the same test as v > d || v < -d, and copysign(d, v) the same
+// bound as hi ? d : -d, so v and the kept edit are those of clamp_edit.
+template <typename R>
+__device__ __forceinline__ R clamp_edit_if(R& v, R d, bool& keep) {
+    const bool h = fabs(v) > d;
+    const R c = h ? copysign(d, v) : v;
+    keep = keep & h;
+    const R e = keep ? c - v : R(0);
+    v = c;
+    return e;
+}
+
 template <typename R, int NT = FFT_THREADS>
 __device__ __forceinline__ void block_max_atomic(R v, unsigned long long* target) {
     __shared__ R red[(NT + 63) / 64];
