@@ -499,8 +499,11 @@ __device__ __forceinline__ void zz_body(const ZArgs<R, R>& a, uint32_t blk) {
             const R x0 = fabs(v[b][r].x) - eps, x1 = fabs(v[b][r].y) - eps;
             if constexpr (DL::LR % TP == 0) vm = fmax(vm, fmax(x0, x1));
             else vm = (j < DL::LR) ? fmax(vm, fmax(x0, x1)) : vm;
-            const bool l0 = ok & (fabs(v[b][r].x) > eps) & (x0 < tolr);
-            const bool l1 = ok & (fabs(v[b][r].y) > eps) & (x1 < tolr);
+            // x0 > 0 is |u| > eps (the excess of a value not above eps is
+            // never positive: denormals are kept); the clamp below tests
+            // |u| > eps itself, so these masks end here
+            const bool l0 = ok & (x0 > R(0)) & (x0 < tolr);
+            const bool l1 = ok & (x1 > R(0)) & (x1 < tolr);
             const int bit = 2 * (b * DL::RAD + r);
             dmask |= (l0 ? DMask(1) << bit : DMask(0)) | (l1 ? DMask(2) << bit : DMask(0));
         }
@@ -579,36 +582,50 @@ __device__ __forceinline__ void zz_body(const ZArgs<R, R>& a, uint32_t blk) {
     // through an offset the compiler cannot see is zero, so they are formed
     // again rather than kept live from the inverse transform (which would
     // double the kernel's registers).
+    // The thread's and row's indices go through the same zero (the thread
+    // index kept in [0, TP) where TP is a power of two), so no address of the
+    // inverse half stays live into this one.
     int zoff;
     asm volatile("s_mov_b32 %0, 0" : "=s"(zoff));
     cplx<R>* bf = buf + zoff;
-    auto lldf = [&](int n) { return bf[bidx(n, p)]; };
-    auto lstf = [&](int n, cplx<R> u) { bf[bidx(n, p)] = u; };
+    const int tf = (TP & (TP - 1)) == 0 ? ((t + zoff) & (TP - 1)) : t + zoff;
+    const int pf = p + zoff;
+    auto lldf = [&](int n) { return bf[bidx(n, pf)]; };
+    auto lstf = [&](int n, cplx<R> u) { bf[bidx(n, pf)] = u; };
 #pragma unroll
     for (int b = 0; b < DF::NB; ++b) {
-        const int j = t + b * TP;
+        const int j = tf + b * TP;
         if ((DF::LR % TP == 0) || j < DF::LR) dftR<R, DF::RAD>(v[b], -1);
     }
-    pen::stage_store<R, LZ, TP, 0, true>(lstf, v, t);
+    pen::stage_store<R, LZ, TP, 0, true>(lstf, v, tf);
     __syncthreads();
     if constexpr (NST > 1) {
-        pen::transform<R, LZ, TP, true, true, false, 1, true>(lldf, lstf, lldf, lstf, tw + zoff, t, -1);
+        pen::transform<R, LZ, TP, true, true, false, 1, true>(lldf, lstf, lldf, lstf, tw + zoff, tf, -1);
         __syncthreads();
     }
     // R2C split X[k] = A[k] + W_nz^k B[k] into the thread's own row, bins k
     // and M-k together: with A, B from Z[k] and Z[M-k], X[M-k] is
     // conj(A - W^k B) (W^(M-k) = -conj W^k), so one product serves both; k = 0
-    // gives the DC and Nyquist bins
+    // gives the DC and Nyquist bins.  Rounds of TP bins unrolled; for an even
+    // M with TP dividing M/2 the last round is the lone bin M/2 (one lane),
+    // and every store is the row's uniform base plus a 32-bit offset.
     if (rv) {
-        cplx<R>* rowp = spec_t + (size_t)p * mp;
-        for (int k = t; k <= M / 2; k += TP) {
-            const cplx<R> zk = bf[bidx(k, p)];
-            const cplx<R> zm = bf[bidx(k == 0 ? 0 : M - k, p)];
+        const uint32_t ro = (uint32_t)pf * mp;
+        auto split = [&](int k) {
+            const cplx<R> zk = bf[bidx(k, pf)];
+            const cplx<R> zm = bf[bidx(k == 0 ? 0 : M - k, pf)];
             const cplx<R> A = mk(R(0.5) * (zk.x + zm.x), R(0.5) * (zk.y - zm.y));
             const cplx<R> B = mk(R(0.5) * (zk.y + zm.y), R(0.5) * (zm.x - zk.x));
             const cplx<R> wb = cmul(tw2[k], B);
-            rowp[k] = A + wb;
-            if (k != M - k) rowp[M - k] = conjg(A - wb);
+            tile_at(spec_t, ro + (uint32_t)k) = A + wb;
+            if (k != M - k) tile_at(spec_t, ro + (uint32_t)(M - k)) = conjg(A - wb);
+        };
+        constexpr int KB = M / 2 + 1;  // bins k = 0 .. M/2
+#pragma unroll
+        for (int i = 0; i < KB / TP; ++i) split(tf + i * TP);
+        if constexpr (KB % TP != 0) {
+            const int k = tf + (KB / TP) * TP;
+            if (k < KB) split(k);
         }
     }
 }
