@@ -437,14 +437,18 @@ __device__ __forceinline__ void zz_body(const ZArgs<R, R>& a, uint32_t blk) {
     // X[M-n] of the thread is loaded before any is used.
     constexpr int NB0 = pen::first_nb<LZ, TP>(), R0 = pen::radix(LZ, 0);
     cplx<R> xa[NB0][R0], xb[NB0][R0];
+    // A row past the field's end reads the field's last row instead (no
+    // per-lane zeroing or masked loads): its values go through the transforms
+    // into its own tile slots, and it stores, logs and counts nothing.
+    const uint32_t lro = (uint32_t)(rv ? p : (int)(a.rows - 1 - row0)) * mp;
 #pragma unroll
     for (int b = 0; b < NB0; ++b) {
-        const bool has = rv && pen::first_has<LZ, TP>(t, b);
+        const bool has = pen::first_has<LZ, TP>(t, b);
 #pragma unroll
         for (int r = 0; r < R0; ++r) {
             const int n = pen::first_elem<LZ>(t, b, r, TP);
-            xa[b][r] = has ? tile_at(spec_t, (uint32_t)p * mp + (uint32_t)n) : mk(R(0), R(0));
-            xb[b][r] = has ? tile_at(spec_t, (uint32_t)p * mp + (uint32_t)(M - n)) : mk(R(0), R(0));
+            xa[b][r] = has ? tile_at(spec_t, lro + (uint32_t)n) : mk(R(0), R(0));
+            xb[b][r] = has ? tile_at(spec_t, lro + (uint32_t)(M - n)) : mk(R(0), R(0));
         }
     }
     auto zload = [&](int n, int b, int r) {
@@ -485,9 +489,9 @@ __device__ __forceinline__ void zz_body(const ZArgs<R, R>& a, uint32_t blk) {
     DMask dmask = 0;
     // Branch-free tests (bitwise, not short-circuit, so no per-element exec
     // mask branches): a component fires where |u| > eps (u > eps or
-    // u < -eps) at a valid position.  Absent rows were loaded as zeros and
-    // transform to zeros, so their excess |0| - eps < 0 leaves vm alone; only
-    // butterflies beyond the stage's count (j >= LR) hold no values.
+    // u < -eps) at a valid position.  Butterflies beyond the stage's count
+    // (j >= LR) hold no values; a row past the field's end is dropped from vm
+    // once, after the loop.
 #pragma unroll
     for (int b = 0; b < DL::NB; ++b) {
         const int j = t + b * TP;
@@ -537,12 +541,16 @@ __device__ __forceinline__ void zz_body(const ZArgs<R, R>& a, uint32_t blk) {
     // the clamp itself (v clamped in place for the forward transform), with
     // the edits of the clamps not deferred: the store of a fresh pass and the
     // atomic adds of a later one in loops of their own
+    // the bound held in a vector register once (copysign's bit select cannot
+    // take it and its mask both from scalar registers)
+    R epsv = eps;
+    if constexpr (sizeof(R) == 4) asm volatile("v_mov_b32 %0, %1" : "=v"(epsv) : "s"(eps));
     auto clamp_pair = [&](int b, int r, bool ok, R& e0, R& e1, bool& f0, bool& f1) {
         const int bit = 2 * (b * DL::RAD + r);
         f0 = ok & !((dmask >> bit) & 1u);
         f1 = ok & !((dmask >> (bit + 1)) & 1u);
-        e0 = clamp_edit_if(v[b][r].x, eps, f0);
-        e1 = clamp_edit_if(v[b][r].y, eps, f1);
+        e0 = clamp_edit_if(v[b][r].x, epsv, f0);
+        e1 = clamp_edit_if(v[b][r].y, epsv, f1);
     };
     if (fresh) {
 #pragma unroll
@@ -573,7 +581,7 @@ __device__ __forceinline__ void zz_body(const ZArgs<R, R>& a, uint32_t blk) {
             }
         }
     }
-    block_max_store<R, NT>(vm, &a.vpart[gb]);
+    block_max_store<R, NT>(rv ? vm : R(0), &a.vpart[gb]);
     // every thread is past its reads of the tile (block_max_store's barrier)
     if (tid == 0) a.log_cnt[gb] = nlog;
     // Forward transform, radices reversed: its first stage butterflies the
