@@ -3,6 +3,7 @@
 #include <exception>
 
 #include "batch.hpp"
+#include "fft_fast.hpp"
 
 namespace ffcz {
 
@@ -39,9 +40,10 @@ void Batch<T>::solve(int mode, const T* const* e0, int max_iter, T tol, bool use
     for (auto* f : f_) FFCZ_HIP(hipStreamSynchronize(f->stream()));
     for (size_t i = 0; i < f_.size(); ++i) f_[i]->prepare_solve(mode, e0[i], max_iter, tol, s_, false);
     // the batched fields run the fused iteration form (its batched launches
-    // use tile variant 0); the others the five-pass form
+    // use the batch's z tile variant, which sets the log layout); the others
+    // the five-pass form
     for (auto* f : f_)
-        if (batch_fusable(f->fft())) f->set_fused_run(0);
+        if (batch_fusable(f->fft())) f->set_fused_run(fast_zb_var(f->fft().Lz));
     build(use_graph && max_iter > 0);
     batch_reset_states(plan_, max_iter, (double)tol, s_);
     if (max_iter > 0) batch_prologue(plan_, s_);

@@ -1111,7 +1111,7 @@ void batch_plan_build(BatchPlan<R>& bp, const std::vector<const Fft3d<R>*>& fiel
                 z.log_cnt = ia.log_cnt;
                 tab.push_back(z);
                 start.push_back((uint32_t)tiles);
-                tiles += ceil_div(f.rows, (size_t)fast_z_rows(f.Lz, f.half));
+                tiles += ceil_div(f.rows, (size_t)fast_z_rows(f.Lz, f.half, fast_zb_var(f.Lz)));
             }
             FFCZ_REQUIRE(tiles < (1ull << 31), ERR_INVALID, "batch too large for one launch");
             Pending p{};
@@ -1179,7 +1179,7 @@ void batch_plan_build(BatchPlan<R>& bp, const std::vector<const Fft3d<R>*>& fiel
     std::vector<FinEntry<R>> fin;
     for (size_t i : batched) {
         const Fft3d<R>& f = *fields[i];
-        fin.push_back({args[i].vpart, ceil_div(f.rows, (size_t)fast_z_rows(f.Lz, f.half)), args[i].st});
+        fin.push_back({args[i].vpart, ceil_div(f.rows, (size_t)fast_z_rows(f.Lz, f.half, fast_zb_var(f.Lz))), args[i].st});
     }
     size_t fin_off = 0;
     if (!fin.empty()) fin_off = put(fin.data(), fin.size() * sizeof(fin[0]));
@@ -1409,7 +1409,7 @@ void certify_batch(const std::vector<const Fft3d<double>*>& fs, const std::vecto
                     for (size_t i : g.second) {
                         tab.push_back(make(i));
                         start.push_back((uint32_t)tiles);
-                        tiles += ceil_div(fs[i]->rows, (size_t)fast_z_rows(fs[i]->Lz, true, 0));
+                        tiles += ceil_div(fs[i]->rows, (size_t)fast_z_rows(fs[i]->Lz, true, fast_zb_var(fs[i]->Lz)));
                     }
                     FFCZ_REQUIRE(tiles < (1ull << 31), ERR_INVALID, "batch too large for one launch");
                     gs.push_back({kind, 0, g.first, (unsigned)tiles, put(tab.data(), tab.size() * sizeof(tab[0])),

@@ -74,5 +74,7 @@ bool fast_fft_enabled();
 // rows per workgroup of the specialised z kernels for half length Lz and tile
 // variant var (0: the generic kernels run)
 int fast_z_rows(int Lz, bool half, int var = 0);
+// the z tile variant the batched launches use for a half length
+int fast_zb_var(int Lz);
 
 }  // namespace ffcz

@@ -63,6 +63,18 @@ int fast_z_rows(int Lz, bool half, int var) {
     }
 }
 
+int fast_zb_var(int Lz) {
+    switch (Lz) {
+#define FFCZ_ZB(Lv) \
+    case Lv:        \
+        return zb_var(Lv);
+        FFCZ_FAST_Z_LENGTHS(FFCZ_ZB)
+#undef FFCZ_ZB
+        default:
+            return 0;
+    }
+}
+
 Tuning& tuning() {
     // defaults from the environment: FFCZ_GENERIC_FFT=1, FFCZ_SLAB_MB=<MiB>
     static Tuning t = [] {
