@@ -101,6 +101,8 @@ void Solver<T>::release() {
         if (p) (void)hipFree(p);
     if (xhost_) (void)hipHostFree(xhost_);
     if (chost_) (void)hipHostFree(chost_);
+    if (sthost_) (void)hipHostFree(sthost_);
+    sthost_ = nullptr;
     xglob_ = nullptr;
     xhost_ = nullptr;
     chost_ = nullptr;
@@ -459,14 +461,23 @@ void Solver<T>::solve_loop(int mode, const T* e0, int max_iter, T tol) {
             else run_iteration(a, fused, first, pass_timing_ ? plog_.get() : nullptr);
         }
         launched += k;
-        FFCZ_HIP(hipMemcpyAsync(&h, st_, sizeof(h), hipMemcpyDeviceToHost, s_));
-        FFCZ_HIP(hipStreamSynchronize(s_));
+        h = read_state(s_);
         known = true;
         if (h.done) break;
         chunk = std::min(chunk * 2, 16);
     }
     if (pass_timing_ && known) account_pass_times((int)h.iter);
     finish_solve(s_, known && h.done ? &h : nullptr);
+}
+
+// The device state, through a pinned host copy (a pageable target would
+// stage the copy), after everything queued on s.
+template <typename T>
+DevState Solver<T>::read_state(hipStream_t s) {
+    if (!sthost_) FFCZ_HIP(hipHostMalloc((void**)&sthost_, sizeof(DevState), hipHostMallocDefault));
+    FFCZ_HIP(hipMemcpyAsync(sthost_, st_, sizeof(DevState), hipMemcpyDeviceToHost, s));
+    FFCZ_HIP(hipStreamSynchronize(s));
+    return *sthost_;
 }
 
 template <typename T>
@@ -531,8 +542,7 @@ void Solver<T>::finish_solve(hipStream_t s, const DevState* known) {
     if (known) {
         h = *known;  // the epilogue leaves the state as it is
     } else {
-        FFCZ_HIP(hipMemcpyAsync(&h, st_, sizeof(h), hipMemcpyDeviceToHost, s));
-        FFCZ_HIP(hipStreamSynchronize(s));
+        h = read_state(s);
     }
     e_src_ = nullptr;
     if (ds_fresh_) {

@@ -218,6 +218,8 @@ class Solver {
     T* ds_ = nullptr;
     cplx<T>* spec_ = nullptr;
     DevState* st_ = nullptr;
+    DevState* sthost_ = nullptr;  // pinned host copy of st_ (the loop's checks)
+    DevState read_state(hipStream_t s);
     T* vpart_ = nullptr;  // per-workgroup violation partials of the inverse z pass
     T* delta_dense_ = nullptr;   // direct mode (and certification of it)
     cplx<T>* orig_ = nullptr;    // F(x), compact layout (ptw modes)
