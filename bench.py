@@ -24,7 +24,7 @@ per-pass times and forward-FFT error against the FP64 transform of the field.
 reports the size of the self-describing container of those streams and the
 time to decode it back into x_hat on the GPU.
 
-Prints one JSON line.  ``--gpus`` > 1 is rejected: this build runs on one GPU.
+Prints one JSON line.  ``--gpus`` > 1 is rejected: the benchmark runs on one GPU.
 """
 from __future__ import annotations
 
@@ -461,7 +461,7 @@ def cpu_baseline_batch(wls, iters, cores, n_total):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        print(json.dumps({"error": "the benchmark runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
     import torch
 

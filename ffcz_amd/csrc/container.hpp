@@ -36,8 +36,8 @@ ContainerInfo container_info(const uint8_t* buf, size_t len);
 // before any device work.
 void container_decode(const uint8_t* buf, size_t len, const void* x_prime, void* x_hat, hipStream_t s);
 
-// The base compressor that produces x' for the solver (the survey's "next"
-// row F4): dual-quantisation Lorenzo.  x' = round(x/(2 eps)) * 2 eps is
+// The base compressor that produces x' for the solver (the reference takes
+// x' from an external compressor): dual-quantisation Lorenzo.  x' = round(x/(2 eps)) * 2 eps is
 // written to x_prime (host or device, N elements), and the returned stream
 // codes the 3D Lorenzo residuals of the integers (Huffman + ZSTD, outliers
 // beyond 16 bits listed).  Layout: container.hip.

@@ -16,7 +16,7 @@
 namespace ffcz {
 namespace {
 
-// Identity of this build of the library (its compile date and time, one
+// Identity of this compile of the library (its compile date and time, one
 // whitespace-free token): tile variants are compile-time kernel shapes, so a
 // plan only holds for the build that timed it.
 const std::string& build_id() {
@@ -42,7 +42,7 @@ bool same(const Entry& e, const std::string& dev, const PlanKey& k) {
 
 std::mutex g_mu;
 std::vector<Entry> g_process;  // tuned or adopted in this process
-std::vector<Entry> g_disk;     // the file's plans for this build, read once
+std::vector<Entry> g_disk;     // the file's plans for this compile, read once
 bool g_disk_read = false;
 bool g_path_set = false;
 std::string g_path;
@@ -87,7 +87,7 @@ std::string format(const Entry& e) {
     return o.str();
 }
 
-// One line of the file: true for a well-formed plan of this build.
+// One line of the file: true for a well-formed plan of this compile.
 bool parse(const std::string& line, Entry& e) {
     std::istringstream in(line);
     std::string build;

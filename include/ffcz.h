@@ -297,8 +297,7 @@ int ffcz_huffman_zstd_decompress_u8(const uint8_t* comp, size_t comp_len, const 
 /* ---- container: the coded edits with everything a decoder needs ---------
  * No counterpart in the reference, whose code tables live in memory only and
  * which writes neither shape, type, bounds nor quantiser range
- * (HuffmanZSTDCoder.cuh:240-246, projection_example.cu:202-246; the survey's
- * "next" row F1).  The four streams are the ones the reference driver codes,
+ * (HuffmanZSTDCoder.cuh:240-246, projection_example.cu:202-246).  The four streams are the ones the reference driver codes,
  * each Huffman-coded bit-exactly as HuffmanZSTDCoder.cuh:48-357 and
  * ZSTD-compressed (zstd_level <= 0: stored raw).  Layout: container.hip. */
 int ffcz_container_pack(ffcz_solver* h, int zstd_level, ffcz_blob** out); /* after ffcz_extract */
@@ -315,8 +314,7 @@ int ffcz_container_decode(const uint8_t* buf, size_t len, const void* x_prime, v
 
 /* ---- base compressor producing x' on the GPU ----------------------------
  * No counterpart in the reference, which takes x' from an external base
- * compressor through a file (projection_example.cu:164-176; the survey's
- * "next" row F4).  Dual-quantisation Lorenzo: x' = round(x/(2 eps)) * 2 eps
+ * compressor through a file (projection_example.cu:164-176).  Dual-quantisation Lorenzo: x' = round(x/(2 eps)) * 2 eps
  * goes to x_prime (host or device, N elements, |x' - x| <= eps up to the
  * rounding of the product), and the blob codes the 3D Lorenzo residuals of
  * the integers (Huffman + ZSTD).  x' can feed ffcz_init_* / ffcz_solve_*

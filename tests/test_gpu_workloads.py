@@ -24,7 +24,7 @@ bounds exactly as the solver rounds them.
 
 Tolerances.  FP64 loops (c5) match the oracle to 1e-9 relative, norm-wise.
 FP32 loops are compared with the FP64 oracle: iterate ``e`` and frequency
-edits ``df`` within 1e-6 relative (the north-star FP32 tolerance).  The
+edits ``df`` within 1e-6 relative (the stated FP32 tolerance).  The
 spatial edits ``ds`` are where an FP32 transform's rounding shows most: the
 loop clamps ``e`` back into the box after a round trip that moved it by a
 few 1e-7 of |e|, so ``ds`` (an eighth of eps at most) carries that rounding
