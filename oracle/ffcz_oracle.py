@@ -1,6 +1,6 @@
 """CPU oracle for the FFCz alternating-projection post-processor.
 
-TEST INFRASTRUCTURE ONLY.  This module is the checker: only ``tests/``,
+TEST INFRASTRUCTURE ONLY.  This module checks the GPU results: only ``tests/``,
 ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
 import it.  The product path (``ffcz_amd``) never routes through it.
 
