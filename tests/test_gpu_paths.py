@@ -40,7 +40,10 @@ def test_fast_matches_generic_f64(gpu, restore_tuning, shape):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-@pytest.mark.parametrize("shape", [(32, 64, 128), (16, 32, 256)])
+@pytest.mark.parametrize("shape", [(32, 64, 128), (16, 32, 256),
+                                   # rows (nx*ny) not a multiple of the fused z tile's rows: the last
+                                   # tile is partial (half lengths 64, 256 and 75)
+                                   (1, 60, 128), (1, 72, 512), (2, 60, 150)])
 def test_fused_form_matches_five_pass_form(gpu, restore_tuning, shape, dtype):
     """The whole-field loop runs the fused form (inverse z, spatial clamp and
     forward z in one pass, speculative clamps undone after convergence); a
