@@ -113,6 +113,8 @@ SIGNATURES = {
     "ffcz_get_fft_precision": [_h, C.POINTER(_i)],
     "ffcz_certify": [_h, _i, C.POINTER(Cert)],
     "ffcz_certify_both": [_h, C.POINTER(Cert)],
+    "ffcz_certify_both_enqueue": [_h],
+    "ffcz_certify_both_results": [_h, C.POINTER(Cert)],
     "ffcz_set_tuning": [_i, C.c_long],
     "ffcz_get_tuning": [C.POINTER(_i), C.POINTER(C.c_long)],
     "ffcz_get_stream": [_h, C.POINTER(_vp)],

@@ -488,19 +488,35 @@ int ffcz_set_margins(ffcz_solver* h, double sm, double fm) {
         with(h, [&](auto& s) { s.set_margins(sm, fm); });
     });
 }
+static void cert_out(const CertResult* r, ffcz_cert* out) {
+    for (int i = 0; i < 2; ++i) {
+        out[i].spatial_violations = r[i].spatial_violations;
+        out[i].freq_violations = r[i].freq_violations;
+        out[i].spatial_max_excess = r[i].spatial_max_excess;
+        out[i].freq_max_excess = r[i].freq_max_excess;
+        out[i].ok = r[i].ok;
+    }
+}
 int ffcz_certify_both(ffcz_solver* h, ffcz_cert* out) {
     return guard([&] {
         FFCZ_REQUIRE(out != nullptr, ERR_INVALID, "null result");
         with(h, [&](auto& s) {
             CertResult r[2];
             s.certify_both(r);
-            for (int i = 0; i < 2; ++i) {
-                out[i].spatial_violations = r[i].spatial_violations;
-                out[i].freq_violations = r[i].freq_violations;
-                out[i].spatial_max_excess = r[i].spatial_max_excess;
-                out[i].freq_max_excess = r[i].freq_max_excess;
-                out[i].ok = r[i].ok;
-            }
+            cert_out(r, out);
+        });
+    });
+}
+int ffcz_certify_both_enqueue(ffcz_solver* h) {
+    return guard([&] { with(h, [&](auto& s) { s.certify_both_enqueue(); }); });
+}
+int ffcz_certify_both_results(ffcz_solver* h, ffcz_cert* out) {
+    return guard([&] {
+        FFCZ_REQUIRE(out != nullptr, ERR_INVALID, "null result");
+        with(h, [&](auto& s) {
+            CertResult r[2];
+            s.certify_both_results(r);
+            cert_out(r, out);
         });
     });
 }

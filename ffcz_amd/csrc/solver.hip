@@ -106,6 +106,7 @@ void Solver<T>::release() {
     xglob_ = nullptr;
     xhost_ = nullptr;
     chost_ = nullptr;
+    cert_pending_ = false;
     xpending_ = false;
     x_ = e_ = e0_ = ds_ = df_ = vpart_ = nullptr;
     spec_ = nullptr;
@@ -720,6 +721,7 @@ CertArgs<T> Solver<T>::certify_prepare(bool quantized) {
 
 template <typename T>
 void Solver<T>::certify_enqueue(bool quantized) {
+    FFCZ_REQUIRE(!cert_pending_, ERR_STATE, "the queued certification's results have not been read");
     const CertArgs<T> c = certify_prepare(quantized);
     ffcz::certify<T>(*f64_, c, s_);
     certify_copy_stats(quantized, s_);
@@ -728,6 +730,13 @@ void Solver<T>::certify_enqueue(bool quantized) {
 template <typename T>
 void Solver<T>::certify_both(CertResult* out) {
     FFCZ_REQUIRE(out != nullptr, ERR_INVALID, "null result");
+    certify_both_enqueue();
+    certify_both_results(out);
+}
+
+template <typename T>
+void Solver<T>::certify_both_enqueue() {
+    FFCZ_REQUIRE(!cert_pending_, ERR_STATE, "the queued certification's results have not been read");
     CertArgs<T> a = certify_prepare(false);
     CertArgs<T> b = certify_prepare(true);
     b.work_spec = cert_spec_ + f64_->spec_elems();
@@ -738,7 +747,15 @@ void Solver<T>::certify_both(CertResult* out) {
     ffcz::certify_pair<T>(*f64_, a, b, s_);
     // both results in one copy (cstats_ and chost_ hold unquantised, quantised)
     FFCZ_HIP(hipMemcpyAsync(chost_, cstats_, 2 * sizeof(CertStats), hipMemcpyDeviceToHost, s_));
+    cert_pending_ = true;
+}
+
+template <typename T>
+void Solver<T>::certify_both_results(CertResult* out) {
+    FFCZ_REQUIRE(out != nullptr, ERR_INVALID, "null result");
+    FFCZ_REQUIRE(cert_pending_, ERR_STATE, "no certification queued");
     FFCZ_HIP(hipStreamSynchronize(s_));
+    cert_pending_ = false;
     out[0] = certify_read(false);
     out[1] = certify_read(true);
 }

@@ -84,6 +84,11 @@ class Solver {
     // both certifications (unquantised [0], quantised [1]) in one paired
     // sequence; requires the extraction
     void certify_both(CertResult* out2);
+    // the same sequence queued without waiting, its results read (waiting for
+    // them) by certify_both_results; work queued in between (the next field's
+    // solve) runs after it on the solver's stream
+    void certify_both_enqueue();
+    void certify_both_results(CertResult* out2);
     // mean ms per launch of the five iteration passes, then per whole iteration
     void time_iteration(int reps, float* ms6);
     // Per-pass timing inside real solves: HIP events between the launches of
@@ -251,6 +256,7 @@ class Solver {
     double* cert_real2_ = nullptr;         // second C2R work field (paired, frequency edits present)
     CertStats* cstats_ = nullptr;  // [2]: unquantised, quantised
     CertStats* chost_ = nullptr;   // pinned host copies
+    bool cert_pending_ = false;    // a queued certification's results not yet read
     // parameters
     bool initialized_ = false;
     int init_mode_ = -1;

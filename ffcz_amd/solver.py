@@ -323,6 +323,19 @@ class ProjectionSolver:
         return tuple(CertResult(bool(c.ok), c.spatial_violations, c.freq_violations, c.spatial_max_excess,
                                 c.freq_max_excess) for c in r)
 
+    def certify_both_enqueue(self) -> None:
+        """Queue certify_both's sequence without waiting (read it with
+        certify_both_results; work queued in between runs after it)."""
+        call("ffcz_certify_both_enqueue", self._h)
+
+    def certify_both_results(self):
+        """Wait for the queued certify_both_enqueue and return its
+        (unquantised, quantised) results."""
+        r = (_lib.Cert * 2)()
+        call("ffcz_certify_both_results", self._h, r)
+        return tuple(CertResult(bool(c.ok), c.spatial_violations, c.freq_violations, c.spatial_max_excess,
+                                c.freq_max_excess) for c in r)
+
     def stream_handle(self) -> int:
         s = C.c_void_p()
         call("ffcz_get_stream", self._h, C.byref(s))

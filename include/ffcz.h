@@ -147,6 +147,15 @@ int ffcz_certify(ffcz_solver* h, int quantized, ffcz_cert* out);
  * quantised, as one paired FP64 sequence (one y launch over both spectra;
  * the two forward z transforms in one launch or two, as ffcz_autotune chose) */
 int ffcz_certify_both(ffcz_solver* h, ffcz_cert* out2);
+/* ffcz_certify_both in two halves, for callers that keep the GPU busy while
+ * they read results: _enqueue queues the sequence and the copy of its results
+ * without waiting; _results waits for them and fills out2 as ffcz_certify_both
+ * does.  Work queued in between on the solver (the next field's reset and
+ * solve) runs after the sequence; a second certification before the results
+ * are read is an error (one result buffer).  No counterpart in the reference,
+ * which has no certification. */
+int ffcz_certify_both_enqueue(ffcz_solver* h);
+int ffcz_certify_both_results(ffcz_solver* h, ffcz_cert* out2);
 /* ---- batches: many fields of uneven shapes corrected together ----------
  * No counterpart in the reference (its driver, projection_example.cu:157-265,
  * runs one field per ProjectionSolver).  Each field is a borrowed solver

@@ -272,3 +272,43 @@ def test_extraction_of_untouched_frequency_edits(gpu, dtype, quant):
                 assert np.array_equal(comp, ref.compact)
                 assert np.array_equal(qsym, ref.quant)
                 assert lo == ref.vmin and hi == ref.vmax
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_queued_certification_across_solves(gpu, dtype):
+    """certify_both_enqueue / certify_both_results: a field's certification
+    queued, the next field's solve queued behind it, then its results read
+    (the pipelined order of the benchmark's timed steps).  Each equals the
+    synchronous certify_both of the same solve; a second certification before
+    the results are read, or a read with nothing queued, is refused."""
+    shape = (16, 20, 30)
+    cases = [binding_case(shape, seed=s, dtype=dtype, spread=1.4, dfrac=0.8) for s in (21, 22)]
+    x, _, eps, delta = cases[0]
+    with ProjectionSolver(*shape, dtype=dtype) as s:
+        s.initialize_all_abs(x, eps, delta)
+        s.set_margins(0.125, 0.125)
+        s.set_quantizer(QUANT_STEP, delta / 8, eps / 8)
+        expected = []
+        for _, e0, _, _ in cases:
+            s.reset_edit_accumulators()
+            s.solve_loop(MODE_ABS, e0, 60, 1e-9)
+            s.extract_compact_representation()
+            expected.append(s.certify_both())
+        with pytest.raises(FFCzError):
+            s.certify_both_results()
+        got = []
+        for k, (_, e0, _, _) in enumerate(cases):
+            s.reset_edit_accumulators()
+            s.solve_loop(MODE_ABS, e0, 60, 1e-9)
+            if k > 0:
+                got.append(s.certify_both_results())
+            s.extract_compact_representation()
+            s.certify_both_enqueue()
+            if k == 0:
+                with pytest.raises(FFCzError):
+                    s.certify_both_enqueue()
+                with pytest.raises(FFCzError):
+                    s.certify(False)
+        got.append(s.certify_both_results())
+        assert got == expected
+        assert any(v.freq_violations or v.spatial_violations or v.ok for pair in got for v in pair)
