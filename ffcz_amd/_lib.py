@@ -121,6 +121,7 @@ SIGNATURES = {
     "ffcz_time_iteration": [_h, _i, C.POINTER(_f)],
     "ffcz_set_pass_timing": [_h, _i],
     "ffcz_set_graph": [_h, _i],
+    "ffcz_set_borrow_e0": [_h, _i],
     "ffcz_get_pass_timing": [_h, C.POINTER(_f), C.POINTER(C.c_long)],
     "ffcz_get_pass_counts": [_h, C.POINTER(C.c_long)],
     "ffcz_huffman_zstd_compress_u16": [_vp, _sz, _i, C.POINTER(_h)],

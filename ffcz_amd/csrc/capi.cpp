@@ -622,6 +622,9 @@ int ffcz_get_stream(ffcz_solver* h, void** stream) {
 int ffcz_set_graph(ffcz_solver* h, int enable) {
     return guard([&] { with(h, [&](auto& s) { s.set_graph(enable != 0); }); });
 }
+int ffcz_set_borrow_e0(ffcz_solver* h, int borrow) {
+    return guard([&] { with(h, [&](auto& s) { s.set_borrow_e0(borrow != 0); }); });
+}
 int ffcz_set_pass_timing(ffcz_solver* h, int enable) {
     return guard([&] { with(h, [&](auto& s) { s.set_pass_timing(enable != 0); }); });
 }

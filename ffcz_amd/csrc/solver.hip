@@ -109,6 +109,7 @@ void Solver<T>::release() {
     cert_pending_ = false;
     xpending_ = false;
     x_ = e_ = e0_ = ds_ = df_ = vpart_ = nullptr;
+    e0_ext_ = nullptr;
     spec_ = nullptr;
     st_ = nullptr;
     delta_dense_ = nullptr;
@@ -301,7 +302,7 @@ IterArgs<T> Solver<T>::iter_args() const {
     IterArgs<T> a{};
     a.e = e_;
     a.e_init = e_src_ ? e_src_ : e0_;
-    a.e_copy = e_src_ ? e0_ : nullptr;
+    a.e_copy = (e_src_ && !e0_ext_) ? e0_ : nullptr;
     a.ds_fresh = ds_fresh_ ? 1 : 0;
     a.ds = ds_;
     a.df = dfc_;
@@ -364,6 +365,7 @@ void Solver<T>::prepare_solve(int mode, const T* e0, int max_iter, T tol, hipStr
     // reads the initial error directly (its inverse pass writes the iterate
     // e_); a device-resident e0 is copied into e0_ by that same pass.
     e_src_ = nullptr;
+    e0_ext_ = nullptr;
     fused_ = false;
     e_lazy_ = false;
     hipPointerAttribute_t attr{};
@@ -372,6 +374,8 @@ void Solver<T>::prepare_solve(int mode, const T* e0, int max_iter, T tol, hipStr
         e_src_ = e0;
     else
         (void)hipGetLastError();  // a host pointer is not an error here
+    // a borrowed device e0 stays where it is: no copy for the certifications
+    if (e_src_ && borrow_e0_) e0_ext_ = e0;
     if (!e_src_) FFCZ_HIP(hipMemcpyAsync(e0_, e0, f_.N * sizeof(T), hipMemcpyDefault, s));
     if (max_iter <= 0) FFCZ_HIP(hipMemcpyAsync(e_, e0_, f_.N * sizeof(T), hipMemcpyDeviceToDevice, s));
     if (reset) reset_state<T>(st_, max_iter, (double)tol, s);
@@ -773,7 +777,7 @@ CertArgs<T> Solver<T>::cert_args(bool quantized) const {
     c.nx = f_.nx;
     c.ny = f_.ny;
     c.nz = f_.nz;
-    c.e0 = e0_;
+    c.e0 = e0_ext_ ? e0_ext_ : e0_;
     c.ds = ds_;
     c.df = dfc_;
     if (quantized) {

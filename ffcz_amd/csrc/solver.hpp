@@ -189,6 +189,10 @@ class Solver {
     // and captured again whenever the iteration's arguments, tile variants or
     // FFT variant change.  Per-pass timing runs the launches eagerly.
     void set_graph(bool on);
+    // borrow=true: a device-resident initial error passed to a solve is read
+    // in place by the certifications that follow, instead of being copied
+    // (the caller keeps it unchanged until the next solve)
+    void set_borrow_e0(bool borrow) { borrow_e0_ = borrow; }
     bool graph() const { return graph_on_; }
 
    private:
@@ -274,6 +278,8 @@ class Solver {
     // lazily, or stored whole by the next solve's first spatial clamp
     mutable bool ds_fresh_ = false;
     const T* e_src_ = nullptr;  // device initial error read by the running solve's first iteration
+    bool borrow_e0_ = false;
+    const T* e0_ext_ = nullptr;  // the borrowed initial error of the last solve (null: e0_ holds it)
     void materialize_ds(hipStream_t s = nullptr) const;
     // Fused iteration form (iteration_fused): its log of deferred clamps, whether the last
     // solve ran it, and the iterate it leaves in the spectrum rows until asked for

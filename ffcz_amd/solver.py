@@ -341,6 +341,12 @@ class ProjectionSolver:
         call("ffcz_get_stream", self._h, C.byref(s))
         return s.value or 0
 
+    def set_borrow_e0(self, borrow: bool = True):
+        """Read a device-resident initial error in place in the certifications
+        after a solve instead of copying it (the caller keeps it unchanged
+        until the next solve)."""
+        call("ffcz_set_borrow_e0", self._h, int(bool(borrow)))
+
     def set_graph(self, enable: bool = True):
         """Replay each iteration after a solve's first as one HIP graph launch
         (the default), or launch every kernel eagerly."""

@@ -273,6 +273,13 @@ int ffcz_set_pass_timing(ffcz_solver* h, int enable);
  * the iteration's buffers, bounds, mode, tile variants or FFT variant change.
  * Per-pass timing (above) launches eagerly.  Results are identical either way. */
 int ffcz_set_graph(ffcz_solver* h, int enable);
+/* borrow=1: a device-resident initial error passed to a solve is not copied;
+ * the certifications after that solve read it in place, so the caller keeps
+ * it allocated and unchanged until the next solve (the copy is one N-element
+ * write in the solve's first pass).  0, the default, copies it as the
+ * reference copies its host input (projection_algorithm_impl.cuh:816-858);
+ * host arrays are always copied. */
+int ffcz_set_borrow_e0(ffcz_solver* h, int borrow);
 int ffcz_get_pass_timing(ffcz_solver* h, float* ms6, long* iterations);
 /* n6[6]: the number of timed launches behind each of those six means */
 int ffcz_get_pass_counts(ffcz_solver* h, long* n6);

@@ -312,3 +312,29 @@ def test_queued_certification_across_solves(gpu, dtype):
         got.append(s.certify_both_results())
         assert got == expected
         assert any(v.freq_violations or v.spatial_violations or v.ok for pair in got for v in pair)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_borrowed_initial_error(gpu, dtype):
+    """set_borrow_e0: a device initial error read in place by the
+    certifications instead of copied gives the same iterations, edits and
+    certifications as the copying default; a host initial error is copied
+    either way."""
+    shape = (16, 20, 30)
+    x, e0, eps, delta = binding_case(shape, seed=31, dtype=dtype, spread=1.4, dfrac=0.8)
+    e0_dev = torch.from_numpy(e0).cuda()
+    out = []
+    for borrow, src in ((False, e0_dev), (True, e0_dev), (True, e0)):
+        with ProjectionSolver(*shape, dtype=dtype) as s:
+            s.set_borrow_e0(borrow)
+            s.initialize_all_abs(x, eps, delta)
+            s.set_margins(0.125, 0.125)
+            s.set_quantizer(QUANT_STEP, delta / 8, eps / 8)
+            s.solve_loop(MODE_ABS, src, 60, 1e-9)
+            s.extract_compact_representation()
+            out.append((s.iteration_count, s.get_spatial_edits_full(), s.get_freq_edits_full(), s.certify_both()))
+    for o in out[1:]:
+        assert o[0] == out[0][0]
+        assert np.array_equal(o[1], out[0][1]) and np.array_equal(o[2], out[0][2])
+        assert o[3] == out[0][3]
+    assert any(v.freq_violations or v.spatial_violations or v.ok for v in out[0][3])
