@@ -16,7 +16,7 @@ import pytest
 import torch
 
 import ffcz_amd
-from ffcz_amd import MODE_ABS, MODE_DIRECT, ProjectionBatch, ProjectionSolver
+from ffcz_amd import MODE_ABS, MODE_DIRECT, QUANT_STEP, ProjectionBatch, ProjectionSolver
 from ffcz_amd._lib import FFCzError, load
 from oracle import ffcz_oracle as O
 from tests._cases import binding_case
@@ -262,3 +262,29 @@ def test_batched_extraction_of_untouched_frequency_edits(gpu, dtype):
                                   rs.vmin, rs.vmax)):
                 assert np.array_equal(g, r)
         assert 0 < untouched < len(shapes)
+
+
+def test_batch_borrowed_initial_errors(gpu):
+    """Fields that borrow their device initial errors (set_borrow_e0) solve,
+    extract and certify exactly like fields that copy them."""
+    dtype = np.float32
+    shapes = SHAPES[:4] + SHAPES[5:]
+    cases = _cases(shapes, dtype, seed0=70)
+    e0_dev = [torch.from_numpy(c[1]).cuda() for c in cases]
+    res = []
+    for borrow in (False, True):
+        with ProjectionBatch(shapes, dtype=dtype) as b:
+            for f, (x, e0, eps, delta) in zip(b.fields, cases):
+                f.set_borrow_e0(borrow)
+                f.initialize_all_abs(x, eps, delta)
+                f.set_margins(0.125, 0.125)
+                f.set_quantizer(QUANT_STEP, delta / 8, eps / 8)
+            b.solve(MODE_ABS, e0_dev, 80, 1e-7)
+            its = b.status()[0]
+            c1 = b.certify_all(False)
+            b.extract_all()
+            c2 = b.certify_all(True)
+            res.append((its, c1, c2, _batch_results(b)))
+    assert res[0][:3] == res[1][:3]
+    for (e, ds, df), (e1, ds1, df1) in zip(res[0][3], res[1][3]):
+        assert np.array_equal(e, e1) and np.array_equal(ds, ds1) and np.array_equal(df, df1)
