@@ -37,9 +37,23 @@ static FftPlan make_plan(int L) {
     while (rem % 2 == 0) { push(2); rem /= 2; }
     while (rem % 5 == 0) { push(5); rem /= 5; }
     while (rem % 3 == 0) { push(3); rem /= 3; }
+    // primes up to FFT_MAX_PRIME (composites are gone by now): direct DFT stages
+    for (int q = 7; q <= FFT_MAX_PRIME; q += 2)
+        while (rem % q == 0) { push(q); rem /= q; }
     FFCZ_REQUIRE(rem == 1, ERR_INVALID,
-                 "FFT axis length " + std::to_string(L) + " is not 2/3/5-smooth");
+                 "FFT axis length " + std::to_string(L) + " has a prime factor above " +
+                     std::to_string(FFT_MAX_PRIME));
     return p;
+}
+
+// Butterflies per pencil of the plan's widest prime stage (0 without one): a
+// prime stage runs one butterfly per thread, so a tile of P pencils needs
+// span * P <= FFT_THREADS.
+static int prime_span(const FftPlan& p) {
+    int span = 0;
+    for (int s = 0; s < p.nst; ++s)
+        if (p.radix[s] > 5 && p.radix[s] != 8) span = std::max(span, p.L / p.radix[s]);
+    return span;
 }
 
 template <typename R>
@@ -90,17 +104,21 @@ void Fft3d<R>::create(size_t nx_, size_t ny_, size_t nz_) {
     const size_t zrows = half ? (size_t)Lz + 1 : nz;
     const size_t ztab = ((size_t)Lz + nz / 2 + 1) * cs;
     Pz = 0;
+    const size_t span_z = (size_t)prime_span(pz);
     for (int P = 256; P >= 1; P >>= 1) {
-        if ((size_t)Lz * P <= (size_t)FFT_MAX_TILE && zrows * (P + 1) * cs + ztab <= LDS_BUDGET) { Pz = P; break; }
+        if ((size_t)Lz * P <= (size_t)FFT_MAX_TILE && zrows * (P + 1) * cs + ztab <= LDS_BUDGET &&
+            span_z * P <= (size_t)FFT_THREADS) { Pz = P; break; }
     }
     FFCZ_REQUIRE(Pz >= 1, ERR_INVALID, "z axis too long for one LDS tile");
-    auto pickW = [&](size_t L) {
+    auto pickW = [&](size_t L, const FftPlan& pl) {
+        const size_t span = (size_t)prime_span(pl);
         for (int W = 16; W >= 1; W >>= 1)
-            if (L * W <= (size_t)FFT_MAX_TILE && (L * W + L) * cs <= LDS_BUDGET) return W;
+            if (L * W <= (size_t)FFT_MAX_TILE && (L * W + L) * cs <= LDS_BUDGET && span * W <= (size_t)FFT_THREADS)
+                return W;
         return 0;
     };
-    Wy = pickW(ny);
-    Wx = pickW(nx);
+    Wy = pickW(ny, py);
+    Wx = pickW(nx, px);
     FFCZ_REQUIRE(Wy >= 1 && Wx >= 1, ERR_INVALID, "x/y axis too long for one LDS tile");
     d_tz = upload_table<R>(Lz, Lz);
     d_tz2 = upload_table<R>((int)nz, (int)(nz / 2 + 1), nz % 2 == 0);

@@ -1,4 +1,5 @@
-// LDS-resident mixed-radix (2/3/4/5/8) Stockham FFT over a tile of pencils.
+// LDS-resident mixed-radix (2/3/4/5/8, and primes up to 31) Stockham FFT over
+// a tile of pencils.
 //
 // A tile holds P pencils of length L in LDS; element (p, n) sits at
 // buf[p*sp + n*sn].  Every stage is the self-sorting Stockham step (decimation
@@ -21,6 +22,7 @@ constexpr int FFT_MAX_STAGES = 16;
 constexpr int FFT_THREADS = 256;     // threads per workgroup for every FFT kernel
 constexpr int FFT_EPT = 16;          // max tile elements per thread (tile <= 4096 elements)
 constexpr int FFT_MAX_TILE = FFT_THREADS * FFT_EPT;
+constexpr int FFT_MAX_PRIME = 31;    // largest prime radix (direct DFT stage)
 
 struct FftPlan {
     int L;                 // transform length
@@ -177,6 +179,68 @@ __device__ __forceinline__ void fft_stage(cplx<R>* buf, const cplx<R>* tw, int L
     __syncthreads();
 }
 
+// One Stockham stage of a prime radix 7 <= rad <= FFT_MAX_PRIME (a factor the
+// fixed radices leave): the butterfly's DFT is evaluated directly,
+//     out_m = sum_r v_r W_rad^(-s r m),   W_rad^q = W_L^(q L/rad) from the table,
+// rad^2 complex products per butterfly.  One butterfly per thread: the plan
+// keeps (L/rad) * P <= FFT_THREADS, so every input is in a register before the
+// barrier and each output is formed and stored after it.
+template <typename R>
+__device__ __forceinline__ void fft_stage_prime(cplx<R>* buf, const cplx<R>* tw, int L, int P, int sp, int sn, int rad,
+                                                 int Ns, int twstride, const FastDiv& nsdiv, const FastDiv& pdiv,
+                                                 const FastDiv& lrdiv, bool pfast, int sign) {
+    const int t = threadIdx.x;
+    const int LR = L / rad;
+    const bool on = t < LR * P;
+    cplx<R> v[FFT_MAX_PRIME];
+    int obase = 0;
+    if (on) {
+        int p, j;
+        if (pfast) {
+            j = (int)fdiv((uint32_t)t, pdiv);
+            p = t - j * P;
+        } else {
+            p = (int)fdiv((uint32_t)t, lrdiv);
+            j = t - p * LR;
+        }
+        const int q = (int)fdiv((uint32_t)j, nsdiv);
+        const int k = j - q * Ns;
+        const int ib = p * sp + j * sn;
+#pragma unroll
+        for (int r = 0; r < FFT_MAX_PRIME; ++r) {
+            if (r < rad) {
+                cplx<R> x = buf[ib + r * LR * sn];
+                if (r > 0 && Ns > 1) {
+                    cplx<R> w = tw[k * r * twstride];
+                    if (sign > 0) w.y = -w.y;
+                    x = cmul(x, w);
+                }
+                v[r] = x;
+            }
+        }
+        obase = p * sp + (q * Ns * rad + k) * sn;
+    }
+    __syncthreads();
+    if (on) {
+        for (int m = 0; m < rad; ++m) {
+            cplx<R> acc = v[0];
+            int e = 0;  // r * m mod rad, stepped
+#pragma unroll
+            for (int r = 1; r < FFT_MAX_PRIME; ++r) {
+                if (r < rad) {
+                    e += m;
+                    if (e >= rad) e -= rad;
+                    cplx<R> w = tw[e * LR];
+                    if (sign > 0) w.y = -w.y;
+                    acc = acc + cmul(v[r], w);
+                }
+            }
+            buf[obase + m * Ns * sn] = acc;
+        }
+    }
+    __syncthreads();
+}
+
 // Full transform of all P pencils of the tile (caller has synchronised after
 // filling buf and tw).  Leaves the result in natural order in buf; ends with a
 // barrier.
@@ -192,7 +256,10 @@ __device__ void tile_fft(cplx<R>* buf, const cplx<R>* tw, const FftPlan& pl, int
             case 2: fft_stage<R, 2>(buf, tw, pl.L, P, sp, sn, pl.Ns[s], pl.twstride[s], pl.ns_div[s], pdiv, lrdiv, pfast, sign); break;
             case 5: fft_stage<R, 5>(buf, tw, pl.L, P, sp, sn, pl.Ns[s], pl.twstride[s], pl.ns_div[s], pdiv, lrdiv, pfast, sign); break;
             case 3: fft_stage<R, 3>(buf, tw, pl.L, P, sp, sn, pl.Ns[s], pl.twstride[s], pl.ns_div[s], pdiv, lrdiv, pfast, sign); break;
-            default: break;
+            default:
+                fft_stage_prime<R>(buf, tw, pl.L, P, sp, sn, rad, pl.Ns[s], pl.twstride[s], pl.ns_div[s], pdiv, lrdiv,
+                                   pfast, sign);
+                break;
         }
     }
 }

@@ -165,3 +165,75 @@ def test_converged_iteration_clamps_are_dropped(gpu, shape, dtype):
         # element sits within rounding of eps)
         if dtype == np.float64:
             assert np.count_nonzero(np.abs(e) > eps) == np.count_nonzero(np.abs(ref.e) > eps)
+
+
+DEGENERATE = [(1, 1, 1), (1, 1, 2), (2, 1, 1), (1, 3, 1), (7, 1, 5), (1, 1, 17), (3, 2, 1), (1, 2, 64), (2, 2, 2)]
+
+
+@pytest.mark.parametrize("shape", DEGENERATE)
+def test_degenerate_axes_match_oracle(gpu, shape):
+    """Axes of length 1 and 2 (a lone DC bin, a DC and Nyquist pair, lines and
+    planes along every axis): both bound modes against the FP64 oracle, then
+    the certification of the margin-shrunk solve against the oracle's."""
+    x, e0, eps, delta = binding_case(shape, seed=9)
+    ref = O.solve(x, e0, eps, mode="abs", delta=delta, max_iter=50, tol=1e-12)
+    with ProjectionSolver(*shape, dtype=np.float64) as s:
+        s.initialize_all_abs(x, eps, delta)
+        s.solve_all_abs(e0, 50, 1e-12)
+        assert s.iteration_count == ref.iterations
+        assert rel(s.get_results(), ref.e.ravel()) < 1e-9
+        assert rel(s.get_freq_edits_full(), ref.df) < 1e-9
+        assert rel(s.get_spatial_edits_full(), ref.ds) < 1e-9
+        s.reset_edit_accumulators()
+        s.set_margins(0.125, 0.125)
+        s.solve_all_abs(e0, 50, 1e-12)
+        c = s.certify(False)
+        r = O.certify(O.reconstruct_edit(shape, e0, s.get_freq_edits_full(), s.get_spatial_edits_full()), eps, delta)
+        assert (c.ok, c.spatial_violations, c.freq_violations) == (r["ok"], r["spatial_violations"],
+                                                                   r["freq_violations"])
+        assert c.ok
+    x, e0, eps, idx, dk = direct_case(shape, seed=9)
+    ref = O.solve(x, e0, eps, mode="direct", idx=idx, delta_k=dk, max_iter=50, tol=1e-12)
+    with ProjectionSolver(*shape, dtype=np.float64) as s:
+        s.initialize(x, idx, dk, eps)
+        s.solve(e0, 50, 1e-12)
+        assert s.iteration_count == ref.iterations
+        assert rel(s.get_results(), ref.e.ravel()) < 1e-9
+        assert rel(s.get_freq_edits_full(), ref.df) < 1e-9
+        assert rel(s.get_spatial_edits_full(), ref.ds) < 1e-9
+
+
+PRIME_LOOP_SHAPES = [(7, 11, 13), (1, 69, 115), (2, 98, 14), (23, 6, 46)]
+
+
+@pytest.mark.parametrize("shape", PRIME_LOOP_SHAPES)
+def test_prime_factor_axes_match_oracle(gpu, shape):
+    """Axis lengths with prime factors 7..31 (69 = 3·23, 98 = 2·7², 115 = 5·23,
+    half-length z over 7 and 23): both bound modes in FP64 against the oracle,
+    the abs loop in FP32 against the FP32 oracle."""
+    x, e0, eps, delta = binding_case(shape, seed=13)
+    ref = O.solve(x, e0, eps, mode="abs", delta=delta, max_iter=50, tol=1e-12)
+    with ProjectionSolver(*shape, dtype=np.float64) as s:
+        s.initialize_all_abs(x, eps, delta)
+        s.solve_all_abs(e0, 50, 1e-12)
+        assert s.iteration_count == ref.iterations
+        assert rel(s.get_results(), ref.e.ravel()) < 1e-9
+        assert rel(s.get_freq_edits_full(), ref.df) < 1e-9
+        assert rel(s.get_spatial_edits_full(), ref.ds) < 1e-9
+    x, e0, eps, idx, dk = direct_case(shape, seed=13)
+    ref = O.solve(x, e0, eps, mode="direct", idx=idx, delta_k=dk, max_iter=50, tol=1e-12)
+    with ProjectionSolver(*shape, dtype=np.float64) as s:
+        s.initialize(x, idx, dk, eps)
+        s.solve(e0, 50, 1e-12)
+        assert s.iteration_count == ref.iterations
+        assert rel(s.get_results(), ref.e.ravel()) < 1e-9
+        assert rel(s.get_freq_edits_full(), ref.df) < 1e-9
+    x, e0, eps, delta = binding_case(shape, seed=14, dtype=np.float32)
+    ref = O.solve(x, e0, eps, mode="abs", delta=delta, max_iter=30, tol=1e-7, dtype=np.float32)
+    with ProjectionSolver(*shape, dtype=np.float32) as s:
+        s.initialize_all_abs(x, eps, delta)
+        s.solve_all_abs(e0, 30, 1e-7)
+        assert abs(s.iteration_count - ref.iterations) <= 1
+        assert rel(s.get_freq_edits_full(), ref.df) < 1e-5
+        assert rel(s.get_spatial_edits_full(), ref.ds) < 1e-5
+        assert rel(s.get_results(), ref.e.ravel()) < 1e-5
