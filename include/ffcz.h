@@ -275,8 +275,9 @@ int ffcz_set_pass_timing(ffcz_solver* h, int enable);
 int ffcz_set_graph(ffcz_solver* h, int enable);
 /* borrow=1: a device-resident initial error passed to a solve is not copied;
  * the certifications after that solve read it in place, so the caller keeps
- * it allocated and unchanged until the next solve (the copy is one N-element
- * write in the solve's first pass).  0, the default, copies it as the
+ * it allocated and unchanged until the last of them has run (for a queued
+ * certification: until ffcz_certify_both_results returns) or the next solve
+ * starts (the copy is one N-element write in the solve's first pass).  0, the default, copies it as the
  * reference copies its host input (projection_algorithm_impl.cuh:816-858);
  * host arrays are always copied. */
 int ffcz_set_borrow_e0(ffcz_solver* h, int borrow);
