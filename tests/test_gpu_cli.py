@@ -10,8 +10,10 @@ from oracle import ffcz_oracle as O
 pytestmark = pytest.mark.gpu
 
 
+# (69, 69, 115): a field shape with prime factor 23 (run-time planned kernels)
 @pytest.mark.parametrize("shape, flag, dtype, tflag", [((16, 20, 30), "-3", np.float32, "-f"),
-                                                       ((1, 64, 64), "-2", np.float64, "-d")])
+                                                       ((1, 64, 64), "-2", np.float64, "-d"),
+                                                       ((69, 69, 115), "-3", np.float32, "-f")])
 def test_compress_then_decompress(gpu, tmp_path, shape, flag, dtype, tflag):
     rng = np.random.default_rng(5)
     x = rng.random(shape).astype(dtype)
