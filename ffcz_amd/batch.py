@@ -46,13 +46,22 @@ class ProjectionBatch:
         if len(spatial_errors) != len(self.fields):
             raise ValueError("one initial error per field")
         keep = []
+        devices = set()
         ptrs = (C.c_void_p * len(self.fields))()
         for i, (f, e) in enumerate(zip(self.fields, spatial_errors)):
-            p, k = as_pointer(e, self.dtype)
+            p, k = as_pointer(e, self.dtype, sync=False)
             if int(np.prod(k.shape)) != f.N:
                 raise ValueError(f"initial error {i} has the wrong size")
             keep.append(k)
             ptrs[i] = p.value
+            if getattr(k, "is_cuda", False):
+                devices.add(k.device)
+        if devices:
+            import torch
+
+            # the library works on its own non-blocking streams: one wait per device
+            for d in devices:
+                torch.cuda.current_stream(d).synchronize()
         call("ffcz_batch_solve", self._h, int(mode), ptrs, int(max_iterations), float(convergence_tolerance),
              int(bool(use_graph)))
         del keep

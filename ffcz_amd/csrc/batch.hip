@@ -78,8 +78,20 @@ void Batch<T>::build(bool use_graph) {
         ffts.push_back(&f->fft());
         args.push_back(f->iter_args());
     }
-    release_graph();
     batch_plan_build(plan_, ffts, args);
+    // The captured iteration reads every batched field's arguments from the
+    // plan's device tables, which the build above rewrote in place.  The graph
+    // is captured again only when its launches change (axis lengths, grids,
+    // table addresses) or when fields run outside the batched kernels (their
+    // arguments are kernel parameters inside the graph).
+    std::vector<uintptr_t> sig;
+    for (const auto& g : plan_.groups)
+        sig.insert(sig.end(), {(uintptr_t)g.pass, (uintptr_t)g.L, (uintptr_t)g.smode, (uintptr_t)g.grid,
+                               (uintptr_t)g.args, (uintptr_t)g.start, (uintptr_t)g.n});
+    sig.insert(sig.end(), {(uintptr_t)plan_.fin, (uintptr_t)plan_.nfin});
+    if (use_graph && gexec_ && plan_.fallback.empty() && sig == gsig_) return;
+    release_graph();
+    gsig_.clear();
     if (use_graph) {
         hipGraph_t g = nullptr;
         FFCZ_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
@@ -94,6 +106,7 @@ void Batch<T>::build(bool use_graph) {
         const hipError_t e = hipGraphInstantiate(&gexec_, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         FFCZ_HIP(e);
+        gsig_ = std::move(sig);
     }
 }
 

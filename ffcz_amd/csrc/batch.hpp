@@ -4,10 +4,12 @@
 // on its own); Batch::solve runs their projection loops in lock step: every
 // iteration is one launch per pass and axis length over all fields
 // (fft3d.hpp BatchPlan), each field gated by its own convergence state on the
-// device, and the iteration is captured once in a HIP graph that is replayed
-// between the host's convergence checks.
+// device, and the iteration is captured in a HIP graph that is replayed
+// between the host's convergence checks (and kept for later solves whose
+// launches are the same).
 #pragma once
 
+#include <cstdint>
 #include <vector>
 
 #include "solver.hpp"
@@ -49,6 +51,7 @@ class Batch {
     hipStream_t s_ = nullptr;
     BatchPlan<T> plan_;
     hipGraphExec_t gexec_ = nullptr;
+    std::vector<uintptr_t> gsig_;  // the launches gexec_ was captured with
     DevState* hstate_ = nullptr;  // pinned host copies of the fields' states
     CertStats* cert_host_ = nullptr;  // pinned host copies of the fields' certification stats
     int launched_ = 0;

@@ -28,8 +28,10 @@ def _np_dtype(dtype) -> np.dtype:
     return d
 
 
-def as_pointer(a, dtype=None):
-    """(void* address, keep-alive object) for a numpy array or torch tensor."""
+def as_pointer(a, dtype=None, sync=True):
+    """(void* address, keep-alive object) for a numpy array or torch tensor.
+    sync=False leaves the wait for torch's current stream on a device tensor
+    to the caller (one wait per device for many tensors)."""
     if a is None:
         return None, None
     try:
@@ -40,7 +42,7 @@ def as_pointer(a, dtype=None):
                 a = a.contiguous()
             if dtype is not None and a.dtype != getattr(torch, np.dtype(dtype).name):
                 raise TypeError(f"tensor dtype {a.dtype} does not match {np.dtype(dtype)}")
-            if a.is_cuda:
+            if a.is_cuda and sync:
                 # the library works on its own non-blocking stream
                 torch.cuda.current_stream(a.device).synchronize()
             return C.c_void_p(a.data_ptr()), a

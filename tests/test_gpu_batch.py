@@ -288,3 +288,34 @@ def test_batch_borrowed_initial_errors(gpu):
     assert res[0][:3] == res[1][:3]
     for (e, ds, df), (e1, ds1, df1) in zip(res[0][3], res[1][3]):
         assert np.array_equal(e, e1) and np.array_equal(ds, ds1) and np.array_equal(df, df1)
+
+
+def test_graph_kept_across_solves_with_new_initial_errors(gpu):
+    """The captured iteration is kept across solves whose launches are the same
+    and reads each solve's arguments from the plan's tables: three solves on
+    alternating device initial errors (borrowed, so the tables point at other
+    arrays each time) give the same bits with the graph as with eager launches."""
+    dtype = np.float32
+    shapes = SHAPES[:4]
+    cases = _cases(shapes, dtype, seed0=90)
+    e_a = [torch.from_numpy(c[1]).cuda() for c in cases]
+    e_b = [torch.from_numpy(np.ascontiguousarray(c[1][::-1])).cuda() for c in cases]
+    out = {}
+    for use_graph in (False, True):
+        with ProjectionBatch(shapes, dtype=dtype) as b:
+            for f, (x, e0, eps, delta) in zip(b.fields, cases):
+                f.set_borrow_e0(True)
+                f.initialize_all_abs(x, eps, delta)
+            res = []
+            for es in (e_a, e_b, e_a):
+                b.solve(MODE_ABS, es, 60, 1e-7, use_graph=use_graph)
+                res.append((b.status()[0], _batch_results(b)))
+                for f in b.fields:
+                    f.reset_edit_accumulators()
+            out[use_graph] = res
+    assert out[False][0][0] != out[False][1][0] or not all(
+        np.array_equal(a[0], b[0]) for a, b in zip(out[False][0][1], out[False][1][1]))
+    for (it0, r0), (it1, r1) in zip(out[False], out[True]):
+        assert it0 == it1
+        for (e, ds, df), (e1, ds1, df1) in zip(r0, r1):
+            assert np.array_equal(e, e1) and np.array_equal(ds, ds1) and np.array_equal(df, df1)
