@@ -36,8 +36,14 @@ void Batch<T>::release_graph() {
 template <typename T>
 void Batch<T>::solve(int mode, const T* const* e0, int max_iter, T tol, bool use_graph) {
     FFCZ_REQUIRE(e0 != nullptr, ERR_INVALID, "null initial-error table");
-    // the fields' own streams may still run their initialisation
-    for (auto* f : f_) FFCZ_HIP(hipStreamSynchronize(f->stream()));
+    // the fields' own streams may still run their initialisation (a query
+    // first: most are idle, and waiting on an idle stream still costs a
+    // round trip per field)
+    for (auto* f : f_) {
+        const hipError_t q = hipStreamQuery(f->stream());
+        if (q == hipErrorNotReady) FFCZ_HIP(hipStreamSynchronize(f->stream()));
+        else FFCZ_HIP(q);
+    }
     for (size_t i = 0; i < f_.size(); ++i) f_[i]->prepare_solve(mode, e0[i], max_iter, tol, s_, false);
     // the batched fields run the fused iteration form (its batched launches
     // use the batch's z tile variant, which sets the log layout); the others
